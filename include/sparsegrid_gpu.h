@@ -1,0 +1,213 @@
+/*
+ * sparsegrid_gpu.h -- C ABI of the MI355X (gfx950) block-sparse grid library
+ * (libsg_hip.so).  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * The library re-implements the SparseGridGpu hot path of openfpm_data:
+ * chunk-key insertion + flush (sort, duplicate merge with smax/sadd/...,
+ * merge into the sorted chunk map), findNeighbours, tagBoundaries,
+ * applyStencils/conv over edge^dim chunks, and point get.  The C++ class
+ * template in SparseGridGpu.hpp (same directory) is the drop-in facade over
+ * these entry points; every entry point below names the reference interface
+ * it replaces (paths relative to the openfpm_data source tree).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative SG_E* code; the
+ *    message of the last failure on the calling thread is sg_last_error().
+ *    (The reference prints to std::cout and continues,
+ *    src/SparseGridGpu/SparseGridGpu.hpp:2350, :3158; an error code never
+ *    throws across the ABI.)
+ *  - Work is stream ordered on the handle's HIP stream (sg_set_stream);
+ *    calls that return sizes to the host synchronise that stream.
+ *  - `mem` arguments say where a caller buffer lives: SG_MEM_HOST or
+ *    SG_MEM_DEVICE (a device pointer valid on the handle's device).
+ *  - Point keys are grid_smb linear ids: chunkId * edge^dim + localId, with
+ *    chunkId row-major x-fastest over ceil(size/edge) chunks and localId
+ *    x-fastest inside the chunk (src/Grid/Geometry/grid_smb.hpp:175-189).
+ *  - Layout in device memory (chunk-major SoA, as the reference's
+ *    memory_traits_inte DataBlock arrays, src/SparseGridGpu/BlockMapGpu.hpp:63-69):
+ *    one array per property of (nChunks+1)*edge^dim scalars and one uint8
+ *    mask array of the same shape; chunk nChunks is the background chunk
+ *    (src/Vector/map_vector_sparse.hpp:1356-1364).
+ */
+#ifndef SPARSEGRID_GPU_H
+#define SPARSEGRID_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sg_grid sg_grid;
+
+/* status codes */
+#define SG_OK 0
+#define SG_E_ARG (-1)     /* invalid argument */
+#define SG_E_HIP (-2)     /* HIP runtime failure */
+#define SG_E_OOM (-3)     /* device allocation failed */
+#define SG_E_STATE (-4)   /* call not valid in this state */
+#define SG_E_RANGE (-5)   /* size or coordinate out of the supported range */
+
+/* where a caller buffer lives */
+#define SG_MEM_HOST 0
+#define SG_MEM_DEVICE 1
+
+/* property scalar types */
+typedef enum {
+    SG_F32 = 0, SG_F64 = 1, SG_I32 = 2, SG_U32 = 3, SG_I64 = 4,
+    SG_U64 = 5, SG_I8 = 6, SG_U8 = 7, SG_I16 = 8, SG_U16 = 9
+} sg_dtype;
+
+/* flush reductions: smax_, smin_, sadd_, sRight_, sLeft_, sBitwiseOr_
+ * (src/Vector/cuda/map_vector_sparse_cuda_kernels.cuh:50-450).
+ * SG_OP_NONE marks a property absent from the reference's v_reduce list;
+ * the reference leaves such a property unwritten (undefined) for merged
+ * chunks; this library defines it as "last inserted value wins, points
+ * without an insert keep their old value". */
+typedef enum {
+    SG_OP_NONE = 0, SG_OP_MAX = 1, SG_OP_MIN = 2, SG_OP_ADD = 3,
+    SG_OP_RIGHT = 4, SG_OP_LEFT = 5, SG_OP_OR = 6
+} sg_op;
+
+/* mask bits (src/SparseGridGpu/SparseGridGpu_kernels.cuh:21-27) */
+#define SG_EXIST 1
+#define SG_PADDING 2
+
+/* neighbourhood types: NNStar (2*dim face chunks) and NNFull (3^dim chunks)
+ * (src/SparseGridGpu/SparseGridGpu_ker_util.hpp:91-157,
+ *  src/SparseGridGpu/SparseGridGpu.hpp:262-352) */
+#define SG_NN_STAR 0
+#define SG_NN_FULL 1
+
+/* tagBoundaries option (src/SparseGridGpu/SparseGridGpu.hpp:31-35) */
+#define SG_NO_CALCULATE_EXISTING_POINTS 0
+#define SG_CALCULATE_EXISTING_POINTS 1
+
+/* Stencil descriptors.  The reference takes an arbitrary __device__ functor
+ * (src/SparseGridGpu/SparseGridGpu.hpp:2704-2734); a C ABI cannot carry one,
+ * so the stencils used by the reference's tests and benchmarks are named
+ * here and evaluated by hand-written kernels. */
+typedef enum {
+    /* dst = c0*u + sum_d c1*(u[-e_d] + u[+e_d]); conv with the 7-point
+     * Laplacian of HeatStencil (tests/utils/SparseGridGpu_util_test.cuh:240-255) */
+    SG_ST_LAP = 0,
+    /* dst = u + coef[0] * (-2*dim*u + sum_d (u[-e_d] + u[+e_d]))
+     * (HeatStencil, tests/utils/SparseGridGpu_util_test.cuh:201-260) */
+    SG_ST_HEAT = 1,
+    /* two-property Gray-Scott reaction-diffusion step (conv2 form,
+     * src/SparseGridGpu/SparseGridGpu.hpp:2668-2683):
+     * coef = {Du, Dv, F, k, dt};  src = {u, v}, dst = {u', v'} */
+    SG_ST_GRAY_SCOTT = 2,
+    /* 3^dim box convolution, tot = sum_{i,j,k} coef[i*9+j*3+k]*u[i-1,j-1,k-1]
+     * (Conv3x3x3, tests/utils/SparseGridGpu_util_test.cuh:357-420);
+     * integer properties use icoef[] and wrapping integer arithmetic */
+    SG_ST_BOX = 3,
+    /* red-black SOR sweep of -lap(u) = f, in place on src[0] with f = src[1]:
+     * u = (1-w)*u + w*(f + sum of 2*dim neighbours)/(2*dim) on points with
+     * (x+y+z)%2 == color; coef = {w}, color = icoef[0] */
+    SG_ST_RBSOR = 4,
+    /* residual r = f - A u with A = -lap (7 point): dst[0] = src[1] -
+     * (2*dim*u - sum neighbours), u = src[0] */
+    SG_ST_RESIDUAL = 5
+} sg_stencil_kind;
+
+/* which points a stencil writes */
+#define SG_APPLY_INTERIOR 0  /* EXIST && !PADDING  (conv/conv2 functors, SparseGridGpu_kernels.cuh:281) */
+#define SG_APPLY_EXISTING 1  /* mask != 0          (Conv3x3x3 called with bool(mask)) */
+
+typedef struct {
+    int32_t kind;          /* sg_stencil_kind */
+    int32_t apply;         /* SG_APPLY_* */
+    int32_t src[4];        /* property indices read */
+    int32_t dst[4];        /* property indices written */
+    int32_t use_box;       /* 0: whole grid; 1: restrict to [box_lo, box_hi] inclusive */
+    int32_t box_lo[3];
+    int32_t box_hi[3];
+    int32_t pad_;
+    double coef[27];       /* floating coefficients (see kinds) */
+    int64_t icoef[27];     /* integer coefficients / integer parameters */
+} sg_stencil_desc;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* SparseGridGpu(const size_t (&res)[dim], stencilSupportRadius)
+ * (src/SparseGridGpu/SparseGridGpu.hpp:1694-1698) for
+ * aggregate<dtypes...> and chunk edge block_edge (dim 2 or 3;
+ * block_edge^dim <= 1024; stencil_radius must be 1). */
+int sg_create(int dim, const int64_t *size, int block_edge, int nprops,
+              const int32_t *dtypes, int stencil_radius, sg_grid **out);
+int sg_destroy(sg_grid *g);
+const char *sg_last_error(void);
+/* set the HIP stream (hipStream_t as void*) all later work goes to; NULL = default stream */
+int sg_set_stream(sg_grid *g, void *stream);
+int sg_synchronize(sg_grid *g);
+
+/* setBackgroundValue<p> (src/SparseGridGpu/SparseGridGpu.hpp:2878-2884,
+ * BlockMapGpu.hpp:306-326); value points to one host scalar of the property type */
+int sg_set_background(sg_grid *g, int prop, const void *value);
+
+/* ---- insertion + flush ---------------------------------------------------- */
+
+/* Point insertion: sg.insert<p>(coord) per point, one insert slot per point
+ * (src/SparseGridGpu/BlockMapGpu_ker.cuh:171-197,
+ *  src/Vector/cuda/map_vector_sparse_cuda_ker.cuh:309-324).  Records are
+ * appended to the handle's insert buffer in call order and merged by
+ * sg_flush in that order.  masks may be NULL (every record EXIST);
+ * values[p] may be NULL (value unspecified; the record only sets mask bits). */
+int sg_insert_points(sg_grid *g, int64_t n, const int64_t *keys,
+                     const uint8_t *masks, const void *const *values, int mem);
+/* same, with integer coordinates [n][dim] instead of linear keys */
+int sg_insert_coords(sg_grid *g, int64_t n, const int32_t *coords,
+                     const uint8_t *masks, const void *const *values, int mem);
+/* Chunk insertion: insertBlock(blockId) per chunk slot
+ * (src/SparseGridGpu/BlockMapGpu_ker.cuh:199-227); masks is [n][edge^dim],
+ * values[p] is [n][edge^dim]. */
+int sg_insert_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids,
+                     const uint8_t *masks, const void *const *values, int mem);
+/* number of records waiting in the insert buffer */
+int64_t sg_pending_inserts(sg_grid *g);
+/* flush<v_reduce...>(ctx, FLUSH_ON_DEVICE) (src/SparseGridGpu/SparseGridGpu.hpp:772-779,
+ * src/Vector/map_vector_sparse.hpp:1192-1226); ops[p] for every property. */
+int sg_flush(sg_grid *g, const int32_t *ops);
+
+/* ---- neighbourhood, boundaries, stencils ---------------------------------- */
+
+/* findNeighbours<NN>() (src/SparseGridGpu/SparseGridGpu.hpp:2396-2420) */
+int sg_find_neighbours(sg_grid *g, int nn_type);
+/* setNNType<NN>() (src/SparseGridGpu/SparseGridGpu.hpp:1825-1829) */
+int sg_set_nn_type(sg_grid *g, int nn_type);
+/* tagBoundaries<NN>(ctx, No_check(), opt) (src/SparseGridGpu/SparseGridGpu.hpp:2290-2394) */
+int sg_tag_boundaries(sg_grid *g, int nn_type, int opt);
+/* applyStencils / conv / conv2 (src/SparseGridGpu/SparseGridGpu.hpp:2568-2734) */
+int sg_apply_stencil(sg_grid *g, const sg_stencil_desc *desc);
+
+/* ---- queries ---------------------------------------------------------------- */
+
+int64_t sg_num_chunks(sg_grid *g);
+int sg_chunk_size(sg_grid *g);
+/* sorted chunk ids (private_get_index_array), [nChunks] int64 */
+int sg_get_chunk_ids(sg_grid *g, int64_t *out, int mem);
+/* masks [nChunks][edge^dim] (private_get_data_array<pMask>) */
+int sg_get_masks(sg_grid *g, uint8_t *out, int mem);
+/* property p [nChunks][edge^dim] (deviceToHost<p>) */
+int sg_get_prop(sg_grid *g, int prop, void *out, int mem);
+/* overwrite property p of the existing chunks (hostToDevice<p>) */
+int sg_set_prop(sg_grid *g, int prop, const void *in, int mem);
+/* get<p>(coord) for n point keys; background where the point does not exist
+ * (src/SparseGridGpu/SparseGridGpu.hpp:1897-1901, BlockMapGpu.hpp:108-126) */
+int sg_get_points(sg_grid *g, int prop, int64_t n, const int64_t *keys,
+                  void *out, int mem);
+/* neighbour table [nChunks][nNN] int32 built by sg_find_neighbours
+ * (private_get_neighborhood_array); nNN = 2^dim (star) or 3^dim (full) */
+int sg_get_neighbours(sg_grid *g, int nn_type, int32_t *out, int mem);
+/* device address of property p's array (prop == -1: the mask array) */
+int sg_device_ptr(sg_grid *g, int prop, void **out);
+/* countExistingElements / countBoundaryElements (src/SparseGridGpu/SparseGridGpu.hpp:2422-2490) */
+int sg_count_existing(sg_grid *g, int64_t *out);
+int sg_count_boundary(sg_grid *g, int64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPARSEGRID_GPU_H */

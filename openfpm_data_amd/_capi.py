@@ -1,0 +1,156 @@
+"""ctypes binding of libsg_hip.so (the C ABI declared in include/sparsegrid_gpu.h).
+
+The library is built in-tree by ``openfpm_data_amd/csrc/Makefile`` (``__graft_entry__.build()``).
+Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsg_hip.so")
+
+SG_OK = 0
+SG_MEM_HOST = 0
+SG_MEM_DEVICE = 1
+
+# dtypes (sg_dtype)
+SG_F32, SG_F64, SG_I32, SG_U32, SG_I64, SG_U64, SG_I8, SG_U8, SG_I16, SG_U16 = range(10)
+DTYPE_CODES = {
+    np.dtype(np.float32): SG_F32,
+    np.dtype(np.float64): SG_F64,
+    np.dtype(np.int32): SG_I32,
+    np.dtype(np.uint32): SG_U32,
+    np.dtype(np.int64): SG_I64,
+    np.dtype(np.uint64): SG_U64,
+    np.dtype(np.int8): SG_I8,
+    np.dtype(np.uint8): SG_U8,
+    np.dtype(np.int16): SG_I16,
+    np.dtype(np.uint16): SG_U16,
+}
+CODE_DTYPES = {v: k for k, v in DTYPE_CODES.items()}
+
+# reductions (sg_op)
+OP_NONE, OP_MAX, OP_MIN, OP_ADD, OP_RIGHT, OP_LEFT, OP_OR = range(7)
+OPS = {"none": OP_NONE, "smax": OP_MAX, "smin": OP_MIN, "sadd": OP_ADD,
+       "sright": OP_RIGHT, "sleft": OP_LEFT, "sor": OP_OR}
+
+EXIST = 1
+PADDING = 2
+NN_STAR = 0
+NN_FULL = 1
+NO_CALCULATE_EXISTING_POINTS = 0
+CALCULATE_EXISTING_POINTS = 1
+
+# stencil kinds (sg_stencil_kind) and apply modes
+ST_LAP, ST_HEAT, ST_GRAY_SCOTT, ST_BOX, ST_RBSOR, ST_RESIDUAL = range(6)
+APPLY_INTERIOR = 0
+APPLY_EXISTING = 1
+
+
+class StencilDesc(ctypes.Structure):
+    """sg_stencil_desc (include/sparsegrid_gpu.h)."""
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("apply", ctypes.c_int32),
+        ("src", ctypes.c_int32 * 4),
+        ("dst", ctypes.c_int32 * 4),
+        ("use_box", ctypes.c_int32),
+        ("box_lo", ctypes.c_int32 * 3),
+        ("box_hi", ctypes.c_int32 * 3),
+        ("pad_", ctypes.c_int32),
+        ("coef", ctypes.c_double * 27),
+        ("icoef", ctypes.c_int64 * 27),
+    ]
+
+
+def make_desc(kind, src, dst=(), apply=APPLY_INTERIOR, box=None, coef=(), icoef=()):
+    d = StencilDesc()
+    d.kind = kind
+    d.apply = apply
+    for i, s in enumerate(src):
+        d.src[i] = s
+    for i, s in enumerate(dst):
+        d.dst[i] = s
+    if box is not None:
+        d.use_box = 1
+        lo, hi = box
+        for i in range(len(lo)):
+            d.box_lo[i] = int(lo[i])
+            d.box_hi[i] = int(hi[i])
+    for i, c in enumerate(coef):
+        d.coef[i] = float(c)
+    for i, c in enumerate(icoef):
+        d.icoef[i] = int(c)
+    return d
+
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_INT = ctypes.c_int
+
+_SIGS = {
+    "sg_create": (_INT, [_INT, ctypes.POINTER(_I64), _INT, _INT, ctypes.POINTER(_I32), _INT, ctypes.POINTER(_P)]),
+    "sg_destroy": (_INT, [_P]),
+    "sg_last_error": (ctypes.c_char_p, []),
+    "sg_set_stream": (_INT, [_P, _P]),
+    "sg_synchronize": (_INT, [_P]),
+    "sg_set_background": (_INT, [_P, _INT, _P]),
+    "sg_insert_points": (_INT, [_P, _I64, _P, _P, _P, _INT]),
+    "sg_insert_coords": (_INT, [_P, _I64, _P, _P, _P, _INT]),
+    "sg_insert_chunks": (_INT, [_P, _I64, _P, _P, _P, _INT]),
+    "sg_pending_inserts": (_I64, [_P]),
+    "sg_flush": (_INT, [_P, ctypes.POINTER(_I32)]),
+    "sg_find_neighbours": (_INT, [_P, _INT]),
+    "sg_set_nn_type": (_INT, [_P, _INT]),
+    "sg_tag_boundaries": (_INT, [_P, _INT, _INT]),
+    "sg_apply_stencil": (_INT, [_P, ctypes.POINTER(StencilDesc)]),
+    "sg_num_chunks": (_I64, [_P]),
+    "sg_chunk_size": (_INT, [_P]),
+    "sg_get_chunk_ids": (_INT, [_P, _P, _INT]),
+    "sg_get_masks": (_INT, [_P, _P, _INT]),
+    "sg_get_prop": (_INT, [_P, _INT, _P, _INT]),
+    "sg_set_prop": (_INT, [_P, _INT, _P, _INT]),
+    "sg_get_points": (_INT, [_P, _INT, _I64, _P, _P, _INT]),
+    "sg_get_neighbours": (_INT, [_P, _INT, _P, _INT]),
+    "sg_device_ptr": (_INT, [_P, _INT, ctypes.POINTER(_P)]),
+    "sg_count_existing": (_INT, [_P, ctypes.POINTER(_I64)]),
+    "sg_count_boundary": (_INT, [_P, ctypes.POINTER(_I64)]),
+}
+
+_lib = None
+
+
+def exported_symbols():
+    """Names of the entry points this binding expects the library to export."""
+    return sorted(_SIGS)
+
+
+def load():
+    """Load libsg_hip.so once; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libsg_hip.so is not built (expected %s); run __graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class SgError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != SG_OK:
+        msg = load().sg_last_error()
+        raise SgError("libsg_hip error %d: %s" % (rc, msg.decode() if msg else ""))
+    return rc

@@ -1,0 +1,355 @@
+// sg_capi.hip -- the extern "C" entry points of libsg_hip.so (include/sparsegrid_gpu.h).
+// Argument checking, handle lifetime, host<->device copies; the kernels live in
+// sg_flush.hip, sg_stencil.hip and sg_primitives.hip.
+#include "sg_internal.hpp"
+
+#include <algorithm>
+
+namespace sg {
+
+static thread_local std::string t_err;
+
+void set_error(const std::string &msg) { t_err = msg; }
+
+int fail(int code, const std::string &msg)
+{
+    t_err = msg;
+    return code;
+}
+
+void *DevicePool::alloc(size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    bytes = (bytes + 255) & ~(size_t)255;
+    // reuse a cached block of at least `bytes` and at most twice that
+    auto it = cache_.lower_bound(bytes);
+    if (it != cache_.end() && it->first <= 2 * bytes) {
+        void *p = it->second;
+        cache_.erase(it);
+        live_ += sizes_[p];
+        return p;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        // drop the cache and retry once
+        for (auto &kv : cache_) {
+            (void)hipFree(kv.second);
+            sizes_.erase(kv.second);
+        }
+        cache_.clear();
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    sizes_[p] = bytes;
+    live_ += bytes;
+    return p;
+}
+
+void DevicePool::free(void *p)
+{
+    if (!p) return;
+    auto it = sizes_.find(p);
+    if (it == sizes_.end()) return;
+    live_ -= it->second;
+    cache_.emplace(it->second, p);
+}
+
+void DevicePool::release()
+{
+    for (auto &kv : sizes_) (void)hipFree(kv.first);
+    sizes_.clear();
+    cache_.clear();
+    live_ = 0;
+}
+
+int ensure(DevicePool &pool, DevBuf &b, size_t bytes)
+{
+    if (b.bytes >= bytes && b.p) return SG_OK;
+    pool.free(b.p);
+    size_t want = std::max(bytes, b.bytes + b.bytes / 2);
+    b.p = pool.alloc(want);
+    if (!b.p) {
+        b.bytes = 0;
+        return fail(SG_E_OOM, "scratch allocation failed");
+    }
+    b.bytes = want;
+    return SG_OK;
+}
+
+}  // namespace sg
+
+using namespace sg;
+
+namespace {
+
+bool valid_prop(const sg_grid *g, int p) { return g && p >= 0 && p < g->np; }
+
+int alloc_chunk_arrays(sg_grid *g)
+{
+    // empty grid: the background chunk only
+    g->cap = 0;
+    g->nchunks = 0;
+    g->d_keys = (int64_t *)g->pool.alloc(sizeof(int64_t));
+    g->d_mask = (uint8_t *)g->pool.alloc(g->csz);
+    if (!g->d_keys || !g->d_mask) return fail(SG_E_OOM, "allocation failed");
+    SG_HIP(hipMemsetAsync(g->d_mask, 0, g->csz, g->stream));
+    for (int p = 0; p < g->np; ++p) {
+        g->d_prop[p] = g->pool.alloc((size_t)g->csz * g->ts[p]);
+        if (!g->d_prop[p]) return fail(SG_E_OOM, "allocation failed");
+        int rc = fill_background_chunk(g, p);
+        if (rc) return rc;
+    }
+    return SG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sg_last_error(void) { return t_err.c_str(); }
+
+int sg_create(int dim, const int64_t *size, int block_edge, int nprops, const int32_t *dtypes, int stencil_radius,
+              sg_grid **out)
+{
+    if (!out || !size) return fail(SG_E_ARG, "null argument");
+    *out = nullptr;
+    if (dim < 2 || dim > 3) return fail(SG_E_ARG, "dim must be 2 or 3");
+    if (block_edge < 2) return fail(SG_E_ARG, "block_edge must be >= 2");
+    int64_t csz = 1;
+    for (int d = 0; d < dim; ++d) csz *= block_edge;
+    if (csz > 1024) return fail(SG_E_RANGE, "block_edge^dim must be <= 1024");
+    if (stencil_radius != 1) return fail(SG_E_ARG, "stencil_radius must be 1 (the reference's kernels assume 1)");
+    if (nprops < 0 || nprops > kMaxProps) return fail(SG_E_ARG, "nprops must be in [0, 16]");
+    if (nprops > 0 && !dtypes) return fail(SG_E_ARG, "null dtypes");
+    sg_grid *g = new sg_grid();
+    g->dim = dim;
+    g->edge = block_edge;
+    g->csz = (int)csz;
+    g->r = stencil_radius;
+    int64_t nblocks = 1;
+    for (int d = 0; d < dim; ++d) {
+        if (size[d] <= 0) {
+            delete g;
+            return fail(SG_E_ARG, "size must be positive");
+        }
+        g->sz[d] = size[d];
+        g->bsz[d] = size[d] / block_edge + ((size[d] % block_edge) != 0);  // grid_smb.hpp:47-57
+        nblocks *= g->bsz[d];
+    }
+    if (nblocks > ((int64_t)1 << 31) - 2) {
+        delete g;
+        return fail(SG_E_RANGE, "more than 2^31 chunks in the domain");
+    }
+    g->np = nprops;
+    for (int p = 0; p < nprops; ++p) {
+        g->dt[p] = dtypes[p];
+        g->ts[p] = (int)dtype_size(dtypes[p]);
+        if (!g->ts[p]) {
+            delete g;
+            return fail(SG_E_ARG, "unknown dtype");
+        }
+        memset(g->bg[p], 0, sizeof(g->bg[p]));
+    }
+    int rc = alloc_chunk_arrays(g);
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return SG_OK;
+}
+
+int sg_destroy(sg_grid *g)
+{
+    if (!g) return SG_OK;
+    (void)hipStreamSynchronize(g->stream);
+    g->pool.release();
+    delete g;
+    return SG_OK;
+}
+
+int sg_set_stream(sg_grid *g, void *stream)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    // work already queued on the old stream must finish before the new stream runs
+    SG_HIP(hipStreamSynchronize(g->stream));
+    g->stream = (hipStream_t)stream;
+    return SG_OK;
+}
+
+int sg_synchronize(sg_grid *g)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    SG_HIP(hipStreamSynchronize(g->stream));
+    return SG_OK;
+}
+
+int sg_set_background(sg_grid *g, int prop, const void *value)
+{
+    if (!valid_prop(g, prop) || !value) return fail(SG_E_ARG, "bad property or value");
+    memcpy(g->bg[prop], value, g->ts[prop]);
+    // background chunk + every slot without a point takes the new value
+    int rc = normalize_prop(g, prop);
+    if (rc) return rc;
+    return fill_background_chunk(g, prop);
+}
+
+int sg_insert_points(sg_grid *g, int64_t n, const int64_t *keys, const uint8_t *masks, const void *const *values,
+                     int mem)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    int rc = append_points(g, n, keys, masks, values, mem);
+    if (rc == SG_OK && mem == SG_MEM_HOST) SG_HIP(hipStreamSynchronize(g->stream));
+    return rc;
+}
+
+int sg_insert_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *masks, const void *const *values,
+                     int mem)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    int rc = append_coords(g, n, coords, masks, values, mem);
+    if (rc == SG_OK && mem == SG_MEM_HOST) SG_HIP(hipStreamSynchronize(g->stream));
+    return rc;
+}
+
+int sg_insert_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t *masks,
+                     const void *const *values, int mem)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    int rc = append_chunks(g, n, chunk_ids, masks, values, mem);
+    if (rc == SG_OK && mem == SG_MEM_HOST) SG_HIP(hipStreamSynchronize(g->stream));
+    return rc;
+}
+
+int64_t sg_pending_inserts(sg_grid *g) { return g ? g->ins_n : 0; }
+
+int sg_flush(sg_grid *g, const int32_t *ops)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    for (int p = 0; p < g->np; ++p) {
+        int op = ops ? ops[p] : SG_OP_NONE;
+        if (op < SG_OP_NONE || op > SG_OP_OR) return fail(SG_E_ARG, "unknown reduction");
+        if (op == SG_OP_OR && dtype_is_float(g->dt[p])) return fail(SG_E_ARG, "bitwise or on a float property");
+    }
+    return flush_impl(g, ops);
+}
+
+int sg_find_neighbours(sg_grid *g, int nn_type)
+{
+    if (!g || (nn_type != SG_NN_STAR && nn_type != SG_NN_FULL)) return fail(SG_E_ARG, "bad grid or nn type");
+    return find_neighbours_impl(g, nn_type);
+}
+
+int sg_set_nn_type(sg_grid *g, int nn_type)
+{
+    if (!g || (nn_type != SG_NN_STAR && nn_type != SG_NN_FULL)) return fail(SG_E_ARG, "bad grid or nn type");
+    // the kernels always stage the ghost cells their own neighbourhood needs
+    g->ghost_type = nn_type;
+    return SG_OK;
+}
+
+int sg_tag_boundaries(sg_grid *g, int nn_type, int opt)
+{
+    if (!g || (nn_type != SG_NN_STAR && nn_type != SG_NN_FULL)) return fail(SG_E_ARG, "bad grid or nn type");
+    if (opt != SG_NO_CALCULATE_EXISTING_POINTS && opt != SG_CALCULATE_EXISTING_POINTS)
+        return fail(SG_E_ARG, "bad option");
+    return tag_boundaries_impl(g, nn_type);
+}
+
+int sg_apply_stencil(sg_grid *g, const sg_stencil_desc *desc)
+{
+    if (!g || !desc) return fail(SG_E_ARG, "null argument");
+    return apply_stencil_impl(g, desc);
+}
+
+int64_t sg_num_chunks(sg_grid *g) { return g ? g->nchunks : 0; }
+
+int sg_chunk_size(sg_grid *g) { return g ? g->csz : 0; }
+
+static hipMemcpyKind out_kind(int mem) { return mem == SG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost; }
+
+int sg_get_chunk_ids(sg_grid *g, int64_t *out, int mem)
+{
+    if (!g || !out) return fail(SG_E_ARG, "null argument");
+    if (g->nchunks)
+        SG_HIP(hipMemcpyAsync(out, g->d_keys, g->nchunks * sizeof(int64_t), out_kind(mem), g->stream));
+    SG_HIP(hipStreamSynchronize(g->stream));
+    return SG_OK;
+}
+
+int sg_get_masks(sg_grid *g, uint8_t *out, int mem)
+{
+    if (!g || !out) return fail(SG_E_ARG, "null argument");
+    if (g->nchunks) SG_HIP(hipMemcpyAsync(out, g->d_mask, g->nchunks * g->csz, out_kind(mem), g->stream));
+    SG_HIP(hipStreamSynchronize(g->stream));
+    return SG_OK;
+}
+
+int sg_get_prop(sg_grid *g, int prop, void *out, int mem)
+{
+    if (!valid_prop(g, prop) || !out) return fail(SG_E_ARG, "bad property or output");
+    if (g->nchunks)
+        SG_HIP(hipMemcpyAsync(out, g->d_prop[prop], (size_t)g->nchunks * g->csz * g->ts[prop], out_kind(mem),
+                              g->stream));
+    SG_HIP(hipStreamSynchronize(g->stream));
+    return SG_OK;
+}
+
+int sg_set_prop(sg_grid *g, int prop, const void *in, int mem)
+{
+    if (!valid_prop(g, prop) || !in) return fail(SG_E_ARG, "bad property or input");
+    if (g->nchunks) {
+        SG_HIP(hipMemcpyAsync(g->d_prop[prop], in, (size_t)g->nchunks * g->csz * g->ts[prop],
+                              mem == SG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, g->stream));
+        // slots without a point hold the background
+        int rc = normalize_prop(g, prop);
+        if (rc) return rc;
+    }
+    if (mem == SG_MEM_HOST) SG_HIP(hipStreamSynchronize(g->stream));
+    return SG_OK;
+}
+
+int sg_get_points(sg_grid *g, int prop, int64_t n, const int64_t *keys, void *out, int mem)
+{
+    if (!valid_prop(g, prop) || (n > 0 && (!keys || !out))) return fail(SG_E_ARG, "bad argument");
+    return get_points_impl(g, prop, n, keys, out, mem);
+}
+
+int sg_get_neighbours(sg_grid *g, int nn_type, int32_t *out, int mem)
+{
+    if (!g || !out || (nn_type != SG_NN_STAR && nn_type != SG_NN_FULL)) return fail(SG_E_ARG, "bad argument");
+    if (!g->nn_valid[nn_type]) {
+        int rc = find_neighbours_impl(g, nn_type);
+        if (rc) return rc;
+    }
+    const size_t bytes = (size_t)g->nchunks * g->nnn(nn_type) * sizeof(int32_t);
+    if (bytes) SG_HIP(hipMemcpyAsync(out, g->d_nn[nn_type], bytes, out_kind(mem), g->stream));
+    SG_HIP(hipStreamSynchronize(g->stream));
+    return SG_OK;
+}
+
+int sg_device_ptr(sg_grid *g, int prop, void **out)
+{
+    if (!g || !out) return fail(SG_E_ARG, "null argument");
+    if (prop == -1) *out = g->d_mask;
+    else if (valid_prop(g, prop)) *out = g->d_prop[prop];
+    else return fail(SG_E_ARG, "bad property");
+    return SG_OK;
+}
+
+int sg_count_existing(sg_grid *g, int64_t *out)
+{
+    if (!g || !out) return fail(SG_E_ARG, "null argument");
+    return count_mask_impl(g, 0, out);
+}
+
+int sg_count_boundary(sg_grid *g, int64_t *out)
+{
+    if (!g || !out) return fail(SG_E_ARG, "null argument");
+    return count_mask_impl(g, 1, out);
+}
+
+}  // extern "C"

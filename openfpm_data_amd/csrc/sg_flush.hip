@@ -1,0 +1,795 @@
+// sg_flush.hip -- insertion buffer and flush for gfx950.
+//
+// Reference pipeline (src/Vector/map_vector_sparse.hpp:1192-1226 ->
+// src/SparseGridGpu/BlockMapGpu_kernels.cuh:700-807): every insert claims a whole
+// chunk slot; slots are compacted, stably radix-sorted by chunk key, run-length
+// segmented, merged with the old sorted chunk list, and each output chunk is the
+// per-offset fold of its slots in sorted (= insertion) order followed by a fold
+// with the old chunk: a = op(a,b) when both exist, a = b when only b exists, mask =
+// OR of all masks.
+//
+// Here the records are points, not chunk slots (a 2^24-insert batch as chunk slots
+// would need 2^24 * 4.6 KB): records are (point key, mask byte, values).  The fold a
+// point receives is the same: the records of that point in insertion order, then
+// the old value.  A chunk slot of the reference contributes to an offset only
+// through its mask there (mask 0 leaves the fold unchanged), so splitting slots into
+// points preserves the result at every existing point.
+//
+// Steps: stable LSD radix sort of (point key, record id) -> run heads + scan ->
+// unique points -> chunk heads + scan -> unique chunks -> binary-search merge with
+// the old chunk list (new chunks get positions by a merge-path count) -> one
+// thread per unique point folds its records and the old value in place.
+#include "sg_internal.hpp"
+
+#include <algorithm>
+
+namespace sg {
+namespace {
+
+constexpr int kWG = 256;
+
+template <typename T> __device__ inline T dev_add(T a, T b)
+{
+    if constexpr (std::is_integral<T>::value) {
+        typedef typename std::make_unsigned<T>::type U;
+        return (T)(U)((U)a + (U)b);
+    } else {
+        return a + b;
+    }
+}
+
+// reduction operators of the reference (map_vector_sparse_cuda_kernels.cuh:42-450);
+// max/min follow moderngpu's (a < b) ? b : a and (b < a) ? b : a
+template <typename T> __device__ inline T dev_reduce(int op, T a, T b)
+{
+    switch (op) {
+    case SG_OP_MAX: return (a < b) ? b : a;
+    case SG_OP_MIN: return (b < a) ? b : a;
+    case SG_OP_ADD: return dev_add(a, b);
+    case SG_OP_RIGHT: return b;
+    case SG_OP_LEFT: return a;
+    case SG_OP_OR:
+        if constexpr (std::is_integral<T>::value) return (T)(a | b);
+        else return a;
+    default: return b;
+    }
+}
+
+// ---- insert buffer ------------------------------------------------------------
+__global__ void fill_u8_kernel(uint8_t *p, uint8_t v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename T> __global__ void fill_kernel(T *p, T v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// coordinates -> grid_smb linear ids (grid_smb.hpp:175-189); out-of-range -> flag
+__global__ void coords_to_keys_kernel(const int32_t *coords, int64_t *keys, int64_t n, int dim, int edge, int64_t sz0,
+                                      int64_t sz1, int64_t sz2, int64_t bs0, int64_t bs1, int csz, int *bad)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t c[3] = {0, 0, 0};
+    int64_t sz[3] = {sz0, sz1, sz2};
+    bool ok = true;
+    for (int d = 0; d < dim; ++d) {
+        c[d] = coords[i * dim + d];
+        ok = ok && c[d] >= 0 && c[d] < sz[d];
+    }
+    if (!ok) {
+        *bad = 1;
+        keys[i] = 0;
+        return;
+    }
+    int64_t b, l;
+    if (dim == 3) {
+        b = ((c[2] / edge) * bs1 + c[1] / edge) * bs0 + c[0] / edge;
+        l = ((c[2] % edge) * edge + c[1] % edge) * edge + c[0] % edge;
+    } else {
+        b = (c[1] / edge) * bs0 + c[0] / edge;
+        l = (c[1] % edge) * edge + c[0] % edge;
+    }
+    keys[i] = b * csz + l;
+}
+
+__global__ void check_keys_kernel(const int64_t *keys, int64_t n, int64_t limit, int *bad)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (keys[i] < 0 || keys[i] >= limit)) *bad = 1;
+}
+
+// chunk records -> count of point records (non-zero masks; an all-zero chunk keeps
+// one record so that the chunk is still created, as the reference creates it)
+__global__ __launch_bounds__(kWG) void chunk_count_kernel(const uint8_t *masks, int csz, uint32_t *cnt)
+{
+    __shared__ uint32_t s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    const uint8_t *m = masks + (int64_t)blockIdx.x * csz;
+    uint32_t c = 0;
+    for (int o = threadIdx.x; o < csz; o += blockDim.x) c += (m[o] != 0);
+    atomicAdd(&s, c);
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = s ? s : 1;
+}
+
+// expand chunk records into point records in offset order (one workgroup per chunk;
+// a wave-ordered compaction keeps offsets ascending)
+__global__ __launch_bounds__(kWG) void chunk_expand_kernel(const int64_t *chunk_ids, const uint8_t *masks,
+                                                          const uint32_t *start, int csz, int64_t *okey,
+                                                          uint8_t *omask, uint32_t *osrc, int64_t base)
+{
+    __shared__ uint32_t s_run;
+    __shared__ uint32_t s_wave[kWG / 64];
+    const int64_t c = blockIdx.x;
+    const uint8_t *m = masks + c * csz;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_run = 0;
+    __syncthreads();
+    bool any = false;
+    for (int o0 = 0; o0 < csz; o0 += kWG) {
+        const int o = o0 + threadIdx.x;
+        const bool has = (o < csz) && m[o] != 0;
+        const uint64_t bal = __ballot(has);
+        const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+        if (lane == 0) s_wave[w] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t pre = s_run;
+        uint32_t tot = 0;
+        for (int q = 0; q < kWG / 64; ++q) {
+            if (q < w) pre += s_wave[q];
+            tot += s_wave[q];
+        }
+        if (has) {
+            uint32_t pos = start[c] + pre + (uint32_t)__popcll(bal & lt);
+            okey[pos] = chunk_ids[c] * csz + o;
+            omask[pos] = m[o];
+            osrc[pos] = (uint32_t)(c * csz + o);
+        }
+        any = any || tot != 0;
+        __syncthreads();
+        if (threadIdx.x == 0) s_run += tot;
+        __syncthreads();
+    }
+    if (!any && threadIdx.x == 0) {
+        uint32_t pos = start[c];
+        okey[pos] = chunk_ids[c] * csz;
+        omask[pos] = 0;
+        osrc[pos] = (uint32_t)(c * csz);
+    }
+    (void)base;
+}
+
+template <typename T> __global__ void gather_kernel(const T *src, const uint32_t *idx, T *dst, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+// ---- flush kernels -------------------------------------------------------------
+template <typename K> __global__ void keys_to_sort_kernel(const int64_t *keys, K *sk, uint32_t *sv, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        sk[i] = (K)keys[i];
+        sv[i] = (uint32_t)i;
+    }
+}
+
+// run-start predicate (find_buffer_offsets_for_scan, src/util/cuda/kernels.cuh:57-68)
+template <typename K> __global__ void run_head_kernel(const K *sk, uint32_t *head, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) head[i] = (i == 0 || sk[i - 1] != sk[i]) ? 1u : 0u;
+}
+
+// unique points (construct_index_unique, src/util/cuda/kernels.cuh:34-54)
+template <typename K>
+__global__ void unique_points_kernel(const K *sk, const uint32_t *head, const uint32_t *hscan, int64_t n,
+                                     int64_t *ukey, uint32_t *rstart, uint32_t nu)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && head[i]) {
+        ukey[hscan[i]] = (int64_t)sk[i];
+        rstart[hscan[i]] = (uint32_t)i;
+    }
+    if (i == 0) rstart[nu] = (uint32_t)n;
+}
+
+__global__ void chunk_head_kernel(const int64_t *ukey, uint32_t *head, int64_t nu, int csz)
+{
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nu) head[u] = (u == 0 || ukey[u - 1] / csz != ukey[u] / csz) ? 1u : 0u;
+}
+
+__global__ void unique_chunks_kernel(const int64_t *ukey, const uint32_t *head, const uint32_t *hscan, int64_t nu,
+                                     int csz, int64_t *nkey)
+{
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nu && head[u]) nkey[hscan[u]] = ukey[u] / csz;
+}
+
+__device__ inline int64_t lower_bound_dev(const int64_t *a, int64_t n, int64_t x)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// new chunk j: position in the old list or -1, and the brand-new flag
+__global__ void old_lookup_kernel(const int64_t *nkey, int64_t nn, const int64_t *okeys, int64_t m, int64_t *oldpos,
+                                  uint32_t *isnew)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nn) return;
+    int64_t k = nkey[j];
+    int64_t lb = lower_bound_dev(okeys, m, k);
+    bool found = lb < m && okeys[lb] == k;
+    oldpos[j] = found ? lb : -1;
+    isnew[j] = found ? 0u : 1u;
+}
+
+__global__ void brand_new_kernel(const int64_t *nkey, const uint32_t *isnew, const uint32_t *nscan, int64_t nn,
+                                 int64_t *bn)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nn && isnew[j]) bn[nscan[j]] = nkey[j];
+}
+
+// final chunk position of each new-touched chunk (merge-path rank in the union)
+__global__ void chunk_pos_kernel(const int64_t *nkey, const int64_t *oldpos, const uint32_t *nscan, int64_t nn,
+                                 const int64_t *okeys, int64_t m, const int64_t *bn, int64_t nb, int64_t *cpos)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nn) return;
+    int64_t k = nkey[j];
+    int64_t op = oldpos[j];
+    if (op >= 0) cpos[j] = op + lower_bound_dev(bn, nb, k);
+    else cpos[j] = (int64_t)nscan[j] + lower_bound_dev(okeys, m, k);
+}
+
+// scatter the old chunks into the enlarged arrays (copy_old_ker, BlockMapGpu_kernels.cuh:489-513)
+__global__ void move_old_keys_kernel(const int64_t *okeys, int64_t m, const int64_t *bn, int64_t nb, int64_t *nkeys,
+                                     int64_t *opos)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int64_t p = i + lower_bound_dev(bn, nb, okeys[i]);
+    nkeys[p] = okeys[i];
+    opos[i] = p;
+}
+
+__global__ void place_new_keys_kernel(const int64_t *bn, int64_t nb, const int64_t *okeys, int64_t m, int64_t *nkeys,
+                                      int64_t *bpos)
+{
+    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int64_t p = b + lower_bound_dev(okeys, m, bn[b]);
+    nkeys[p] = bn[b];
+    bpos[b] = p;
+}
+
+// one workgroup per chunk: copy chunk `src` of an array to chunk `dst` of another
+template <typename V>
+__global__ __launch_bounds__(kWG) void move_chunks_kernel(const V *src, V *dst, const int64_t *dpos, int64_t m,
+                                                         int vec_per_chunk)
+{
+    const int64_t c = blockIdx.x;
+    if (c >= m) return;
+    const int64_t d = dpos[c];
+    for (int i = threadIdx.x; i < vec_per_chunk; i += kWG) dst[d * vec_per_chunk + i] = src[c * vec_per_chunk + i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void init_chunks_kernel(T *dst, const int64_t *dpos, int64_t nb, int csz, T v)
+{
+    const int64_t b = blockIdx.x;
+    if (b >= nb) return;
+    const int64_t d = dpos ? dpos[b] : b;
+    for (int o = threadIdx.x; o < csz; o += kWG) dst[d * csz + o] = v;
+}
+
+// fold the records of each unique point, then the old value, into the chunk arrays
+// (segreduce_total_with_mask + applyOp, BlockMapGpu_kernels.cuh:326-477,
+//  BlockMapGpu_dimensionalityWrappers.cuh:214-226)
+template <typename T>
+__global__ void fold_points_kernel(const int64_t *ukey, const uint32_t *rstart, const uint32_t *perm,
+                                   const uint32_t *uchunk, const int64_t *cpos, int64_t nu, int csz,
+                                   const uint8_t *rmask, const T *rval, T *out, const uint8_t *omask, int op, T bgv)
+{
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const uint32_t r0 = rstart[u], r1 = rstart[u + 1];
+    uint32_t rec = perm[r0];
+    T a = rval[rec];
+    uint8_t am = rmask[rec];
+    for (uint32_t i = r0 + 1; i < r1; ++i) {
+        rec = perm[i];
+        const uint8_t bm = rmask[rec];
+        const bool aE = am & 1, bE = bm & 1;
+        const T b = rval[rec];
+        if (op == SG_OP_NONE) {
+            if (bE) a = b;
+        } else if (aE && bE) {
+            a = dev_reduce<T>(op, a, b);
+        } else if (bE) {
+            a = b;
+        }
+        am |= bm;
+    }
+    const int64_t slot = cpos[uchunk[u]] * csz + (ukey[u] % csz);
+    const uint8_t om = omask[slot];
+    const bool aE = am & 1, oE = om & 1;
+    if (op == SG_OP_NONE) {
+        if (!aE && oE) a = out[slot];
+    } else if (aE && oE) {
+        a = dev_reduce<T>(op, a, out[slot]);
+    } else if (oE) {
+        a = out[slot];
+    }
+    out[slot] = ((am | om) == 0) ? bgv : a;
+}
+
+__global__ void fold_masks_kernel(const int64_t *ukey, const uint32_t *rstart, const uint32_t *perm,
+                                  const uint32_t *uchunk, const int64_t *cpos, int64_t nu, int csz,
+                                  const uint8_t *rmask, uint8_t *omask)
+{
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    uint8_t am = 0;
+    for (uint32_t i = rstart[u]; i < rstart[u + 1]; ++i) am |= rmask[perm[i]];
+    const int64_t slot = cpos[uchunk[u]] * csz + (ukey[u] % csz);
+    omask[slot] |= am;
+}
+
+__global__ void uchunk_index_kernel(const uint32_t *hscan, const uint32_t *head, int64_t nu, uint32_t *uchunk)
+{
+    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nu) uchunk[u] = hscan[u] + head[u] - 1;
+}
+
+template <typename F> void dispatch_type(int dt, F &&f)
+{
+    switch (dt) {
+    case SG_F32: f(float()); break;
+    case SG_F64: f(double()); break;
+    case SG_I32: f(int32_t()); break;
+    case SG_U32: f(uint32_t()); break;
+    case SG_I64: f(int64_t()); break;
+    case SG_U64: f(uint64_t()); break;
+    case SG_I8: f(int8_t()); break;
+    case SG_U8: f(uint8_t()); break;
+    case SG_I16: f(int16_t()); break;
+    case SG_U16: f(uint16_t()); break;
+    }
+}
+
+int grow_insert_buffer(sg_grid *g, int64_t need)
+{
+    if (need <= g->ins_cap) return SG_OK;
+    int64_t ncap = std::max<int64_t>(need, std::max<int64_t>(1024, g->ins_cap * 2));
+    int64_t *nk = (int64_t *)g->pool.alloc(ncap * sizeof(int64_t));
+    uint8_t *nm = (uint8_t *)g->pool.alloc(ncap);
+    if (!nk || !nm) return fail(SG_E_OOM, "insert buffer allocation failed");
+    void *nv[kMaxProps] = {};
+    for (int p = 0; p < g->np; ++p) {
+        nv[p] = g->pool.alloc(ncap * g->ts[p]);
+        if (!nv[p]) return fail(SG_E_OOM, "insert buffer allocation failed");
+    }
+    if (g->ins_n) {
+        SG_HIP(hipMemcpyAsync(nk, g->d_ins_key, g->ins_n * sizeof(int64_t), hipMemcpyDeviceToDevice, g->stream));
+        SG_HIP(hipMemcpyAsync(nm, g->d_ins_mask, g->ins_n, hipMemcpyDeviceToDevice, g->stream));
+        for (int p = 0; p < g->np; ++p)
+            SG_HIP(hipMemcpyAsync(nv[p], g->d_ins_val[p], g->ins_n * g->ts[p], hipMemcpyDeviceToDevice, g->stream));
+    }
+    g->pool.free(g->d_ins_key);
+    g->pool.free(g->d_ins_mask);
+    for (int p = 0; p < g->np; ++p) g->pool.free(g->d_ins_val[p]);
+    g->d_ins_key = nk;
+    g->d_ins_mask = nm;
+    for (int p = 0; p < g->np; ++p) g->d_ins_val[p] = nv[p];
+    g->ins_cap = ncap;
+    return SG_OK;
+}
+
+hipMemcpyKind kind_from(int mem) { return mem == SG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice; }
+
+int64_t key_limit(const sg_grid *g)
+{
+    int64_t nb = 1;
+    for (int d = 0; d < g->dim; ++d) nb *= g->bsz[d];
+    return nb * g->csz;
+}
+
+// append masks and values of n records at insert position `at`
+int append_mask_values(sg_grid *g, int64_t at, int64_t n, const uint8_t *masks, const void *const *values, int mem)
+{
+    if (masks) SG_HIP(hipMemcpyAsync(g->d_ins_mask + at, masks, n, kind_from(mem), g->stream));
+    else {
+        hipLaunchKernelGGL(fill_u8_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_mask + at,
+                           (uint8_t)SG_EXIST, n);
+        SG_HIP(hipGetLastError());
+    }
+    for (int p = 0; p < g->np; ++p) {
+        unsigned char *dst = (unsigned char *)g->d_ins_val[p] + at * g->ts[p];
+        if (values && values[p]) {
+            SG_HIP(hipMemcpyAsync(dst, values[p], n * g->ts[p], kind_from(mem), g->stream));
+        } else {
+            dispatch_type(g->dt[p], [&](auto tag) {
+                typedef decltype(tag) T;
+                T v;
+                memcpy(&v, g->bg[p], sizeof(T));
+                hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, (T *)dst, v, n);
+            });
+            SG_HIP(hipGetLastError());
+        }
+    }
+    return SG_OK;
+}
+
+}  // namespace
+
+int append_points(sg_grid *g, int64_t n, const int64_t *keys, const uint8_t *masks, const void *const *values,
+                  int mem)
+{
+    if (n < 0) return fail(SG_E_ARG, "negative record count");
+    if (n == 0) return SG_OK;
+    if (!keys) return fail(SG_E_ARG, "null keys");
+    const int64_t limit = key_limit(g);
+    if (mem == SG_MEM_HOST) {
+        for (int64_t i = 0; i < n; ++i)
+            if (keys[i] < 0 || keys[i] >= limit) return fail(SG_E_RANGE, "point key outside the grid");
+    }
+    int rc = grow_insert_buffer(g, g->ins_n + n);
+    if (rc) return rc;
+    const int64_t at = g->ins_n;
+    SG_HIP(hipMemcpyAsync(g->d_ins_key + at, keys, n * sizeof(int64_t), kind_from(mem), g->stream));
+    rc = append_mask_values(g, at, n, masks, values, mem);
+    if (rc) return rc;
+    g->ins_n += n;
+    return SG_OK;
+}
+
+int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *masks, const void *const *values,
+                  int mem)
+{
+    if (n < 0) return fail(SG_E_ARG, "negative record count");
+    if (n == 0) return SG_OK;
+    if (!coords) return fail(SG_E_ARG, "null coords");
+    int rc = grow_insert_buffer(g, g->ins_n + n);
+    if (rc) return rc;
+    const int64_t at = g->ins_n;
+    const int32_t *dc = coords;
+    DevBuf &cb = g->scratch[9];
+    if (mem == SG_MEM_HOST) {
+        for (int64_t i = 0; i < n * g->dim; ++i) {
+            int d = (int)(i % g->dim);
+            if (coords[i] < 0 || coords[i] >= g->sz[d]) return fail(SG_E_RANGE, "coordinate outside the grid");
+        }
+        rc = ensure(g->pool, cb, n * g->dim * sizeof(int32_t) + 64);
+        if (rc) return rc;
+        SG_HIP(hipMemcpyAsync(cb.p, coords, n * g->dim * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+        dc = (const int32_t *)cb.p;
+    } else {
+        rc = ensure(g->pool, cb, 64);
+        if (rc) return rc;
+    }
+    int *bad = (int *)((char *)cb.p + ((mem == SG_MEM_HOST) ? n * g->dim * sizeof(int32_t) : 0));
+    bad = (int *)(((uintptr_t)bad + 7) & ~(uintptr_t)7);
+    SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
+    hipLaunchKernelGGL(coords_to_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, dc,
+                       g->d_ins_key + at, n, g->dim, g->edge, g->sz[0], g->sz[1], g->sz[2], g->bsz[0], g->bsz[1],
+                       g->csz, bad);
+    SG_HIP(hipGetLastError());
+    if (mem == SG_MEM_DEVICE) {
+        int hbad = 0;
+        SG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+        SG_HIP(hipStreamSynchronize(g->stream));
+        if (hbad) return fail(SG_E_RANGE, "coordinate outside the grid");
+    }
+    rc = append_mask_values(g, at, n, masks, values, mem);
+    if (rc) return rc;
+    g->ins_n += n;
+    return SG_OK;
+}
+
+int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t *masks, const void *const *values,
+                  int mem)
+{
+    if (n < 0) return fail(SG_E_ARG, "negative record count");
+    if (n == 0) return SG_OK;
+    if (!chunk_ids || !masks) return fail(SG_E_ARG, "null chunk ids or masks");
+    const int csz = g->csz;
+    int64_t nblocks = key_limit(g) / csz;
+    if (mem == SG_MEM_HOST)
+        for (int64_t i = 0; i < n; ++i)
+            if (chunk_ids[i] < 0 || chunk_ids[i] >= nblocks) return fail(SG_E_RANGE, "chunk id outside the grid");
+    // stage chunk data on the device
+    size_t bytes = n * sizeof(int64_t) + n * csz;
+    for (int p = 0; p < g->np; ++p) bytes += (values && values[p]) ? n * csz * g->ts[p] : 0;
+    DevBuf &st = g->scratch[8];
+    int rc = ensure(g->pool, st, bytes + 256 + (size_t)(n + 1) * 4 * 2);
+    if (rc) return rc;
+    char *q = (char *)st.p;
+    const int64_t *dids = chunk_ids;
+    const uint8_t *dm = masks;
+    const void *dv[kMaxProps] = {};
+    if (mem == SG_MEM_HOST) {
+        SG_HIP(hipMemcpyAsync(q, chunk_ids, n * sizeof(int64_t), hipMemcpyHostToDevice, g->stream));
+        dids = (const int64_t *)q;
+        q += n * sizeof(int64_t);
+        SG_HIP(hipMemcpyAsync(q, masks, n * csz, hipMemcpyHostToDevice, g->stream));
+        dm = (const uint8_t *)q;
+        q += n * csz;
+        for (int p = 0; p < g->np; ++p) {
+            if (values && values[p]) {
+                q = (char *)(((uintptr_t)q + 15) & ~(uintptr_t)15);
+                SG_HIP(hipMemcpyAsync(q, values[p], n * csz * g->ts[p], hipMemcpyHostToDevice, g->stream));
+                dv[p] = q;
+                q += n * csz * g->ts[p];
+            }
+        }
+    } else {
+        for (int p = 0; p < g->np; ++p) dv[p] = (values ? values[p] : nullptr);
+    }
+    q = (char *)(((uintptr_t)q + 15) & ~(uintptr_t)15);
+    uint32_t *cnt = (uint32_t *)q;
+    uint32_t *start = cnt + (n + 1);
+    hipLaunchKernelGGL(chunk_count_kernel, dim3((unsigned)n), dim3(kWG), 0, g->stream, dm, csz, cnt);
+    SG_HIP(hipGetLastError());
+    uint32_t total = 0;
+    rc = exclusive_scan_u32(g, cnt, start, n, &total);
+    if (rc) return rc;
+    rc = grow_insert_buffer(g, g->ins_n + total);
+    if (rc) return rc;
+    const int64_t at = g->ins_n;
+    DevBuf &sb = g->scratch[7];
+    rc = ensure(g->pool, sb, (size_t)total * 4 + 64);
+    if (rc) return rc;
+    uint32_t *src = (uint32_t *)sb.p;
+    hipLaunchKernelGGL(chunk_expand_kernel, dim3((unsigned)n), dim3(kWG), 0, g->stream, dids, dm, start, csz,
+                       g->d_ins_key + at, g->d_ins_mask + at, src, (int64_t)0);
+    SG_HIP(hipGetLastError());
+    for (int p = 0; p < g->np; ++p) {
+        void *dst = (unsigned char *)g->d_ins_val[p] + at * g->ts[p];
+        dispatch_type(g->dt[p], [&](auto tag) {
+            typedef decltype(tag) T;
+            if (dv[p]) {
+                hipLaunchKernelGGL(gather_kernel<T>, dim3(grid_1d(total, kWG)), dim3(kWG), 0, g->stream,
+                                   (const T *)dv[p], src, (T *)dst, (int64_t)total);
+            } else {
+                T v;
+                memcpy(&v, g->bg[p], sizeof(T));
+                hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_1d(total, kWG)), dim3(kWG), 0, g->stream, (T *)dst, v,
+                                   (int64_t)total);
+            }
+        });
+        SG_HIP(hipGetLastError());
+    }
+    g->ins_n += total;
+    // the staging buffers are reused by the next call on this stream; keep them alive
+    return SG_OK;
+}
+
+int flush_impl(sg_grid *g, const int32_t *ops)
+{
+    const int64_t n = g->ins_n;
+    if (n == 0) return SG_OK;  // nothing inserted (map_vector_sparse.hpp:1199-1213)
+    if (n >= ((int64_t)1 << 31)) return fail(SG_E_RANGE, "more than 2^31 pending inserts");
+    const int csz = g->csz;
+    const int64_t limit = key_limit(g);
+    const int64_t m = g->nchunks;
+    int rc;
+
+    // validate device-side keys
+    {
+        rc = ensure(g->pool, g->scratch[6], 64);
+        if (rc) return rc;
+        int *bad = (int *)g->scratch[6].p;
+        SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
+        hipLaunchKernelGGL(check_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key, n, limit,
+                           bad);
+        SG_HIP(hipGetLastError());
+        int hbad = 0;
+        SG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+        SG_HIP(hipStreamSynchronize(g->stream));
+        if (hbad) {
+            g->ins_n = 0;
+            return fail(SG_E_RANGE, "point key outside the grid in the insert buffer");
+        }
+    }
+    int bits = 0;
+    while (bits < 63 && (((int64_t)1 << bits) < limit)) ++bits;
+    const bool k32 = bits <= 32;
+
+    // scratch layout
+    DevBuf &b_keys = g->scratch[0], &b_vals = g->scratch[1], &b_tk = g->scratch[2], &b_tv = g->scratch[3];
+    DevBuf &b_u = g->scratch[4], &b_c = g->scratch[5];
+    const size_t kbytes = (k32 ? 4 : 8) * (size_t)n;
+    if ((rc = ensure(g->pool, b_keys, kbytes)) || (rc = ensure(g->pool, b_tk, kbytes)) ||
+        (rc = ensure(g->pool, b_vals, 4 * (size_t)n)) || (rc = ensure(g->pool, b_tv, 4 * (size_t)n)))
+        return rc;
+
+    void *sk = nullptr;
+    uint32_t *perm = nullptr;
+    if (k32) {
+        hipLaunchKernelGGL(keys_to_sort_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           g->d_ins_key, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+        SG_HIP(hipGetLastError());
+        uint32_t *ko;
+        rc = radix_sort_pairs_u32(g, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, (uint32_t *)b_tk.p,
+                                  (uint32_t *)b_tv.p, n, bits, &ko, &perm);
+        sk = ko;
+    } else {
+        hipLaunchKernelGGL(keys_to_sort_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           g->d_ins_key, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+        SG_HIP(hipGetLastError());
+        uint64_t *ko;
+        rc = radix_sort_pairs_u64(g, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, (uint64_t *)b_tk.p,
+                                  (uint32_t *)b_tv.p, n, bits, &ko, &perm);
+        sk = ko;
+    }
+    if (rc) return rc;
+    // the temporaries not holding the sorted result are free for reuse
+    uint32_t *spare_a = (perm == (uint32_t *)b_vals.p) ? (uint32_t *)b_tv.p : (uint32_t *)b_vals.p;
+
+    // unique points: run heads + scan (head -> spare_a, scan -> b_u)
+    if ((rc = ensure(g->pool, b_u, (size_t)n * (4 + 4 + 8 + 4 + 4) + 64))) return rc;
+    uint32_t *head = spare_a;
+    uint32_t *hscan = (uint32_t *)b_u.p;
+    if (k32)
+        hipLaunchKernelGGL(run_head_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint32_t *)sk, head, n);
+    else
+        hipLaunchKernelGGL(run_head_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint64_t *)sk, head, n);
+    SG_HIP(hipGetLastError());
+    uint32_t nu32 = 0;
+    if ((rc = exclusive_scan_u32(g, head, hscan, n, &nu32))) return rc;
+    const int64_t nu = nu32;
+    int64_t *ukey = (int64_t *)((char *)b_u.p + (((size_t)n * 4 + 15) & ~(size_t)15));
+    uint32_t *rstart = (uint32_t *)(ukey + n);
+    if (k32)
+        hipLaunchKernelGGL(unique_points_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint32_t *)sk, head, hscan, n, ukey, rstart, nu32);
+    else
+        hipLaunchKernelGGL(unique_points_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint64_t *)sk, head, hscan, n, ukey, rstart, nu32);
+    SG_HIP(hipGetLastError());
+
+    // unique chunks among the unique points (head -> hscan region reused)
+    uint32_t *chead = head;  // head no longer needed
+    uint32_t *cscan = hscan;
+    hipLaunchKernelGGL(chunk_head_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, chead, nu, csz);
+    SG_HIP(hipGetLastError());
+    uint32_t nn32 = 0;
+    if ((rc = exclusive_scan_u32(g, chead, cscan, nu, &nn32))) return rc;
+    const int64_t nnew = nn32;
+    uint32_t *uchunk = rstart + (n + 1);
+    hipLaunchKernelGGL(uchunk_index_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, cscan, chead, nu, uchunk);
+    SG_HIP(hipGetLastError());
+
+    // per new chunk: key, old position, brand-new flag and its scan, final position
+    if ((rc = ensure(g->pool, b_c, (size_t)nnew * (8 + 8 + 4 + 4 + 8 + 8) + 128))) return rc;
+    int64_t *nkey = (int64_t *)b_c.p;
+    int64_t *oldpos = nkey + nnew;
+    uint32_t *isnew = (uint32_t *)(oldpos + nnew);
+    uint32_t *nscan = isnew + nnew;
+    int64_t *cpos = (int64_t *)(((uintptr_t)(nscan + nnew) + 15) & ~(uintptr_t)15);
+    int64_t *bn = cpos + nnew;
+    hipLaunchKernelGGL(unique_chunks_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, chead, cscan, nu,
+                       csz, nkey);
+    SG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(old_lookup_kernel, dim3(grid_1d(nnew, kWG)), dim3(kWG), 0, g->stream, nkey, nnew, g->d_keys,
+                       m, oldpos, isnew);
+    SG_HIP(hipGetLastError());
+    uint32_t nb32 = 0;
+    if ((rc = exclusive_scan_u32(g, isnew, nscan, nnew, &nb32))) return rc;
+    const int64_t nb = nb32;
+    hipLaunchKernelGGL(brand_new_kernel, dim3(grid_1d(nnew, kWG)), dim3(kWG), 0, g->stream, nkey, isnew, nscan, nnew,
+                       bn);
+    SG_HIP(hipGetLastError());
+    hipLaunchKernelGGL(chunk_pos_kernel, dim3(grid_1d(nnew, kWG)), dim3(kWG), 0, g->stream, nkey, oldpos, nscan, nnew,
+                       g->d_keys, m, bn, nb, cpos);
+    SG_HIP(hipGetLastError());
+
+    if (nb > 0) {
+        // the chunk set grows: build the union arrays (old chunks moved, new chunks
+        // initialised to mask 0 / background), then fold into them
+        const int64_t total = m + nb;
+        int64_t ncap = std::max<int64_t>(total, g->cap);
+        if (total > g->cap) ncap = std::max<int64_t>(total, (int64_t)(g->cap * 1.25) + 16);
+        int64_t *nkeys = (int64_t *)g->pool.alloc(ncap * sizeof(int64_t));
+        uint8_t *nmask = (uint8_t *)g->pool.alloc((size_t)(ncap + 1) * csz);
+        void *nprop[kMaxProps] = {};
+        if (!nkeys || !nmask) return fail(SG_E_OOM, "chunk array allocation failed");
+        for (int p = 0; p < g->np; ++p) {
+            nprop[p] = g->pool.alloc((size_t)(ncap + 1) * csz * g->ts[p]);
+            if (!nprop[p]) return fail(SG_E_OOM, "chunk array allocation failed");
+        }
+        DevBuf &b_o = g->scratch[6];
+        if ((rc = ensure(g->pool, b_o, (size_t)(m + nb) * 8 + 128))) return rc;
+        int64_t *opos = (int64_t *)b_o.p;
+        int64_t *bpos = opos + m;
+        if (m > 0) {
+            hipLaunchKernelGGL(move_old_keys_kernel, dim3(grid_1d(m, kWG)), dim3(kWG), 0, g->stream, g->d_keys, m, bn,
+                               nb, nkeys, opos);
+            SG_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(place_new_keys_kernel, dim3(grid_1d(nb, kWG)), dim3(kWG), 0, g->stream, bn, nb, g->d_keys,
+                           m, nkeys, bpos);
+        SG_HIP(hipGetLastError());
+        // chunk data: copy as 16-byte vectors when the chunk size allows
+        auto move = [&](const void *src, void *dst, size_t chunk_bytes) -> int {
+            if (m == 0) return SG_OK;
+            if (chunk_bytes % 16 == 0)
+                hipLaunchKernelGGL(move_chunks_kernel<uint4>, dim3((unsigned)m), dim3(kWG), 0, g->stream,
+                                   (const uint4 *)src, (uint4 *)dst, opos, m, (int)(chunk_bytes / 16));
+            else
+                hipLaunchKernelGGL(move_chunks_kernel<uint8_t>, dim3((unsigned)m), dim3(kWG), 0, g->stream,
+                                   (const uint8_t *)src, (uint8_t *)dst, opos, m, (int)chunk_bytes);
+            SG_HIP(hipGetLastError());
+            return SG_OK;
+        };
+        if ((rc = move(g->d_mask, nmask, csz))) return rc;
+        hipLaunchKernelGGL(init_chunks_kernel<uint8_t>, dim3((unsigned)nb), dim3(kWG), 0, g->stream, nmask, bpos, nb,
+                           csz, (uint8_t)0);
+        SG_HIP(hipGetLastError());
+        hipLaunchKernelGGL(init_chunks_kernel<uint8_t>, dim3(1), dim3(kWG), 0, g->stream, nmask + total * csz,
+                           (const int64_t *)nullptr, (int64_t)1, csz, (uint8_t)0);
+        SG_HIP(hipGetLastError());
+        for (int p = 0; p < g->np; ++p) {
+            if ((rc = move(g->d_prop[p], nprop[p], (size_t)csz * g->ts[p]))) return rc;
+            dispatch_type(g->dt[p], [&](auto tag) {
+                typedef decltype(tag) T;
+                T v;
+                memcpy(&v, g->bg[p], sizeof(T));
+                hipLaunchKernelGGL(init_chunks_kernel<T>, dim3((unsigned)nb), dim3(kWG), 0, g->stream, (T *)nprop[p],
+                                   bpos, nb, csz, v);
+                hipLaunchKernelGGL(init_chunks_kernel<T>, dim3(1), dim3(kWG), 0, g->stream,
+                                   (T *)nprop[p] + total * csz, (const int64_t *)nullptr, (int64_t)1, csz, v);
+            });
+            SG_HIP(hipGetLastError());
+        }
+        g->pool.free(g->d_keys);
+        g->pool.free(g->d_mask);
+        for (int p = 0; p < g->np; ++p) g->pool.free(g->d_prop[p]);
+        g->d_keys = nkeys;
+        g->d_mask = nmask;
+        for (int p = 0; p < g->np; ++p) g->d_prop[p] = nprop[p];
+        g->nchunks = total;
+        g->cap = ncap;
+    }
+
+    // fold: values of every property (they read the old masks), then the masks
+    for (int p = 0; p < g->np; ++p) {
+        const int op = ops ? ops[p] : SG_OP_NONE;
+        dispatch_type(g->dt[p], [&](auto tag) {
+            typedef decltype(tag) T;
+            T v;
+            memcpy(&v, g->bg[p], sizeof(T));
+            hipLaunchKernelGGL(fold_points_kernel<T>, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart,
+                               perm, uchunk, cpos, nu, csz, g->d_ins_mask, (const T *)g->d_ins_val[p],
+                               (T *)g->d_prop[p], g->d_mask, op, v);
+        });
+        SG_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(fold_masks_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart, perm, uchunk,
+                       cpos, nu, csz, g->d_ins_mask, g->d_mask);
+    SG_HIP(hipGetLastError());
+
+    g->ins_n = 0;
+    g->nn_valid[0] = g->nn_valid[1] = false;  // flush marks the neighbour table stale (SparseGridGpu.hpp:778)
+    return SG_OK;
+}
+
+}  // namespace sg

@@ -1,0 +1,144 @@
+// sg_internal.hpp -- shared definitions of libsg_hip.so (host side + device helpers).
+//
+// Device layout of one grid (chunk-major SoA, the reference's memory_traits_inte
+// DataBlock arrays, src/SparseGridGpu/BlockMapGpu.hpp:63-69):
+//   keys  int64 [cap]                 sorted chunk ids (map_vector_sparse.hpp:790-791)
+//   prop  T     [(cap+1) * csz]       one array per property, chunk c at c*csz
+//   mask  uint8 [(cap+1) * csz]       EXIST bit 0, PADDING bit 1
+//   chunk nChunks of every array is the background chunk (mask 0, values = background)
+// Invariant kept by every kernel of this library: a slot whose mask byte is 0 holds
+// the property's background value.  The reference substitutes the background for
+// such slots whenever it reads them (loadGhostBlock_impl, SparseGridGpu_ker_util.hpp:551-552;
+// BlockMapGpu_ker::get, BlockMapGpu_ker.cuh:119-135), so results are identical while
+// the stencil kernels read neighbour data without reading neighbour masks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sparsegrid_gpu.h"
+
+namespace sg {
+
+constexpr int kMaxProps = 16;
+
+// ---- errors ------------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define SG_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::sg::fail(SG_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline size_t dtype_size(int dt)
+{
+    switch (dt) {
+    case SG_F32: case SG_I32: case SG_U32: return 4;
+    case SG_F64: case SG_I64: case SG_U64: return 8;
+    case SG_I8: case SG_U8: return 1;
+    case SG_I16: case SG_U16: return 2;
+    default: return 0;
+    }
+}
+inline bool dtype_is_float(int dt) { return dt == SG_F32 || dt == SG_F64; }
+
+// ---- a per-handle caching device allocator -------------------------------------
+// hipMalloc/hipFree synchronise the device; a grid reallocates its arrays on
+// every flush that adds chunks, so freed blocks are kept and reused.  All work of a
+// handle is ordered on the handle's stream, so reuse by the same handle is safe.
+class DevicePool {
+public:
+    ~DevicePool() { release(); }
+    void *alloc(size_t bytes);
+    void free(void *p);
+    void release();
+    size_t live_bytes() const { return live_; }
+
+private:
+    std::multimap<size_t, void *> cache_;
+    std::map<void *, size_t> sizes_;
+    size_t live_ = 0;
+};
+
+// a device buffer owned by a pool, grown on demand
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+int ensure(DevicePool &pool, DevBuf &b, size_t bytes);
+
+}  // namespace sg
+
+struct sg_grid {
+    int dim = 3, edge = 8, csz = 512, r = 1;
+    int64_t sz[3] = {1, 1, 1}, bsz[3] = {1, 1, 1};
+    int np = 0;
+    int dt[sg::kMaxProps];
+    int ts[sg::kMaxProps];
+    unsigned char bg[sg::kMaxProps][8];
+
+    int64_t nchunks = 0, cap = 0;
+    int64_t *d_keys = nullptr;
+    void *d_prop[sg::kMaxProps] = {};
+    uint8_t *d_mask = nullptr;
+
+    // neighbour tables: NNStar [n][2^dim], NNFull [n][3^dim], int32 positions
+    int32_t *d_nn[2] = {nullptr, nullptr};
+    int64_t nn_cap[2] = {0, 0};
+    bool nn_valid[2] = {false, false};
+    int ghost_type = SG_NN_STAR;
+
+    // insert buffer (point records)
+    int64_t ins_n = 0, ins_cap = 0;
+    int64_t *d_ins_key = nullptr;
+    uint8_t *d_ins_mask = nullptr;
+    void *d_ins_val[sg::kMaxProps] = {};
+
+    hipStream_t stream = nullptr;
+    sg::DevicePool pool;
+    sg::DevBuf scratch[12];  // flush / sort / scan temporaries
+    sg::DevBuf hostlike;     // small device staging
+
+    int nnn(int type) const { return type == SG_NN_STAR ? (1 << dim) : (dim == 2 ? 9 : 27); }
+};
+
+namespace sg {
+
+// host launchers implemented in the .hip translation units
+int flush_impl(sg_grid *g, const int32_t *ops);
+int append_points(sg_grid *g, int64_t n, const int64_t *keys, const uint8_t *masks,
+                  const void *const *values, int mem);
+int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *masks,
+                  const void *const *values, int mem);
+int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t *masks,
+                  const void *const *values, int mem);
+int find_neighbours_impl(sg_grid *g, int type);
+int tag_boundaries_impl(sg_grid *g, int type);
+int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d);
+int get_points_impl(sg_grid *g, int prop, int64_t n, const int64_t *keys, void *out, int mem);
+int fill_background_chunk(sg_grid *g, int prop);
+int normalize_prop(sg_grid *g, int prop);
+int count_mask_impl(sg_grid *g, int which, int64_t *out);
+
+// device-wide primitives (sg_primitives.hip)
+// exclusive scan of n uint32 values; *total (host) receives the sum when non-null
+int exclusive_scan_u32(sg_grid *g, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total);
+// stable LSD radix sort of (key, value) pairs on the low `bits` bits of the key
+int radix_sort_pairs_u64(sg_grid *g, uint64_t *keys, uint32_t *vals, uint64_t *keys_tmp,
+                         uint32_t *vals_tmp, int64_t n, int bits, uint64_t **keys_out,
+                         uint32_t **vals_out);
+int radix_sort_pairs_u32(sg_grid *g, uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp,
+                         uint32_t *vals_tmp, int64_t n, int bits, uint32_t **keys_out,
+                         uint32_t **vals_out);
+
+inline unsigned grid_1d(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+}  // namespace sg

@@ -1,0 +1,268 @@
+// sg_primitives.hip -- device-wide exclusive scan and stable LSD radix sort for gfx950.
+//
+// These replace the CUB/hipCUB calls of the reference flush
+// (openfpm::scan -> DeviceScan::ExclusiveSum, src/util/cuda/scan_ofp.cuh:32-78;
+//  openfpm::sort -> DeviceRadixSort::SortPairs, src/util/cuda/sort_ofp.cuh:262-402).
+//
+// Radix sort: 8-bit digits, one tile of 4096 keys per 256-thread workgroup.
+//   pass = histogram (LDS atomics) -> device scan of the digit-major count table
+//          -> scatter with a stable in-tile rank computed by wave64 ballot matching
+//             (for each item, the lanes holding the same digit are found with 8
+//             ballots; rank = popcount of lower peer lanes + counts of earlier waves
+//             and earlier items of the tile).
+// Stability: items are ranked in index order inside a tile and tiles are ranked in
+// tile order by the scanned table, so equal keys keep their input order (as
+// CUB SortPairs does), which the flush relies on for its merge order.
+#include "sg_internal.hpp"
+
+namespace sg {
+namespace {
+
+constexpr int kScanWG = 256;
+constexpr int kScanIPT = 16;
+constexpr int kScanTile = kScanWG * kScanIPT;
+
+constexpr int kSortWG = 256;
+constexpr int kSortIPT = 16;
+constexpr int kSortTile = kSortWG * kSortIPT;
+
+// inclusive scan across a wave64 with cross-lane shuffles
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// exclusive scan of one value per thread over a 256-thread block; returns the
+// block total in *total
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *total)
+{
+    __shared__ uint32_t s_wave[kScanWG / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) s_wave[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kScanWG / 64; ++i) {
+        uint32_t s = s_wave[i];
+        if (i < w) pre += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - v;
+}
+
+__global__ __launch_bounds__(kScanWG) void scan_reduce_kernel(const uint32_t *in, uint32_t *block_sums, int64_t n)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanIPT; ++k) {
+        int64_t i = base + k;
+        if (i < n) s += in[i];
+    }
+    uint32_t tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanWG) void scan_down_kernel(const uint32_t *in, uint32_t *out, const uint32_t *block_offsets,
+                                                            int64_t n, uint32_t *total_out)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    uint32_t v[kScanIPT];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanIPT; ++k) {
+        int64_t i = base + k;
+        v[k] = (i < n) ? in[i] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t pre = block_exclusive_scan(s, &tot);
+    pre += block_offsets ? block_offsets[blockIdx.x] : 0u;
+#pragma unroll
+    for (int k = 0; k < kScanIPT; ++k) {
+        int64_t i = base + k;
+        if (i < n) out[i] = pre;
+        pre += v[k];
+        if (total_out && i == n - 1) *total_out = pre;
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortWG) void radix_hist_kernel(const K *keys, uint32_t *counts, int64_t n, int shift,
+                                                             int ntiles)
+{
+    __shared__ uint32_t s_hist[256];
+    const int t = threadIdx.x;
+    s_hist[t] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int j = 0; j < kSortIPT; ++j) {
+        int64_t i = base + (int64_t)j * kSortWG + t;
+        if (i < n) {
+            uint32_t d = (uint32_t)(keys[i] >> shift) & 255u;
+            atomicAdd(&s_hist[d], 1u);
+        }
+    }
+    __syncthreads();
+    counts[(int64_t)t * ntiles + blockIdx.x] = s_hist[t];
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortWG) void radix_scatter_kernel(const K *keys, const uint32_t *vals, K *okeys,
+                                                                uint32_t *ovals, const uint32_t *offsets, int64_t n,
+                                                                int shift, int ntiles)
+{
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_wcnt[kSortWG / 64][256];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    s_base[t] = offsets[(int64_t)t * ntiles + blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < kSortWG / 64; ++q) s_wcnt[q][t] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int j = 0; j < kSortIPT; ++j) {
+        const int64_t i = base + (int64_t)j * kSortWG + t;
+        const bool valid = i < n;
+        K key = valid ? keys[i] : (K)0;
+        uint32_t val = valid ? vals[i] : 0u;
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) s_wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = s_base[d] + rank;
+            for (int q = 0; q < w; ++q) pos += s_wcnt[q][d];
+            okeys[pos] = key;
+            ovals[pos] = val;
+        }
+        __syncthreads();
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < kSortWG / 64; ++q) {
+            sum += s_wcnt[q][t];
+            s_wcnt[q][t] = 0;
+        }
+        s_base[t] += sum;
+        __syncthreads();
+    }
+}
+
+template <typename K>
+int radix_sort_pairs(sg_grid *g, K *keys, uint32_t *vals, K *keys_tmp, uint32_t *vals_tmp, int64_t n, int bits,
+                     K **keys_out, uint32_t **vals_out)
+{
+    *keys_out = keys;
+    *vals_out = vals;
+    if (n <= 1 || bits <= 0) return SG_OK;
+    if (n >= (int64_t)1 << 31) return fail(SG_E_RANGE, "radix sort supports fewer than 2^31 items");
+    const int ntiles = (int)((n + kSortTile - 1) / kSortTile);
+    DevBuf &cbuf = g->scratch[10];
+    int rc = ensure(g->pool, cbuf, (size_t)256 * ntiles * sizeof(uint32_t) * 2);
+    if (rc) return rc;
+    uint32_t *counts = (uint32_t *)cbuf.p;
+    uint32_t *offs = counts + (size_t)256 * ntiles;
+    K *src_k = keys, *dst_k = keys_tmp;
+    uint32_t *src_v = vals, *dst_v = vals_tmp;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(radix_hist_kernel<K>, dim3(ntiles), dim3(kSortWG), 0, g->stream, src_k, counts, n, shift,
+                           ntiles);
+        SG_HIP(hipGetLastError());
+        rc = exclusive_scan_u32(g, counts, offs, (int64_t)256 * ntiles, nullptr);
+        if (rc) return rc;
+        hipLaunchKernelGGL(radix_scatter_kernel<K>, dim3(ntiles), dim3(kSortWG), 0, g->stream, src_k, src_v, dst_k,
+                           dst_v, offs, n, shift, ntiles);
+        SG_HIP(hipGetLastError());
+        std::swap(src_k, dst_k);
+        std::swap(src_v, dst_v);
+    }
+    *keys_out = src_k;
+    *vals_out = src_v;
+    return SG_OK;
+}
+
+}  // namespace
+
+int exclusive_scan_u32(sg_grid *g, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total)
+{
+    if (n <= 0) {
+        if (total) *total = 0;
+        return SG_OK;
+    }
+    // level sizes: n, ceil(n/T), ceil(ceil(n/T)/T), ... until one tile
+    std::vector<int64_t> sizes{n};
+    while (sizes.back() > kScanTile) sizes.push_back((sizes.back() + kScanTile - 1) / kScanTile);
+    size_t need = 0;
+    for (size_t l = 1; l < sizes.size(); ++l) need += 2 * (size_t)sizes[l];
+    need += 2;
+    DevBuf &sb = g->scratch[11];
+    int rc = ensure(g->pool, sb, need * sizeof(uint32_t));
+    if (rc) return rc;
+    uint32_t *pool = (uint32_t *)sb.p;
+    // per level l >= 1: sums[l] (block sums of level l-1) and offs[l] (their scan)
+    std::vector<uint32_t *> sums(sizes.size(), nullptr), offs(sizes.size(), nullptr);
+    size_t at = 0;
+    for (size_t l = 1; l < sizes.size(); ++l) {
+        sums[l] = pool + at;
+        at += sizes[l];
+        offs[l] = pool + at;
+        at += sizes[l];
+    }
+    uint32_t *d_total = pool + at;
+    const uint32_t *lin = in;
+    for (size_t l = 0; l + 1 < sizes.size(); ++l) {
+        hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)sizes[l + 1]), dim3(kScanWG), 0, g->stream, lin,
+                           sums[l + 1], sizes[l]);
+        SG_HIP(hipGetLastError());
+        lin = sums[l + 1];
+    }
+    for (size_t l = sizes.size() - 1; l >= 1; --l) {
+        // scan level l (its single top tile or using offsets from level l+1)
+        const uint32_t *bo = (l + 1 < sizes.size()) ? offs[l + 1] : nullptr;
+        hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)((sizes[l] + kScanTile - 1) / kScanTile)), dim3(kScanWG),
+                           0, g->stream, sums[l], offs[l], bo, sizes[l], (uint32_t *)nullptr);
+        SG_HIP(hipGetLastError());
+    }
+    const uint32_t *bo0 = (sizes.size() > 1) ? offs[1] : nullptr;
+    hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)((n + kScanTile - 1) / kScanTile)), dim3(kScanWG), 0,
+                       g->stream, in, out, bo0, n, total ? d_total : (uint32_t *)nullptr);
+    SG_HIP(hipGetLastError());
+    if (total) {
+        // the inclusive total is written by the thread owning item n-1 (works in place)
+        SG_HIP(hipMemcpyAsync(total, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+        SG_HIP(hipStreamSynchronize(g->stream));
+    }
+    return SG_OK;
+}
+
+int radix_sort_pairs_u64(sg_grid *g, uint64_t *keys, uint32_t *vals, uint64_t *keys_tmp, uint32_t *vals_tmp,
+                         int64_t n, int bits, uint64_t **keys_out, uint32_t **vals_out)
+{
+    return radix_sort_pairs<uint64_t>(g, keys, vals, keys_tmp, vals_tmp, n, bits, keys_out, vals_out);
+}
+
+int radix_sort_pairs_u32(sg_grid *g, uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
+                         int64_t n, int bits, uint32_t **keys_out, uint32_t **vals_out)
+{
+    return radix_sort_pairs<uint32_t>(g, keys, vals, keys_tmp, vals_tmp, n, bits, keys_out, vals_out);
+}
+
+}  // namespace sg

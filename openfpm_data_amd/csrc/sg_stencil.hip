@@ -1,0 +1,867 @@
+// sg_stencil.hip -- neighbour tables, tagBoundaries, stencils and point gets for gfx950.
+//
+// Generic path (any dim 2/3, any chunk edge): one workgroup per chunk stages the
+// chunk and its one-cell ghost layer in LDS (the reference's enlarged block,
+// SparseGridGpu_kernels.cuh:258-276) and evaluates the stencil per point.
+//
+// Fast path (3D, 8^3 chunks, star stencils, f32/f64): one wave64 per chunk, lane l
+// owns the x-row (y = l & 7, z = l >> 3) of 8 points in registers:
+//   - own rows: two (f32) / four (f64) 16-byte loads per lane, the wave reads the
+//     chunk's 2 KB / 4 KB contiguously (chunk-major SoA);
+//   - +-y / +-z ghost faces: 32 rows of 8 values, one 16-byte load per lane per
+//     half row, staged with the own rows in a 10x10-row LDS tile;
+//   - +-x ghost faces: one scalar per lane from the x-neighbour chunks, kept in
+//     registers (x-1 / x+1 of the row ends);
+//   - neighbour chunk positions come from the findNeighbours table (scalar loads,
+//     uniform per wave); a missing neighbour is the background chunk, whose values
+//     are the background, so no mask of a neighbour is read (see the invariant in
+//     sg_internal.hpp).
+// The mask row (8 bytes) decides which points are written (EXIST && !PADDING, or
+// mask != 0), as applyStencilInPlace does (SparseGridGpu.hpp:1005-1035).
+#include "sg_internal.hpp"
+
+namespace sg {
+namespace {
+
+constexpr int kWG = 256;
+
+template <typename T> __device__ inline T dev_add(T a, T b)
+{
+    if constexpr (std::is_integral<T>::value) {
+        typedef typename std::make_unsigned<T>::type U;
+        return (T)(U)((U)a + (U)b);
+    } else {
+        return a + b;
+    }
+}
+template <typename T> __device__ inline T dev_mul(T a, T b)
+{
+    if constexpr (std::is_integral<T>::value) {
+        typedef typename std::make_unsigned<T>::type U;
+        return (T)(U)((U)a * (U)b);
+    } else {
+        return a * b;
+    }
+}
+
+struct GeomArgs {
+    int dim, edge, csz;
+    int64_t bsz0, bsz1, bsz2;
+};
+
+__device__ inline int64_t lower_bound_dev(const int64_t *a, int64_t n, int64_t x)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// get_sparse: position of a chunk key or the background position n
+// (map_vector_sparse_cuda_ker.cuh:75-92)
+__device__ inline int64_t find_chunk(const int64_t *keys, int64_t n, int64_t key)
+{
+    int64_t lb = lower_bound_dev(keys, n, key);
+    return (lb < n && keys[lb] == key) ? lb : n;
+}
+
+// findNeighbours (SparseGridGpu_kernels.cuh:1009-1035) with NNStar::getNNpos
+// (SparseGridGpu_ker_util.hpp:91-112) or NNFull::getNNpos (SparseGridGpu.hpp:266-287).
+// BlockLinId (grid_smb.hpp:258-276) returns -1 for a coordinate >= blockSz and does not
+// check for -1: NNStar guards dPos < 0 itself, NNFull does not, so at the low x / y
+// faces its linear id aliases another chunk; this table reproduces both.
+__global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs ga, int type, int nnn, int32_t *nn)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * nnn) return;
+    const int64_t c = t / nnn;
+    const int k = (int)(t % nnn);
+    const int dim = ga.dim;
+    int64_t bs[3] = {ga.bsz0, ga.bsz1, ga.bsz2};
+    int64_t bc[3] = {0, 0, 0};
+    int64_t b = keys[c];
+    for (int d = 0; d < dim; ++d) {
+        bc[d] = b % bs[d];
+        b /= bs[d];
+    }
+    auto block_lin_id = [&](const int64_t *q) -> int64_t {
+        int64_t v = q[dim - 1];
+        if (v >= bs[dim - 1]) return -1;
+        for (int d = dim - 2; d >= 0; --d) {
+            v *= bs[d];
+            if (q[d] >= bs[d]) return -1;
+            v += q[d];
+        }
+        return v;
+    };
+    int64_t pos;
+    if (type == SG_NN_STAR) {
+        pos = -1;
+        if (k < 2 * dim) {
+            const int d = k / 2;
+            const int64_t dPos = bc[d] + (k % 2) * 2 - 1;
+            bc[d] = dPos;
+            int64_t bl = block_lin_id(bc);
+            bl = (dPos < 0) ? -1 : bl;
+            pos = find_chunk(keys, n, bl);
+        }
+    } else {
+        pos = n;
+        if (k != nnn / 2) {
+            int cnt = k;
+            for (int d = 0; d < dim; ++d) {
+                bc[d] += cnt % 3 - 1;
+                cnt /= 3;
+            }
+            pos = find_chunk(keys, n, block_lin_id(bc));
+        }
+    }
+    nn[t] = (int32_t)pos;
+}
+
+// neighbour chunk of an enlarged-block cell; dir[d] in {-1,0,1}
+// star tables index face neighbours as 2*d + (dir>0); full tables sum (dir+1)*3^d.
+// Returns -1 when the cell is an edge/corner cell under a star table (never read by
+// star stencils; the reference loads arbitrary data there).
+__device__ inline int64_t ghost_source(const int32_t *nnrow, int type, int dim, const int *dir, int64_t self)
+{
+    int nz = 0, k = 0, acc = 1, star = -1;
+    for (int d = 0; d < dim; ++d) {
+        if (dir[d] != 0) {
+            ++nz;
+            star = 2 * d + (dir[d] > 0);
+        }
+        k += (dir[d] + 1) * acc;
+        acc *= 3;
+    }
+    if (nz == 0) return self;
+    if (type == SG_NN_STAR) return (nz == 1) ? (int64_t)nnrow[star] : -1;
+    return (int64_t)nnrow[k];
+}
+
+// stage the enlarged block of one chunk (edge + 2 cells per dimension) in LDS
+template <typename T>
+__device__ void load_enlarged(T *eb, const T *arr, const int32_t *nnrow, int type, int64_t c, int64_t bgpos, int dim,
+                              int edge, int csz, T bgv)
+{
+    const int e = edge + 2;
+    const int ebs = (dim == 3) ? e * e * e : e * e;
+    for (int i = threadIdx.x; i < ebs; i += blockDim.x) {
+        int crd[3] = {0, 0, 0}, dir[3] = {0, 0, 0}, t = i, off = 0, acc = 1;
+        for (int d = 0; d < dim; ++d) {
+            crd[d] = t % e - 1;
+            t /= e;
+            dir[d] = (crd[d] < 0) ? -1 : (crd[d] >= edge ? 1 : 0);
+            int w = crd[d] - dir[d] * edge;
+            off += w * acc;
+            acc *= edge;
+        }
+        int64_t src = ghost_source(nnrow, type, dim, dir, c);
+        T v = bgv;
+        if (src >= 0) v = arr[src * csz + off];
+        eb[i] = v;
+    }
+    (void)bgpos;
+}
+
+struct StencilArgs {
+    int kind, apply, use_box;
+    int lo[3], hi[3];
+    double coef[27];
+    int64_t icoef[27];
+};
+
+// generic stencil: one workgroup per chunk, enlarged blocks of up to two
+// properties in dynamic LDS.  dst_alias[q] != 0 means dst q is also a source: it is
+// then written for every slot (new value where active, the old value elsewhere)
+// into a separate buffer that replaces the property afterwards.
+template <typename T>
+__global__ __launch_bounds__(kWG) void stencil_generic_kernel(const int64_t *keys, int64_t n, GeomArgs ga,
+                                                             const int32_t *nn, int nnn, int type, const uint8_t *mask,
+                                                             const T *src0, const T *src1, T *dst0, T *dst1,
+                                                             int alias0, int alias1, T bg0, T bg1, StencilArgs sa)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T *eb0 = (T *)smem_raw;
+    const int dim = ga.dim, edge = ga.edge, csz = ga.csz, e = edge + 2;
+    const int ebs = (dim == 3) ? e * e * e : e * e;
+    T *eb1 = eb0 + ebs;
+    const int64_t c = blockIdx.x;
+    if (c >= n) return;
+    const int32_t *nnrow = nn + c * nnn;
+    load_enlarged<T>(eb0, src0, nnrow, type, c, n, dim, edge, csz, bg0);
+    if (src1) load_enlarged<T>(eb1, src1, nnrow, type, c, n, dim, edge, csz, bg1);
+    __syncthreads();
+    int64_t bs[3] = {ga.bsz0, ga.bsz1, ga.bsz2};
+    int64_t base[3] = {0, 0, 0};
+    {
+        int64_t b = keys[c];
+        for (int d = 0; d < dim; ++d) {
+            base[d] = (b % bs[d]) * edge;
+            b /= bs[d];
+        }
+    }
+    const int kind = sa.kind;
+    for (int o = threadIdx.x; o < csz; o += blockDim.x) {
+        int t = o, lin = 0, acc = 1;
+        int64_t pc[3] = {0, 0, 0}, par = 0;
+        for (int d = 0; d < dim; ++d) {
+            const int lc = t % edge;
+            pc[d] = base[d] + lc;
+            par += pc[d];
+            lin += (lc + 1) * acc;
+            t /= edge;
+            acc *= e;
+        }
+        uint8_t cm = mask[c * csz + o];
+        if (sa.use_box)
+            for (int d = 0; d < dim; ++d)
+                if (pc[d] < sa.lo[d] || pc[d] > sa.hi[d]) cm = 0;
+        bool active = (sa.apply == SG_APPLY_EXISTING) ? (cm != 0) : ((cm & SG_EXIST) && !(cm & SG_PADDING));
+        const T cur = eb0[lin];
+        T r0 = alias0 ? eb0[lin] : (T)0, r1 = alias1 ? eb1[lin] : (T)0;
+        if (alias0 == 2) r0 = eb1[lin];
+        if (alias1 == 1) r1 = eb0[lin];
+        if (kind == SG_ST_RBSOR && (int)(par & 1) != (int)sa.icoef[0]) active = false;
+        if (active) {
+            if constexpr (!std::is_integral<T>::value) {
+                if (kind == SG_ST_LAP || kind == SG_ST_HEAT || kind == SG_ST_RESIDUAL) {
+                    T a;
+                    if (kind == SG_ST_LAP) a = (T)sa.coef[0] * cur;
+                    else if (kind == SG_ST_HEAT) a = (T)(-2.0 * dim) * cur;
+                    else a = (T)(2.0 * dim) * cur;
+                    int s = 1;
+                    for (int d = 0; d < dim; ++d) {
+                        T pair = eb0[lin - s] + eb0[lin + s];
+                        if (kind == SG_ST_LAP) a = a + (T)sa.coef[1] * pair;
+                        else if (kind == SG_ST_HEAT) a = a + pair;
+                        else a = a - pair;
+                        s *= e;
+                    }
+                    if (kind == SG_ST_LAP) r0 = a;
+                    else if (kind == SG_ST_HEAT) r0 = cur + (T)sa.coef[0] * a;
+                    else r0 = eb1[lin] - a;
+                } else if (kind == SG_ST_GRAY_SCOTT) {
+                    const T cu = cur, cv = eb1[lin];
+                    T lu = (T)(-2.0 * dim) * cu, lv = (T)(-2.0 * dim) * cv;
+                    int s = 1;
+                    for (int d = 0; d < dim; ++d) {
+                        lu = lu + (eb0[lin - s] + eb0[lin + s]);
+                        lv = lv + (eb1[lin - s] + eb1[lin + s]);
+                        s *= e;
+                    }
+                    const T Du = (T)sa.coef[0], Dv = (T)sa.coef[1], F = (T)sa.coef[2], k = (T)sa.coef[3],
+                            dt = (T)sa.coef[4];
+                    const T uvv = (cu * cv) * cv;
+                    r0 = cu + dt * ((Du * lu - uvv) + F * ((T)1 - cu));
+                    r1 = cv + dt * ((Dv * lv + uvv) - (F + k) * cv);
+                } else if (kind == SG_ST_RBSOR) {
+                    T nbs = 0;
+                    int s = 1;
+                    for (int d = 0; d < dim; ++d) {
+                        nbs = nbs + (eb0[lin - s] + eb0[lin + s]);
+                        s *= e;
+                    }
+                    const T w = (T)sa.coef[0];
+                    const T gs = (eb1[lin] + nbs) / (T)(2 * dim);
+                    r0 = cur + w * (gs - cur);
+                }
+            }
+            if (kind == SG_ST_BOX) {
+                // Conv3x3x3 order: x offset outermost, z innermost
+                T tot = 0;
+                const int nk = (dim == 3) ? 3 : 1;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        for (int k = 0; k < nk; ++k) {
+                            const int off = (i - 1) + (j - 1) * e + ((dim == 3) ? (k - 1) * e * e : 0);
+                            const int ci = (dim == 3) ? (i * 9 + j * 3 + k) : (i * 3 + j);
+                            const T cf = std::is_integral<T>::value ? (T)sa.icoef[ci] : (T)sa.coef[ci];
+                            tot = dev_add<T>(tot, dev_mul<T>(eb0[lin + off], cf));
+                        }
+                r0 = tot;
+            }
+        }
+        const int64_t slot = c * csz + o;
+        if (dst0 && (active || alias0)) dst0[slot] = r0;
+        if (dst1 && (active || alias1)) dst1[slot] = r1;
+    }
+}
+
+// tagBoundaries (SparseGridGpu_kernels.cuh:667-737): padding = an existing point with
+// a star (NNStar::isPadding) or full (NNfull_is_padding_impl) neighbour that does not
+// exist; the enlarged mask block takes missing chunks from the background (mask 0).
+__global__ __launch_bounds__(kWG) void tag_boundaries_kernel(int64_t n, GeomArgs ga, const int32_t *nn, int nnn,
+                                                            int type, uint8_t *mask)
+{
+    extern __shared__ unsigned char smem_raw[];
+    uint8_t *eb = smem_raw;
+    const int dim = ga.dim, edge = ga.edge, csz = ga.csz, e = edge + 2;
+    const int64_t c = blockIdx.x;
+    if (c >= n) return;
+    load_enlarged<uint8_t>(eb, mask, nn + c * nnn, type, c, n, dim, edge, csz, (uint8_t)0);
+    __syncthreads();
+    for (int o = threadIdx.x; o < csz; o += blockDim.x) {
+        int t = o, lin = 0, acc = 1;
+        for (int d = 0; d < dim; ++d) {
+            lin += (t % edge + 1) * acc;
+            t /= edge;
+            acc *= e;
+        }
+        const uint8_t cur = eb[lin];
+        if (!(cur & 1)) continue;
+        bool pad = false;
+        if (type == SG_NN_STAR) {
+            int s = 1;
+            for (int d = 0; d < dim; ++d) {
+                pad = pad || !(eb[lin + s] & 1) || !(eb[lin - s] & 1);
+                s *= e;
+            }
+        } else {
+            const int nb = (dim == 3) ? 27 : 9;
+            for (int k = 0; k < nb; ++k) {
+                int kk = k, off = 0, s = 1;
+                for (int d = 0; d < dim; ++d) {
+                    off += (kk % 3 - 1) * s;
+                    kk /= 3;
+                    s *= e;
+                }
+                pad = pad || !(eb[lin + off] & 1);
+            }
+        }
+        mask[c * csz + o] = pad ? (uint8_t)(cur | 2) : (uint8_t)(cur & ~2);
+    }
+}
+
+// ---- 3D 8^3 star fast path ------------------------------------------------------
+// LDS tile per property: rows (z, y) with z, y in [-1, 8] -> index (z+1)*10 + (y+1),
+// each row 8 values.
+template <typename T> struct Row8 {
+    T v[8];
+};
+
+template <typename T> __device__ inline void load_row(Row8<T> &r, const T *p)
+{
+    if constexpr (sizeof(T) == 4) {
+        const float4 *q = (const float4 *)p;
+        float4 a = q[0], b = q[1];
+        r.v[0] = __builtin_bit_cast(T, a.x); r.v[1] = __builtin_bit_cast(T, a.y);
+        r.v[2] = __builtin_bit_cast(T, a.z); r.v[3] = __builtin_bit_cast(T, a.w);
+        r.v[4] = __builtin_bit_cast(T, b.x); r.v[5] = __builtin_bit_cast(T, b.y);
+        r.v[6] = __builtin_bit_cast(T, b.z); r.v[7] = __builtin_bit_cast(T, b.w);
+    } else {
+        const double2 *q = (const double2 *)p;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double2 a = q[i];
+            r.v[2 * i] = __builtin_bit_cast(T, a.x);
+            r.v[2 * i + 1] = __builtin_bit_cast(T, a.y);
+        }
+    }
+}
+
+template <typename T> __device__ inline void store_row(T *p, const Row8<T> &r)
+{
+    if constexpr (sizeof(T) == 4) {
+        float4 *q = (float4 *)p;
+        q[0] = make_float4(__builtin_bit_cast(float, r.v[0]), __builtin_bit_cast(float, r.v[1]),
+                           __builtin_bit_cast(float, r.v[2]), __builtin_bit_cast(float, r.v[3]));
+        q[1] = make_float4(__builtin_bit_cast(float, r.v[4]), __builtin_bit_cast(float, r.v[5]),
+                           __builtin_bit_cast(float, r.v[6]), __builtin_bit_cast(float, r.v[7]));
+    } else {
+        double2 *q = (double2 *)p;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            q[i] = make_double2(__builtin_bit_cast(double, r.v[2 * i]), __builtin_bit_cast(double, r.v[2 * i + 1]));
+    }
+}
+
+// stage one property of one chunk: own row -> registers + LDS, y/z ghost rows -> LDS,
+// x ghosts -> registers
+template <typename T>
+__device__ inline void stage_prop(const T *__restrict__ arr, int64_t c, const int64_t *nbp, T *tile, int lane,
+                                  Row8<T> &own, T &gxm, T &gxp)
+{
+    const int y = lane & 7, z = lane >> 3;
+    load_row(own, arr + c * 512 + lane * 8);
+    // ghost rows: lane -> (face, index, half): 4 faces x 8 rows x 2 halves = 64 lanes
+    {
+        const int face = lane >> 4;      // 0:-y 1:+y 2:-z 3:+z
+        const int idx = (lane >> 1) & 7; // z for y-faces, y for z-faces
+        const int half = lane & 1;
+        int64_t nbc;
+        int srow, trow;
+        if (face == 0) { nbc = nbp[2]; srow = idx * 8 + 7; trow = (idx + 1) * 10 + 0; }
+        else if (face == 1) { nbc = nbp[3]; srow = idx * 8 + 0; trow = (idx + 1) * 10 + 9; }
+        else if (face == 2) { nbc = nbp[4]; srow = 7 * 8 + idx; trow = 0 * 10 + (idx + 1); }
+        else { nbc = nbp[5]; srow = 0 * 8 + idx; trow = 9 * 10 + (idx + 1); }
+        const T *src = arr + nbc * 512 + srow * 8 + half * 4;
+        T *dst = tile + trow * 8 + half * 4;
+        if constexpr (sizeof(T) == 4) {
+            *(float4 *)dst = *(const float4 *)src;
+        } else {
+            ((double2 *)dst)[0] = ((const double2 *)src)[0];
+            ((double2 *)dst)[1] = ((const double2 *)src)[1];
+        }
+    }
+    gxm = arr[nbp[0] * 512 + lane * 8 + 7];
+    gxp = arr[nbp[1] * 512 + lane * 8 + 0];
+    store_row(tile + ((z + 1) * 10 + (y + 1)) * 8, own);
+}
+
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Laplacian pieces at x of a row: returns sum over d of (m_d + p_d) in the
+// reference's accumulation order (x pair, then y pair, then z pair).
+template <typename T>
+__device__ inline void lap_terms(const Row8<T> &own, T gxm, T gxp, const T *tile, int lane, T (&pxy)[8], T (&py)[8],
+                                 T (&pz)[8])
+{
+    const int y = lane & 7, z = lane >> 3;
+    Row8<T> ym, yp, zm, zp;
+    load_row(ym, tile + ((z + 1) * 10 + y) * 8);
+    load_row(yp, tile + ((z + 1) * 10 + y + 2) * 8);
+    load_row(zm, tile + (z * 10 + y + 1) * 8);
+    load_row(zp, tile + ((z + 2) * 10 + y + 1) * 8);
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        const T xm = (x == 0) ? gxm : own.v[x - 1];
+        const T xp = (x == 7) ? gxp : own.v[x + 1];
+        pxy[x] = xm + xp;
+        py[x] = ym.v[x] + yp.v[x];
+        pz[x] = zm.v[x] + zp.v[x];
+    }
+}
+
+// wave-per-chunk star stencil over 8^3 chunks.  KIND: LAP, HEAT, GRAY_SCOTT.
+// Chunks are assigned so that each XCD (workgroups are dealt to the 8 XCDs round
+// robin) walks a contiguous range of the sorted chunk list: neighbours in x and y
+// then tend to be resident in the same XCD's L2.
+template <typename T, int KIND>
+__global__ __launch_bounds__(64) void stencil_fast_kernel(const int32_t *__restrict__ nn, int64_t n,
+                                                          const uint8_t *__restrict__ mask, const T *__restrict__ src0,
+                                                          const T *__restrict__ src1, T *__restrict__ dst0,
+                                                          T *__restrict__ dst1, int alias, StencilArgs sa,
+                                                          const int64_t *__restrict__ keys, int64_t bsz0, int64_t bsz1,
+                                                          int64_t per_xcd)
+{
+    __shared__ __attribute__((aligned(16))) T tile0[100 * 8];
+    __shared__ __attribute__((aligned(16))) T tile1[(KIND == SG_ST_GRAY_SCOTT) ? 100 * 8 : 8];
+    const int64_t b = blockIdx.x;
+    const int64_t c = (b & 7) * per_xcd + (b >> 3);
+    if (c >= n) return;
+    const int lane = threadIdx.x;
+    const int32_t *nnrow = nn + c * 8;
+    int64_t nbp[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) nbp[k] = nnrow[k];
+
+    Row8<T> u, v;
+    T uxm, uxp, vxm = 0, vxp = 0;
+    stage_prop<T>(src0, c, nbp, tile0, lane, u, uxm, uxp);
+    if constexpr (KIND == SG_ST_GRAY_SCOTT) stage_prop<T>(src1, c, nbp, tile1, lane, v, vxm, vxp);
+    const uint64_t mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
+    wave_lds_sync();
+
+    // activity per point
+    bool act[8];
+    {
+        bool inbox[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) inbox[x] = true;
+        if (sa.use_box) {
+            int64_t kb = keys[c];
+            const int64_t bx = kb % bsz0;
+            kb /= bsz0;
+            const int64_t by = kb % bsz1;
+            const int64_t bz = kb / bsz1;
+            const int64_t py = by * 8 + (lane & 7), pz = bz * 8 + (lane >> 3);
+            const bool yz = py >= sa.lo[1] && py <= sa.hi[1] && pz >= sa.lo[2] && pz <= sa.hi[2];
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const int64_t px = bx * 8 + x;
+                inbox[x] = yz && px >= sa.lo[0] && px <= sa.hi[0];
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            const uint8_t cm = (uint8_t)(mrow >> (8 * x));
+            const bool a = (sa.apply == SG_APPLY_EXISTING) ? (cm != 0) : ((cm & SG_EXIST) && !(cm & SG_PADDING));
+            act[x] = a && inbox[x];
+        }
+    }
+
+    T px[8], py[8], pz[8];
+    lap_terms<T>(u, uxm, uxp, tile0, lane, px, py, pz);
+    Row8<T> r0, r1;
+    if constexpr (KIND == SG_ST_LAP) {
+        const T c0 = (T)sa.coef[0], c1 = (T)sa.coef[1];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            T a = c0 * u.v[x];
+            a = a + c1 * px[x];
+            a = a + c1 * py[x];
+            a = a + c1 * pz[x];
+            r0.v[x] = a;
+        }
+    } else if constexpr (KIND == SG_ST_HEAT) {
+        const T dt = (T)sa.coef[0];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            T a = (T)(-6.0) * u.v[x];
+            a = a + px[x];
+            a = a + py[x];
+            a = a + pz[x];
+            r0.v[x] = u.v[x] + dt * a;
+        }
+    } else {
+        T qx[8], qy[8], qz[8];
+        lap_terms<T>(v, vxm, vxp, tile1, lane, qx, qy, qz);
+        const T Du = (T)sa.coef[0], Dv = (T)sa.coef[1], F = (T)sa.coef[2], k = (T)sa.coef[3], dt = (T)sa.coef[4];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            const T cu = u.v[x], cv = v.v[x];
+            T lu = (T)(-6.0) * cu;
+            lu = lu + px[x];
+            lu = lu + py[x];
+            lu = lu + pz[x];
+            T lv = (T)(-6.0) * cv;
+            lv = lv + qx[x];
+            lv = lv + qy[x];
+            lv = lv + qz[x];
+            const T uvv = (cu * cv) * cv;
+            r0.v[x] = cu + dt * ((Du * lu - uvv) + F * ((T)1 - cu));
+            r1.v[x] = cv + dt * ((Dv * lv + uvv) - (F + k) * cv);
+        }
+    }
+
+    bool all = true;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) all = all && act[x];
+    T *o0 = dst0 + c * 512 + lane * 8;
+    T *o1 = (KIND == SG_ST_GRAY_SCOTT) ? dst1 + c * 512 + lane * 8 : nullptr;
+    if (alias) {
+        // destination replaces a source: every slot is written (old value where inactive)
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            r0.v[x] = act[x] ? r0.v[x] : u.v[x];
+            if constexpr (KIND == SG_ST_GRAY_SCOTT) r1.v[x] = act[x] ? r1.v[x] : v.v[x];
+        }
+        store_row(o0, r0);
+        if constexpr (KIND == SG_ST_GRAY_SCOTT) store_row(o1, r1);
+    } else if (all) {
+        store_row(o0, r0);
+        if constexpr (KIND == SG_ST_GRAY_SCOTT) store_row(o1, r1);
+    } else {
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            if (act[x]) {
+                o0[x] = r0.v[x];
+                if constexpr (KIND == SG_ST_GRAY_SCOTT) o1[x] = r1.v[x];
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void get_points_kernel(const int64_t *keys, int64_t n, const uint8_t *mask, const T *arr, int csz,
+                                  const int64_t *pk, int64_t m, T *out, T bgv)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int64_t key = pk[i];
+    const int64_t pos = find_chunk(keys, n, key / csz);
+    const int64_t slot = pos * csz + key % csz;
+    out[i] = (pos < n && (mask[slot] & 1)) ? arr[slot] : bgv;
+}
+
+template <typename T> __global__ void fill_range_kernel(T *p, int64_t n, T v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename T> __global__ void normalize_kernel(T *p, const uint8_t *mask, int64_t n, T v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && mask[i] == 0) p[i] = v;
+}
+
+__global__ __launch_bounds__(kWG) void count_mask_kernel(const uint8_t *mask, int64_t n, int which,
+                                                        unsigned long long *out)
+{
+    __shared__ unsigned long long s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint8_t m = mask[i];
+        c += (which == 0) ? (m & 1) : ((m & 3) == 3);
+    }
+    atomicAdd(&s, c);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, s);
+}
+
+template <typename F> void dispatch_type(int dt, F &&f)
+{
+    switch (dt) {
+    case SG_F32: f(float()); break;
+    case SG_F64: f(double()); break;
+    case SG_I32: f(int32_t()); break;
+    case SG_U32: f(uint32_t()); break;
+    case SG_I64: f(int64_t()); break;
+    case SG_U64: f(uint64_t()); break;
+    case SG_I8: f(int8_t()); break;
+    case SG_U8: f(uint8_t()); break;
+    case SG_I16: f(int16_t()); break;
+    case SG_U16: f(uint16_t()); break;
+    }
+}
+
+GeomArgs geom(const sg_grid *g)
+{
+    GeomArgs ga;
+    ga.dim = g->dim;
+    ga.edge = g->edge;
+    ga.csz = g->csz;
+    ga.bsz0 = g->bsz[0];
+    ga.bsz1 = g->bsz[1];
+    ga.bsz2 = g->bsz[2];
+    return ga;
+}
+
+int ensure_nn(sg_grid *g, int type)
+{
+    if (g->nn_valid[type]) return SG_OK;
+    return find_neighbours_impl(g, type);
+}
+
+}  // namespace
+
+int find_neighbours_impl(sg_grid *g, int type)
+{
+    const int nnn = g->nnn(type);
+    const int64_t n = g->nchunks;
+    if (n * nnn > g->nn_cap[type]) {
+        g->pool.free(g->d_nn[type]);
+        int64_t cap = std::max<int64_t>(n * nnn, 64);
+        g->d_nn[type] = (int32_t *)g->pool.alloc(cap * sizeof(int32_t));
+        if (!g->d_nn[type]) return fail(SG_E_OOM, "neighbour table allocation failed");
+        g->nn_cap[type] = cap;
+    }
+    if (n > 0) {
+        hipLaunchKernelGGL(find_neighbours_kernel, dim3(grid_1d(n * nnn, kWG)), dim3(kWG), 0, g->stream, g->d_keys, n,
+                           geom(g), type, nnn, g->d_nn[type]);
+        SG_HIP(hipGetLastError());
+    }
+    g->nn_valid[type] = true;
+    return SG_OK;
+}
+
+int tag_boundaries_impl(sg_grid *g, int type)
+{
+    const int64_t n = g->nchunks;
+    if (n == 0) return SG_OK;
+    int rc = ensure_nn(g, type);
+    if (rc) return rc;
+    const int e = g->edge + 2;
+    const size_t ebs = (g->dim == 3) ? (size_t)e * e * e : (size_t)e * e;
+    hipLaunchKernelGGL(tag_boundaries_kernel, dim3((unsigned)n), dim3(kWG), ebs, g->stream, n, geom(g),
+                       g->d_nn[type], g->nnn(type), type, g->d_mask);
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
+{
+    const int kind = d->kind;
+    if (kind < SG_ST_LAP || kind > SG_ST_RESIDUAL) return fail(SG_E_ARG, "unknown stencil kind");
+    const int nsrc = (kind == SG_ST_GRAY_SCOTT || kind == SG_ST_RBSOR || kind == SG_ST_RESIDUAL) ? 2 : 1;
+    const int ndst = (kind == SG_ST_GRAY_SCOTT) ? 2 : (kind == SG_ST_RBSOR ? 0 : 1);
+    for (int q = 0; q < nsrc; ++q)
+        if (d->src[q] < 0 || d->src[q] >= g->np) return fail(SG_E_ARG, "bad source property");
+    for (int q = 0; q < ndst; ++q)
+        if (d->dst[q] < 0 || d->dst[q] >= g->np) return fail(SG_E_ARG, "bad destination property");
+    if (ndst == 2 && d->dst[0] == d->dst[1]) return fail(SG_E_ARG, "the two destinations must differ");
+    const int dt = g->dt[d->src[0]];
+    for (int q = 0; q < nsrc; ++q)
+        if (g->dt[d->src[q]] != dt) return fail(SG_E_ARG, "stencil properties must share one type");
+    for (int q = 0; q < ndst; ++q)
+        if (g->dt[d->dst[q]] != dt) return fail(SG_E_ARG, "stencil properties must share one type");
+    if (kind != SG_ST_BOX && !dtype_is_float(dt)) return fail(SG_E_ARG, "this stencil needs f32 or f64 properties");
+    if (d->apply != SG_APPLY_INTERIOR && d->apply != SG_APPLY_EXISTING) return fail(SG_E_ARG, "bad apply mode");
+    const int64_t n = g->nchunks;
+    if (n == 0) return SG_OK;
+    const int type = (kind == SG_ST_BOX) ? SG_NN_FULL : SG_NN_STAR;
+    int rc = ensure_nn(g, type);
+    if (rc) return rc;
+
+    StencilArgs sa;
+    sa.kind = kind;
+    sa.apply = d->apply;
+    sa.use_box = d->use_box;
+    for (int i = 0; i < 3; ++i) {
+        sa.lo[i] = d->box_lo[i];
+        sa.hi[i] = d->box_hi[i];
+    }
+    for (int i = 0; i < 27; ++i) {
+        sa.coef[i] = d->coef[i];
+        sa.icoef[i] = d->icoef[i];
+    }
+    // a destination that is also a source is computed into a fresh array that then
+    // replaces the property (every chunk reads the pre-sweep state)
+    int alias[2] = {0, 0};
+    for (int q = 0; q < ndst; ++q)
+        for (int s = 0; s < nsrc; ++s)
+            if (d->dst[q] == d->src[s]) alias[q] = s + 1;
+    void *outp[2] = {nullptr, nullptr};
+    const size_t arr_bytes = (size_t)(g->cap + 1) * g->csz * g->ts[d->src[0]];
+    for (int q = 0; q < ndst; ++q) {
+        if (alias[q]) {
+            outp[q] = g->pool.alloc(arr_bytes);
+            if (!outp[q]) return fail(SG_E_OOM, "stencil output allocation failed");
+        } else {
+            outp[q] = g->d_prop[d->dst[q]];
+        }
+    }
+    // the fast kernel writes old values of dst q from src q, so it takes dst q
+    // aliasing src q (or no aliasing) only
+    const bool alias_ok = (ndst == 1) ? (alias[0] == 0 || alias[0] == 1)
+                                      : ((alias[0] == 0 && alias[1] == 0) || (alias[0] == 1 && alias[1] == 2));
+    const bool fast = g->dim == 3 && g->edge == 8 && type == SG_NN_STAR &&
+                      (kind == SG_ST_LAP || kind == SG_ST_HEAT || kind == SG_ST_GRAY_SCOTT) && alias_ok;
+    dispatch_type(dt, [&](auto tag) {
+        typedef decltype(tag) T;
+        const T *s0 = (const T *)g->d_prop[d->src[0]];
+        const T *s1 = nsrc > 1 ? (const T *)g->d_prop[d->src[1]] : nullptr;
+        T *o0 = (T *)(ndst > 0 ? outp[0] : (kind == SG_ST_RBSOR ? g->d_prop[d->src[0]] : nullptr));
+        T *o1 = (T *)(ndst > 1 ? outp[1] : nullptr);
+        if constexpr (std::is_same<T, float>::value || std::is_same<T, double>::value) {
+            if (fast) {
+                const int64_t per_xcd = (n + 7) / 8;
+                const unsigned nblk = (unsigned)(per_xcd * 8);
+                const int al = alias[0] != 0;
+                if (kind == SG_ST_LAP)
+                    hipLaunchKernelGGL((stencil_fast_kernel<T, SG_ST_LAP>), dim3(nblk), dim3(64), 0, g->stream,
+                                       g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
+                                       g->bsz[1], per_xcd);
+                else if (kind == SG_ST_HEAT)
+                    hipLaunchKernelGGL((stencil_fast_kernel<T, SG_ST_HEAT>), dim3(nblk), dim3(64), 0, g->stream,
+                                       g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
+                                       g->bsz[1], per_xcd);
+                else
+                    hipLaunchKernelGGL((stencil_fast_kernel<T, SG_ST_GRAY_SCOTT>), dim3(nblk), dim3(64), 0,
+                                       g->stream, g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys,
+                                       g->bsz[0], g->bsz[1], per_xcd);
+                return;
+            }
+        }
+        T bg0, bg1;
+        memcpy(&bg0, g->bg[d->src[0]], sizeof(T));
+        memcpy(&bg1, g->bg[nsrc > 1 ? d->src[1] : d->src[0]], sizeof(T));
+        const int e = g->edge + 2;
+        const size_t ebs = (g->dim == 3) ? (size_t)e * e * e : (size_t)e * e;
+        hipLaunchKernelGGL(stencil_generic_kernel<T>, dim3((unsigned)n), dim3(kWG), ebs * sizeof(T) * nsrc, g->stream,
+                           g->d_keys, n, geom(g), g->d_nn[type], g->nnn(type), type, g->d_mask, s0, s1, o0, o1,
+                           alias[0], alias[1], bg0, bg1, sa);
+    });
+    SG_HIP(hipGetLastError());
+    for (int q = 0; q < ndst; ++q) {
+        if (!alias[q]) continue;
+        const int p = d->dst[q];
+        g->pool.free(g->d_prop[p]);
+        g->d_prop[p] = outp[q];
+        rc = fill_background_chunk(g, p);
+        if (rc) return rc;
+    }
+    return SG_OK;
+}
+
+int fill_background_chunk(sg_grid *g, int p)
+{
+    int rc = SG_OK;
+    dispatch_type(g->dt[p], [&](auto tag) {
+        typedef decltype(tag) T;
+        T v;
+        memcpy(&v, g->bg[p], sizeof(T));
+        T *base = (T *)g->d_prop[p] + g->nchunks * g->csz;
+        hipLaunchKernelGGL(fill_range_kernel<T>, dim3(grid_1d(g->csz, kWG)), dim3(kWG), 0, g->stream, base,
+                           (int64_t)g->csz, v);
+    });
+    SG_HIP(hipGetLastError());
+    return rc;
+}
+
+int normalize_prop(sg_grid *g, int p)
+{
+    const int64_t total = (g->nchunks + 1) * g->csz;
+    dispatch_type(g->dt[p], [&](auto tag) {
+        typedef decltype(tag) T;
+        T v;
+        memcpy(&v, g->bg[p], sizeof(T));
+        hipLaunchKernelGGL(normalize_kernel<T>, dim3(grid_1d(total, kWG)), dim3(kWG), 0, g->stream, (T *)g->d_prop[p],
+                           g->d_mask, total, v);
+    });
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+int get_points_impl(sg_grid *g, int prop, int64_t m, const int64_t *keys, void *out, int mem)
+{
+    if (m <= 0) return SG_OK;
+    const size_t ts = g->ts[prop];
+    DevBuf &b = g->scratch[9];
+    int rc = ensure(g->pool, b, (size_t)m * (8 + ts) + 64);
+    if (rc) return rc;
+    const int64_t *dk = keys;
+    void *dout = out;
+    if (mem == SG_MEM_HOST) {
+        SG_HIP(hipMemcpyAsync(b.p, keys, m * sizeof(int64_t), hipMemcpyHostToDevice, g->stream));
+        dk = (const int64_t *)b.p;
+        dout = (char *)b.p + m * sizeof(int64_t);
+    }
+    dispatch_type(g->dt[prop], [&](auto tag) {
+        typedef decltype(tag) T;
+        T v;
+        memcpy(&v, g->bg[prop], sizeof(T));
+        hipLaunchKernelGGL(get_points_kernel<T>, dim3(grid_1d(m, kWG)), dim3(kWG), 0, g->stream, g->d_keys,
+                           g->nchunks, g->d_mask, (const T *)g->d_prop[prop], g->csz, dk, m, (T *)dout, v);
+    });
+    SG_HIP(hipGetLastError());
+    if (mem == SG_MEM_HOST) {
+        SG_HIP(hipMemcpyAsync(out, dout, m * ts, hipMemcpyDeviceToHost, g->stream));
+        SG_HIP(hipStreamSynchronize(g->stream));
+    }
+    return SG_OK;
+}
+
+int count_mask_impl(sg_grid *g, int which, int64_t *out)
+{
+    DevBuf &b = g->scratch[9];
+    int rc = ensure(g->pool, b, 64);
+    if (rc) return rc;
+    unsigned long long *d = (unsigned long long *)b.p;
+    SG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), g->stream));
+    const int64_t total = g->nchunks * g->csz;
+    if (total > 0) {
+        unsigned blocks = (unsigned)std::min<int64_t>(grid_1d(total, kWG), 4096);
+        hipLaunchKernelGGL(count_mask_kernel, dim3(blocks), dim3(kWG), 0, g->stream, g->d_mask, total, which, d);
+        SG_HIP(hipGetLastError());
+    }
+    unsigned long long h = 0;
+    SG_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+    SG_HIP(hipStreamSynchronize(g->stream));
+    *out = (int64_t)h;
+    return SG_OK;
+}
+
+}  // namespace sg

@@ -1,0 +1,235 @@
+"""Scenarios run identically on the oracle (oracle.pysgcpu.SparseGridCpu) and on the
+MI355X library (openfpm_data_amd.SparseGridGpu): both expose the same methods."""
+import numpy as np
+
+from openfpm_data_amd import _capi as C
+
+
+def known_tag_boundaries(G, case, nn=C.NN_STAR):
+    """SparseGridGpu_tests.cu:284-470: several flushes of single points, then tagBoundaries."""
+    g = G(case["dim"], case["size"], [np.float32], block_edge=case["block_edge"])
+    g.set_background_value(0, case["background"])
+    for pts in case["flushes"]:
+        pts = np.asarray(pts, dtype=np.int32)
+        g.insert_coords(pts, [np.full(len(pts), case["value"], np.float32)])
+        g.flush(["smax"])
+    g.find_neighbours(nn)
+    g.tag_boundaries(nn)
+    return g
+
+
+def padding_ids(g, npoints):
+    keys = np.arange(npoints, dtype=np.int64)
+    ids = g.chunk_ids()
+    m = g.masks()
+    cs = g.chunk_size
+    pos = np.searchsorted(ids, keys // cs)
+    ok = (pos < ids.size) & (ids[np.minimum(pos, ids.size - 1)] == keys // cs)
+    out = np.zeros(npoints, dtype=bool)
+    out[ok] = (m[pos[ok], keys[ok] % cs] & C.PADDING) != 0
+    return np.nonzero(out)[0]
+
+
+def fill_all(G, dim, size, block_edge, props, value_fn, op="smax"):
+    """Insert every point of a dense domain (insertValues / insertConstantValue)."""
+    g = G(dim, size, props, block_edge=block_edge)
+    grids = np.meshgrid(*[np.arange(s) for s in size[::-1]], indexing="ij")
+    coords = np.stack([grids[dim - 1 - d].reshape(-1) for d in range(dim)], axis=1).astype(np.int32)
+    return g, coords
+
+
+def random_flush_program(seed, dim, edge, size, dtypes, nflush=3, nrec=200, with_chunks=True, ops=None):
+    """A random insert/flush program (deterministic): list of steps."""
+    rng = np.random.default_rng(seed)
+    cs = edge ** dim
+    bsz = [-(-s // edge) for s in size]
+    nblocks = int(np.prod(bsz))
+    steps = []
+    for f in range(nflush):
+        recs = []
+        # point records concentrated in a few chunks so that duplicates occur
+        hot = rng.integers(0, nblocks, size=max(1, nblocks // 3))
+        for _ in range(rng.integers(1, 4)):
+            n = int(rng.integers(1, nrec))
+            ch = rng.choice(hot, n)
+            off = rng.integers(0, cs, n)
+            keys = ch.astype(np.int64) * cs + off
+            masks = rng.choice(np.array([1, 1, 1, 3, 0, 2], dtype=np.uint8), n)
+            vals = [_rand_vals(rng, dt, n) for dt in dtypes]
+            recs.append(("points", keys, masks, vals))
+            if with_chunks and rng.random() < 0.5:
+                m = int(rng.integers(1, 4))
+                cids = rng.choice(hot, m).astype(np.int64)
+                cm = rng.choice(np.array([0, 0, 1, 1, 3], dtype=np.uint8), (m, cs))
+                if rng.random() < 0.3:
+                    cm[0, :] = 0  # an all-empty chunk slot still creates the chunk
+                cv = [_rand_vals(rng, dt, m * cs).reshape(m, cs) for dt in dtypes]
+                recs.append(("chunks", cids, cm, cv))
+        fops = ops if ops is not None else [
+            str(rng.choice(["smax", "smin", "sadd", "sright", "sleft", "none"] +
+                           (["sor"] if np.issubdtype(dt, np.integer) else []))) for dt in dtypes]
+        steps.append((recs, fops))
+    return steps
+
+
+def _rand_vals(rng, dt, n):
+    dt = np.dtype(dt)
+    if np.issubdtype(dt, np.floating):
+        # small integers keep float sums exact whatever the order
+        return rng.integers(-50, 50, n).astype(dt)
+    info = np.iinfo(dt)
+    lo, hi = max(info.min, -1000), min(info.max, 1000)
+    if dt == np.uint32:
+        return rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    return rng.integers(lo, hi, n).astype(dt)
+
+
+def run_flush_program(G, dim, edge, size, dtypes, steps, background=None):
+    g = G(dim, size, dtypes, block_edge=edge)
+    if background is not None:
+        for p, b in enumerate(background):
+            g.set_background_value(p, b)
+    for recs, ops in steps:
+        for kind, a, m, v in recs:
+            if kind == "points":
+                g.insert(a, v, m)
+            else:
+                g.insert_block(a, m, v)
+        g.flush(ops)
+    return g
+
+
+def model_flush(dim, edge, size, dtypes, steps, background=None):
+    """Brute-force dictionary model of the flush semantics: per point, records folded in
+    insertion order (a = op(a, b) when both exist, a = b when only b exists), then the
+    old value; mask = OR of everything; chunks are created by any record."""
+    cs = edge ** dim
+    chunks = set()
+    vals = {}   # key -> list of values
+    masks = {}  # key -> mask
+    bg = [np.zeros(1, dt)[0] for dt in dtypes] if background is None else \
+        [np.array([b], dt)[0] for b, dt in zip(background, dtypes)]
+
+    def red(op, a, b, dt):
+        if op == "smax":
+            return b if a < b else a
+        if op == "smin":
+            return b if b < a else a
+        if op == "sadd":
+            with np.errstate(over="ignore"):
+                return (np.array([a], dt) + np.array([b], dt))[0]
+        if op == "sright":
+            return b
+        if op == "sleft":
+            return a
+        if op == "sor":
+            return np.array([a], dt)[0] | np.array([b], dt)[0]
+        raise ValueError(op)
+
+    for recs, ops in steps:
+        new = {}  # key -> (mask, [vals])
+        order = []
+        for kind, a, m, v in recs:
+            if kind == "points":
+                for i in range(len(a)):
+                    order.append((int(a[i]), int(m[i]), [vv[i] for vv in v]))
+                    chunks.add(int(a[i]) // cs)
+            else:
+                for j in range(len(a)):
+                    chunks.add(int(a[j]))
+                    for o in range(cs):
+                        if m[j, o]:
+                            order.append((int(a[j]) * cs + o, int(m[j, o]), [vv[j, o] for vv in v]))
+        for key, mk, vs in order:
+            if key not in new:
+                new[key] = [mk, list(vs)]
+                continue
+            am, av = new[key]
+            aE, bE = am & 1, mk & 1
+            for p, dt in enumerate(dtypes):
+                if ops[p] == "none":
+                    if bE:
+                        av[p] = vs[p]
+                elif aE and bE:
+                    av[p] = red(ops[p], av[p], vs[p], np.dtype(dt))
+                elif bE:
+                    av[p] = vs[p]
+            new[key] = [am | mk, av]
+        for key, (am, av) in new.items():
+            om = masks.get(key, 0)
+            ov = vals.get(key, list(bg))
+            aE, oE = am & 1, om & 1
+            res = list(av)
+            for p, dt in enumerate(dtypes):
+                if ops[p] == "none":
+                    if not aE and oE:
+                        res[p] = ov[p]
+                elif aE and oE:
+                    res[p] = red(ops[p], av[p], ov[p], np.dtype(dt))
+                elif oE:
+                    res[p] = ov[p]
+            masks[key] = am | om
+            vals[key] = res
+    return chunks, masks, vals, cs
+
+
+def compare_to_model(g, model, dtypes):
+    chunks, masks, vals, cs = model
+    ids = g.chunk_ids()
+    assert list(ids) == sorted(chunks)
+    m = g.masks()
+    exp_m = np.zeros_like(m)
+    pos = {int(c): i for i, c in enumerate(ids)}
+    for key, mk in masks.items():
+        exp_m[pos[key // cs], key % cs] = mk
+    np.testing.assert_array_equal(m, exp_m)
+    for p, dt in enumerate(dtypes):
+        arr = g.prop(p)
+        for key, mk in masks.items():
+            if mk & 1:
+                got = arr[pos[key // cs], key % cs]
+                exp = np.array([vals[key][p]], dtype=dt)[0]
+                assert got == exp or (np.isnan(got) and np.isnan(exp)), (p, key, got, exp)
+
+
+def grid_state(g):
+    """(chunk ids, masks, [props]) of a grid."""
+    return g.chunk_ids(), g.masks(), [g.prop(p) for p in range(len(g.dtypes))]
+
+
+def assert_same_state(a, b, props=None, exist_only=True, rtol=None):
+    ia, ma, pa = grid_state(a)
+    ib, mb, pb = grid_state(b)
+    np.testing.assert_array_equal(ia, ib)
+    np.testing.assert_array_equal(ma, mb)
+    sel = (ma & 1) != 0 if exist_only else np.ones_like(ma, dtype=bool)
+    for p in (range(len(pa)) if props is None else props):
+        if rtol is None:
+            np.testing.assert_array_equal(pa[p][sel], pb[p][sel], err_msg="property %d" % p)
+        else:
+            np.testing.assert_allclose(pa[p][sel], pb[p][sel], rtol=rtol, atol=0, err_msg="property %d" % p)
+
+
+def dense_of(g, p, fill=0.0):
+    """Dense array [z][y][x] (or [y][x]) of property p (background/fill where absent)."""
+    dt = g.dtypes[p]
+    out = np.full(g.size[::-1], fill, dtype=dt)
+    ids = g.chunk_ids()
+    m = g.masks()
+    v = g.prop(p)
+    keys = (ids[:, None] * g.chunk_size + np.arange(g.chunk_size)[None, :]).reshape(-1)
+    ex = (m.reshape(-1) & 1) != 0
+    crd = g.inv_lin_id(keys[ex])
+    out[tuple(crd[:, d] for d in range(g.dim - 1, -1, -1))] = v.reshape(-1)[ex]
+    return out
+
+
+def exist_dense(g):
+    out = np.zeros(g.size[::-1], dtype=np.uint8)
+    ids = g.chunk_ids()
+    m = g.masks()
+    keys = (ids[:, None] * g.chunk_size + np.arange(g.chunk_size)[None, :]).reshape(-1)
+    mm = m.reshape(-1)
+    crd = g.inv_lin_id(keys)
+    out[tuple(crd[:, d] for d in range(g.dim - 1, -1, -1))] = mm
+    return out
