@@ -182,6 +182,22 @@ int sg_tag_boundaries(sg_grid *g, int nn_type, int opt);
 /* applyStencils / conv / conv2 (src/SparseGridGpu/SparseGridGpu.hpp:2568-2734) */
 int sg_apply_stencil(sg_grid *g, const sg_stencil_desc *desc);
 
+/* ---- launch tuning (no counterpart in the reference: its launch shapes are
+ *      compile-time template parameters, SparseGridGpu.hpp:483-489) -------------- */
+
+/* workgroups (of 4 waves) per XCD of the persistent 8^3 star-stencil kernel */
+#define SG_TUNE_STENCIL_WG_PER_XCD 1
+/* 8^3 star-stencil kernel variant: 0 one chunk per 64-lane workgroup, 1 persistent
+ * software-pipelined waves, 2 as 1 with registers capped for 4 waves per SIMD,
+ * 3 / 4 / 5 workgroups of 4 / 8 / 2 consecutive chunks sharing x faces in LDS */
+#define SG_TUNE_STENCIL_VARIANT 2
+/* 8^3 star-stencil stores: 0 active points only, 1 whole 8-point rows where every point
+ * is active or absent (absent points keep the background), 2 as 1, non-temporal */
+#define SG_TUNE_STENCIL_STORES 3
+/* 8^3 star-stencil loads: 1 = read the mask first and skip rows without points */
+#define SG_TUNE_STENCIL_SKIP_EMPTY_ROWS 4
+int sg_set_tuning(sg_grid *g, int key, int64_t value);
+
 /* ---- queries ---------------------------------------------------------------- */
 
 int64_t sg_num_chunks(sg_grid *g);

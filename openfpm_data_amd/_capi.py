@@ -36,6 +36,11 @@ OP_NONE, OP_MAX, OP_MIN, OP_ADD, OP_RIGHT, OP_LEFT, OP_OR = range(7)
 OPS = {"none": OP_NONE, "smax": OP_MAX, "smin": OP_MIN, "sadd": OP_ADD,
        "sright": OP_RIGHT, "sleft": OP_LEFT, "sor": OP_OR}
 
+TUNE_STENCIL_WG_PER_XCD = 1
+TUNE_STENCIL_VARIANT = 2
+TUNE_STENCIL_STORES = 3
+TUNE_STENCIL_SKIP_EMPTY_ROWS = 4
+
 EXIST = 1
 PADDING = 2
 NN_STAR = 0
@@ -107,6 +112,7 @@ _SIGS = {
     "sg_set_nn_type": (_INT, [_P, _INT]),
     "sg_tag_boundaries": (_INT, [_P, _INT, _INT]),
     "sg_apply_stencil": (_INT, [_P, ctypes.POINTER(StencilDesc)]),
+    "sg_set_tuning": (_INT, [_P, _INT, _I64]),
     "sg_num_chunks": (_I64, [_P]),
     "sg_chunk_size": (_INT, [_P]),
     "sg_get_chunk_ids": (_INT, [_P, _P, _INT]),

@@ -265,6 +265,32 @@ int sg_apply_stencil(sg_grid *g, const sg_stencil_desc *desc)
     return apply_stencil_impl(g, desc);
 }
 
+int sg_set_tuning(sg_grid *g, int key, int64_t value)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    if (key == SG_TUNE_STENCIL_WG_PER_XCD) {
+        if (value < 1 || value > 4096) return fail(SG_E_RANGE, "workgroups per XCD must be in [1, 4096]");
+        g->fast_wg_per_xcd = value;
+        return SG_OK;
+    }
+    if (key == SG_TUNE_STENCIL_VARIANT) {
+        if (value < 0 || value > 5) return fail(SG_E_RANGE, "stencil variant must be in [0, 5]");
+        g->fast_variant = (int)value;
+        return SG_OK;
+    }
+    if (key == SG_TUNE_STENCIL_STORES) {
+        if (value < 0 || value > 2) return fail(SG_E_RANGE, "store mode must be 0, 1 or 2");
+        g->fast_wmode = (int)value;
+        return SG_OK;
+    }
+    if (key == SG_TUNE_STENCIL_SKIP_EMPTY_ROWS) {
+        if (value < 0 || value > 1) return fail(SG_E_RANGE, "skip-empty-rows must be 0 or 1");
+        g->fast_lskip = (int)value;
+        return SG_OK;
+    }
+    return fail(SG_E_ARG, "unknown tuning key");
+}
+
 int64_t sg_num_chunks(sg_grid *g) { return g ? g->nchunks : 0; }
 
 int sg_chunk_size(sg_grid *g) { return g ? g->csz : 0; }

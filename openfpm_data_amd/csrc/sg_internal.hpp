@@ -102,6 +102,18 @@ struct sg_grid {
     uint8_t *d_ins_mask = nullptr;
     void *d_ins_val[sg::kMaxProps] = {};
 
+    // launch shape of the persistent 8^3 star-stencil kernel: workgroups (4 waves)
+    // per XCD; set through sg_set_tuning (the autotuner picks it per workload)
+    int64_t fast_wg_per_xcd = 160;
+    // 0: one chunk per 64-lane workgroup; 1: persistent + pipelined; 2: persistent +
+    // pipelined with registers capped for 4 waves per SIMD
+    int fast_variant = 4;
+    // stores of the 8^3 star-stencil kernel: 0 active points only, 1 whole rows where
+    // every slot is active or empty, 2 as 1 with non-temporal stores
+    int fast_wmode = 2;
+    // read the mask first and skip loading 8-point rows without points
+    int fast_lskip = 0;
+
     hipStream_t stream = nullptr;
     sg::DevicePool pool;
     sg::DevBuf scratch[12];  // flush / sort / scan temporaries

@@ -169,7 +169,11 @@ __device__ void load_enlarged(T *eb, const T *arr, const int32_t *nnrow, int typ
 
 struct StencilArgs {
     int kind, apply, use_box;
+    int wmode;  // 8^3 fast path stores: 0 active points only, 1 whole rows, 2 whole rows non-temporal
     int lo[3], hi[3];
+    double bgd[2];  // destination backgrounds
+    double bgs[2];  // source backgrounds
+    int lskip;      // 8^3 fast path: read the mask first and skip loading rows without points
     double coef[27];
     int64_t icoef[27];
 };
@@ -379,37 +383,24 @@ template <typename T> __device__ inline void store_row(T *p, const Row8<T> &r)
     }
 }
 
-// stage one property of one chunk: own row -> registers + LDS, y/z ghost rows -> LDS,
-// x ghosts -> registers
-template <typename T>
-__device__ inline void stage_prop(const T *__restrict__ arr, int64_t c, const int64_t *nbp, T *tile, int lane,
-                                  Row8<T> &own, T &gxm, T &gxp)
+template <typename T> __device__ inline void store_row_nt(T *p, const Row8<T> &r)
 {
-    const int y = lane & 7, z = lane >> 3;
-    load_row(own, arr + c * 512 + lane * 8);
-    // ghost rows: lane -> (face, index, half): 4 faces x 8 rows x 2 halves = 64 lanes
-    {
-        const int face = lane >> 4;      // 0:-y 1:+y 2:-z 3:+z
-        const int idx = (lane >> 1) & 7; // z for y-faces, y for z-faces
-        const int half = lane & 1;
-        int64_t nbc;
-        int srow, trow;
-        if (face == 0) { nbc = nbp[2]; srow = idx * 8 + 7; trow = (idx + 1) * 10 + 0; }
-        else if (face == 1) { nbc = nbp[3]; srow = idx * 8 + 0; trow = (idx + 1) * 10 + 9; }
-        else if (face == 2) { nbc = nbp[4]; srow = 7 * 8 + idx; trow = 0 * 10 + (idx + 1); }
-        else { nbc = nbp[5]; srow = 0 * 8 + idx; trow = 9 * 10 + (idx + 1); }
-        const T *src = arr + nbc * 512 + srow * 8 + half * 4;
-        T *dst = tile + trow * 8 + half * 4;
-        if constexpr (sizeof(T) == 4) {
-            *(float4 *)dst = *(const float4 *)src;
-        } else {
-            ((double2 *)dst)[0] = ((const double2 *)src)[0];
-            ((double2 *)dst)[1] = ((const double2 *)src)[1];
+    if constexpr (sizeof(T) == 4) {
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        v4f a = {__builtin_bit_cast(float, r.v[0]), __builtin_bit_cast(float, r.v[1]),
+                 __builtin_bit_cast(float, r.v[2]), __builtin_bit_cast(float, r.v[3])};
+        v4f b = {__builtin_bit_cast(float, r.v[4]), __builtin_bit_cast(float, r.v[5]),
+                 __builtin_bit_cast(float, r.v[6]), __builtin_bit_cast(float, r.v[7])};
+        __builtin_nontemporal_store(a, (v4f *)p);
+        __builtin_nontemporal_store(b, (v4f *)p + 1);
+    } else {
+        typedef double v2d __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            v2d a = {__builtin_bit_cast(double, r.v[2 * i]), __builtin_bit_cast(double, r.v[2 * i + 1])};
+            __builtin_nontemporal_store(a, (v2d *)p + i);
         }
     }
-    gxm = arr[nbp[0] * 512 + lane * 8 + 7];
-    gxp = arr[nbp[1] * 512 + lane * 8 + 0];
-    store_row(tile + ((z + 1) * 10 + (y + 1)) * 8, own);
 }
 
 __device__ inline void wave_lds_sync()
@@ -419,11 +410,85 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Laplacian pieces at x of a row: returns sum over d of (m_d + p_d) in the
-// reference's accumulation order (x pair, then y pair, then z pair).
+template <typename T> struct Half4 {
+    T v[4];
+};
+
+// Per-lane inputs of one chunk for one property: own x-row, one half ghost row of the
+// +-y / +-z faces (lane -> face = lane >> 4, row = (lane >> 1) & 7, half = lane & 1),
+// and the two x-ghost scalars of the own row.
+template <typename T> struct PropIn {
+    Row8<T> own;
+    Half4<T> gh;
+    T xm, xp;
+};
+
+// Ghost half rows of the +-y / +-z faces: lane -> face = lane >> 4 (0: -y, 1: +y, 2: -z,
+// 3: +z), row index idx = (lane >> 1) & 7 (z for the y faces, y for the z faces), half
+// = lane & 1.  Source row inside the neighbour chunk (row = z*8 + y) and destination
+// row of the LDS tile ((z+1)*10 + (y+1)), chosen without branches.
+__device__ inline int ghost_src_row(int lane)
+{
+    const int face = lane >> 4, idx = (lane >> 1) & 7;
+    const int ry = idx * 8 + ((face == 0) ? 7 : 0);
+    const int rz = ((face == 2) ? 56 : 0) + idx;
+    return (face < 2) ? ry : rz;
+}
+
+__device__ inline int ghost_tile_row(int lane)
+{
+    const int face = lane >> 4, idx = (lane >> 1) & 7;
+    const int ty = (idx + 1) * 10 + ((face == 0) ? 0 : 9);
+    const int tz = ((face == 2) ? 0 : 90) + idx + 1;
+    return (face < 2) ? ty : tz;
+}
+
+template <typename T> __device__ inline void load_half(Half4<T> &h, const T *src)
+{
+    if constexpr (sizeof(T) == 4) {
+        const float4 a = *(const float4 *)src;
+        h.v[0] = __builtin_bit_cast(T, a.x); h.v[1] = __builtin_bit_cast(T, a.y);
+        h.v[2] = __builtin_bit_cast(T, a.z); h.v[3] = __builtin_bit_cast(T, a.w);
+    } else {
+        const double2 a = ((const double2 *)src)[0], b = ((const double2 *)src)[1];
+        h.v[0] = __builtin_bit_cast(T, a.x); h.v[1] = __builtin_bit_cast(T, a.y);
+        h.v[2] = __builtin_bit_cast(T, b.x); h.v[3] = __builtin_bit_cast(T, b.y);
+    }
+}
+
+// neighbour-dependent loads of one property: the face half row and (optionally) the
+// two x-ghost scalars of the own row
 template <typename T>
-__device__ inline void lap_terms(const Row8<T> &own, T gxm, T gxp, const T *tile, int lane, T (&pxy)[8], T (&py)[8],
-                                 T (&pz)[8])
+__device__ inline void issue_ghosts(PropIn<T> &r, const T *__restrict__ arr, const int64_t (&nbp)[6], int lane,
+                                    bool load_xm, bool load_xp)
+{
+    const int face = lane >> 4;
+    const int64_t nbc = (face == 0) ? nbp[2] : (face == 1) ? nbp[3] : (face == 2) ? nbp[4] : nbp[5];
+    load_half(r.gh, arr + nbc * 512 + ghost_src_row(lane) * 8 + (lane & 1) * 4);
+    if (load_xm) r.xm = arr[nbp[0] * 512 + lane * 8 + 7];
+    if (load_xp) r.xp = arr[nbp[1] * 512 + lane * 8 + 0];
+}
+
+// LDS tile per property: rows (z, y) with z, y in [-1, 8] -> row index (z+1)*10 + (y+1),
+// 8 values per row.  Own rows and the 32 face rows are written; corner rows are unused.
+template <typename T> __device__ inline void stage_lds(const PropIn<T> &r, T *tile, int lane)
+{
+    const int y = lane & 7, z = lane >> 3;
+    store_row(tile + ((z + 1) * 10 + (y + 1)) * 8, r.own);
+    T *dst = tile + ghost_tile_row(lane) * 8 + (lane & 1) * 4;
+    if constexpr (sizeof(T) == 4) {
+        *(float4 *)dst = make_float4(__builtin_bit_cast(float, r.gh.v[0]), __builtin_bit_cast(float, r.gh.v[1]),
+                                     __builtin_bit_cast(float, r.gh.v[2]), __builtin_bit_cast(float, r.gh.v[3]));
+    } else {
+        ((double2 *)dst)[0] = make_double2(__builtin_bit_cast(double, r.gh.v[0]), __builtin_bit_cast(double, r.gh.v[1]));
+        ((double2 *)dst)[1] = make_double2(__builtin_bit_cast(double, r.gh.v[2]), __builtin_bit_cast(double, r.gh.v[3]));
+    }
+}
+
+// Laplacian pair sums at x of a row, in the reference's accumulation order
+// (x pair, then y pair, then z pair; HeatStencil, SparseGridGpu_util_test.cuh:240-255)
+template <typename T>
+__device__ inline void lap_terms(const PropIn<T> &r, const T *tile, int lane, T (&pxy)[8], T (&py)[8], T (&pz)[8])
 {
     const int y = lane & 7, z = lane >> 3;
     Row8<T> ym, yp, zm, zp;
@@ -433,45 +498,77 @@ __device__ inline void lap_terms(const Row8<T> &own, T gxm, T gxp, const T *tile
     load_row(zp, tile + ((z + 2) * 10 + y + 1) * 8);
 #pragma unroll
     for (int x = 0; x < 8; ++x) {
-        const T xm = (x == 0) ? gxm : own.v[x - 1];
-        const T xp = (x == 7) ? gxp : own.v[x + 1];
+        const T xm = (x == 0) ? r.xm : r.own.v[x - 1];
+        const T xp = (x == 7) ? r.xp : r.own.v[x + 1];
         pxy[x] = xm + xp;
         py[x] = ym.v[x] + yp.v[x];
         pz[x] = zm.v[x] + zp.v[x];
     }
 }
 
-// wave-per-chunk star stencil over 8^3 chunks.  KIND: LAP, HEAT, GRAY_SCOTT.
-// Chunks are assigned so that each XCD (workgroups are dealt to the 8 XCDs round
-// robin) walks a contiguous range of the sorted chunk list: neighbours in x and y
-// then tend to be resident in the same XCD's L2.
-template <typename T, int KIND>
-__global__ __launch_bounds__(64) void stencil_fast_kernel(const int32_t *__restrict__ nn, int64_t n,
-                                                          const uint8_t *__restrict__ mask, const T *__restrict__ src0,
-                                                          const T *__restrict__ src1, T *__restrict__ dst0,
-                                                          T *__restrict__ dst1, int alias, StencilArgs sa,
-                                                          const int64_t *__restrict__ keys, int64_t bsz0, int64_t bsz1,
-                                                          int64_t per_xcd)
-{
-    __shared__ __attribute__((aligned(16))) T tile0[100 * 8];
-    __shared__ __attribute__((aligned(16))) T tile1[(KIND == SG_ST_GRAY_SCOTT) ? 100 * 8 : 8];
-    const int64_t b = blockIdx.x;
-    const int64_t c = (b & 7) * per_xcd + (b >> 3);
-    if (c >= n) return;
-    const int lane = threadIdx.x;
-    const int32_t *nnrow = nn + c * 8;
-    int64_t nbp[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) nbp[k] = nnrow[k];
+template <typename T, int KIND> struct ChunkIn {
+    static constexpr bool kTwo = (KIND == SG_ST_GRAY_SCOTT);
+    PropIn<T> u;
+    PropIn<T> v;
+    uint64_t mrow;
+};
 
-    Row8<T> u, v;
-    T uxm, uxp, vxm = 0, vxp = 0;
-    stage_prop<T>(src0, c, nbp, tile0, lane, u, uxm, uxp);
-    if constexpr (KIND == SG_ST_GRAY_SCOTT) stage_prop<T>(src1, c, nbp, tile1, lane, v, vxm, vxp);
-    const uint64_t mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
+template <typename T, int KIND>
+__device__ inline void issue_own(ChunkIn<T, KIND> &in, const uint8_t *__restrict__ mask, const T *__restrict__ src0,
+                                 const T *__restrict__ src1, int64_t c, int lane, const StencilArgs &sa)
+{
+    if (sa.lskip) {
+        // a row without points holds the background (library invariant): not loaded
+        in.mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
+        if (in.mrow != 0) {
+            load_row(in.u.own, src0 + c * 512 + lane * 8);
+            if constexpr (ChunkIn<T, KIND>::kTwo) load_row(in.v.own, src1 + c * 512 + lane * 8);
+        } else {
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                in.u.own.v[x] = (T)sa.bgs[0];
+                if constexpr (ChunkIn<T, KIND>::kTwo) in.v.own.v[x] = (T)sa.bgs[1];
+            }
+        }
+        return;
+    }
+    load_row(in.u.own, src0 + c * 512 + lane * 8);
+    if constexpr (ChunkIn<T, KIND>::kTwo) load_row(in.v.own, src1 + c * 512 + lane * 8);
+    in.mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
+}
+
+template <typename T, int KIND>
+__device__ inline void read_nn(int64_t (&nbp)[6], const int32_t *__restrict__ nn, int64_t c)
+{
+    const int32_t *row = nn + c * 8;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) nbp[k] = row[k];
+}
+
+template <typename T, int KIND>
+__device__ inline void issue_chunk(ChunkIn<T, KIND> &in, const int32_t *__restrict__ nn, const uint8_t *__restrict__ mask,
+                                   const T *__restrict__ src0, const T *__restrict__ src1, int64_t c, int lane,
+                                   const StencilArgs &sa)
+{
+    // own rows and mask first: they do not depend on the neighbour table
+    issue_own<T, KIND>(in, mask, src0, src1, c, lane, sa);
+    __builtin_amdgcn_sched_barrier(0);
+    int64_t nbp[6];
+    read_nn<T, KIND>(nbp, nn, c);
+    issue_ghosts<T>(in.u, src0, nbp, lane, true, true);
+    if constexpr (ChunkIn<T, KIND>::kTwo) issue_ghosts<T>(in.v, src1, nbp, lane, true, true);
+}
+
+template <typename T, int KIND>
+__device__ inline void process_chunk(const ChunkIn<T, KIND> &in, int64_t c, T *tile0, T *tile1, T *__restrict__ dst0,
+                                     T *__restrict__ dst1, int alias, const StencilArgs &sa,
+                                     const int64_t *__restrict__ keys, int64_t bsz0, int64_t bsz1, int lane)
+{
+    constexpr bool kTwo = ChunkIn<T, KIND>::kTwo;
+    stage_lds<T>(in.u, tile0, lane);
+    if constexpr (kTwo) stage_lds<T>(in.v, tile1, lane);
     wave_lds_sync();
 
-    // activity per point
     bool act[8];
     {
         bool inbox[8];
@@ -493,20 +590,20 @@ __global__ __launch_bounds__(64) void stencil_fast_kernel(const int32_t *__restr
         }
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
-            const uint8_t cm = (uint8_t)(mrow >> (8 * x));
+            const uint8_t cm = (uint8_t)(in.mrow >> (8 * x));
             const bool a = (sa.apply == SG_APPLY_EXISTING) ? (cm != 0) : ((cm & SG_EXIST) && !(cm & SG_PADDING));
             act[x] = a && inbox[x];
         }
     }
 
     T px[8], py[8], pz[8];
-    lap_terms<T>(u, uxm, uxp, tile0, lane, px, py, pz);
+    lap_terms<T>(in.u, tile0, lane, px, py, pz);
     Row8<T> r0, r1;
     if constexpr (KIND == SG_ST_LAP) {
         const T c0 = (T)sa.coef[0], c1 = (T)sa.coef[1];
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
-            T a = c0 * u.v[x];
+            T a = c0 * in.u.own.v[x];
             a = a + c1 * px[x];
             a = a + c1 * py[x];
             a = a + c1 * pz[x];
@@ -516,19 +613,19 @@ __global__ __launch_bounds__(64) void stencil_fast_kernel(const int32_t *__restr
         const T dt = (T)sa.coef[0];
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
-            T a = (T)(-6.0) * u.v[x];
+            T a = (T)(-6.0) * in.u.own.v[x];
             a = a + px[x];
             a = a + py[x];
             a = a + pz[x];
-            r0.v[x] = u.v[x] + dt * a;
+            r0.v[x] = in.u.own.v[x] + dt * a;
         }
     } else {
         T qx[8], qy[8], qz[8];
-        lap_terms<T>(v, vxm, vxp, tile1, lane, qx, qy, qz);
+        lap_terms<T>(in.v, tile1, lane, qx, qy, qz);
         const T Du = (T)sa.coef[0], Dv = (T)sa.coef[1], F = (T)sa.coef[2], k = (T)sa.coef[3], dt = (T)sa.coef[4];
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
-            const T cu = u.v[x], cv = v.v[x];
+            const T cu = in.u.own.v[x], cv = in.v.own.v[x];
             T lu = (T)(-6.0) * cu;
             lu = lu + px[x];
             lu = lu + py[x];
@@ -542,33 +639,169 @@ __global__ __launch_bounds__(64) void stencil_fast_kernel(const int32_t *__restr
             r1.v[x] = cv + dt * ((Dv * lv + uvv) - (F + k) * cv);
         }
     }
+    // the LDS tiles are rewritten by the next chunk of this wave
+    wave_lds_sync();
 
-    bool all = true;
+    bool all = true, whole = true;
 #pragma unroll
-    for (int x = 0; x < 8; ++x) all = all && act[x];
+    for (int x = 0; x < 8; ++x) {
+        all = all && act[x];
+        // a slot without a point holds the destination's background (library invariant),
+        // so rewriting it with the background changes nothing
+        whole = whole && (act[x] || (uint8_t)(in.mrow >> (8 * x)) == 0);
+    }
     T *o0 = dst0 + c * 512 + lane * 8;
-    T *o1 = (KIND == SG_ST_GRAY_SCOTT) ? dst1 + c * 512 + lane * 8 : nullptr;
-    if (alias) {
+    T *o1 = kTwo ? dst1 + c * 512 + lane * 8 : nullptr;
+    if (sa.wmode != 0 && !alias && in.mrow == 0) {
+        // no point in the row: the destination row already holds the background
+    } else if (sa.wmode != 0 && !alias && whole) {
+        const T b0 = (T)sa.bgd[0], b1 = (T)sa.bgd[1];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            r0.v[x] = act[x] ? r0.v[x] : b0;
+            if constexpr (kTwo) r1.v[x] = act[x] ? r1.v[x] : b1;
+        }
+        if (sa.wmode == 2) {
+            store_row_nt(o0, r0);
+            if constexpr (kTwo) store_row_nt(o1, r1);
+        } else {
+            store_row(o0, r0);
+            if constexpr (kTwo) store_row(o1, r1);
+        }
+    } else if (alias) {
         // destination replaces a source: every slot is written (old value where inactive)
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
-            r0.v[x] = act[x] ? r0.v[x] : u.v[x];
-            if constexpr (KIND == SG_ST_GRAY_SCOTT) r1.v[x] = act[x] ? r1.v[x] : v.v[x];
+            r0.v[x] = act[x] ? r0.v[x] : in.u.own.v[x];
+            if constexpr (kTwo) r1.v[x] = act[x] ? r1.v[x] : in.v.own.v[x];
         }
         store_row(o0, r0);
-        if constexpr (KIND == SG_ST_GRAY_SCOTT) store_row(o1, r1);
+        if constexpr (kTwo) store_row(o1, r1);
     } else if (all) {
         store_row(o0, r0);
-        if constexpr (KIND == SG_ST_GRAY_SCOTT) store_row(o1, r1);
+        if constexpr (kTwo) store_row(o1, r1);
     } else {
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
             if (act[x]) {
                 o0[x] = r0.v[x];
-                if constexpr (KIND == SG_ST_GRAY_SCOTT) o1[x] = r1.v[x];
+                if constexpr (kTwo) o1[x] = r1.v[x];
             }
         }
     }
+}
+
+// Persistent wave-per-chunk star stencil over 8^3 chunks (KIND: LAP, HEAT, GRAY_SCOTT).
+// Workgroups are dealt to the 8 XCDs round robin, so workgroup b works in XCD b & 7:
+// XCD x owns the contiguous range [x*per_xcd, (x+1)*per_xcd) of the sorted chunk list,
+// and each of its waves walks `per_wave` consecutive chunks.  Neighbour chunks in x and
+// y are then mostly read from the same XCD's L2.  The loads of the next chunk are
+// issued before the current chunk is computed (software pipelining across chunks).
+// MINW: minimum waves per SIMD the register allocation must allow (1 = unconstrained).
+template <typename T, int KIND, int MINW>
+__global__ __launch_bounds__(256, MINW) void stencil_fast_kernel(const int32_t *__restrict__ nn, int64_t n,
+                                                                 const uint8_t *__restrict__ mask,
+                                                                 const T *__restrict__ src0, const T *__restrict__ src1,
+                                                                 T *__restrict__ dst0, T *__restrict__ dst1, int alias,
+                                                                 StencilArgs sa, const int64_t *__restrict__ keys,
+                                                                 int64_t bsz0, int64_t bsz1, int64_t per_xcd,
+                                                                 int64_t per_wave)
+{
+    constexpr bool kTwo = (KIND == SG_ST_GRAY_SCOTT);
+    __shared__ __attribute__((aligned(16))) T tiles0[4][100 * 8];
+    __shared__ __attribute__((aligned(16))) T tiles1[4][kTwo ? 100 * 8 : 8];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int64_t lo = xcd * per_xcd;
+    const int64_t hi = (lo + per_xcd < n) ? lo + per_xcd : n;
+    const int64_t begin = lo + (slot * 4 + w) * per_wave;
+    const int64_t end = (begin + per_wave < hi) ? begin + per_wave : hi;
+    if (begin >= end) return;
+    T *tile0 = tiles0[w];
+    T *tile1 = tiles1[w];
+    ChunkIn<T, KIND> cur, nxt;
+    issue_chunk<T, KIND>(cur, nn, mask, src0, src1, begin, lane, sa);
+    for (int64_t c = begin; c < end; ++c) {
+        if (c + 1 < end) issue_chunk<T, KIND>(nxt, nn, mask, src0, src1, c + 1, lane, sa);
+        process_chunk<T, KIND>(cur, c, tile0, tile1, dst0, dst1, alias, sa, keys, bsz0, bsz1, lane);
+        cur = nxt;
+    }
+}
+
+// One chunk per 64-lane workgroup (no pipelining, low register use, many waves);
+// workgroup b handles chunk (b & 7) * per_xcd + (b >> 3) so each XCD walks a
+// contiguous range of the sorted chunk list.
+template <typename T, int KIND>
+__global__ __launch_bounds__(64) void stencil_wave_kernel(const int32_t *__restrict__ nn, int64_t n,
+                                                          const uint8_t *__restrict__ mask, const T *__restrict__ src0,
+                                                          const T *__restrict__ src1, T *__restrict__ dst0,
+                                                          T *__restrict__ dst1, int alias, StencilArgs sa,
+                                                          const int64_t *__restrict__ keys, int64_t bsz0, int64_t bsz1,
+                                                          int64_t per_xcd)
+{
+    constexpr bool kTwo = (KIND == SG_ST_GRAY_SCOTT);
+    __shared__ __attribute__((aligned(16))) T tile0[100 * 8];
+    __shared__ __attribute__((aligned(16))) T tile1[kTwo ? 100 * 8 : 8];
+    const int64_t b = blockIdx.x;
+    const int64_t c = (b & 7) * per_xcd + (b >> 3);
+    if (c >= n) return;
+    ChunkIn<T, KIND> cur;
+    issue_chunk<T, KIND>(cur, nn, mask, src0, src1, c, threadIdx.x, sa);
+    process_chunk<T, KIND>(cur, c, tile0, tile1, dst0, dst1, alias, sa, keys, bsz0, bsz1, threadIdx.x);
+}
+
+// W waves per workgroup on W consecutive chunks of the sorted list.  Consecutive chunks
+// are usually x-neighbours, so each wave publishes its x = 0 and x = 7 columns in LDS
+// and takes its x-ghosts from the neighbouring wave when the neighbour table says the
+// -x / +x chunk is the previous / next chunk of the group; only the other x-ghosts are
+// loaded from memory.  Group g runs in XCD g's slot (g & 7 of the launch order).
+template <typename T, int KIND, int W>
+__global__ __launch_bounds__(64 * W) void stencil_group_kernel(const int32_t *__restrict__ nn, int64_t n,
+                                                               const uint8_t *__restrict__ mask,
+                                                               const T *__restrict__ src0, const T *__restrict__ src1,
+                                                               T *__restrict__ dst0, T *__restrict__ dst1, int alias,
+                                                               StencilArgs sa, const int64_t *__restrict__ keys,
+                                                               int64_t bsz0, int64_t bsz1, int64_t groups_per_xcd)
+{
+    constexpr bool kTwo = (KIND == SG_ST_GRAY_SCOTT);
+    constexpr int NP = kTwo ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) T tiles0[W][100 * 8];
+    __shared__ __attribute__((aligned(16))) T tiles1[W][kTwo ? 100 * 8 : 8];
+    __shared__ T cols[W][NP][2][64];  // [wave][property][x = 0 | x = 7][row]
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t b = blockIdx.x;
+    const int64_t grp = (b & 7) * groups_per_xcd + (b >> 3);
+    const int64_t c = grp * W + w;
+    const bool valid = c < n;  // every wave reaches the barrier
+    ChunkIn<T, KIND> in;
+    bool xm_grp = false, xp_grp = false;
+    if (valid) {
+        issue_own<T, KIND>(in, mask, src0, src1, c, lane, sa);
+        __builtin_amdgcn_sched_barrier(0);
+        int64_t nbp[6];
+        read_nn<T, KIND>(nbp, nn, c);
+        xm_grp = (w > 0) && nbp[0] == c - 1;
+        xp_grp = (w < W - 1) && (c + 1 < n) && nbp[1] == c + 1;
+        issue_ghosts<T>(in.u, src0, nbp, lane, !xm_grp, !xp_grp);
+        if constexpr (kTwo) issue_ghosts<T>(in.v, src1, nbp, lane, !xm_grp, !xp_grp);
+        cols[w][0][0][lane] = in.u.own.v[0];
+        cols[w][0][1][lane] = in.u.own.v[7];
+        if constexpr (kTwo) {
+            cols[w][NP - 1][0][lane] = in.v.own.v[0];
+            cols[w][NP - 1][1][lane] = in.v.own.v[7];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    if (xm_grp) {
+        in.u.xm = cols[w - 1][0][1][lane];
+        if constexpr (kTwo) in.v.xm = cols[w - 1][NP - 1][1][lane];
+    }
+    if (xp_grp) {
+        in.u.xp = cols[w + 1][0][0][lane];
+        if constexpr (kTwo) in.v.xp = cols[w + 1][NP - 1][0][lane];
+    }
+    process_chunk<T, KIND>(in, c, tiles0[w], tiles1[w], dst0, dst1, alias, sa, keys, bsz0, bsz1, lane);
 }
 
 template <typename T>
@@ -709,6 +942,28 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
     sa.kind = kind;
     sa.apply = d->apply;
     sa.use_box = d->use_box;
+    sa.wmode = g->fast_wmode;
+    sa.bgd[0] = sa.bgd[1] = 0.0;
+    sa.bgs[0] = sa.bgs[1] = 0.0;
+    sa.lskip = g->fast_lskip;
+    for (int q = 0; q < nsrc && q < 2; ++q) {
+        if (g->dt[d->src[q]] == SG_F32) {
+            float f;
+            memcpy(&f, g->bg[d->src[q]], sizeof(f));
+            sa.bgs[q] = f;
+        } else if (g->dt[d->src[q]] == SG_F64) {
+            memcpy(&sa.bgs[q], g->bg[d->src[q]], sizeof(double));
+        }
+    }
+    for (int q = 0; q < ndst && q < 2; ++q) {
+        if (g->dt[d->dst[q]] == SG_F32) {
+            float f;
+            memcpy(&f, g->bg[d->dst[q]], sizeof(f));
+            sa.bgd[q] = f;
+        } else if (g->dt[d->dst[q]] == SG_F64) {
+            memcpy(&sa.bgd[q], g->bg[d->dst[q]], sizeof(double));
+        }
+    }
     for (int i = 0; i < 3; ++i) {
         sa.lo[i] = d->box_lo[i];
         sa.hi[i] = d->box_hi[i];
@@ -748,20 +1003,49 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
         if constexpr (std::is_same<T, float>::value || std::is_same<T, double>::value) {
             if (fast) {
                 const int64_t per_xcd = (n + 7) / 8;
-                const unsigned nblk = (unsigned)(per_xcd * 8);
                 const int al = alias[0] != 0;
-                if (kind == SG_ST_LAP)
-                    hipLaunchKernelGGL((stencil_fast_kernel<T, SG_ST_LAP>), dim3(nblk), dim3(64), 0, g->stream,
-                                       g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
-                                       g->bsz[1], per_xcd);
-                else if (kind == SG_ST_HEAT)
-                    hipLaunchKernelGGL((stencil_fast_kernel<T, SG_ST_HEAT>), dim3(nblk), dim3(64), 0, g->stream,
-                                       g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
-                                       g->bsz[1], per_xcd);
-                else
-                    hipLaunchKernelGGL((stencil_fast_kernel<T, SG_ST_GRAY_SCOTT>), dim3(nblk), dim3(64), 0,
-                                       g->stream, g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys,
-                                       g->bsz[0], g->bsz[1], per_xcd);
+                auto launch = [&](auto kind_tag) {
+                    constexpr int K = decltype(kind_tag)::value;
+                    if (g->fast_variant == 0) {
+                        // one chunk per 64-lane workgroup
+                        hipLaunchKernelGGL((stencil_wave_kernel<T, K>), dim3((unsigned)(per_xcd * 8)), dim3(64), 0,
+                                           g->stream, g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys,
+                                           g->bsz[0], g->bsz[1], per_xcd);
+                        return;
+                    }
+                    if (g->fast_variant >= 3) {
+                        // groups of W consecutive chunks sharing x faces through LDS
+                        auto grouped = [&](auto wtag) {
+                            constexpr int GW = decltype(wtag)::value;
+                            const int64_t ngroups = (n + GW - 1) / GW;
+                            const int64_t gpx = (ngroups + 7) / 8;
+                            hipLaunchKernelGGL((stencil_group_kernel<T, K, GW>), dim3((unsigned)(gpx * 8)),
+                                               dim3(64 * GW), 0, g->stream, g->d_nn[type], n, g->d_mask, s0, s1, o0,
+                                               o1, al, sa, g->d_keys, g->bsz[0], g->bsz[1], gpx);
+                        };
+                        if (g->fast_variant == 3) grouped(std::integral_constant<int, 4>());
+                        else if (g->fast_variant == 4) grouped(std::integral_constant<int, 8>());
+                        else grouped(std::integral_constant<int, 2>());
+                        return;
+                    }
+                    // persistent waves: wg_per_xcd workgroups of 4 waves per XCD, each wave
+                    // walking per_wave consecutive chunks
+                    const int64_t wg_per_xcd =
+                        std::max<int64_t>(1, std::min<int64_t>(g->fast_wg_per_xcd, (per_xcd + 3) / 4));
+                    const int64_t per_wave = (per_xcd + wg_per_xcd * 4 - 1) / (wg_per_xcd * 4);
+                    const unsigned nblk = (unsigned)(wg_per_xcd * 8);
+                    if (g->fast_variant == 1)
+                        hipLaunchKernelGGL((stencil_fast_kernel<T, K, 1>), dim3(nblk), dim3(256), 0, g->stream,
+                                           g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
+                                           g->bsz[1], per_xcd, per_wave);
+                    else
+                        hipLaunchKernelGGL((stencil_fast_kernel<T, K, 4>), dim3(nblk), dim3(256), 0, g->stream,
+                                           g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
+                                           g->bsz[1], per_xcd, per_wave);
+                };
+                if (kind == SG_ST_LAP) launch(std::integral_constant<int, SG_ST_LAP>());
+                else if (kind == SG_ST_HEAT) launch(std::integral_constant<int, SG_ST_HEAT>());
+                else launch(std::integral_constant<int, SG_ST_GRAY_SCOTT>());
                 return;
             }
         }

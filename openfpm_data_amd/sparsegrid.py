@@ -74,6 +74,9 @@ class SparseGridGpu:
         raw = getattr(stream, "cuda_stream", stream)
         C.check(self._lib.sg_set_stream(self._h, ctypes.c_void_p(raw)))
 
+    def set_tuning(self, key, value):
+        C.check(self._lib.sg_set_tuning(self._h, int(key), int(value)))
+
     def synchronize(self):
         C.check(self._lib.sg_synchronize(self._h))
 

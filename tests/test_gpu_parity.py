@@ -232,3 +232,35 @@ def test_blobs_flush_subsample(G, oracle):
         g.conv_box(1, 2, np.ones(27, np.int64))
         gs.append(g)
     S.assert_same_state(gs[0], gs[1], props=[0, 1, 2])
+
+
+@pytest.mark.parametrize("variant,stores,lskip", [(0, 0, 0), (0, 1, 1), (1, 2, 0), (2, 1, 0), (3, 2, 1),
+                                                  (4, 2, 0), (4, 0, 1), (5, 1, 0)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_fast_stencil_variants(G, oracle, variant, stores, lskip, dtype):
+    """Every selectable launch shape of the 8^3 star-stencil kernel gives the oracle's
+    result (Gray-Scott ping-pong, heat with a box, Laplacian in place after tagBoundaries)."""
+    rng = np.random.default_rng(71)
+    pts = W.shell_points(56, 12.0, 22.0)
+    u0 = rng.uniform(0.3, 1.0, len(pts)).astype(dtype)
+    v0 = rng.uniform(0.0, 0.4, len(pts)).astype(dtype)
+    gs = []
+    for cls in (G, oracle.SparseGridCpu):
+        g = cls(3, (56, 56, 56), [dtype] * 4, block_edge=8)
+        g.set_background_value(2, 0.5)
+        if cls is G:
+            g.set_tuning(C.TUNE_STENCIL_VARIANT, variant)
+            g.set_tuning(C.TUNE_STENCIL_STORES, stores)
+            g.set_tuning(C.TUNE_STENCIL_SKIP_EMPTY_ROWS, lskip)
+            g.set_tuning(C.TUNE_STENCIL_WG_PER_XCD, 3)
+        g.insert_coords(pts, [u0, v0, np.zeros_like(u0), np.zeros_like(u0)])
+        g.flush(["smax"] * 4)
+        for _ in range(3):
+            g.conv2_gray_scott(0, 1, 2, 3, **W.GS_PARAMS)
+            g.conv2_gray_scott(2, 3, 0, 1, **W.GS_PARAMS)
+        g.conv_heat(0, 2, 0.05, box=((3, 10, 0), (50, 44, 40)))
+        g.tag_boundaries()
+        g.conv_lap(2, 2)
+        g.conv_lap(1, 3, -4.0, 0.5)
+        gs.append(g)
+    S.assert_same_state(gs[0], gs[1])
