@@ -1,0 +1,454 @@
+/*
+ * SparseGridGpu.hpp -- C++ drop-in facade of openfpm_data's SparseGridGpu over the C ABI of
+ * libsg_hip.so (sparsegrid_gpu.h).  Host-only header: user code compiles with any C++17
+ * compiler and links libsg_hip.so; the kernels behind it are hand-written gfx950 HIP.
+ *
+ * Kept from the reference (src/SparseGridGpu/SparseGridGpu.hpp):
+ *   SparseGridGpu<dim, aggregate<T...>, blockEdgeSize, threadBlockSize, indexT> (:483-489),
+ *   setBackgroundValue<p> (:2878), setGPUInsertBuffer (:2270), flush<v_reduce...>(ctx, opt)
+ *   (:772), findNeighbours<NN> (:2396), setNNType<NN> (:1825), tagBoundaries<NN>(ctx, chk, opt)
+ *   (:2290), applyStencils<Stencil>(box, mode, args...) (:2704), get<p>(coord) (:1897),
+ *   insertFlush<p>(coord) (:2832), getLinId / getCoord (:1849-1857), getBox (:2690),
+ *   countExistingElements / countBoundaryElements / measureBlockOccupancy (:2422-2552),
+ *   reduction tags smax_ / smin_ / sadd_ / sRight_ / sLeft_ / sBitwiseOr_
+ *   (src/Vector/cuda/map_vector_sparse_cuda_kernels.cuh), NNStar / NNFull, the test
+ *   stencils HeatStencil and Conv3x3x3 (src/SparseGridGpu/tests/utils/SparseGridGpu_util_test.cuh).
+ * Differences (a C ABI carries no device code):
+ *   - insertion is host-batched (insertPoints / insertBlocks) instead of user __global__
+ *     kernels calling insert<p>() on toKernel();
+ *   - applyStencils takes the named stencils of this header (descriptors evaluated by
+ *     library kernels), not arbitrary __device__ functors or lambdas;
+ *   - failures throw sg::error (the reference prints and continues).
+ */
+#ifndef SPARSEGRIDGPU_FACADE_HPP
+#define SPARSEGRIDGPU_FACADE_HPP
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <initializer_list>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "sparsegrid_gpu.h"
+
+namespace sg {
+
+struct error : std::runtime_error {
+    explicit error(const std::string &m) : std::runtime_error(m) {}
+};
+
+inline void check(int rc)
+{
+    if (rc != SG_OK) throw error(std::string("libsg_hip: ") + sg_last_error());
+}
+
+template <typename T> struct dtype_of;
+template <> struct dtype_of<float> { static constexpr int value = SG_F32; };
+template <> struct dtype_of<double> { static constexpr int value = SG_F64; };
+template <> struct dtype_of<int32_t> { static constexpr int value = SG_I32; };
+template <> struct dtype_of<uint32_t> { static constexpr int value = SG_U32; };
+template <> struct dtype_of<int64_t> { static constexpr int value = SG_I64; };
+template <> struct dtype_of<uint64_t> { static constexpr int value = SG_U64; };
+template <> struct dtype_of<int8_t> { static constexpr int value = SG_I8; };
+template <> struct dtype_of<uint8_t> { static constexpr int value = SG_U8; };
+template <> struct dtype_of<int16_t> { static constexpr int value = SG_I16; };
+template <> struct dtype_of<uint16_t> { static constexpr int value = SG_U16; };
+
+}  // namespace sg
+
+// ---- reference vocabulary -------------------------------------------------------------
+
+template <typename... T> struct aggregate {
+    typedef std::tuple<T...> type;
+    static constexpr unsigned int max_prop = sizeof...(T);
+};
+
+template <unsigned int dim, typename T = long int> class grid_key_dx
+{
+    T k[dim];
+
+public:
+    grid_key_dx() : k{} {}
+    grid_key_dx(std::initializer_list<T> l)
+    {
+        unsigned int i = 0;
+        for (T v : l)
+            if (i < dim) k[i++] = v;
+        for (; i < dim; ++i) k[i] = 0;
+    }
+    T get(unsigned int i) const { return k[i]; }
+    void set_d(unsigned int i, T v) { k[i] = v; }
+};
+
+template <unsigned int dim, typename T> class Box
+{
+    T lo[dim], hi[dim];
+
+public:
+    Box() : lo{}, hi{} {}
+    void setLow(unsigned int i, T v) { lo[i] = v; }
+    void setHigh(unsigned int i, T v) { hi[i] = v; }
+    T getLow(unsigned int i) const { return lo[i]; }
+    T getHigh(unsigned int i) const { return hi[i]; }
+};
+
+enum flush_type { FLUSH_ON_HOST = 0, FLUSH_ON_DEVICE = 1, FLUSH_NO_DATA = 2 };
+enum tag_boundaries { NO_CALCULATE_EXISTING_POINTS = SG_NO_CALCULATE_EXISTING_POINTS,
+                      CALCULATE_EXISTING_POINTS = SG_CALCULATE_EXISTING_POINTS };
+enum StencilMode { STENCIL_MODE_INPLACE = 1, STENCIL_MODE_INPLACE_NO_SHARED = 3 };
+enum mask_sparse { NOT_EXIST = 0, EXIST = 1, PADDING = 2, EXIST_AND_PADDING = 3 };
+
+struct No_check {};
+
+namespace gpu {
+// stream/temporary-storage context of the reference; the library keeps its own per grid
+struct ofp_context_t {};
+}  // namespace gpu
+
+template <unsigned int p> struct smax_ { static constexpr unsigned int prop = p; static constexpr int op = SG_OP_MAX; };
+template <unsigned int p> struct smin_ { static constexpr unsigned int prop = p; static constexpr int op = SG_OP_MIN; };
+template <unsigned int p> struct sadd_ { static constexpr unsigned int prop = p; static constexpr int op = SG_OP_ADD; };
+template <unsigned int p> struct sRight_ { static constexpr unsigned int prop = p; static constexpr int op = SG_OP_RIGHT; };
+template <unsigned int p> struct sLeft_ { static constexpr unsigned int prop = p; static constexpr int op = SG_OP_LEFT; };
+template <unsigned int p> struct sBitwiseOr_ { static constexpr unsigned int prop = p; static constexpr int op = SG_OP_OR; };
+
+template <unsigned int dim> struct NNStar {
+    static constexpr int type = SG_NN_STAR;
+    static constexpr int nNN = 1 << dim;
+};
+template <unsigned int dim> struct NNFull {
+    static constexpr int type = SG_NN_FULL;
+    static constexpr int nNN = (dim == 2) ? 9 : 27;
+};
+
+// ---- named stencils for applyStencils ----------------------------------------------------
+
+namespace sg {
+template <unsigned int dim> inline void box_to_desc(sg_stencil_desc &d, const Box<dim, int> &box)
+{
+    d.use_box = 1;
+    for (unsigned int i = 0; i < dim; ++i) {
+        d.box_lo[i] = box.getLow(i);
+        d.box_hi[i] = box.getHigh(i);
+    }
+}
+inline sg_stencil_desc blank_desc(int kind, int apply)
+{
+    sg_stencil_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.kind = kind;
+    d.apply = apply;
+    return d;
+}
+}  // namespace sg
+
+// u_dst = u + dt * lap(u) (tests/utils/SparseGridGpu_util_test.cuh:201-260)
+template <unsigned int dim, unsigned int p_src, unsigned int p_dst> struct HeatStencil {
+    typedef NNStar<dim> stencil_type;
+    static constexpr unsigned int supportRadius = 1;
+    static constexpr unsigned int flops = 3 + 2 * dim;
+    static sg_stencil_desc desc(const Box<dim, int> &box, double dt)
+    {
+        sg_stencil_desc d = sg::blank_desc(SG_ST_HEAT, SG_APPLY_INTERIOR);
+        sg::box_to_desc<dim>(d, box);
+        d.src[0] = p_src;
+        d.dst[0] = p_dst;
+        d.coef[0] = dt;
+        return d;
+    }
+};
+
+// u_dst = c0 * u + c1 * sum of the 2*dim face neighbours (conv with a 7-point Laplacian)
+template <unsigned int dim, unsigned int p_src, unsigned int p_dst> struct LaplacianStencil {
+    typedef NNStar<dim> stencil_type;
+    static constexpr unsigned int supportRadius = 1;
+    static sg_stencil_desc desc(const Box<dim, int> &box, double c0 = -2.0 * dim, double c1 = 1.0)
+    {
+        sg_stencil_desc d = sg::blank_desc(SG_ST_LAP, SG_APPLY_INTERIOR);
+        sg::box_to_desc<dim>(d, box);
+        d.src[0] = p_src;
+        d.dst[0] = p_dst;
+        d.coef[0] = c0;
+        d.coef[1] = c1;
+        return d;
+    }
+};
+
+// two-property Gray-Scott step (conv2 form)
+template <unsigned int dim, unsigned int pu, unsigned int pv, unsigned int pu_out, unsigned int pv_out>
+struct GrayScottStencil {
+    typedef NNStar<dim> stencil_type;
+    static constexpr unsigned int supportRadius = 1;
+    static sg_stencil_desc desc(const Box<dim, int> &box, double Du, double Dv, double F, double k, double dt)
+    {
+        sg_stencil_desc d = sg::blank_desc(SG_ST_GRAY_SCOTT, SG_APPLY_INTERIOR);
+        sg::box_to_desc<dim>(d, box);
+        d.src[0] = pu;
+        d.src[1] = pv;
+        d.dst[0] = pu_out;
+        d.dst[1] = pv_out;
+        d.coef[0] = Du;
+        d.coef[1] = Dv;
+        d.coef[2] = F;
+        d.coef[3] = k;
+        d.coef[4] = dt;
+        return d;
+    }
+};
+
+struct conv_coeff {
+    float coeff[3][3][3];
+};
+
+// sum_{i,j,k} coeff[i][j][k] * u[x+i-1, y+j-1, z+k-1] on every existing point
+// (Conv3x3x3, tests/utils/SparseGridGpu_util_test.cuh:357-420)
+template <unsigned int dim, unsigned int p_src, unsigned int p_dst> struct Conv3x3x3 {
+    typedef NNFull<dim> stencil_type;
+    static constexpr unsigned int supportRadius = 1;
+    static constexpr unsigned int flops = 2 * 27;
+    static sg_stencil_desc desc(const Box<dim, int> &box, const conv_coeff &cc)
+    {
+        sg_stencil_desc d = sg::blank_desc(SG_ST_BOX, SG_APPLY_EXISTING);
+        sg::box_to_desc<dim>(d, box);
+        d.src[0] = p_src;
+        d.dst[0] = p_dst;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) {
+                    d.coef[i * 9 + j * 3 + k] = cc.coeff[i][j][k];
+                    d.icoef[i * 9 + j * 3 + k] = (int64_t)std::llround(cc.coeff[i][j][k]);
+                }
+        return d;
+    }
+};
+
+// ---- the grid ---------------------------------------------------------------------------
+
+template <unsigned int dim> struct default_edge { static constexpr unsigned int value = dim == 3 ? 8 : (dim == 2 ? 16 : 256); };
+
+template <unsigned int dim, typename AggregateT, unsigned int blockEdgeSize = default_edge<dim>::value,
+          unsigned int threadBlockSize = 512, typename indexT = long int>
+class SparseGridGpu
+{
+public:
+    static constexpr unsigned int dims = dim;
+    static constexpr unsigned int blockEdgeSize_ = blockEdgeSize;
+    typedef typename AggregateT::type props_tuple;
+    template <unsigned int p> using prop_type = typename std::tuple_element<p, props_tuple>::type;
+    static constexpr unsigned int nprops = std::tuple_size<props_tuple>::value;
+
+    static constexpr unsigned int chunk_size()
+    {
+        unsigned int s = 1;
+        for (unsigned int i = 0; i < dim; ++i) s *= blockEdgeSize;
+        return s;
+    }
+
+    explicit SparseGridGpu(const size_t (&res)[dim], unsigned int stencilSupportRadius = 1)
+    {
+        int64_t sz[3] = {1, 1, 1};
+        for (unsigned int i = 0; i < dim; ++i) {
+            sz[i] = (int64_t)res[i];
+            size_[i] = res[i];
+            bsz_[i] = res[i] / blockEdgeSize + ((res[i] % blockEdgeSize) != 0);
+        }
+        int32_t dts[nprops > 0 ? nprops : 1];
+        fill_dtypes(dts, std::make_index_sequence<nprops>());
+        sg::check(sg_create((int)dim, sz, (int)blockEdgeSize, (int)nprops, dts, (int)stencilSupportRadius, &h_));
+    }
+    ~SparseGridGpu() { sg_destroy(h_); }
+    SparseGridGpu(const SparseGridGpu &) = delete;
+    SparseGridGpu &operator=(const SparseGridGpu &) = delete;
+    SparseGridGpu(SparseGridGpu &&o) noexcept : h_(o.h_)
+    {
+        o.h_ = nullptr;
+        for (unsigned int i = 0; i < dim; ++i) {
+            size_[i] = o.size_[i];
+            bsz_[i] = o.bsz_[i];
+        }
+    }
+
+    sg_grid *handle() { return h_; }
+    void setStream(void *hip_stream) { sg::check(sg_set_stream(h_, hip_stream)); }
+
+    // -- geometry (grid_smb) --
+    static constexpr unsigned int getBlockEdgeSize() { return blockEdgeSize; }
+    constexpr unsigned int getBlockSize() const { return chunk_size(); }
+    template <typename CoordT> indexT getLinId(const grid_key_dx<dim, CoordT> &c) const
+    {
+        indexT b = c.get(dim - 1) / blockEdgeSize, l = c.get(dim - 1) % blockEdgeSize;
+        for (int d = (int)dim - 2; d >= 0; --d) {
+            b = b * (indexT)bsz_[d] + c.get(d) / blockEdgeSize;
+            l = l * blockEdgeSize + c.get(d) % blockEdgeSize;
+        }
+        return b * (indexT)chunk_size() + l;
+    }
+    grid_key_dx<dim, int> getCoord(indexT lin) const
+    {
+        grid_key_dx<dim, int> c;
+        indexT b = lin / chunk_size(), l = lin % chunk_size();
+        for (unsigned int d = 0; d < dim; ++d) {
+            c.set_d(d, (int)((b % (indexT)bsz_[d]) * blockEdgeSize + l % blockEdgeSize));
+            b /= (indexT)bsz_[d];
+            l /= blockEdgeSize;
+        }
+        return c;
+    }
+    Box<dim, int> getBox() const
+    {
+        Box<dim, int> b;
+        for (unsigned int i = 0; i < dim; ++i) {
+            b.setLow(i, 0);
+            b.setHigh(i, (int)size_[i]);
+        }
+        return b;
+    }
+
+    // -- background / insertion / flush --
+    template <unsigned int p> void setBackgroundValue(prop_type<p> v) { sg::check(sg_set_background(h_, (int)p, &v)); }
+    template <typename dim3T> void setGPUInsertBuffer(dim3T, dim3T) {}  // the insert buffer grows on demand
+    void preFlush() {}
+
+    // point inserts (insert<p>(coord) once per point): keys are getLinId values, one
+    // host value array per property (nullptr: leave unset), masks default to EXIST
+    void insertPoints(const indexT *keys, size_t n, const void *const *values, const uint8_t *masks = nullptr)
+    {
+        std::vector<int64_t> k(keys, keys + n);
+        sg::check(sg_insert_points(h_, (int64_t)n, k.data(), masks, values, SG_MEM_HOST));
+    }
+    // chunk inserts (insertBlock(blockId)): masks and values are [n][blockSize]
+    void insertBlocks(const indexT *chunk_ids, size_t n, const uint8_t *masks, const void *const *values)
+    {
+        std::vector<int64_t> k(chunk_ids, chunk_ids + n);
+        sg::check(sg_insert_chunks(h_, (int64_t)n, k.data(), masks, values, SG_MEM_HOST));
+    }
+
+    template <typename... v_reduce> void flush(gpu::ofp_context_t &, flush_type opt = FLUSH_ON_DEVICE)
+    {
+        (void)opt;
+        int32_t ops[nprops > 0 ? nprops : 1];
+        for (unsigned int i = 0; i < nprops; ++i) ops[i] = SG_OP_NONE;
+        int dummy[] = {0, (ops[v_reduce::prop] = v_reduce::op, 0)...};
+        (void)dummy;
+        sg::check(sg_flush(h_, ops));
+    }
+
+    // insertFlush<p>(coord) = value: insert one point and flush it immediately
+    template <unsigned int p> struct insert_ref {
+        SparseGridGpu *g;
+        indexT key;
+        insert_ref &operator=(prop_type<p> v)
+        {
+            const void *vals[nprops > 0 ? nprops : 1] = {};
+            vals[p] = &v;
+            g->insertPoints(&key, 1, vals);
+            int32_t ops[nprops > 0 ? nprops : 1];
+            for (unsigned int i = 0; i < nprops; ++i) ops[i] = SG_OP_NONE;
+            sg::check(sg_flush(g->h_, ops));
+            return *this;
+        }
+    };
+    template <unsigned int p, typename CoordT> insert_ref<p> insertFlush(const grid_key_dx<dim, CoordT> &coord)
+    {
+        return insert_ref<p>{this, getLinId(coord)};
+    }
+
+    // -- neighbourhood, boundaries, stencils --
+    template <typename NN = NNStar<dim>> void findNeighbours() { sg::check(sg_find_neighbours(h_, NN::type)); }
+    template <typename NN> void setNNType() { sg::check(sg_set_nn_type(h_, NN::type)); }
+    template <typename NN = NNStar<dim>, typename checker_type = No_check>
+    void tagBoundaries(gpu::ofp_context_t &, checker_type = checker_type(),
+                       tag_boundaries opt = NO_CALCULATE_EXISTING_POINTS)
+    {
+        sg::check(sg_tag_boundaries(h_, NN::type, (int)opt));
+    }
+    template <typename Stencil, typename... Args> void applyStencils(const Box<dim, int> &box, StencilMode, Args... args)
+    {
+        sg_stencil_desc d = Stencil::desc(box, args...);
+        sg::check(sg_apply_stencil(h_, &d));
+    }
+
+    // -- access --
+    template <unsigned int p, typename CoordT> prop_type<p> get(const grid_key_dx<dim, CoordT> &coord) const
+    {
+        int64_t k = (int64_t)getLinId(coord);
+        prop_type<p> v;
+        sg::check(sg_get_points(h_, (int)p, 1, &k, &v, SG_MEM_HOST));
+        return v;
+    }
+    size_t size() const { return (size_t)sg_num_chunks(h_) * chunk_size(); }
+    std::vector<indexT> private_get_index_array() const
+    {
+        std::vector<int64_t> k((size_t)sg_num_chunks(h_));
+        if (!k.empty()) sg::check(sg_get_chunk_ids(h_, k.data(), SG_MEM_HOST));
+        return std::vector<indexT>(k.begin(), k.end());
+    }
+    template <unsigned int p> std::vector<prop_type<p>> private_get_data_array() const
+    {
+        std::vector<prop_type<p>> v((size_t)sg_num_chunks(h_) * chunk_size());
+        if (!v.empty()) sg::check(sg_get_prop(h_, (int)p, v.data(), SG_MEM_HOST));
+        return v;
+    }
+    std::vector<uint8_t> private_get_mask_array() const
+    {
+        std::vector<uint8_t> m((size_t)sg_num_chunks(h_) * chunk_size());
+        if (!m.empty()) sg::check(sg_get_masks(h_, m.data(), SG_MEM_HOST));
+        return m;
+    }
+    template <unsigned int p> void hostToDevice(const std::vector<prop_type<p>> &v)
+    {
+        sg::check(sg_set_prop(h_, (int)p, v.data(), SG_MEM_HOST));
+    }
+    template <unsigned int... p> void deviceToHost() { sg::check(sg_synchronize(h_)); }
+
+    size_t countExistingElements() const
+    {
+        int64_t c = 0;
+        sg::check(sg_count_existing(h_, &c));
+        return (size_t)c;
+    }
+    size_t countBoundaryElements() const
+    {
+        int64_t c = 0;
+        sg::check(sg_count_boundary(h_, &c));
+        return (size_t)c;
+    }
+    // mean and standard deviation of the per-chunk share of EXIST && !PADDING points
+    void measureBlockOccupancy(double &mean, double &deviation) const
+    {
+        std::vector<uint8_t> m = private_get_mask_array();
+        const size_t nb = m.size() / chunk_size();
+        std::vector<double> occ(nb, 0.0);
+        for (size_t b = 0; b < nb; ++b) {
+            size_t c = 0;
+            for (size_t o = 0; o < chunk_size(); ++o) c += (m[b * chunk_size() + o] & 3) == 1;
+            occ[b] = (double)c / chunk_size();
+        }
+        mean = 0.0;
+        for (double o : occ) mean += o;
+        mean = nb ? mean / nb : 0.0;
+        double v = 0.0;
+        for (double o : occ) v += (o - mean) * (o - mean);
+        deviation = nb > 1 ? std::sqrt(v / (nb - 1)) : 0.0;
+    }
+
+private:
+    template <size_t... I> static void fill_dtypes(int32_t *dts, std::index_sequence<I...>)
+    {
+        int dummy[] = {0, (dts[I] = sg::dtype_of<typename std::tuple_element<I, props_tuple>::type>::value, 0)...};
+        (void)dummy;
+        (void)dts;
+    }
+
+    sg_grid *h_ = nullptr;
+    size_t size_[dim];
+    size_t bsz_[dim];
+};
+
+#endif  // SPARSEGRIDGPU_FACADE_HPP
