@@ -106,6 +106,105 @@ def run_gpu(args, ids, masks, data, dtype):
     return res
 
 
+def run_flush_secondary(cpu_threads, with_cpu):
+    """2^24 point inserts into 1024^3 (64 Gaussian blobs, 30% repeats), int32 smax + uint32
+    sadd flush, NNFull tagBoundaries, 27-point uint32 box sum; inserts resident in HBM."""
+    import torch
+    from openfpm_data_amd import SparseGridGpu, _capi as C
+    from openfpm_data_amd import workloads as W
+
+    w = W.blobs1024()
+    n = w["coords"].shape[0]
+    d_c = torch.from_numpy(w["coords"]).cuda()
+    d_i = torch.from_numpy(w["int32"]).cuda()
+    d_u = torch.from_numpy(w["uint32"].view(np.int32)).cuda()
+    z = torch.zeros_like(d_u)
+    stream = torch.cuda.Stream()
+    times = []
+    for rep in range(4):
+        g = SparseGridGpu(3, w["size"], [np.int32, np.uint32, np.uint32], block_edge=8)
+        g.set_stream(stream)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev[0].record(stream)
+        g.insert_coords(d_c, [d_i, d_u, z])
+        g.flush(["smax", "sadd", "none"])
+        ev[1].record(stream)
+        g.find_neighbours(C.NN_FULL)
+        g.set_nn_type(C.NN_FULL)
+        ev[2].record(stream)
+        g.tag_boundaries(C.NN_FULL)
+        ev[3].record(stream)
+        g.conv_box(1, 2, np.ones(27, np.int64))
+        ev[4].record(stream)
+        ev[4].synchronize()
+        times.append([ev[i].elapsed_time(ev[i + 1]) for i in range(4)])
+        nch, npts = g.num_chunks(), g.count_existing_elements()
+        g.close()
+    t = np.median(np.array(times[1:]), axis=0)  # first repetition warms the allocator
+    out = {"workload": "blobs1024_flush", "inserts": n, "chunks": nch, "points": npts,
+           "insert_flush_ms": round(float(t[0]), 3), "inserts_per_s": round(n / (t[0] * 1e-3) / 1e9, 4),
+           "unit": "G inserts/s", "find_neighbours_full_ms": round(float(t[1]), 3),
+           "tag_boundaries_full_ms": round(float(t[2]), 3), "box27_uint32_ms": round(float(t[3]), 3)}
+    if with_cpu:
+        from oracle import pysgcpu
+        pysgcpu.set_threads(cpu_threads)
+        g = pysgcpu.SparseGridCpu(3, w["size"], [np.int32, np.uint32, np.uint32], block_edge=8)
+        t0 = time.perf_counter()
+        g.insert_coords(w["coords"], [w["int32"], w["uint32"], np.zeros(n, np.uint32)])
+        g.flush(["smax", "sadd", "none"])
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(n / el / 1e9, 5), "unit": "G inserts/s", "cores": cpu_threads,
+                               "kind": "port", "sample": "oracle insert + flush of the same 2^24 records, %.1f s" % el}
+        g.close()
+    return out
+
+
+def run_poisson_secondary():
+    """256 independent Poisson grids (edge log-uniform in [32, 256], 5-40% of the chunks
+    active, f uniform(-1, 1), seed 3) solved to relative residual 1e-10: all grids in one
+    continuous batch in the mixed form (FP32 red-black SOR + FP64 defect correction, inner
+    sweeps replayed from a hipGraph) against one plain FP64 solve per grid."""
+    import torch
+    from openfpm_data_amd import poisson as P, _capi as C
+    from openfpm_data_amd import workloads as W
+
+    specs = W.poisson_batch(256)
+    fs = [torch.from_numpy(s["f"]).cuda() for s in specs]
+
+    def grids():
+        gs = [P.poisson_grid(s["size"], s["chunk_ids"], f) for s, f in zip(specs, fs)]
+        torch.cuda.synchronize()
+        return gs
+
+    def timed(mode, batched, graph):
+        gs = grids()
+        t0 = time.perf_counter()
+        if batched:
+            res = P.solve(gs, P.make_params(mode=mode, tol=1e-10, use_graph=graph))
+        else:
+            res = [P.solve([g], P.make_params(mode=mode, tol=1e-10, use_graph=graph))[0] for g in gs]
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        for g in gs:
+            g.close()
+        return el, res
+
+    timed(C.POISSON_MIXED, True, True)  # warm-up (allocator, code objects)
+    t_mix, r_mix = timed(C.POISSON_MIXED, True, True)
+    t_plain, r_plain = timed(C.POISSON_PLAIN_FP64, False, False)
+    sw = np.array([r["sweeps"] for r in r_mix])
+    return {"workload": "poisson_batch256", "grids": len(specs),
+            "chunks": int(sum(s["chunk_ids"].size for s in specs)), "tol": 1e-10,
+            "solves_per_s": round(len(specs) / t_mix, 2), "unit": "solves/s",
+            "batch_s": round(t_mix, 4), "converged": int(sum(r["converged"] for r in r_mix)),
+            "max_rel_residual": max(r["rel_residual"] for r in r_mix),
+            "sweeps_mean": round(float(sw.mean()), 1), "sweeps_max": int(sw.max()),
+            "per_grid_plain_fp64_solves_per_s": round(len(specs) / t_plain, 2),
+            "per_grid_plain_fp64_converged": int(sum(r["converged"] for r in r_plain)),
+            "speedup_vs_per_grid_plain_fp64": round(t_plain / t_mix, 3)}
+
+
 def run_cpu_baseline(ids, masks, data, seconds, threads):
     from oracle import pysgcpu
     from openfpm_data_amd import workloads as W
@@ -142,6 +241,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the insertion/flush workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 20 device copies of 1 GiB (reference for the FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -189,6 +289,11 @@ def main():
                "sample": "oracle (oracle/sgcpu.cpp, OpenMP) on the same 512^3 shell FP32 Gray-Scott grid, "
                          "%d steps in %.1f s" % (c["steps"], c["seconds"])}
 
+    secondary = None
+    if not args.no_secondary:
+        secondary = {"flush": run_flush_secondary(args.cpu_threads or min(16, os.cpu_count() or 1), not args.no_cpu),
+                     "poisson": run_poisson_secondary()}
+
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -208,6 +313,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "secondary": secondary,
         "wall_s": round(r["wall_s"], 4),
         "setup_s": round(gen_s, 2),
     }

@@ -182,6 +182,55 @@ int sg_tag_boundaries(sg_grid *g, int nn_type, int opt);
 /* applyStencils / conv / conv2 (src/SparseGridGpu/SparseGridGpu.hpp:2568-2734) */
 int sg_apply_stencil(sg_grid *g, const sg_stencil_desc *desc);
 
+/* ---- batched Poisson solves (no counterpart in the reference: a solver built from the
+ *      stencil sweeps of this path) ------------------------------------------------------
+ * -lap(u) = f on the existing points, u = 0 outside; 3D grids with 8^3 chunks.
+ * SG_POISSON_MIXED: FP32 red-black SOR on the correction e with right-hand side r, FP64
+ * only for the defect correction u <- u + e, the residual r = f - A u and its norm.
+ * SG_POISSON_PLAIN_FP64: red-black SOR on u in FP64, residual checked between blocks.
+ * Property layout of each grid: u, u2, f double; e, r float (u2, e, r unused by the plain
+ * form); u holds the initial guess when added (e, r, u2 need no initial values).
+ * A grid's iterates do not depend on the other grids of its batch: it is solved to the
+ * same bits alone or in any mix.  The stream may be null (the batch then makes its own;
+ * graphs cannot be captured on the legacy null stream). */
+#define SG_POISSON_MIXED 0
+#define SG_POISSON_PLAIN_FP64 1
+
+typedef struct {
+    int32_t mode;          /* SG_POISSON_* */
+    int32_t u_prop, u2_prop, f_prop, e_prop, r_prop;
+    double tol;            /* relative residual ||f - A u|| / ||f|| to reach */
+    double omega;          /* SOR relaxation; <= 0: estimated per grid */
+    double omega0;         /* relaxation of the first block when estimating (default 1.5) */
+    double inner_target;   /* residual reduction asked of one inner block (default 1e-5 mixed, 1e-3 plain) */
+    int32_t first_sweeps;  /* sweeps of the first block (default 24) */
+    int32_t max_sweeps;    /* per grid (default 20000) */
+    int32_t max_outer;     /* per grid (default 100) */
+    int32_t use_graph;     /* capture the inner sweep sequence in a hipGraph */
+} sg_poisson_params;
+
+typedef struct {
+    int32_t converged;     /* 1 when rel_residual <= tol */
+    int32_t outer;         /* outer iterations (residual evaluations - 1) */
+    int32_t sweeps;        /* red-black sweeps */
+    int32_t solution_prop; /* property holding the solution u */
+    double rel_residual;   /* final ||f - A u|| / ||f|| (FP64) */
+    double omega;          /* relaxation used */
+} sg_poisson_result;
+
+typedef struct sg_batch sg_batch;
+/* continuous batching: grids join with sg_batch_add and leave when converged; every
+ * sg_batch_step runs one outer iteration for all live grids in shared launches */
+int sg_batch_create(const sg_poisson_params *p, void *stream, sg_batch **out);
+int sg_batch_destroy(sg_batch *b);
+/* tol > 0: this grid's relative residual target (else the batch's p->tol) */
+int sg_batch_add(sg_batch *b, sg_grid *g, int64_t tag, double tol);
+int sg_batch_live(sg_batch *b);
+int sg_batch_step(sg_batch *b);
+int sg_batch_poll(sg_batch *b, int64_t *tags, sg_poisson_result *results, int max, int *n);
+/* solve all grids to p->tol in one batch (results[i] for grids[i]) */
+int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, sg_poisson_result *results);
+
 /* ---- launch tuning (no counterpart in the reference: its launch shapes are
  *      compile-time template parameters, SparseGridGpu.hpp:483-489) -------------- */
 

@@ -91,6 +91,39 @@ def make_desc(kind, src, dst=(), apply=APPLY_INTERIOR, box=None, coef=(), icoef=
     return d
 
 
+POISSON_MIXED = 0
+POISSON_PLAIN_FP64 = 1
+
+
+class PoissonParams(ctypes.Structure):
+    """sg_poisson_params (include/sparsegrid_gpu.h)."""
+    _fields_ = [
+        ("mode", ctypes.c_int32),
+        ("u_prop", ctypes.c_int32), ("u2_prop", ctypes.c_int32), ("f_prop", ctypes.c_int32),
+        ("e_prop", ctypes.c_int32), ("r_prop", ctypes.c_int32),
+        ("tol", ctypes.c_double),
+        ("omega", ctypes.c_double),
+        ("omega0", ctypes.c_double),
+        ("inner_target", ctypes.c_double),
+        ("first_sweeps", ctypes.c_int32),
+        ("max_sweeps", ctypes.c_int32),
+        ("max_outer", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+
+class PoissonResult(ctypes.Structure):
+    """sg_poisson_result (include/sparsegrid_gpu.h)."""
+    _fields_ = [
+        ("converged", ctypes.c_int32),
+        ("outer", ctypes.c_int32),
+        ("sweeps", ctypes.c_int32),
+        ("solution_prop", ctypes.c_int32),
+        ("rel_residual", ctypes.c_double),
+        ("omega", ctypes.c_double),
+    ]
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _I32 = ctypes.c_int32
@@ -124,6 +157,13 @@ _SIGS = {
     "sg_device_ptr": (_INT, [_P, _INT, ctypes.POINTER(_P)]),
     "sg_count_existing": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_count_boundary": (_INT, [_P, ctypes.POINTER(_I64)]),
+    "sg_batch_create": (_INT, [ctypes.POINTER(PoissonParams), _P, ctypes.POINTER(_P)]),
+    "sg_batch_destroy": (_INT, [_P]),
+    "sg_batch_add": (_INT, [_P, _P, _I64, ctypes.c_double]),
+    "sg_batch_live": (_INT, [_P]),
+    "sg_batch_step": (_INT, [_P]),
+    "sg_batch_poll": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(PoissonResult), _INT, ctypes.POINTER(_INT)]),
+    "sg_poisson_solve": (_INT, [ctypes.POINTER(_P), _INT, ctypes.POINTER(PoissonParams), ctypes.POINTER(PoissonResult)]),
 }
 
 _lib = None

@@ -75,3 +75,41 @@ def blobs1024(n_insert=1 << 24, n=1024, nblobs=64, sigma=24.0, dup_frac=0.3, see
 
 # Gray-Scott parameters of the 512^3 reaction-diffusion workload
 GS_PARAMS = {"Du": 2e-5, "Dv": 1e-5, "F": 0.04, "k": 0.06, "dt": 1.0}
+
+
+def random_chunk_grid(rng, edge_lo, edge_hi, occ_lo=0.05, occ_hi=0.40):
+    """One sparse grid of the batched / served Poisson workloads: domain edge log-uniform in
+    [edge_lo, edge_hi] rounded to a multiple of 8, a random subset of round(occ * nb^3)
+    chunks (occ uniform in [occ_lo, occ_hi]) fully active."""
+    e = float(np.exp(rng.uniform(np.log(edge_lo), np.log(edge_hi))))
+    edge = int(min(edge_hi, max(edge_lo, 8 * round(e / 8))))
+    nb = edge // 8
+    occ = float(rng.uniform(occ_lo, occ_hi))
+    k = max(1, int(round(occ * nb ** 3)))
+    ids = np.sort(rng.choice(nb ** 3, size=k, replace=False)).astype(np.int64)
+    return edge, occ, ids
+
+
+def poisson_batch(n=256, seed=3):
+    """256 independent grids (edge log-uniform in [32, 256], chunk occupancy in [5%, 40%]),
+    f uniform(-1, 1) on the active points (FP64), u = 0 outside."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        edge, occ, ids = random_chunk_grid(rng, 32, 256)
+        f = rng.uniform(-1.0, 1.0, (ids.size, 512))
+        out.append({"size": (edge, edge, edge), "occupancy": occ, "chunk_ids": ids, "f": f})
+    return out
+
+
+def serving_requests(n, seed=4):
+    """Request stream of the served Poisson workload: edge log-uniform in [32, 384],
+    occupancy in [5%, 40%], FP32 RHS uniform(-1, 1), tolerance 1e-10 or 1e-6 (50/50)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        edge, occ, ids = random_chunk_grid(rng, 32, 384)
+        f = rng.uniform(-1.0, 1.0, (ids.size, 512)).astype(np.float32)
+        tol = 1e-10 if rng.random() < 0.5 else 1e-6
+        out.append({"id": i, "size": (edge, edge, edge), "occupancy": occ, "chunk_ids": ids, "f": f, "tol": tol})
+    return out

@@ -264,3 +264,21 @@ def test_fast_stencil_variants(G, oracle, variant, stores, lskip, dtype):
         g.conv_lap(1, 3, -4.0, 0.5)
         gs.append(g)
     S.assert_same_state(gs[0], gs[1])
+
+
+def test_blobs1024_full_flush_tag_box(G, oracle):
+    """The full insertion workload: 2^24 inserts into 1024^3 from 64 Gaussian blobs (30%
+    repeats), int32 smax + uint32 sadd, NNFull tagBoundaries, 27-point uint32 box sum:
+    chunk map, masks and every property bit-exact against the oracle."""
+    w = W.blobs1024()
+    gs = []
+    for cls in (G, oracle.SparseGridCpu):
+        g = cls(3, w["size"], [np.int32, np.uint32, np.uint32], block_edge=8)
+        g.insert_coords(w["coords"], [w["int32"], w["uint32"], np.zeros(len(w["uint32"]), np.uint32)])
+        g.flush(["smax", "sadd", "none"])
+        g.set_nn_type(C.NN_FULL)
+        g.tag_boundaries(C.NN_FULL)
+        g.conv_box(1, 2, np.ones(27, np.int64))
+        gs.append(g)
+    assert gs[0].num_chunks() == gs[1].num_chunks() == 317793
+    S.assert_same_state(gs[0], gs[1])
