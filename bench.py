@@ -205,6 +205,31 @@ def run_poisson_secondary():
             "speedup_vs_per_grid_plain_fp64": round(t_plain / t_mix, 3)}
 
 
+def run_serving_secondary(n_open=192, n_cap=64, budget_gib=32.0):
+    """Served Poisson requests (edge log-uniform in [32, 384], 5-40% of the chunks active,
+    FP32 f, tolerance 1e-10 or 1e-6, seed 4), mixed method, HBM budget `budget_gib`:
+    capacity measured closed-loop (first `n_cap` requests all present at t = 0), then
+    `n_open` requests arriving as a Poisson process at 80% of that capacity (seed 5)."""
+    from openfpm_data_amd import serving as S
+    from openfpm_data_amd import workloads as W
+
+    reqs = W.serving_requests(n_open)
+    budget = int(budget_gib * (1 << 30))
+    srv = S.PoissonServer(budget)
+    srv.run(reqs[:8], np.zeros(8))  # warm-up
+    _, cap = srv.run(reqs[:n_cap], np.zeros(n_cap))
+    rate = 0.8 * cap["solves_per_s"]
+    _, s = srv.run(reqs, S.poisson_arrivals(n_open, rate, 5))
+    srv.close()
+    return {"workload": "poisson_serving", "requests": n_open, "budget_gib": budget_gib,
+            "capacity_solves_per_s": round(cap["solves_per_s"], 2), "arrival_rate_per_s": round(rate, 2),
+            "solves_per_s": round(s["solves_per_s"], 2), "unit": "solves/s",
+            "p50_latency_ms": round(s["p50_latency_s"] * 1e3, 3), "p99_latency_ms": round(s["p99_latency_s"] * 1e3, 3),
+            "point_updates_per_s": round(s["point_updates_per_s"] / 1e9, 3), "point_updates_unit": "G/s",
+            "served": s["served"], "converged": s["converged"], "rejected": s["rejected"],
+            "max_live": s["max_live"], "peak_reserved_gib": round(s["peak_reserved_bytes"] / (1 << 30), 3)}
+
+
 def run_cpu_baseline(ids, masks, data, seconds, threads):
     from oracle import pysgcpu
     from openfpm_data_amd import workloads as W
@@ -292,7 +317,8 @@ def main():
     secondary = None
     if not args.no_secondary:
         secondary = {"flush": run_flush_secondary(args.cpu_threads or min(16, os.cpu_count() or 1), not args.no_cpu),
-                     "poisson": run_poisson_secondary()}
+                     "poisson": run_poisson_secondary(),
+                     "serving": run_serving_secondary()}
 
     line = {
         "metric": METRIC,

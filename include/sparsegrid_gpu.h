@@ -165,7 +165,8 @@ int sg_insert_coords(sg_grid *g, int64_t n, const int32_t *coords,
  * values[p] is [n][edge^dim]. */
 int sg_insert_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids,
                      const uint8_t *masks, const void *const *values, int mem);
-/* number of records waiting in the insert buffer */
+/* number of point records waiting in the insert buffer (a whole-chunk record counts as
+ * its slots with a non-zero mask, at least one); -1 on failure */
 int64_t sg_pending_inserts(sg_grid *g);
 /* flush<v_reduce...>(ctx, FLUSH_ON_DEVICE) (src/SparseGridGpu/SparseGridGpu.hpp:772-779,
  * src/Vector/map_vector_sparse.hpp:1192-1226); ops[p] for every property. */
@@ -271,6 +272,14 @@ int sg_device_ptr(sg_grid *g, int prop, void **out);
 /* countExistingElements / countBoundaryElements (src/SparseGridGpu/SparseGridGpu.hpp:2422-2490) */
 int sg_count_existing(sg_grid *g, int64_t *out);
 int sg_count_boundary(sg_grid *g, int64_t *out);
+
+/* ---- device memory (no counterpart in the reference: its vectors keep their
+ *      capacity; a server admitting grids against an HBM budget needs both) ---- */
+/* wait for the handle's stream, then free the flush temporaries and cached blocks; the
+ * chunk map, properties, masks and neighbour tables stay */
+int sg_trim(sg_grid *g);
+/* bytes of device memory the handle holds (resident arrays + cached temporaries) */
+int sg_device_bytes(sg_grid *g, int64_t *out);
 
 #ifdef __cplusplus
 }

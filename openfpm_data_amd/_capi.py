@@ -157,6 +157,8 @@ _SIGS = {
     "sg_device_ptr": (_INT, [_P, _INT, ctypes.POINTER(_P)]),
     "sg_count_existing": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_count_boundary": (_INT, [_P, ctypes.POINTER(_I64)]),
+    "sg_trim": (_INT, [_P]),
+    "sg_device_bytes": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_batch_create": (_INT, [ctypes.POINTER(PoissonParams), _P, ctypes.POINTER(_P)]),
     "sg_batch_destroy": (_INT, [_P]),
     "sg_batch_add": (_INT, [_P, _P, _I64, ctypes.c_double]),

@@ -65,6 +65,22 @@ void DevicePool::release()
     live_ = 0;
 }
 
+void DevicePool::trim()
+{
+    for (auto &kv : cache_) {
+        (void)hipFree(kv.second);
+        sizes_.erase(kv.second);
+    }
+    cache_.clear();
+}
+
+size_t DevicePool::held_bytes() const
+{
+    size_t n = 0;
+    for (auto &kv : sizes_) n += kv.second;
+    return n;
+}
+
 int ensure(DevicePool &pool, DevBuf &b, size_t bytes)
 {
     if (b.bytes >= bytes && b.p) return SG_OK;
@@ -224,7 +240,13 @@ int sg_insert_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint
     return rc;
 }
 
-int64_t sg_pending_inserts(sg_grid *g) { return g ? g->ins_n : 0; }
+int64_t sg_pending_inserts(sg_grid *g)
+{
+    if (!g) return 0;
+    // chunk records count as the point records they stand for
+    if (g->cins_n > 0 && convert_chunk_records(g) != SG_OK) return -1;
+    return g->ins_n;
+}
 
 int sg_flush(sg_grid *g, const int32_t *ops)
 {
@@ -376,6 +398,49 @@ int sg_count_boundary(sg_grid *g, int64_t *out)
 {
     if (!g || !out) return fail(SG_E_ARG, "null argument");
     return count_mask_impl(g, 1, out);
+}
+
+int sg_trim(sg_grid *g)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    SG_HIP(hipStreamSynchronize(g->stream));
+    if (g->ins_n == 0 && g->ins_cap > 0) {
+        g->pool.free(g->d_ins_key);
+        g->pool.free(g->d_ins_mask);
+        g->d_ins_key = nullptr;
+        g->d_ins_mask = nullptr;
+        for (int p = 0; p < g->np; ++p) {
+            g->pool.free(g->d_ins_val[p]);
+            g->d_ins_val[p] = nullptr;
+        }
+        g->ins_cap = 0;
+    }
+    if (g->cins_n == 0 && g->cins_cap > 0) {
+        g->pool.free(g->d_cins_id);
+        g->pool.free(g->d_cins_mask);
+        g->d_cins_id = nullptr;
+        g->d_cins_mask = nullptr;
+        for (int p = 0; p < g->np; ++p) {
+            g->pool.free(g->d_cins_val[p]);
+            g->d_cins_val[p] = nullptr;
+        }
+        g->cins_cap = 0;
+    }
+    for (auto &b : g->scratch) {
+        g->pool.free(b.p);
+        b = DevBuf();
+    }
+    g->pool.free(g->hostlike.p);
+    g->hostlike = DevBuf();
+    g->pool.trim();
+    return SG_OK;
+}
+
+int sg_device_bytes(sg_grid *g, int64_t *out)
+{
+    if (!g || !out) return fail(SG_E_ARG, "null argument");
+    *out = (int64_t)g->pool.held_bytes();
+    return SG_OK;
 }
 
 }  // extern "C"

@@ -19,6 +19,12 @@
 // unique points -> chunk heads + scan -> unique chunks -> binary-search merge with
 // the old chunk list (new chunks get positions by a merge-path count) -> one
 // thread per unique point folds its records and the old value in place.
+//
+// Whole-chunk inserts (insertBlock) stay chunk records while no point record is
+// pending: only the chunk ids are sorted, and one workgroup per touched chunk folds
+// its records slot by slot with the same rules (a record takes part in the fold of the
+// slots where its mask is non-zero).  Appending point records after chunk records
+// first turns the chunk records into point records, so insertion order is kept.
 #include "sg_internal.hpp"
 
 #include <algorithm>
@@ -357,6 +363,74 @@ __global__ void uchunk_index_kernel(const uint32_t *hscan, const uint32_t *head,
     if (u < nu) uchunk[u] = hscan[u] + head[u] - 1;
 }
 
+// ---- whole-chunk records --------------------------------------------------------
+// A chunk record takes part in the fold of the slots where its mask is non-zero (an
+// all-zero record only creates its chunk), as the oracle's restatement of insertBlock.
+// one workgroup per touched output chunk u: records perm[rstart[u] .. rstart[u+1]) in
+// insertion order, folded per slot as fold_points_kernel folds a point's records
+template <typename T>
+__global__ __launch_bounds__(kWG) void fold_chunks_kernel(const uint32_t *rstart, const uint32_t *perm,
+                                                         const int64_t *cpos, int csz, const uint8_t *rmask,
+                                                         const T *rval, T *out,
+                                                         const uint8_t *omask, int op, T bgv)
+{
+    const int64_t u = blockIdx.x;
+    const uint32_t r0 = rstart[u], r1 = rstart[u + 1];
+    const int64_t base = cpos[u] * csz;
+    for (int o = threadIdx.x; o < csz; o += kWG) {
+        bool have = false;
+        T a = bgv;
+        uint8_t am = 0;
+        for (uint32_t i = r0; i < r1; ++i) {
+            const uint32_t rec = perm[i];
+            const uint8_t bm = rmask[(int64_t)rec * csz + o];
+            if (bm == 0) continue;
+            const T b = rval[(int64_t)rec * csz + o];
+            if (!have) {
+                a = b;
+                am = bm;
+                have = true;
+                continue;
+            }
+            const bool aE = am & 1, bE = bm & 1;
+            if (op == SG_OP_NONE) {
+                if (bE) a = b;
+            } else if (aE && bE) {
+                a = dev_reduce<T>(op, a, b);
+            } else if (bE) {
+                a = b;
+            }
+            am |= bm;
+        }
+        if (!have) continue;
+        const int64_t slot = base + o;
+        const uint8_t om = omask[slot];
+        const bool aE = am & 1, oE = om & 1;
+        if (op == SG_OP_NONE) {
+            if (!aE && oE) a = out[slot];
+        } else if (aE && oE) {
+            a = dev_reduce<T>(op, a, out[slot]);
+        } else if (oE) {
+            a = out[slot];
+        }
+        out[slot] = ((am | om) == 0) ? bgv : a;
+    }
+}
+
+__global__ __launch_bounds__(kWG) void fold_chunk_masks_kernel(const uint32_t *rstart, const uint32_t *perm,
+                                                              const int64_t *cpos, int csz, const uint8_t *rmask,
+                                                              uint8_t *omask)
+{
+    const int64_t u = blockIdx.x;
+    const uint32_t r0 = rstart[u], r1 = rstart[u + 1];
+    const int64_t base = cpos[u] * csz;
+    for (int o = threadIdx.x; o < csz; o += kWG) {
+        uint8_t am = 0;
+        for (uint32_t i = r0; i < r1; ++i) am |= rmask[(int64_t)perm[i] * csz + o];
+        omask[base + o] |= am;
+    }
+}
+
 template <typename F> void dispatch_type(int dt, F &&f)
 {
     switch (dt) {
@@ -449,7 +523,9 @@ int append_points(sg_grid *g, int64_t n, const int64_t *keys, const uint8_t *mas
         for (int64_t i = 0; i < n; ++i)
             if (keys[i] < 0 || keys[i] >= limit) return fail(SG_E_RANGE, "point key outside the grid");
     }
-    int rc = grow_insert_buffer(g, g->ins_n + n);
+    int rc = convert_chunk_records(g);
+    if (rc) return rc;
+    rc = grow_insert_buffer(g, g->ins_n + n);
     if (rc) return rc;
     const int64_t at = g->ins_n;
     SG_HIP(hipMemcpyAsync(g->d_ins_key + at, keys, n * sizeof(int64_t), kind_from(mem), g->stream));
@@ -465,7 +541,9 @@ int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *m
     if (n < 0) return fail(SG_E_ARG, "negative record count");
     if (n == 0) return SG_OK;
     if (!coords) return fail(SG_E_ARG, "null coords");
-    int rc = grow_insert_buffer(g, g->ins_n + n);
+    int rc = convert_chunk_records(g);
+    if (rc) return rc;
+    rc = grow_insert_buffer(g, g->ins_n + n);
     if (rc) return rc;
     const int64_t at = g->ins_n;
     const int32_t *dc = coords;
@@ -502,20 +580,19 @@ int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *m
     return SG_OK;
 }
 
-int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t *masks, const void *const *values,
+namespace {
+
+// whole-chunk records -> point records appended to the point insert buffer
+int expand_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t *masks, const void *const *values,
                   int mem)
 {
-    if (n < 0) return fail(SG_E_ARG, "negative record count");
-    if (n == 0) return SG_OK;
-    if (!chunk_ids || !masks) return fail(SG_E_ARG, "null chunk ids or masks");
     const int csz = g->csz;
-    int64_t nblocks = key_limit(g) / csz;
-    if (mem == SG_MEM_HOST)
-        for (int64_t i = 0; i < n; ++i)
-            if (chunk_ids[i] < 0 || chunk_ids[i] >= nblocks) return fail(SG_E_RANGE, "chunk id outside the grid");
-    // stage chunk data on the device
-    size_t bytes = n * sizeof(int64_t) + n * csz;
-    for (int p = 0; p < g->np; ++p) bytes += (values && values[p]) ? n * csz * g->ts[p] : 0;
+    // stage host chunk data on the device
+    size_t bytes = 0;
+    if (mem == SG_MEM_HOST) {
+        bytes = n * sizeof(int64_t) + n * csz;
+        for (int p = 0; p < g->np; ++p) bytes += (values && values[p]) ? n * csz * g->ts[p] + 16 : 0;
+    }
     DevBuf &st = g->scratch[8];
     int rc = ensure(g->pool, st, bytes + 256 + (size_t)(n + 1) * 4 * 2);
     if (rc) return rc;
@@ -580,115 +657,105 @@ int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t
     return SG_OK;
 }
 
-int flush_impl(sg_grid *g, const int32_t *ops)
+int grow_chunk_buffer(sg_grid *g, int64_t need)
 {
-    const int64_t n = g->ins_n;
-    if (n == 0) return SG_OK;  // nothing inserted (map_vector_sparse.hpp:1199-1213)
-    if (n >= ((int64_t)1 << 31)) return fail(SG_E_RANGE, "more than 2^31 pending inserts");
+    if (need <= g->cins_cap) return SG_OK;
     const int csz = g->csz;
-    const int64_t limit = key_limit(g);
-    const int64_t m = g->nchunks;
-    int rc;
+    int64_t ncap = std::max<int64_t>(need, std::max<int64_t>(16, g->cins_cap * 2));
+    int64_t *ni = (int64_t *)g->pool.alloc(ncap * sizeof(int64_t));
+    uint8_t *nm = (uint8_t *)g->pool.alloc((size_t)ncap * csz);
+    if (!ni || !nm) return fail(SG_E_OOM, "chunk insert buffer allocation failed");
+    void *nv[kMaxProps] = {};
+    for (int p = 0; p < g->np; ++p) {
+        nv[p] = g->pool.alloc((size_t)ncap * csz * g->ts[p]);
+        if (!nv[p]) return fail(SG_E_OOM, "chunk insert buffer allocation failed");
+    }
+    if (g->cins_n) {
+        const int64_t n = g->cins_n;
+        SG_HIP(hipMemcpyAsync(ni, g->d_cins_id, n * sizeof(int64_t), hipMemcpyDeviceToDevice, g->stream));
+        SG_HIP(hipMemcpyAsync(nm, g->d_cins_mask, (size_t)n * csz, hipMemcpyDeviceToDevice, g->stream));
+        for (int p = 0; p < g->np; ++p)
+            SG_HIP(hipMemcpyAsync(nv[p], g->d_cins_val[p], (size_t)n * csz * g->ts[p], hipMemcpyDeviceToDevice,
+                                  g->stream));
+    }
+    g->pool.free(g->d_cins_id);
+    g->pool.free(g->d_cins_mask);
+    for (int p = 0; p < g->np; ++p) g->pool.free(g->d_cins_val[p]);
+    g->d_cins_id = ni;
+    g->d_cins_mask = nm;
+    for (int p = 0; p < g->np; ++p) g->d_cins_val[p] = nv[p];
+    g->cins_cap = ncap;
+    return SG_OK;
+}
 
-    // validate device-side keys
-    {
-        rc = ensure(g->pool, g->scratch[6], 64);
-        if (rc) return rc;
-        int *bad = (int *)g->scratch[6].p;
-        SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
-        hipLaunchKernelGGL(check_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key, n, limit,
-                           bad);
-        SG_HIP(hipGetLastError());
-        int hbad = 0;
-        SG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
-        SG_HIP(hipStreamSynchronize(g->stream));
-        if (hbad) {
-            g->ins_n = 0;
-            return fail(SG_E_RANGE, "point key outside the grid in the insert buffer");
+}  // namespace
+
+int convert_chunk_records(sg_grid *g)
+{
+    if (g->cins_n == 0) return SG_OK;
+    const int64_t n = g->cins_n;
+    const void *vals[kMaxProps];
+    for (int p = 0; p < g->np; ++p) vals[p] = g->d_cins_val[p];
+    int rc = expand_chunks(g, n, g->d_cins_id, g->d_cins_mask, vals, SG_MEM_DEVICE);
+    if (rc) return rc;
+    g->cins_n = 0;
+    return SG_OK;
+}
+
+int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t *masks, const void *const *values,
+                  int mem)
+{
+    if (n < 0) return fail(SG_E_ARG, "negative record count");
+    if (n == 0) return SG_OK;
+    if (!chunk_ids || !masks) return fail(SG_E_ARG, "null chunk ids or masks");
+    const int csz = g->csz;
+    const int64_t nblocks = key_limit(g) / csz;
+    if (mem == SG_MEM_HOST)
+        for (int64_t i = 0; i < n; ++i)
+            if (chunk_ids[i] < 0 || chunk_ids[i] >= nblocks) return fail(SG_E_RANGE, "chunk id outside the grid");
+    // after point records, chunk records join them as points (insertion order is kept)
+    if (g->ins_n > 0) return expand_chunks(g, n, chunk_ids, masks, values, mem);
+    int rc = grow_chunk_buffer(g, g->cins_n + n);
+    if (rc) return rc;
+    const int64_t at = g->cins_n;
+    SG_HIP(hipMemcpyAsync(g->d_cins_id + at, chunk_ids, n * sizeof(int64_t), kind_from(mem), g->stream));
+    SG_HIP(hipMemcpyAsync(g->d_cins_mask + at * csz, masks, (size_t)n * csz, kind_from(mem), g->stream));
+    for (int p = 0; p < g->np; ++p) {
+        unsigned char *dst = (unsigned char *)g->d_cins_val[p] + (size_t)at * csz * g->ts[p];
+        if (values && values[p]) {
+            SG_HIP(hipMemcpyAsync(dst, values[p], (size_t)n * csz * g->ts[p], kind_from(mem), g->stream));
+        } else {
+            dispatch_type(g->dt[p], [&](auto tag) {
+                typedef decltype(tag) T;
+                T v;
+                memcpy(&v, g->bg[p], sizeof(T));
+                hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_1d(n * csz, kWG)), dim3(kWG), 0, g->stream, (T *)dst, v,
+                                   (int64_t)n * csz);
+            });
+            SG_HIP(hipGetLastError());
         }
     }
-    int bits = 0;
-    while (bits < 63 && (((int64_t)1 << bits) < limit)) ++bits;
-    const bool k32 = bits <= 32;
+    g->cins_n += n;
+    return SG_OK;
+}
 
-    // scratch layout
-    DevBuf &b_keys = g->scratch[0], &b_vals = g->scratch[1], &b_tk = g->scratch[2], &b_tv = g->scratch[3];
-    DevBuf &b_u = g->scratch[4], &b_c = g->scratch[5];
-    const size_t kbytes = (k32 ? 4 : 8) * (size_t)n;
-    if ((rc = ensure(g->pool, b_keys, kbytes)) || (rc = ensure(g->pool, b_tk, kbytes)) ||
-        (rc = ensure(g->pool, b_vals, 4 * (size_t)n)) || (rc = ensure(g->pool, b_tv, 4 * (size_t)n)))
-        return rc;
+namespace {
 
-    void *sk = nullptr;
-    uint32_t *perm = nullptr;
-    if (k32) {
-        hipLaunchKernelGGL(keys_to_sort_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           g->d_ins_key, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, n);
-        SG_HIP(hipGetLastError());
-        uint32_t *ko;
-        rc = radix_sort_pairs_u32(g, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, (uint32_t *)b_tk.p,
-                                  (uint32_t *)b_tv.p, n, bits, &ko, &perm);
-        sk = ko;
-    } else {
-        hipLaunchKernelGGL(keys_to_sort_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           g->d_ins_key, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, n);
-        SG_HIP(hipGetLastError());
-        uint64_t *ko;
-        rc = radix_sort_pairs_u64(g, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, (uint64_t *)b_tk.p,
-                                  (uint32_t *)b_tv.p, n, bits, &ko, &perm);
-        sk = ko;
-    }
-    if (rc) return rc;
-    // the temporaries not holding the sorted result are free for reuse
-    uint32_t *spare_a = (perm == (uint32_t *)b_vals.p) ? (uint32_t *)b_tv.p : (uint32_t *)b_vals.p;
-
-    // unique points: run heads + scan (head -> spare_a, scan -> b_u)
-    if ((rc = ensure(g->pool, b_u, (size_t)n * (4 + 4 + 8 + 4 + 4) + 64))) return rc;
-    uint32_t *head = spare_a;
-    uint32_t *hscan = (uint32_t *)b_u.p;
-    if (k32)
-        hipLaunchKernelGGL(run_head_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           (const uint32_t *)sk, head, n);
-    else
-        hipLaunchKernelGGL(run_head_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           (const uint64_t *)sk, head, n);
-    SG_HIP(hipGetLastError());
-    uint32_t nu32 = 0;
-    if ((rc = exclusive_scan_u32(g, head, hscan, n, &nu32))) return rc;
-    const int64_t nu = nu32;
-    int64_t *ukey = (int64_t *)((char *)b_u.p + (((size_t)n * 4 + 15) & ~(size_t)15));
-    uint32_t *rstart = (uint32_t *)(ukey + n);
-    if (k32)
-        hipLaunchKernelGGL(unique_points_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           (const uint32_t *)sk, head, hscan, n, ukey, rstart, nu32);
-    else
-        hipLaunchKernelGGL(unique_points_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           (const uint64_t *)sk, head, hscan, n, ukey, rstart, nu32);
-    SG_HIP(hipGetLastError());
-
-    // unique chunks among the unique points (head -> hscan region reused)
-    uint32_t *chead = head;  // head no longer needed
-    uint32_t *cscan = hscan;
-    hipLaunchKernelGGL(chunk_head_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, chead, nu, csz);
-    SG_HIP(hipGetLastError());
-    uint32_t nn32 = 0;
-    if ((rc = exclusive_scan_u32(g, chead, cscan, nu, &nn32))) return rc;
-    const int64_t nnew = nn32;
-    uint32_t *uchunk = rstart + (n + 1);
-    hipLaunchKernelGGL(uchunk_index_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, cscan, chead, nu, uchunk);
-    SG_HIP(hipGetLastError());
-
-    // per new chunk: key, old position, brand-new flag and its scan, final position
-    if ((rc = ensure(g->pool, b_c, (size_t)nnew * (8 + 8 + 4 + 4 + 8 + 8) + 128))) return rc;
-    int64_t *nkey = (int64_t *)b_c.p;
-    int64_t *oldpos = nkey + nnew;
+// Merge the sorted unique chunk ids `nkey` [nnew] touched by a flush into the chunk map:
+// brand-new chunks are placed by merge-path rank, the arrays grow (old chunks moved, new
+// ones mask 0 / background), and cpos[j] is the final position of nkey[j].
+int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpos_out)
+{
+    const int csz = g->csz;
+    const int64_t m = g->nchunks;
+    int rc;
+    DevBuf &b_c = g->scratch[5];
+    if ((rc = ensure(g->pool, b_c, (size_t)nnew * (8 + 4 + 4 + 8 + 8) + 128))) return rc;
+    int64_t *oldpos = (int64_t *)b_c.p;
     uint32_t *isnew = (uint32_t *)(oldpos + nnew);
     uint32_t *nscan = isnew + nnew;
     int64_t *cpos = (int64_t *)(((uintptr_t)(nscan + nnew) + 15) & ~(uintptr_t)15);
     int64_t *bn = cpos + nnew;
-    hipLaunchKernelGGL(unique_chunks_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, chead, cscan, nu,
-                       csz, nkey);
-    SG_HIP(hipGetLastError());
     hipLaunchKernelGGL(old_lookup_kernel, dim3(grid_1d(nnew, kWG)), dim3(kWG), 0, g->stream, nkey, nnew, g->d_keys,
                        m, oldpos, isnew);
     SG_HIP(hipGetLastError());
@@ -769,6 +836,220 @@ int flush_impl(sg_grid *g, const int32_t *ops)
         g->nchunks = total;
         g->cap = ncap;
     }
+
+    *cpos_out = cpos;
+    return SG_OK;
+}
+
+// flush of whole-chunk records only: stable sort of (chunk id, record id), unique chunk
+// ids + segment starts, merge into the chunk map, one workgroup per touched chunk folds
+// its records slot by slot
+int flush_chunks_impl(sg_grid *g, const int32_t *ops)
+{
+    const int64_t n = g->cins_n;
+    if (n >= ((int64_t)1 << 31)) return fail(SG_E_RANGE, "more than 2^31 pending chunk records");
+    const int csz = g->csz;
+    const int64_t nblocks = key_limit(g) / csz;
+    int rc;
+    {
+        if ((rc = ensure(g->pool, g->scratch[6], 64))) return rc;
+        int *bad = (int *)g->scratch[6].p;
+        SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
+        hipLaunchKernelGGL(check_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_cins_id, n,
+                           nblocks, bad);
+        SG_HIP(hipGetLastError());
+        int hbad = 0;
+        SG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+        SG_HIP(hipStreamSynchronize(g->stream));
+        if (hbad) {
+            g->cins_n = 0;
+            return fail(SG_E_RANGE, "chunk id outside the grid in the insert buffer");
+        }
+    }
+    int bits = 0;
+    while (bits < 63 && (((int64_t)1 << bits) < nblocks)) ++bits;
+    const bool k32 = bits <= 32;
+    DevBuf &b_keys = g->scratch[0], &b_vals = g->scratch[1], &b_tk = g->scratch[2], &b_tv = g->scratch[3];
+    DevBuf &b_u = g->scratch[4];
+    const size_t kbytes = (k32 ? 4 : 8) * (size_t)n;
+    if ((rc = ensure(g->pool, b_keys, kbytes)) || (rc = ensure(g->pool, b_tk, kbytes)) ||
+        (rc = ensure(g->pool, b_vals, 4 * (size_t)n)) || (rc = ensure(g->pool, b_tv, 4 * (size_t)n)))
+        return rc;
+    void *sk = nullptr;
+    uint32_t *perm = nullptr;
+    if (k32) {
+        hipLaunchKernelGGL(keys_to_sort_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           g->d_cins_id, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+        SG_HIP(hipGetLastError());
+        uint32_t *ko;
+        rc = radix_sort_pairs_u32(g, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, (uint32_t *)b_tk.p,
+                                  (uint32_t *)b_tv.p, n, bits, &ko, &perm);
+        sk = ko;
+    } else {
+        hipLaunchKernelGGL(keys_to_sort_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           g->d_cins_id, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+        SG_HIP(hipGetLastError());
+        uint64_t *ko;
+        rc = radix_sort_pairs_u64(g, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, (uint64_t *)b_tk.p,
+                                  (uint32_t *)b_tv.p, n, bits, &ko, &perm);
+        sk = ko;
+    }
+    if (rc) return rc;
+    uint32_t *spare = (perm == (uint32_t *)b_vals.p) ? (uint32_t *)b_tv.p : (uint32_t *)b_vals.p;
+    // b_u: hscan [n] u32 | ukey [n] i64 | rstart [n+1] u32
+    if ((rc = ensure(g->pool, b_u, (size_t)n * (4 + 8 + 4) + 128))) return rc;
+    uint32_t *head = spare;
+    uint32_t *hscan = (uint32_t *)b_u.p;
+    int64_t *ukey = (int64_t *)((char *)b_u.p + (((size_t)n * 4 + 15) & ~(size_t)15));
+    uint32_t *rstart = (uint32_t *)(ukey + n);
+    if (k32)
+        hipLaunchKernelGGL(run_head_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint32_t *)sk, head, n);
+    else
+        hipLaunchKernelGGL(run_head_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint64_t *)sk, head, n);
+    SG_HIP(hipGetLastError());
+    uint32_t nu32 = 0;
+    if ((rc = exclusive_scan_u32(g, head, hscan, n, &nu32))) return rc;
+    const int64_t nu = nu32;
+    if (k32)
+        hipLaunchKernelGGL(unique_points_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint32_t *)sk, head, hscan, n, ukey, rstart, nu32);
+    else
+        hipLaunchKernelGGL(unique_points_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint64_t *)sk, head, hscan, n, ukey, rstart, nu32);
+    SG_HIP(hipGetLastError());
+    int64_t *cpos = nullptr;
+    if ((rc = merge_chunk_set(g, ukey, nu, &cpos))) return rc;
+    for (int p = 0; p < g->np; ++p) {
+        const int op = ops ? ops[p] : SG_OP_NONE;
+        dispatch_type(g->dt[p], [&](auto tag) {
+            typedef decltype(tag) T;
+            T v;
+            memcpy(&v, g->bg[p], sizeof(T));
+            hipLaunchKernelGGL(fold_chunks_kernel<T>, dim3((unsigned)nu), dim3(kWG), 0, g->stream, rstart, perm, cpos,
+                               csz, g->d_cins_mask, (const T *)g->d_cins_val[p], (T *)g->d_prop[p],
+                               g->d_mask, op, v);
+        });
+        SG_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(fold_chunk_masks_kernel, dim3((unsigned)nu), dim3(kWG), 0, g->stream, rstart, perm, cpos, csz,
+                       g->d_cins_mask, g->d_mask);
+    SG_HIP(hipGetLastError());
+    g->cins_n = 0;
+    g->nn_valid[0] = g->nn_valid[1] = false;
+    return SG_OK;
+}
+
+}  // namespace
+
+int flush_impl(sg_grid *g, const int32_t *ops)
+{
+    // chunk records are pending only while no point record is (appending points converts them)
+    if (g->cins_n > 0) return flush_chunks_impl(g, ops);
+    const int64_t n = g->ins_n;
+    if (n == 0) return SG_OK;  // nothing inserted (map_vector_sparse.hpp:1199-1213)
+    if (n >= ((int64_t)1 << 31)) return fail(SG_E_RANGE, "more than 2^31 pending inserts");
+    const int csz = g->csz;
+    const int64_t limit = key_limit(g);
+    int rc;
+
+    // validate device-side keys
+    {
+        rc = ensure(g->pool, g->scratch[6], 64);
+        if (rc) return rc;
+        int *bad = (int *)g->scratch[6].p;
+        SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
+        hipLaunchKernelGGL(check_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key, n, limit,
+                           bad);
+        SG_HIP(hipGetLastError());
+        int hbad = 0;
+        SG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+        SG_HIP(hipStreamSynchronize(g->stream));
+        if (hbad) {
+            g->ins_n = 0;
+            return fail(SG_E_RANGE, "point key outside the grid in the insert buffer");
+        }
+    }
+    int bits = 0;
+    while (bits < 63 && (((int64_t)1 << bits) < limit)) ++bits;
+    const bool k32 = bits <= 32;
+
+    // scratch layout
+    DevBuf &b_keys = g->scratch[0], &b_vals = g->scratch[1], &b_tk = g->scratch[2], &b_tv = g->scratch[3];
+    DevBuf &b_u = g->scratch[4];
+    const size_t kbytes = (k32 ? 4 : 8) * (size_t)n;
+    if ((rc = ensure(g->pool, b_keys, kbytes)) || (rc = ensure(g->pool, b_tk, kbytes)) ||
+        (rc = ensure(g->pool, b_vals, 4 * (size_t)n)) || (rc = ensure(g->pool, b_tv, 4 * (size_t)n)))
+        return rc;
+
+    void *sk = nullptr;
+    uint32_t *perm = nullptr;
+    if (k32) {
+        hipLaunchKernelGGL(keys_to_sort_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           g->d_ins_key, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+        SG_HIP(hipGetLastError());
+        uint32_t *ko;
+        rc = radix_sort_pairs_u32(g, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, (uint32_t *)b_tk.p,
+                                  (uint32_t *)b_tv.p, n, bits, &ko, &perm);
+        sk = ko;
+    } else {
+        hipLaunchKernelGGL(keys_to_sort_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           g->d_ins_key, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+        SG_HIP(hipGetLastError());
+        uint64_t *ko;
+        rc = radix_sort_pairs_u64(g, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, (uint64_t *)b_tk.p,
+                                  (uint32_t *)b_tv.p, n, bits, &ko, &perm);
+        sk = ko;
+    }
+    if (rc) return rc;
+    // the temporaries not holding the sorted result are free for reuse
+    uint32_t *spare_a = (perm == (uint32_t *)b_vals.p) ? (uint32_t *)b_tv.p : (uint32_t *)b_vals.p;
+
+    // unique points: run heads + scan (head -> spare_a, scan -> b_u)
+    if ((rc = ensure(g->pool, b_u, (size_t)n * (4 + 4 + 8 + 4 + 4) + 64))) return rc;
+    uint32_t *head = spare_a;
+    uint32_t *hscan = (uint32_t *)b_u.p;
+    if (k32)
+        hipLaunchKernelGGL(run_head_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint32_t *)sk, head, n);
+    else
+        hipLaunchKernelGGL(run_head_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint64_t *)sk, head, n);
+    SG_HIP(hipGetLastError());
+    uint32_t nu32 = 0;
+    if ((rc = exclusive_scan_u32(g, head, hscan, n, &nu32))) return rc;
+    const int64_t nu = nu32;
+    int64_t *ukey = (int64_t *)((char *)b_u.p + (((size_t)n * 4 + 15) & ~(size_t)15));
+    uint32_t *rstart = (uint32_t *)(ukey + n);
+    if (k32)
+        hipLaunchKernelGGL(unique_points_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint32_t *)sk, head, hscan, n, ukey, rstart, nu32);
+    else
+        hipLaunchKernelGGL(unique_points_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint64_t *)sk, head, hscan, n, ukey, rstart, nu32);
+    SG_HIP(hipGetLastError());
+
+    // unique chunks among the unique points (head -> hscan region reused)
+    uint32_t *chead = head;  // head no longer needed
+    uint32_t *cscan = hscan;
+    hipLaunchKernelGGL(chunk_head_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, chead, nu, csz);
+    SG_HIP(hipGetLastError());
+    uint32_t nn32 = 0;
+    if ((rc = exclusive_scan_u32(g, chead, cscan, nu, &nn32))) return rc;
+    const int64_t nnew = nn32;
+    uint32_t *uchunk = rstart + (n + 1);
+    hipLaunchKernelGGL(uchunk_index_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, cscan, chead, nu, uchunk);
+    SG_HIP(hipGetLastError());
+
+    // unique chunk ids, then their positions in the (grown) chunk map
+    if ((rc = ensure(g->pool, g->scratch[12], (size_t)nnew * 8 + 64))) return rc;
+    int64_t *nkey = (int64_t *)g->scratch[12].p;
+    hipLaunchKernelGGL(unique_chunks_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, chead, cscan, nu,
+                       csz, nkey);
+    SG_HIP(hipGetLastError());
+    int64_t *cpos = nullptr;
+    if ((rc = merge_chunk_set(g, nkey, nnew, &cpos))) return rc;
 
     // fold: values of every property (they read the old masks), then the masks
     for (int p = 0; p < g->np; ++p) {

@@ -60,7 +60,9 @@ public:
     void *alloc(size_t bytes);
     void free(void *p);
     void release();
+    void trim();  // hipFree the cached (not live) blocks
     size_t live_bytes() const { return live_; }
+    size_t held_bytes() const;
 
 private:
     std::multimap<size_t, void *> cache_;
@@ -101,6 +103,12 @@ struct sg_grid {
     int64_t *d_ins_key = nullptr;
     uint8_t *d_ins_mask = nullptr;
     void *d_ins_val[sg::kMaxProps] = {};
+    // insert buffer of whole-chunk records (insertBlock), kept as chunks while no point
+    // record is pending: [n] chunk ids, [n][csz] masks and values
+    int64_t cins_n = 0, cins_cap = 0;
+    int64_t *d_cins_id = nullptr;
+    uint8_t *d_cins_mask = nullptr;
+    void *d_cins_val[sg::kMaxProps] = {};
 
     // launch shape of the persistent 8^3 star-stencil kernel: workgroups (4 waves)
     // per XCD; set through sg_set_tuning (the autotuner picks it per workload)
@@ -116,7 +124,7 @@ struct sg_grid {
 
     hipStream_t stream = nullptr;
     sg::DevicePool pool;
-    sg::DevBuf scratch[12];  // flush / sort / scan temporaries
+    sg::DevBuf scratch[13];  // flush / sort / scan temporaries
     sg::DevBuf hostlike;     // small device staging
 
     int nnn(int type) const { return type == SG_NN_STAR ? (1 << dim) : (dim == 2 ? 9 : 27); }
@@ -126,6 +134,8 @@ namespace sg {
 
 // host launchers implemented in the .hip translation units
 int flush_impl(sg_grid *g, const int32_t *ops);
+// pending whole-chunk records -> point records (before point records are appended)
+int convert_chunk_records(sg_grid *g);
 int append_points(sg_grid *g, int64_t n, const int64_t *keys, const uint8_t *masks,
                   const void *const *values, int mem);
 int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *masks,

@@ -38,33 +38,42 @@ def fill_all(G, dim, size, block_edge, props, value_fn, op="smax"):
     return g, coords
 
 
-def random_flush_program(seed, dim, edge, size, dtypes, nflush=3, nrec=200, with_chunks=True, ops=None):
-    """A random insert/flush program (deterministic): list of steps."""
+def random_flush_program(seed, dim, edge, size, dtypes, nflush=3, nrec=200, with_chunks=True, ops=None,
+                         order="points_first"):
+    """A random insert/flush program (deterministic): list of steps.  order: within each
+    group of records, "points_first" (point records, then sometimes chunk records),
+    "chunks_first" (chunk records, then point records) or "chunks_only"."""
     rng = np.random.default_rng(seed)
     cs = edge ** dim
     bsz = [-(-s // edge) for s in size]
     nblocks = int(np.prod(bsz))
     steps = []
+    cmask_choice = np.array([0, 0, 1, 1, 3] + ([2] if order == "chunks_only" else []), dtype=np.uint8)
     for f in range(nflush):
         recs = []
         # point records concentrated in a few chunks so that duplicates occur
         hot = rng.integers(0, nblocks, size=max(1, nblocks // 3))
         for _ in range(rng.integers(1, 4)):
-            n = int(rng.integers(1, nrec))
-            ch = rng.choice(hot, n)
-            off = rng.integers(0, cs, n)
-            keys = ch.astype(np.int64) * cs + off
-            masks = rng.choice(np.array([1, 1, 1, 3, 0, 2], dtype=np.uint8), n)
-            vals = [_rand_vals(rng, dt, n) for dt in dtypes]
-            recs.append(("points", keys, masks, vals))
-            if with_chunks and rng.random() < 0.5:
+            group = []
+            if order != "chunks_only":
+                n = int(rng.integers(1, nrec))
+                ch = rng.choice(hot, n)
+                off = rng.integers(0, cs, n)
+                keys = ch.astype(np.int64) * cs + off
+                masks = rng.choice(np.array([1, 1, 1, 3, 0, 2], dtype=np.uint8), n)
+                vals = [_rand_vals(rng, dt, n) for dt in dtypes]
+                group.append(("points", keys, masks, vals))
+            if with_chunks and (order != "points_first" or rng.random() < 0.5):
                 m = int(rng.integers(1, 4))
                 cids = rng.choice(hot, m).astype(np.int64)
-                cm = rng.choice(np.array([0, 0, 1, 1, 3], dtype=np.uint8), (m, cs))
+                cm = rng.choice(cmask_choice, (m, cs))
                 if rng.random() < 0.3:
                     cm[0, :] = 0  # an all-empty chunk slot still creates the chunk
                 cv = [_rand_vals(rng, dt, m * cs).reshape(m, cs) for dt in dtypes]
-                recs.append(("chunks", cids, cm, cv))
+                group.append(("chunks", cids, cm, cv))
+            if order == "chunks_first":
+                group.reverse()
+            recs.extend(group)
         fops = ops if ops is not None else [
             str(rng.choice(["smax", "smin", "sadd", "sright", "sleft", "none"] +
                            (["sor"] if np.issubdtype(dt, np.integer) else []))) for dt in dtypes]

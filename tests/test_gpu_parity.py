@@ -57,6 +57,30 @@ def test_flush_random_programs(G, oracle, seed):
     S.compare_to_model(a, S.model_flush(dim, edge, size, dtypes, steps, background=bgs), dtypes)
 
 
+@pytest.mark.parametrize("order", ["chunks_only", "chunks_first"])
+@pytest.mark.parametrize("seed", range(6))
+def test_flush_chunk_record_programs(G, oracle, seed, order):
+    """Whole-chunk records flushed as chunks (chunks_only) and turned into point records
+    when points follow them (chunks_first): same chunk ids, masks and existing values as
+    the oracle (a slot that is PADDING without EXIST has no specified value in the
+    reference, BlockMapGpu_kernels.cuh:361-367), and every mask-0 slot holds the
+    background."""
+    rng = np.random.default_rng(400 + seed)
+    dim = 3 if seed % 2 else 2
+    edge = [2, 4, 8][seed % 3]
+    size = [int(rng.integers(edge, 6 * edge)) for _ in range(dim)]
+    dtypes = [[np.float64], [np.int32, np.uint32], [np.float32, np.int64, np.uint8]][seed % 3]
+    steps = S.random_flush_program(seed, dim, edge, size, dtypes, nflush=3, nrec=120, order=order)
+    bgs = [3, 1, 2][: len(dtypes)]
+    a = S.run_flush_program(G, dim, edge, size, dtypes, steps, background=bgs)
+    b = S.run_flush_program(oracle.SparseGridCpu, dim, edge, size, dtypes, steps, background=bgs)
+    S.assert_same_state(a, b)
+    S.compare_to_model(a, S.model_flush(dim, edge, size, dtypes, steps, background=bgs), dtypes)
+    empty = a.masks() == 0
+    for p, dt in enumerate(dtypes):
+        assert np.all(a.prop(p)[empty] == np.array(bgs[p], dt))
+
+
 def test_flush_empty_and_repeat(G, oracle):
     for cls in (G, oracle.SparseGridCpu):
         g = cls(3, (64, 64, 64), [np.int32], block_edge=8)

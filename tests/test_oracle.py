@@ -70,6 +70,20 @@ def test_flush_against_dictionary_model(oracle, seed):
     S.compare_to_model(g, model, dtypes)
 
 
+@pytest.mark.parametrize("order", ["chunks_only", "chunks_first"])
+@pytest.mark.parametrize("seed", range(4))
+def test_flush_chunk_records_against_model(oracle, seed, order):
+    rng = np.random.default_rng(300 + seed)
+    dim = 3 if seed % 2 else 2
+    edge = [2, 4, 8][seed % 3]
+    size = [int(rng.integers(edge, 5 * edge)) for _ in range(dim)]
+    dtypes = [np.float64, np.int32, np.uint32, np.int16][: 1 + seed % 4]
+    steps = S.random_flush_program(seed, dim, edge, size, dtypes, nflush=3, nrec=80, order=order)
+    bgs = [7, 3, 9, 1][: len(dtypes)]
+    g = S.run_flush_program(oracle.SparseGridCpu, dim, edge, size, dtypes, steps, background=bgs)
+    S.compare_to_model(g, S.model_flush(dim, edge, size, dtypes, steps, background=bgs), dtypes)
+
+
 def _brute_neighbours(ids, bsz, dim, nn):
     """Geometric neighbour positions; NNFull reproduces the reference's unguarded -1 at the
     low faces through the plain linear-id formula (grid_smb::BlockLinId)."""
