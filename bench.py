@@ -25,6 +25,11 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "sparse-grid stencil Gpt-updates/s + % HBM roofline; p50/p99 solve latency"
 
 
+def log(msg):
+    """Progress line on stderr (stdout carries only the JSON line)."""
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def chunk_arrays(size, coords, values, edge=8):
     """Point coordinates -> sorted chunk ids, [n, 512] masks and per-property chunk data."""
     bs = [-(-s // edge) for s in size]
@@ -55,13 +60,13 @@ def traffic_from_profile(name):
     return None
 
 
-def run_gpu(args, ids, masks, data, dtype):
+def run_gpu(args, ids, masks, data, dtype, edge=8):
     import torch
-    from openfpm_data_amd import SparseGridGpu
+    from openfpm_data_amd import SparseGridGpu, autotune
     from openfpm_data_amd import workloads as W
 
     torch.cuda.set_device(0)
-    g = SparseGridGpu(3, (512, 512, 512), [dtype] * 4, block_edge=8)
+    g = SparseGridGpu(3, (512, 512, 512), [dtype] * 4, block_edge=edge)
     stream = torch.cuda.Stream()
     g.set_stream(stream)
     d_ids = torch.from_numpy(ids).cuda()
@@ -85,6 +90,15 @@ def run_gpu(args, ids, masks, data, dtype):
         else:
             g.conv2_gray_scott(2, 3, 0, 1, **P)
 
+    tuned = None
+    if edge == 8:
+        # launch shape by measurement (cached per device and size class)
+        tuned, _ = autotune.tune_stencil(g, step, "gray_scott", dtype, stream)
+        g.synchronize()
+        # restart from the initial state: tuning stepped the grid
+        g.set_prop(0, data[0].astype(dtype))
+        g.set_prop(1, data[1].astype(dtype))
+        g.synchronize()
     for i in range(args.warmup):
         step(i)
     g.synchronize()
@@ -100,7 +114,7 @@ def run_gpu(args, ids, masks, data, dtype):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kern_ms = ev0.elapsed_time(ev1) / args.steps
-    res = {"points": npts, "chunks": nch, "wall_s": wall, "kernel_ms": kern_ms,
+    res = {"points": npts, "chunks": nch, "wall_s": wall, "kernel_ms": kern_ms, "tuned": tuned,
            "checksum_u": float(np.float64(g.prop(0 if args.steps % 2 == 0 else 2)[masks == 1].astype(np.float64).sum()))}
     g.close()
     return res
@@ -177,11 +191,11 @@ def run_poisson_secondary():
         torch.cuda.synchronize()
         return gs
 
-    def timed(mode, batched, graph):
+    def timed(mode, batched, graph, tuning=None):
         gs = grids()
         t0 = time.perf_counter()
         if batched:
-            res = P.solve(gs, P.make_params(mode=mode, tol=1e-10, use_graph=graph))
+            res = P.solve(gs, P.make_params(mode=mode, tol=1e-10, use_graph=graph), tuning=tuning)
         else:
             res = [P.solve([g], P.make_params(mode=mode, tol=1e-10, use_graph=graph))[0] for g in gs]
         torch.cuda.synchronize()
@@ -190,8 +204,14 @@ def run_poisson_secondary():
             g.close()
         return el, res
 
-    timed(C.POISSON_MIXED, True, True)  # warm-up (allocator, code objects)
-    t_mix, r_mix = timed(C.POISSON_MIXED, True, True)
+    from openfpm_data_amd import autotune
+    gs = grids()
+    tuned, _ = autotune.tune_batch(gs)
+    for g in gs:
+        g.close()
+    tuning = {C.TUNE_BATCH_WAVES: tuned["waves"], C.TUNE_BATCH_MAX_WGS: tuned["max_wgs"]}
+    timed(C.POISSON_MIXED, True, True, tuning)  # warm-up (allocator, code objects)
+    t_mix, r_mix = timed(C.POISSON_MIXED, True, True, tuning)
     t_plain, r_plain = timed(C.POISSON_PLAIN_FP64, False, False)
     sw = np.array([r["sweeps"] for r in r_mix])
     return {"workload": "poisson_batch256", "grids": len(specs),
@@ -202,10 +222,10 @@ def run_poisson_secondary():
             "sweeps_mean": round(float(sw.mean()), 1), "sweeps_max": int(sw.max()),
             "per_grid_plain_fp64_solves_per_s": round(len(specs) / t_plain, 2),
             "per_grid_plain_fp64_converged": int(sum(r["converged"] for r in r_plain)),
-            "speedup_vs_per_grid_plain_fp64": round(t_plain / t_mix, 3)}
+            "speedup_vs_per_grid_plain_fp64": round(t_plain / t_mix, 3), "tuned": tuned}
 
 
-def run_serving_secondary(n_open=192, n_cap=64, budget_gib=32.0):
+def run_serving_secondary(n_open=192, n_cap=64, budget_gib=32.0, batch_tuning=None):
     """Served Poisson requests (edge log-uniform in [32, 384], 5-40% of the chunks active,
     FP32 f, tolerance 1e-10 or 1e-6, seed 4), mixed method, HBM budget `budget_gib`:
     capacity measured closed-loop (first `n_cap` requests all present at t = 0), then
@@ -215,7 +235,7 @@ def run_serving_secondary(n_open=192, n_cap=64, budget_gib=32.0):
 
     reqs = W.serving_requests(n_open)
     budget = int(budget_gib * (1 << 30))
-    srv = S.PoissonServer(budget)
+    srv = S.PoissonServer(budget, tuning=batch_tuning)
     srv.run(reqs[:8], np.zeros(8))  # warm-up
     _, cap = srv.run(reqs[:n_cap], np.zeros(n_cap))
     rate = 0.8 * cap["solves_per_s"]
@@ -230,12 +250,12 @@ def run_serving_secondary(n_open=192, n_cap=64, budget_gib=32.0):
             "max_live": s["max_live"], "peak_reserved_gib": round(s["peak_reserved_bytes"] / (1 << 30), 3)}
 
 
-def run_cpu_baseline(ids, masks, data, seconds, threads):
+def run_cpu_baseline(ids, masks, data, seconds, threads, edge=8):
     from oracle import pysgcpu
     from openfpm_data_amd import workloads as W
 
     pysgcpu.set_threads(threads)
-    g = pysgcpu.SparseGridCpu(3, (512, 512, 512), [np.float32] * 4, block_edge=8)
+    g = pysgcpu.SparseGridCpu(3, (512, 512, 512), [np.float32] * 4, block_edge=edge)
     z = np.zeros_like(data[0], dtype=np.float32)
     g.insert_block(ids, masks, [data[0].astype(np.float32), data[1].astype(np.float32), z, z])
     g.flush(["sright"] * 4)
@@ -287,16 +307,31 @@ def main():
         sys.exit(2)
 
     from openfpm_data_amd import workloads as W
+    from openfpm_data_amd import autotune
     t_gen = time.perf_counter()
     w = W.shell512_gs()
-    ids, masks, data = chunk_arrays(w["size"], w["coords"], [w["u"].astype(np.float32), w["v"].astype(np.float32)])
-    del w
     gen_s = time.perf_counter() - t_gen
     dtype = np.float32 if args.dtype == "f32" else np.float64
-    r = run_gpu(args, ids, masks, data, dtype)
+    P_GS = W.GS_PARAMS
+
+    def gs_step(g, i):
+        if i % 2 == 0:
+            g.conv2_gray_scott(0, 1, 2, 3, **P_GS)
+        else:
+            g.conv2_gray_scott(2, 3, 0, 1, **P_GS)
+
+    log("workload generated in %.1f s; tuning the chunk edge" % gen_s)
+    # chunk edge by measurement (cached): one Gray-Scott step per point at edges 4 and 8
+    edge, _ = autotune.tune_chunk_edge(w["size"], w["coords"], [w["u"], w["v"]], dtype, gs_step)
+    ids, masks, data = chunk_arrays(w["size"], w["coords"], [w["u"].astype(np.float32), w["v"].astype(np.float32)],
+                                    edge=edge)
+    del w
+    log("chunk edge %d; stencil run" % edge)
+    r = run_gpu(args, ids, masks, data, dtype, edge=edge)
+    log("stencil %.4f ms/step" % r["kernel_ms"])
 
     npts, nch = r["points"], r["chunks"]
-    slots = nch * 512
+    slots = nch * edge ** 3
     tsz = np.dtype(dtype).itemsize
     # algorithmic bytes of one step: per slot u, v read + u', v' written + mask byte;
     # per chunk the 6 face-neighbour entries of the neighbour table
@@ -309,16 +344,24 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        c = run_cpu_baseline(ids, masks, data, args.cpu_seconds, threads)
+        log("cpu baseline (%d threads)" % threads)
+        c = run_cpu_baseline(ids, masks, data, args.cpu_seconds, threads, edge)
         cpu = {"value": round(c["value"], 6), "unit": "Gpt-updates/s", "cores": threads, "kind": "port",
                "sample": "oracle (oracle/sgcpu.cpp, OpenMP) on the same 512^3 shell FP32 Gray-Scott grid, "
                          "%d steps in %.1f s" % (c["steps"], c["seconds"])}
 
     secondary = None
     if not args.no_secondary:
+        log("flush workload")
         secondary = {"flush": run_flush_secondary(args.cpu_threads or min(16, os.cpu_count() or 1), not args.no_cpu),
-                     "poisson": run_poisson_secondary(),
-                     "serving": run_serving_secondary()}
+                     "poisson": None}
+        log("batched Poisson workload")
+        secondary["poisson"] = run_poisson_secondary()
+        tb = secondary["poisson"]["tuned"]
+        from openfpm_data_amd import _capi as C
+        log("served Poisson workload")
+        secondary["serving"] = run_serving_secondary(
+            batch_tuning={C.TUNE_BATCH_WAVES: tb["waves"], C.TUNE_BATCH_MAX_WGS: tb["max_wgs"]})
 
     line = {
         "metric": METRIC,
@@ -334,7 +377,7 @@ def main():
         "dtype": args.dtype,
         "data": "synthetic",
         "config": {"workload": "shell512_gray_scott", "domain": [512, 512, 512], "shell_radius": [200, 232],
-                   "points": npts, "chunks": nch, "chunk_edge": 8, "properties": 2,
+                   "points": npts, "chunks": nch, "chunk_edge": edge, "properties": 2, "tuned": r["tuned"],
                    "params": W.GS_PARAMS},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

@@ -92,6 +92,19 @@ template <unsigned int dim, typename T> class Box
 
 public:
     Box() : lo{}, hi{} {}
+    Box(std::initializer_list<T> l, std::initializer_list<T> h) : lo{}, hi{}
+    {
+        unsigned int i = 0;
+        for (T v : l) if (i < dim) lo[i++] = v;
+        i = 0;
+        for (T v : h) if (i < dim) hi[i++] = v;
+    }
+    template <typename P> bool isInside(const P &p) const
+    {
+        for (unsigned int i = 0; i < dim; ++i)
+            if ((T)p.get(i) < lo[i] || (T)p.get(i) > hi[i]) return false;
+        return true;
+    }
     void setLow(unsigned int i, T v) { lo[i] = v; }
     void setHigh(unsigned int i, T v) { hi[i] = v; }
     T getLow(unsigned int i) const { return lo[i]; }
@@ -229,6 +242,47 @@ template <unsigned int dim, unsigned int p_src, unsigned int p_dst> struct Conv3
 };
 
 // ---- the grid ---------------------------------------------------------------------------
+
+// Sub-grid iterator of the reference (SparseGridGpu_iterator_sub, getIterator(start, stop)),
+// here only the inclusive box it spans: pack/unpack take their boxes from it.
+template <unsigned int dim> class SparseGridGpu_iterator_sub
+{
+public:
+    SparseGridGpu_iterator_sub(const grid_key_dx<dim, int> &start, const grid_key_dx<dim, int> &stop)
+        : start_(start), stop_(stop) {}
+    const grid_key_dx<dim, int> &getStart() const { return start_; }
+    const grid_key_dx<dim, int> &getStop() const { return stop_; }
+
+private:
+    grid_key_dx<dim, int> start_, stop_;
+};
+
+// Host pack memory (the reference's ExtPreAlloc<CudaMemory> over a preallocated buffer):
+// resize(req) after packCalculate, then pack/packFinalize write into it.
+class PackMemory
+{
+public:
+    void resize(size_t n) { buf_.assign(n, 0); }
+    size_t size() const { return buf_.size(); }
+    unsigned char *getPointer() { return buf_.data(); }
+    const unsigned char *getPointer() const { return buf_.data(); }
+
+private:
+    std::vector<unsigned char> buf_;
+};
+
+// read position in a PackMemory (the reference's Pack_stat / Unpack_stat)
+class Pack_stat
+{
+public:
+    size_t getOffset() const { return off_; }
+    void addOffset(size_t n) { off_ += n; }
+    void reset() { off_ = 0; }
+
+private:
+    size_t off_ = 0;
+};
+typedef Pack_stat Unpack_stat;
 
 template <unsigned int dim> struct default_edge { static constexpr unsigned int value = dim == 3 ? 8 : (dim == 2 ? 16 : 256); };
 
@@ -438,7 +492,96 @@ public:
         deviation = nb > 1 ? std::sqrt(v / (nb - 1)) : 0.0;
     }
 
+    // -- pack / unpack / remove (SparseGridGpu.hpp:3040-3733) --
+    typedef SparseGridGpu_iterator_sub<dim> sub_iterator;
+    sub_iterator getIterator(const grid_key_dx<dim, int> &start, const grid_key_dx<dim, int> &stop) const
+    {
+        return sub_iterator(start, stop);
+    }
+    template <typename CoordT>
+    sub_iterator getIterator(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop) const
+    {
+        grid_key_dx<dim, int> a, b;
+        for (unsigned int d = 0; d < dim; ++d) {
+            a.set_d(d, (int)start.get(d));
+            b.set_d(d, (int)stop.get(d));
+        }
+        return sub_iterator(a, b);
+    }
+    void packReset()
+    {
+        pack_lo_.clear();
+        pack_hi_.clear();
+    }
+    // queue a box to pack (req is left as is, as in the reference)
+    template <int... prp> void packRequest(const sub_iterator &it, size_t &)
+    {
+        for (unsigned int d = 0; d < dim; ++d) {
+            pack_lo_.push_back(it.getStart().get(d));
+            pack_hi_.push_back(it.getStop().get(d));
+        }
+    }
+    // add the bytes of every queued box to req
+    template <int... prp> void packCalculate(size_t &req, gpu::ofp_context_t &)
+    {
+        const int32_t props[] = {prp..., 0};
+        int64_t b = 0;
+        sg::check(sg_pack_size(h_, npack(), pack_lo_.data(), pack_hi_.data(), (int)sizeof...(prp), props,
+                               (int)sizeof(indexT), &b));
+        req += (size_t)b;
+    }
+    // the reference reserves the box's space here; the boxes are written by packFinalize
+    template <int... prp> void pack(PackMemory &, const sub_iterator &, Pack_stat &) {}
+    template <int... prp> void packFinalize(PackMemory &mem, Pack_stat &sts, int = 0, bool = false)
+    {
+        const int32_t props[] = {prp..., 0};
+        int64_t b = 0;
+        sg::check(sg_pack_size(h_, npack(), pack_lo_.data(), pack_hi_.data(), (int)sizeof...(prp), props,
+                               (int)sizeof(indexT), &b));
+        if (sts.getOffset() + (size_t)b > mem.size()) throw sg::error("pack memory too small");
+        sg::check(sg_pack(h_, npack(), pack_lo_.data(), pack_hi_.data(), (int)sizeof...(prp), props,
+                          (int)sizeof(indexT), mem.getPointer() + sts.getOffset(), b, SG_MEM_HOST));
+        sts.addOffset((size_t)b);
+    }
+    // queue a box whose existing points the next removeAddUnpackFinalize removes
+    template <typename CoordT> void remove(const Box<dim, CoordT> &section)
+    {
+        int32_t lo[dim], hi[dim];
+        for (unsigned int d = 0; d < dim; ++d) {
+            lo[d] = (int32_t)section.getLow(d);
+            hi[d] = (int32_t)section.getHigh(d);
+        }
+        sg::check(sg_remove_box(h_, lo, hi));
+    }
+    // the device remove(coord) of the reference for a batch of points
+    void removePoints(const indexT *keys, size_t n)
+    {
+        std::vector<int64_t> k(keys, keys + n);
+        sg::check(sg_remove_points(h_, (int64_t)n, k.data(), SG_MEM_HOST));
+    }
+    void removeAddUnpackReset() { sg::check(sg_remove_add_unpack_reset(h_)); }
+    // unpack the next packed box of mem (at ps) so that it starts at sub's start
+    template <unsigned int... prp>
+    void unpack(PackMemory &mem, const sub_iterator &sub, Unpack_stat &ps, gpu::ofp_context_t &)
+    {
+        const int32_t props[] = {(int32_t)prp..., 0};
+        int32_t lo[dim];
+        for (unsigned int d = 0; d < dim; ++d) lo[d] = sub.getStart().get(d);
+        int64_t used = 0;
+        sg::check(sg_unpack(h_, mem.getPointer() + ps.getOffset(), (int64_t)(mem.size() - ps.getOffset()),
+                            SG_MEM_HOST, lo, (int)sizeof...(prp), props, (int)sizeof(indexT), &used));
+        ps.addOffset((size_t)used);
+    }
+    template <unsigned int... prp> void removeAddUnpackFinalize(gpu::ofp_context_t &, int)
+    {
+        const int32_t props[] = {(int32_t)prp..., 0};
+        sg::check(sg_remove_add_unpack_finalize(h_, (int)sizeof...(prp), props));
+    }
+
 private:
+    int npack() const { return (int)(pack_lo_.size() / dim); }
+    std::vector<int32_t> pack_lo_, pack_hi_;
+
     template <size_t... I> static void fill_dtypes(int32_t *dts, std::index_sequence<I...>)
     {
         int dummy[] = {0, (dts[I] = sg::dtype_of<typename std::tuple_element<I, props_tuple>::type>::value, 0)...};

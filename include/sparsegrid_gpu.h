@@ -227,6 +227,13 @@ int sg_batch_destroy(sg_batch *b);
 /* tol > 0: this grid's relative residual target (else the batch's p->tol) */
 int sg_batch_add(sg_batch *b, sg_grid *g, int64_t tag, double tol);
 int sg_batch_live(sg_batch *b);
+/* launch shape of the batched sweep kernel: waves (64 lanes, one chunk tile each) per
+ * workgroup, 1 / 2 / 4 / 8 (LDS tile per wave: 3.2 KB FP32, 6.4 KB FP64), default 4 */
+#define SG_TUNE_BATCH_WAVES 5
+/* at most this many workgroups per launch, waves then loop over tiles; 0 (default): one
+ * tile per wave */
+#define SG_TUNE_BATCH_MAX_WGS 6
+int sg_batch_set_tuning(sg_batch *b, int key, int64_t value);
 int sg_batch_step(sg_batch *b);
 int sg_batch_poll(sg_batch *b, int64_t *tags, sg_poisson_result *results, int max, int *n);
 /* solve all grids to p->tol in one batch (results[i] for grids[i]) */
@@ -272,6 +279,42 @@ int sg_device_ptr(sg_grid *g, int prop, void **out);
 /* countExistingElements / countBoundaryElements (src/SparseGridGpu/SparseGridGpu.hpp:2422-2490) */
 int sg_count_existing(sg_grid *g, int64_t *out);
 int sg_count_boundary(sg_grid *g, int64_t *out);
+
+/* ---- pack / unpack / remove ---------------------------------------------------
+ * Packed form of one box (packCalculate + pack + packFinalize,
+ * src/SparseGridGpu/SparseGridGpu.hpp:3072-3340; pack_data, SparseGridGpu_kernels.cuh:1658-1697),
+ * sections in this order, A = index_bytes = sizeof(indexT) (4 or 8):
+ *   uint64 n_cnk | int32 box_start[dim] | int32 grid_size[dim] | indexT chunk_ids[n_cnk]
+ *   | uint32 scan[n_cnk + 1] (first point of each chunk, scan[n_cnk] = n_pnt), padded to A
+ *   | per packed property T[n_pnt], one after the other, padded to A
+ *   | int16 offsets[n_pnt], padded to A | uint8 masks[n_pnt], padded to A
+ * Boxes are inclusive [lo, hi] ([nbox][dim] corners, Box::isInside); only existing points
+ * (EXIST) are packed; chunks without such a point are left out.  An array property of the
+ * reference (T[N]) is N consecutive scalar properties here (same bytes). */
+/* packRequest + packCalculate: total bytes of the packed boxes */
+int sg_pack_size(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, int nprop,
+                 const int32_t *props, int index_bytes, int64_t *bytes);
+/* pack + packFinalize: write the packed boxes (in box order) to buf (cap bytes) */
+int sg_pack(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, int nprop,
+            const int32_t *props, int index_bytes, void *buf, int64_t cap, int mem);
+/* unpack (SparseGridGpu.hpp:3665-3733): read the packed box at buf; its points land at
+ * (packed coordinate - box_start + dst_lo) and are queued for
+ * sg_remove_add_unpack_finalize; *consumed = bytes of the section (the next box starts
+ * there).  A point landing outside the grid fails the call (SG_E_RANGE). */
+int sg_unpack(sg_grid *g, const void *buf, int64_t avail, int mem, const int32_t *dst_lo, int nprop,
+              const int32_t *props, int index_bytes, int64_t *consumed);
+/* remove(Box) (SparseGridGpu.hpp:3528-3531): queue an inclusive box whose existing points
+ * the next sg_remove_add_unpack_finalize removes (chunks stay) */
+int sg_remove_box(sg_grid *g, const int32_t *lo, const int32_t *hi);
+/* device remove(coord) (SparseGridGpu_ker.cuh:665-668, BlockMapGpu_ker.cuh:322-332): the
+ * given points get mask 0 now (value -> background); keys of absent chunks are ignored */
+int sg_remove_points(sg_grid *g, int64_t n, const int64_t *keys, int mem);
+/* removeAddUnpackFinalize (SparseGridGpu.hpp:3492-3499): remove the points of the queued
+ * boxes, then merge the pending records: the listed (unpacked) properties take the new
+ * values, the others keep theirs, mask |= EXIST */
+int sg_remove_add_unpack_finalize(sg_grid *g, int nprop, const int32_t *props);
+/* removeAddUnpackReset (SparseGridGpu.hpp:3362-3383): drop queued boxes and pending records */
+int sg_remove_add_unpack_reset(sg_grid *g);
 
 /* ---- device memory (no counterpart in the reference: its vectors keep their
  *      capacity; a server admitting grids against an HBM budget needs both) ---- */

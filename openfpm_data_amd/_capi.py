@@ -40,6 +40,8 @@ TUNE_STENCIL_WG_PER_XCD = 1
 TUNE_STENCIL_VARIANT = 2
 TUNE_STENCIL_STORES = 3
 TUNE_STENCIL_SKIP_EMPTY_ROWS = 4
+TUNE_BATCH_WAVES = 5
+TUNE_BATCH_MAX_WGS = 6
 
 EXIST = 1
 PADDING = 2
@@ -157,12 +159,20 @@ _SIGS = {
     "sg_device_ptr": (_INT, [_P, _INT, ctypes.POINTER(_P)]),
     "sg_count_existing": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_count_boundary": (_INT, [_P, ctypes.POINTER(_I64)]),
+    "sg_pack_size": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),
+    "sg_pack": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, _P, _I64, _INT]),
+    "sg_unpack": (_INT, [_P, _P, _I64, _INT, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),
+    "sg_remove_box": (_INT, [_P, _P, _P]),
+    "sg_remove_points": (_INT, [_P, _I64, _P, _INT]),
+    "sg_remove_add_unpack_finalize": (_INT, [_P, _INT, _P]),
+    "sg_remove_add_unpack_reset": (_INT, [_P]),
     "sg_trim": (_INT, [_P]),
     "sg_device_bytes": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_batch_create": (_INT, [ctypes.POINTER(PoissonParams), _P, ctypes.POINTER(_P)]),
     "sg_batch_destroy": (_INT, [_P]),
     "sg_batch_add": (_INT, [_P, _P, _I64, ctypes.c_double]),
     "sg_batch_live": (_INT, [_P]),
+    "sg_batch_set_tuning": (_INT, [_P, _INT, _I64]),
     "sg_batch_step": (_INT, [_P]),
     "sg_batch_poll": (_INT, [_P, ctypes.POINTER(_I64), ctypes.POINTER(PoissonResult), _INT, ctypes.POINTER(_INT)]),
     "sg_poisson_solve": (_INT, [ctypes.POINTER(_P), _INT, ctypes.POINTER(PoissonParams), ctypes.POINTER(PoissonResult)]),

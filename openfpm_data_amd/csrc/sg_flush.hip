@@ -702,6 +702,16 @@ int convert_chunk_records(sg_grid *g)
     return SG_OK;
 }
 
+int reserve_point_records(sg_grid *g, int64_t n, int64_t *at)
+{
+    int rc = convert_chunk_records(g);
+    if (rc) return rc;
+    rc = grow_insert_buffer(g, g->ins_n + n);
+    if (rc) return rc;
+    *at = g->ins_n;
+    return SG_OK;
+}
+
 int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t *masks, const void *const *values,
                   int mem)
 {

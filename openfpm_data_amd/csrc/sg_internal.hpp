@@ -122,6 +122,9 @@ struct sg_grid {
     // read the mask first and skip loading 8-point rows without points
     int fast_lskip = 0;
 
+    // boxes queued by sg_remove_box, [k][dim] inclusive corners
+    std::vector<int32_t> rem_lo, rem_hi;
+
     hipStream_t stream = nullptr;
     sg::DevicePool pool;
     sg::DevBuf scratch[13];  // flush / sort / scan temporaries
@@ -136,6 +139,8 @@ namespace sg {
 int flush_impl(sg_grid *g, const int32_t *ops);
 // pending whole-chunk records -> point records (before point records are appended)
 int convert_chunk_records(sg_grid *g);
+// room for n more point records at *at of the insert buffer (chunk records converted first)
+int reserve_point_records(sg_grid *g, int64_t n, int64_t *at);
 int append_points(sg_grid *g, int64_t n, const int64_t *keys, const uint8_t *masks,
                   const void *const *values, int mem);
 int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *masks,

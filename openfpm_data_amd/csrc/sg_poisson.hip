@@ -43,18 +43,6 @@ struct PGrid {
     int32_t nsweeps;      // sweeps of this grid in the current inner block
 };
 
-__device__ inline int find_grid(const int64_t *prefix, int ng, int64_t t)
-{
-    // largest g with prefix[g] <= t
-    int lo = 0, hi = ng;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= t) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 template <typename T> struct R8 {
     T v[8];
 };
@@ -142,25 +130,15 @@ __device__ inline void neighbour_rows(const T *tile, int lane, R8<T> &ym, R8<T> 
     ld8(zp, tile + ((z + 2) * 10 + y + 1) * 8);
 }
 
-// One colour of a red-black SOR sweep over all tiles: x += w ((rhs + sum nb) / 6 - x) on
-// the points of colour `color` ((x + y + z) & 1; chunk origins are multiples of 8, so the
-// local coordinates decide).  zero_x: x is taken as 0 everywhere (first half sweep of an
-// inner solve): points of the other colour are written 0.  Whole rows are stored; values
-// of the other colour are written back unchanged (no other wave changes them in this
-// half sweep).
+// One colour of a red-black SOR sweep on one tile (chunk c of grid g): x += w ((rhs + sum
+// nb) / 6 - x) on the points of colour `color` ((x + y + z) & 1; chunk origins are
+// multiples of 8, so the local coordinates decide).  zero_x: x is taken as 0 everywhere
+// (first half sweep of an inner solve): points of the other colour are written 0.  Whole
+// rows are stored; values of the other colour are written back unchanged (no other wave
+// changes them in this half sweep).
 template <typename T>
-__global__ __launch_bounds__(256) void rbsor_batch_kernel(const PGrid *__restrict__ grids,
-                                                          const int64_t *__restrict__ prefix, int ng, int64_t ntiles,
-                                                          int color, int zero_x, int sweep)
+__device__ inline void sweep_tile(const PGrid &g, int64_t c, T *tile, int lane, int color, int zero_x)
 {
-    __shared__ __attribute__((aligned(16))) T tiles[4][100 * 8];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t t = (int64_t)blockIdx.x * 4 + w;
-    if (t >= ntiles) return;
-    const int gi = find_grid(prefix, ng, t);
-    const PGrid g = grids[gi];
-    if (sweep >= g.nsweeps) return;  // this grid's inner block is shorter
-    const int64_t c = t - prefix[gi];
     T *x = (T *)g.x;
     const T *rhs = (const T *)g.rhs;
     const uint64_t mrow = *(const uint64_t *)(g.mask + c * 512 + lane * 8);
@@ -176,7 +154,6 @@ __global__ __launch_bounds__(256) void rbsor_batch_kernel(const PGrid *__restric
         const int32_t *row = g.nn + c * 8;
 #pragma unroll
         for (int k = 0; k < 6; ++k) nbp[k] = row[k];
-        T *tile = tiles[w];
         stage<T>(x, c, nbp, tile, lane, own, xm, xp);
         wave_sync_lds();
         neighbour_rows<T>(tile, lane, ym, yp, zm, zp);
@@ -201,6 +178,36 @@ __global__ __launch_bounds__(256) void rbsor_batch_kernel(const PGrid *__restric
         out.v[i] = (active && mine) ? nv : cur;
     }
     if (mrow != 0) st8(x + c * 512 + lane * 8, out);
+    // the next tile of this wave restages the LDS tile: finish reading it first
+    wave_sync_lds();
+}
+
+// One colour of a red-black SOR sweep over the live grids.  Grids are ordered by
+// decreasing sweep count, so the tiles of the grids still sweeping at sweep `sweep` are
+// [0, act[sweep]); tile t belongs to grid tile_grid[t].  Wave v takes tiles v, v + V, ...
+// (V = waves in the launch), so a launch of any size covers the active tiles.
+template <typename T, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void rbsor_batch_kernel(const PGrid *__restrict__ grids,
+                                                                const int64_t *__restrict__ prefix,
+                                                                const int32_t *__restrict__ tile_grid,
+                                                                const int64_t *__restrict__ act, int color,
+                                                                int zero_x, int sweep)
+{
+    __shared__ __attribute__((aligned(16))) T tiles[WAVES][100 * 8];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t nact = act[sweep];
+    const int64_t nwaves = (int64_t)gridDim.x * WAVES;
+    for (int64_t t = (int64_t)blockIdx.x * WAVES + w; t < nact; t += nwaves) {
+        const int gi = tile_grid[t];
+        sweep_tile<T>(grids[gi], t - prefix[gi], tiles[w], lane, color, zero_x);
+    }
+}
+
+// tile -> grid map of a work list
+__global__ __launch_bounds__(256) void tile_grid_kernel(const int64_t *__restrict__ prefix, int32_t *__restrict__ tg)
+{
+    const int gi = blockIdx.x;
+    for (int64_t t = prefix[gi] + threadIdx.x; t < prefix[gi + 1]; t += 256) tg[t] = gi;
 }
 
 // Defect correction + residual, FP64: u_out = u_in + e, r = f - A u_out, r32 = (float) r;
@@ -208,15 +215,15 @@ __global__ __launch_bounds__(256) void rbsor_batch_kernel(const PGrid *__restric
 // fixed order afterwards, so the norms are deterministic).  e == nullptr: e = 0,
 // u_out == nullptr: residual only.
 __global__ __launch_bounds__(256) void defect_batch_kernel(const PGrid *__restrict__ grids,
-                                                           const int64_t *__restrict__ prefix, int ng, int64_t ntiles,
+                                                           const int64_t *__restrict__ prefix,
+                                                           const int32_t *__restrict__ tile_grid, int64_t ntiles,
                                                            double *__restrict__ partial)
 {
     __shared__ __attribute__((aligned(16))) double tiles[4][100 * 8];
-    __shared__ double red[4][2];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t t = (int64_t)blockIdx.x * 4 + w;
     if (t >= ntiles) return;
-    const int gi = find_grid(prefix, ng, t);
+    const int gi = tile_grid[t];
     const PGrid g = grids[gi];
     const int64_t c = t - prefix[gi];
     int64_t nbp[6];
@@ -292,7 +299,6 @@ __global__ __launch_bounds__(256) void defect_batch_kernel(const PGrid *__restri
         partial[2 * t] = s_r;
         partial[2 * t + 1] = s_f;
     }
-    (void)red;
 }
 
 // per grid: sum of its tiles' partials in a fixed order (strided lanes + fixed tree)
@@ -348,9 +354,13 @@ struct sg_batch {
     std::vector<std::pair<int64_t, sg_poisson_result>> done;
     // device work list (rebuilt when the live set changes)
     sg::DevicePool pool;
-    sg::DevBuf d_grids, d_prefix, d_partial, d_norms;
+    sg::DevBuf d_grids, d_prefix, d_partial, d_norms, d_tg, d_act;
     bool dirty = true;
-    std::vector<int64_t> h_prefix;
+    std::vector<int64_t> h_prefix, h_act;
+    // launch shape of the sweep kernel: waves per workgroup (1, 2, 4, 8) and a cap on the
+    // workgroups of one launch (0: one tile per wave); set by sg_batch_set_tuning
+    int waves = 4;
+    int64_t max_wgs = 0;
     // graph of the inner sweep sequence for the current live set
     hipGraphExec_t graph = nullptr;
     int graph_sweeps = -1;
@@ -366,14 +376,18 @@ namespace {
 
 using namespace sg;
 
-int upload_worklist(sg_batch *b, bool defect_view)
+// Device work list of the live grids in the order `order` (indices into live): grid
+// descriptors, prefix sums of chunk counts, the tile -> grid map and, for the sweeps, the
+// active tile count per sweep index (order must then be by decreasing inner count).
+int upload_worklist(sg_batch *b, bool defect_view, const std::vector<int> &order)
 {
-    const int ng = (int)b->live.size();
+    const int ng = (int)order.size();
     std::vector<PGrid> gd(ng);
     b->h_prefix.assign(ng + 1, 0);
     const bool mixed = b->p.mode == SG_POISSON_MIXED;
+    int max_inner = 0;
     for (int i = 0; i < ng; ++i) {
-        sg_batch::Entry &en = b->live[i];
+        sg_batch::Entry &en = b->live[order[i]];
         sg_grid *g = en.g;
         PGrid &d = gd[i];
         d.nn = g->d_nn[SG_NN_STAR];
@@ -400,38 +414,73 @@ int upload_worklist(sg_batch *b, bool defect_view)
             d.r32 = nullptr;
         }
         b->h_prefix[i + 1] = b->h_prefix[i] + g->nchunks;
+        max_inner = std::max(max_inner, en.inner);
     }
+    // act[s] = tiles of the grids with more than s sweeps (a prefix of the ordered list)
+    b->h_act.assign(std::max(max_inner, 1), 0);
+    if (!defect_view) {
+        int k = ng;
+        for (int sw = 0; sw < max_inner; ++sw) {
+            while (k > 0 && gd[k - 1].nsweeps <= sw) --k;
+            b->h_act[sw] = b->h_prefix[k];
+        }
+    }
+    const int64_t nt = b->h_prefix[ng];
     int rc;
     if ((rc = ensure(b->pool, b->d_grids, sizeof(PGrid) * std::max(ng, 1))) ||
         (rc = ensure(b->pool, b->d_prefix, sizeof(int64_t) * (ng + 1))) ||
-        (rc = ensure(b->pool, b->d_partial, sizeof(double) * 2 * std::max<int64_t>(b->h_prefix[ng], 1))) ||
-        (rc = ensure(b->pool, b->d_norms, sizeof(double) * 2 * std::max(ng, 1))))
+        (rc = ensure(b->pool, b->d_partial, sizeof(double) * 2 * std::max<int64_t>(nt, 1))) ||
+        (rc = ensure(b->pool, b->d_norms, sizeof(double) * 2 * std::max(ng, 1))) ||
+        (rc = ensure(b->pool, b->d_tg, sizeof(int32_t) * std::max<int64_t>(nt, 1))) ||
+        (rc = ensure(b->pool, b->d_act, sizeof(int64_t) * b->h_act.size())))
         return rc;
     SG_HIP(hipMemcpyAsync(b->d_grids.p, gd.data(), sizeof(PGrid) * ng, hipMemcpyHostToDevice, b->stream));
     SG_HIP(hipMemcpyAsync(b->d_prefix.p, b->h_prefix.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice,
                           b->stream));
+    SG_HIP(hipMemcpyAsync(b->d_act.p, b->h_act.data(), sizeof(int64_t) * b->h_act.size(), hipMemcpyHostToDevice,
+                          b->stream));
+    if (ng > 0) {
+        hipLaunchKernelGGL(tile_grid_kernel, dim3((unsigned)ng), dim3(256), 0, b->stream,
+                           (const int64_t *)b->d_prefix.p, (int32_t *)b->d_tg.p);
+        SG_HIP(hipGetLastError());
+    }
     // host vectors are reused on the next call: wait for the copies
     SG_HIP(hipStreamSynchronize(b->stream));
     return SG_OK;
 }
 
+template <typename T, int W> int launch_sweep_kernel(sg_batch *b, unsigned nblk, int color, int zx, int s)
+{
+    hipLaunchKernelGGL((rbsor_batch_kernel<T, W>), dim3(nblk), dim3(64 * W), 0, b->stream,
+                       (const PGrid *)b->d_grids.p, (const int64_t *)b->d_prefix.p, (const int32_t *)b->d_tg.p,
+                       (const int64_t *)b->d_act.p, color, zx, s);
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+template <typename T> int launch_sweep(sg_batch *b, unsigned nblk, int color, int zx, int s)
+{
+    switch (b->waves) {
+    case 1: return launch_sweep_kernel<T, 1>(b, nblk, color, zx, s);
+    case 2: return launch_sweep_kernel<T, 2>(b, nblk, color, zx, s);
+    case 8: return launch_sweep_kernel<T, 8>(b, nblk, color, zx, s);
+    default: return launch_sweep_kernel<T, 4>(b, nblk, color, zx, s);
+    }
+}
+
 int launch_sweeps(sg_batch *b, int sweeps, bool first_zero)
 {
-    const int ng = (int)b->live.size();
-    const int64_t nt = b->h_prefix[ng];
+    const int64_t nt = b->h_act.empty() ? 0 : b->h_act[0];
     if (nt == 0) return SG_OK;
-    const unsigned nblk = (unsigned)((nt + 3) / 4);
+    int64_t nblk = (nt + b->waves - 1) / b->waves;
+    if (b->max_wgs > 0) nblk = std::min(nblk, b->max_wgs);
     const bool mixed = b->p.mode == SG_POISSON_MIXED;
     for (int s = 0; s < sweeps; ++s) {
         for (int color = 0; color < 2; ++color) {
             const int zx = (first_zero && s == 0 && color == 0) ? 1 : 0;
-            if (mixed)
-                hipLaunchKernelGGL(rbsor_batch_kernel<float>, dim3(nblk), dim3(256), 0, b->stream,
-                                   (const PGrid *)b->d_grids.p, (const int64_t *)b->d_prefix.p, ng, nt, color, zx, s);
-            else
-                hipLaunchKernelGGL(rbsor_batch_kernel<double>, dim3(nblk), dim3(256), 0, b->stream,
-                                   (const PGrid *)b->d_grids.p, (const int64_t *)b->d_prefix.p, ng, nt, color, zx, s);
-            SG_HIP(hipGetLastError());
+            int rc = mixed ? launch_sweep<float>(b, (unsigned)nblk, color, zx, s)
+                           : launch_sweep<double>(b, (unsigned)nblk, color, zx, s);
+            if (rc) return rc;
         }
     }
     return SG_OK;
@@ -465,14 +514,16 @@ int run_sweeps(sg_batch *b, int sweeps, bool first_zero)
 int defect_and_norms(sg_batch *b, std::vector<double> &norms, bool defect_view)
 {
     const int ng = (int)b->live.size();
-    int rc = upload_worklist(b, defect_view);
+    std::vector<int> order(ng);
+    for (int i = 0; i < ng; ++i) order[i] = i;
+    int rc = upload_worklist(b, defect_view, order);
     if (rc) return rc;
     const int64_t nt = b->h_prefix[ng];
     norms.assign(2 * ng, 0.0);
     if (nt > 0) {
         hipLaunchKernelGGL(defect_batch_kernel, dim3((unsigned)((nt + 3) / 4)), dim3(256), 0, b->stream,
-                           (const PGrid *)b->d_grids.p, (const int64_t *)b->d_prefix.p, ng, nt,
-                           (double *)b->d_partial.p);
+                           (const PGrid *)b->d_grids.p, (const int64_t *)b->d_prefix.p, (const int32_t *)b->d_tg.p,
+                           nt, (double *)b->d_partial.p);
         SG_HIP(hipGetLastError());
         hipLaunchKernelGGL(norm_reduce_kernel, dim3((unsigned)ng), dim3(256), 0, b->stream,
                            (const double *)b->d_partial.p, (const int64_t *)b->d_prefix.p, ng, (double *)b->d_norms.p);
@@ -587,6 +638,24 @@ int sg_batch_add(sg_batch *b, sg_grid *g, int64_t tag, double tol)
 
 int sg_batch_live(sg_batch *b) { return b ? (int)b->live.size() : 0; }
 
+int sg_batch_set_tuning(sg_batch *b, int key, int64_t value)
+{
+    if (!b) return fail(SG_E_ARG, "null batch");
+    switch (key) {
+    case SG_TUNE_BATCH_WAVES:
+        if (value != 1 && value != 2 && value != 4 && value != 8) return fail(SG_E_ARG, "waves must be 1, 2, 4 or 8");
+        b->waves = (int)value;
+        break;
+    case SG_TUNE_BATCH_MAX_WGS:
+        if (value < 0) return fail(SG_E_ARG, "workgroup cap must be >= 0");
+        b->max_wgs = value;
+        break;
+    default: return fail(SG_E_ARG, "unknown tuning key");
+    }
+    ++b->version;  // the captured sweep graph has the old launch shape
+    return SG_OK;
+}
+
 // One outer iteration for every live grid: residual (+ defect correction in the mixed
 // form), convergence check and retirement, then the inner sweeps of the survivors.
 int sg_batch_step(sg_batch *b)
@@ -641,10 +710,16 @@ int sg_batch_step(sg_batch *b)
     b->dirty = false;
     if (b->live.empty()) return SG_OK;
     // one launch sequence of max(inner) sweeps covers the live set; each grid stops after
-    // its own count (its waves exit), so a grid's iterates do not depend on its batch mates
+    // its own count (its tiles leave the active prefix), so a grid's iterates do not
+    // depend on its batch mates
     int sweeps = 0;
     for (auto &en : b->live) sweeps = std::max(sweeps, en.inner);
-    rc = upload_worklist(b, false);
+    // sweep work list: grids by decreasing inner count (ties in live order)
+    std::vector<int> order(b->live.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return b->live[x].inner > b->live[y].inner; });
+    rc = upload_worklist(b, false, order);
     if (rc) return rc;
     rc = run_sweeps(b, sweeps, mixed);
     if (rc) return rc;

@@ -81,6 +81,9 @@ class PoissonBatch:
     def live(self):
         return self._lib.sg_batch_live(self._h)
 
+    def set_tuning(self, key, value):
+        C.check(self._lib.sg_batch_set_tuning(self._h, int(key), int(value)))
+
     def step(self):
         C.check(self._lib.sg_batch_step(self._h))
 
@@ -103,8 +106,25 @@ class PoissonBatch:
             pass
 
 
-def solve(grids, params):
-    """Solve every grid in one batch; returns one result dict per grid."""
+def solve(grids, params, tuning=None):
+    """Solve every grid in one batch; returns one result dict per grid.  tuning: optional
+    {key: value} of sg_batch_set_tuning (then the batch runs through PoissonBatch)."""
+    if tuning:
+        import torch
+        b = PoissonBatch(params, torch.cuda.current_stream())
+        for k, v in tuning.items():
+            b.set_tuning(k, v)
+        for i, g in enumerate(grids):
+            b.add(g, i)
+        out = [None] * len(grids)
+        while b.live():
+            b.step()
+            for tag, r in b.poll():
+                out[tag] = r
+        for tag, r in b.poll():
+            out[tag] = r
+        b.close()
+        return out
     lib = C.load()
     arr = (ctypes.c_void_p * len(grids))(*[g._h.value for g in grids])
     res = (C.PoissonResult * len(grids))()

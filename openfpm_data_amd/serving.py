@@ -105,14 +105,17 @@ class PoissonServer:
     """Continuous-batching Poisson server on one GPU.
 
     params: ``poisson.make_params(...)`` (method and inner-solve settings; each request
-    carries its own tolerance).  budget_bytes: HBM budget for resident grids."""
+    carries its own tolerance).  budget_bytes: HBM budget for resident grids.  tuning:
+    {key: value} of sg_batch_set_tuning (the sweep kernel's launch shape)."""
 
-    def __init__(self, budget_bytes, params=None, keep_solutions=False):
+    def __init__(self, budget_bytes, params=None, keep_solutions=False, tuning=None):
         import torch
         self.torch = torch
         self.params = params if params is not None else P.make_params()
         self.stream = torch.cuda.Stream()
         self.batch = P.PoissonBatch(self.params, self.stream)
+        for k, v in (tuning or {}).items():
+            self.batch.set_tuning(k, v)
         self.adm = Admission(budget_bytes)
         self.keep = keep_solutions
 

@@ -274,3 +274,65 @@ class SparseGridGpu:
         v = ctypes.c_int64()
         C.check(self._lib.sg_count_boundary(self._h, ctypes.byref(v)))
         return v.value
+
+    # -- pack / unpack / remove (SparseGridGpu.hpp:3040-3733) ---------------------------
+    def _boxes(self, boxes):
+        lo = np.ascontiguousarray(np.array([b[0] for b in boxes], np.int32).reshape(-1, self.dim))
+        hi = np.ascontiguousarray(np.array([b[1] for b in boxes], np.int32).reshape(-1, self.dim))
+        return lo, hi
+
+    def pack_size(self, boxes, props, index_bytes=8):
+        """packRequest + packCalculate: bytes of the packed inclusive boxes [(lo, hi), ...]."""
+        lo, hi = self._boxes(boxes)
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        out = ctypes.c_int64()
+        C.check(self._lib.sg_pack_size(self._h, len(boxes), lo.ctypes.data, hi.ctypes.data, pr.size,
+                                       pr.ctypes.data, int(index_bytes), ctypes.byref(out)))
+        return out.value
+
+    def pack(self, boxes, props, index_bytes=8, out=None):
+        """pack + packFinalize.  Returns a host uint8 array, or fills `out` (a torch uint8
+        tensor on the device or a numpy array) and returns the byte count."""
+        n = self.pack_size(boxes, props, index_bytes)
+        lo, hi = self._boxes(boxes)
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        host = out is None
+        buf = np.zeros(max(n, 1), np.uint8) if host else out
+        cap = (buf.numel() if _is_device(buf) else buf.size)
+        C.check(self._lib.sg_pack(self._h, len(boxes), lo.ctypes.data, hi.ctypes.data, pr.size, pr.ctypes.data,
+                                  int(index_bytes), _ptr(buf), cap, _mem(buf)))
+        return buf[:n] if host else n
+
+    def unpack(self, buf, dst_lo, props, index_bytes=8, offset=0):
+        """unpack one packed box starting at byte `offset` of `buf` (host or device) with its
+        points moved to start at dst_lo; returns the bytes the box occupies."""
+        lo = np.ascontiguousarray(np.asarray(dst_lo, np.int32))
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        if not _is_device(buf):
+            buf = np.ascontiguousarray(np.asarray(buf, np.uint8))
+        size = buf.numel() if _is_device(buf) else buf.size
+        base = _ptr(buf).value + int(offset)
+        used = ctypes.c_int64()
+        C.check(self._lib.sg_unpack(self._h, ctypes.c_void_p(base), size - int(offset), _mem(buf), lo.ctypes.data,
+                                    pr.size, pr.ctypes.data, int(index_bytes), ctypes.byref(used)))
+        return used.value
+
+    def remove_box(self, lo, hi):
+        """remove(Box): queue an inclusive box for the next remove_add_unpack_finalize."""
+        a = np.ascontiguousarray(np.asarray(lo, np.int32))
+        b = np.ascontiguousarray(np.asarray(hi, np.int32))
+        C.check(self._lib.sg_remove_box(self._h, a.ctypes.data, b.ctypes.data))
+
+    def remove_points(self, keys):
+        """Device remove(coord) for linear point keys (host array or device tensor)."""
+        if not _is_device(keys):
+            keys = np.ascontiguousarray(np.asarray(keys, np.int64).reshape(-1))
+        n = keys.numel() if _is_device(keys) else keys.size
+        C.check(self._lib.sg_remove_points(self._h, n, _ptr(keys), _mem(keys)))
+
+    def remove_add_unpack_finalize(self, props):
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        C.check(self._lib.sg_remove_add_unpack_finalize(self._h, pr.size, pr.ctypes.data))
+
+    def remove_add_unpack_reset(self):
+        C.check(self._lib.sg_remove_add_unpack_reset(self._h))

@@ -45,6 +45,13 @@ _SIGS = {
     "sgcpu_lin_id": (_I64, [_P, ctypes.POINTER(_I64)]),
     "sgcpu_inv_lin_id": (_INT, [_P, _I64, ctypes.POINTER(_I64)]),
     "sgcpu_set_threads": (_INT, [_INT]),
+    "sgcpu_pack_size": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),
+    "sgcpu_pack": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, _P, _I64]),
+    "sgcpu_unpack": (_INT, [_P, _P, _I64, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),
+    "sgcpu_remove_box": (_INT, [_P, _P, _P]),
+    "sgcpu_remove_points": (_INT, [_P, _I64, _P]),
+    "sgcpu_remove_add_unpack_finalize": (_INT, [_P, _INT, _P]),
+    "sgcpu_remove_add_unpack_reset": (_INT, [_P]),
 }
 _lib = None
 
@@ -75,6 +82,12 @@ def set_threads(n):
 def _check(rc):
     if rc != 0:
         raise RuntimeError("sgcpu error %d: %s" % (rc, load().sgcpu_last_error().decode()))
+
+
+def _boxes(boxes, dim):
+    lo = np.ascontiguousarray(np.array([b[0] for b in boxes], np.int32).reshape(-1, dim))
+    hi = np.ascontiguousarray(np.array([b[1] for b in boxes], np.int32).reshape(-1, dim))
+    return lo, hi
 
 
 class SparseGridCpu:
@@ -255,3 +268,46 @@ class SparseGridCpu:
         v = ctypes.c_int64()
         _check(self._lib.sgcpu_count_boundary(self._h, ctypes.byref(v)))
         return v.value
+
+    # pack / unpack / remove (SparseGridGpu.hpp:3040-3733)
+    def pack_size(self, boxes, props, index_bytes=8):
+        lo, hi = _boxes(boxes, self.dim)
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        out = ctypes.c_int64()
+        _check(self._lib.sgcpu_pack_size(self._h, len(boxes), lo.ctypes.data, hi.ctypes.data, pr.size, pr.ctypes.data,
+                                         index_bytes, ctypes.byref(out)))
+        return out.value
+
+    def pack(self, boxes, props, index_bytes=8):
+        n = self.pack_size(boxes, props, index_bytes)
+        lo, hi = _boxes(boxes, self.dim)
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        buf = np.zeros(max(n, 1), np.uint8)
+        _check(self._lib.sgcpu_pack(self._h, len(boxes), lo.ctypes.data, hi.ctypes.data, pr.size, pr.ctypes.data,
+                                    index_bytes, buf.ctypes.data, n))
+        return buf[:n]
+
+    def unpack(self, buf, dst_lo, props, index_bytes=8, offset=0):
+        b = np.ascontiguousarray(np.asarray(buf, np.uint8))
+        lo = np.ascontiguousarray(np.asarray(dst_lo, np.int32))
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        used = ctypes.c_int64()
+        _check(self._lib.sgcpu_unpack(self._h, b.ctypes.data + offset, b.size - offset, lo.ctypes.data, pr.size,
+                                      pr.ctypes.data, index_bytes, ctypes.byref(used)))
+        return used.value
+
+    def remove_box(self, lo, hi):
+        a = np.ascontiguousarray(np.asarray(lo, np.int32))
+        b = np.ascontiguousarray(np.asarray(hi, np.int32))
+        _check(self._lib.sgcpu_remove_box(self._h, a.ctypes.data, b.ctypes.data))
+
+    def remove_points(self, keys):
+        k = np.ascontiguousarray(np.asarray(keys, np.int64).reshape(-1))
+        _check(self._lib.sgcpu_remove_points(self._h, k.size, k.ctypes.data))
+
+    def remove_add_unpack_finalize(self, props):
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        _check(self._lib.sgcpu_remove_add_unpack_finalize(self._h, pr.size, pr.ctypes.data))
+
+    def remove_add_unpack_reset(self):
+        _check(self._lib.sgcpu_remove_add_unpack_reset(self._h))

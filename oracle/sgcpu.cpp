@@ -136,6 +136,9 @@ struct Grid {
     std::vector<int64_t> nn[2];
     bool nn_valid[2] = {false, false};
     int ghost_type = SG_NN_STAR;  // setNNType; initialize() uses NNStar (SparseGridGpu.hpp:922)
+
+    // boxes queued by remove(Box) (rem_sects, SparseGridGpu.hpp:3528-3531), [k][dim]
+    std::vector<int32_t> rem_lo, rem_hi;
     std::vector<int> ghost_lin, ghost_nb;
 
     int64_t n() const { return (int64_t)keys.size(); }
@@ -1025,6 +1028,286 @@ int sgcpu_count_boundary(void *h, int64_t *out)
     int64_t c = 0;
     for (int64_t i = 0; i < g.n() * g.csz; ++i) c += (g.mask[i] & 3) == 3;
     *out = c;
+    return SG_OK;
+}
+
+// ---- pack / unpack / remove ------------------------------------------------------
+// Packed form of one box (packCalculate + pack + pack_data, SparseGridGpu.hpp:3072-3289,
+// SparseGridGpu_kernels.cuh:1634-1697), sections in this order, A = index_bytes:
+//   size_t n_cnk | int box_start[dim] | int grid_size[dim] | indexT ids[n_cnk]
+//   | uint32 scan[n_cnk + 1], padded to A | per packed property T[n_pnt] (consecutive),
+//   padded to A | int16 offsets[n_pnt], padded to A | uint8 masks[n_pnt], padded to A
+// ids: chunks (sorted) with at least one existing point inside the box; scan: first
+// point of each chunk, scan[n_cnk] = n_pnt; points of a chunk in increasing offset (the
+// reference orders them by an atomic counter, i.e. in no fixed order).
+static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+static bool inside(const Grid &g, const int64_t *c, const int32_t *lo, const int32_t *hi)
+{
+    for (int d = 0; d < g.dim; ++d)
+        if (c[d] < lo[d] || c[d] > hi[d]) return false;
+    return true;
+}
+
+// points of box k (existing, inside, inclusive box; Box::isInside) per chunk in order
+static void box_points(const Grid &g, const int32_t *lo, const int32_t *hi, std::vector<int64_t> &chunks,
+                       std::vector<uint32_t> &scan, std::vector<int32_t> &offs)
+{
+    chunks.clear();
+    scan.clear();
+    offs.clear();
+    for (int64_t c = 0; c < g.n(); ++c) {
+        const size_t before = offs.size();
+        for (int o = 0; o < g.csz; ++o) {
+            if (!(g.mask[c * g.csz + o] & SG_EXIST)) continue;
+            int64_t pc[3];
+            g.inv_lin(g.keys[c] * g.csz + o, pc);
+            if (inside(g, pc, lo, hi)) offs.push_back(o);
+        }
+        if (offs.size() > before) {
+            chunks.push_back(c);
+            scan.push_back((uint32_t)before);
+        }
+    }
+    scan.push_back((uint32_t)offs.size());
+}
+
+static int64_t section_bytes(const Grid &g, int64_t n_cnk, int64_t n_pnt, int nprop, const int32_t *props, int A)
+{
+    int64_t psz = 0;
+    for (int q = 0; q < nprop; ++q) psz += (int64_t)g.ts[props[q]];
+    return 8 + 2 * g.dim * 4 + A * n_cnk + align_up((n_cnk + 1) * 4, A) + align_up(n_pnt * psz, A) +
+           align_up(n_pnt * 2, A) + align_up(n_pnt, A);
+}
+
+static int check_pack_args(const Grid &g, int nbox, int nprop, const int32_t *props, int A)
+{
+    if (nbox < 0 || nprop < 0) return fail(SG_E_ARG, "negative count");
+    if (A != 4 && A != 8) return fail(SG_E_ARG, "index_bytes must be 4 or 8");
+    for (int q = 0; q < nprop; ++q)
+        if (props[q] < 0 || props[q] >= g.np) return fail(SG_E_ARG, "bad property");
+    if (g.csz > 65536) return fail(SG_E_ARG, "chunk too large for 16-bit offsets");
+    return SG_OK;
+}
+
+int sgcpu_pack_size(void *h, int nbox, const int32_t *lo, const int32_t *hi, int nprop, const int32_t *props,
+                    int index_bytes, int64_t *bytes)
+{
+    Grid &g = *(Grid *)h;
+    int rc = check_pack_args(g, nbox, nprop, props, index_bytes);
+    if (rc) return rc;
+    std::vector<int64_t> ch;
+    std::vector<uint32_t> sc;
+    std::vector<int32_t> of;
+    int64_t tot = 0;
+    for (int k = 0; k < nbox; ++k) {
+        box_points(g, lo + k * g.dim, hi + k * g.dim, ch, sc, of);
+        tot += section_bytes(g, (int64_t)ch.size(), (int64_t)of.size(), nprop, props, index_bytes);
+    }
+    *bytes = tot;
+    return SG_OK;
+}
+
+int sgcpu_pack(void *h, int nbox, const int32_t *lo, const int32_t *hi, int nprop, const int32_t *props,
+               int index_bytes, void *buf, int64_t cap)
+{
+    Grid &g = *(Grid *)h;
+    int rc = check_pack_args(g, nbox, nprop, props, index_bytes);
+    if (rc) return rc;
+    const int A = index_bytes;
+    unsigned char *out = (unsigned char *)buf;
+    int64_t at = 0;
+    std::vector<int64_t> ch;
+    std::vector<uint32_t> sc;
+    std::vector<int32_t> of;
+    for (int k = 0; k < nbox; ++k) {
+        const int32_t *bl = lo + k * g.dim, *bh = hi + k * g.dim;
+        box_points(g, bl, bh, ch, sc, of);
+        const int64_t nc = (int64_t)ch.size(), npt = (int64_t)of.size();
+        const int64_t sb = section_bytes(g, nc, npt, nprop, props, A);
+        if (at + sb > cap) return fail(SG_E_RANGE, "pack buffer too small");
+        unsigned char *s0 = out + at;
+        memset(s0, 0, (size_t)sb);
+        uint64_t ncu = (uint64_t)nc;
+        memcpy(s0, &ncu, 8);
+        int64_t q = 8;
+        for (int d = 0; d < g.dim; ++d, q += 4) memcpy(s0 + q, &bl[d], 4);
+        for (int d = 0; d < g.dim; ++d, q += 4) {
+            int32_t v = (int32_t)g.sz[d];
+            memcpy(s0 + q, &v, 4);
+        }
+        for (int64_t i = 0; i < nc; ++i, q += A) {
+            int64_t id = g.keys[ch[i]];
+            if (A == 8) memcpy(s0 + q, &id, 8);
+            else {
+                int32_t id4 = (int32_t)id;
+                memcpy(s0 + q, &id4, 4);
+            }
+        }
+        memcpy(s0 + q, sc.data(), (size_t)(nc + 1) * 4);
+        q += align_up((nc + 1) * 4, A);
+        // point j of the box: chunk position pos_of[j], offset of[j]
+        std::vector<int64_t> pos_of(npt);
+        for (int64_t i = 0; i < nc; ++i)
+            for (uint32_t j = sc[i]; j < sc[i + 1]; ++j) pos_of[j] = ch[i];
+        int64_t dq = q;
+        for (int pi = 0; pi < nprop; ++pi) {
+            const int p = props[pi];
+            for (int64_t j = 0; j < npt; ++j)
+                memcpy(s0 + dq + j * g.ts[p], g.val(p, pos_of[j], of[j]), g.ts[p]);
+            dq += npt * (int64_t)g.ts[p];
+        }
+        q = dq;
+        int64_t psz = 0;
+        for (int pi = 0; pi < nprop; ++pi) psz += (int64_t)g.ts[props[pi]];
+        q = (q - npt * psz) + align_up(npt * psz, A);
+        for (int64_t j = 0; j < npt; ++j) {
+            int16_t o16 = (int16_t)of[j];
+            memcpy(s0 + q + j * 2, &o16, 2);
+        }
+        q += align_up(npt * 2, A);
+        for (int64_t j = 0; j < npt; ++j) s0[q + j] = g.mask[pos_of[j] * g.csz + of[j]];
+        at += sb;
+    }
+    return SG_OK;
+}
+
+// unpack (SparseGridGpu.hpp:3665-3733 -> addAndConvertPackedChunkToTmp :1415-1560 ->
+// copy_packed_data_to_chunks, SparseGridGpu_kernels.cuh:1591-1632): the point at packed
+// coordinate p lands at p - box_start + dst_lo; it is queued as an insert that the
+// finalize merges (unpacked properties take the packed values, mask |= EXIST).
+int sgcpu_unpack(void *h, const void *buf, int64_t avail, const int32_t *dst_lo, int nprop, const int32_t *props,
+                 int index_bytes, int64_t *consumed)
+{
+    Grid &g = *(Grid *)h;
+    int rc = check_pack_args(g, 1, nprop, props, index_bytes);
+    if (rc) return rc;
+    const int A = index_bytes;
+    const unsigned char *s0 = (const unsigned char *)buf;
+    if (avail < 8 + 2 * g.dim * 4) return fail(SG_E_RANGE, "packed section truncated");
+    uint64_t ncu;
+    memcpy(&ncu, s0, 8);
+    const int64_t nc = (int64_t)ncu;
+    int32_t bstart[3], psize[3];
+    int64_t q = 8;
+    for (int d = 0; d < g.dim; ++d, q += 4) memcpy(&bstart[d], s0 + q, 4);
+    for (int d = 0; d < g.dim; ++d, q += 4) memcpy(&psize[d], s0 + q, 4);
+    if (avail < q + A * nc + (nc + 1) * 4) return fail(SG_E_RANGE, "packed section truncated");
+    const int64_t ids_at = q;
+    q += A * nc;
+    std::vector<uint32_t> sc(nc + 1);
+    memcpy(sc.data(), s0 + q, (size_t)(nc + 1) * 4);
+    const int64_t npt = sc[nc];
+    q += align_up((nc + 1) * 4, A);
+    if (avail < section_bytes(g, nc, npt, nprop, props, A)) return fail(SG_E_RANGE, "packed section truncated");
+    const int64_t data_at = q;
+    int64_t psz = 0;
+    for (int pi = 0; pi < nprop; ++pi) psz += (int64_t)g.ts[props[pi]];
+    q += align_up(npt * psz, A);
+    const int64_t off_at = q;
+    // packed grid geometry: chunk counts of the packing grid (same chunk edge)
+    int64_t pbsz[3];
+    for (int d = 0; d < g.dim; ++d) pbsz[d] = (psize[d] + g.edge - 1) / g.edge;
+    std::vector<int64_t> keys(npt);
+    for (int64_t i = 0; i < nc; ++i) {
+        int64_t id;
+        if (A == 8) memcpy(&id, s0 + ids_at + i * 8, 8);
+        else {
+            int32_t id4;
+            memcpy(&id4, s0 + ids_at + i * 4, 4);
+            id = id4;
+        }
+        int64_t bc[3], b = id;
+        for (int d = 0; d < g.dim; ++d) {
+            bc[d] = b % pbsz[d];
+            b /= pbsz[d];
+        }
+        for (uint32_t j = sc[i]; j < sc[i + 1]; ++j) {
+            int16_t o16;
+            memcpy(&o16, s0 + off_at + j * 2, 2);
+            int64_t l = (uint16_t)o16, c[3];
+            for (int d = 0; d < g.dim; ++d) {
+                c[d] = bc[d] * g.edge + l % g.edge - bstart[d] + dst_lo[d];
+                l /= g.edge;
+                if (c[d] < 0 || c[d] >= g.sz[d]) return fail(SG_E_RANGE, "unpacked point outside the grid");
+            }
+            keys[j] = g.lin_id(c);
+        }
+    }
+    // values: packed properties from the data section, the others the background
+    std::vector<std::vector<unsigned char>> vals(g.np);
+    std::vector<const void *> vp(g.np, nullptr);
+    int64_t dq = data_at;
+    for (int pi = 0; pi < nprop; ++pi) {
+        const int p = props[pi];
+        vals[p].assign(s0 + dq, s0 + dq + npt * g.ts[p]);
+        vp[p] = vals[p].data();
+        dq += npt * (int64_t)g.ts[p];
+    }
+    std::vector<uint8_t> m(npt, SG_EXIST);
+    *consumed = section_bytes(g, nc, npt, nprop, props, A);
+    return sgcpu_insert_points(h, npt, keys.data(), m.data(), vp.data());
+}
+
+int sgcpu_remove_box(void *h, const int32_t *lo, const int32_t *hi)
+{
+    Grid &g = *(Grid *)h;
+    for (int d = 0; d < g.dim; ++d) {
+        g.rem_lo.push_back(lo[d]);
+        g.rem_hi.push_back(hi[d]);
+    }
+    return SG_OK;
+}
+
+static void clear_point(Grid &g, int64_t c, int o)
+{
+    g.mask[c * g.csz + o] = 0;
+    for (int p = 0; p < g.np; ++p) memcpy(g.val(p, c, o), g.bg[p].data(), g.ts[p]);
+}
+
+// device remove(coord) (BlockMapGpu_ker.cuh:322-332): mask of the point set to 0 (the
+// value becomes the background, which the reference substitutes on every read)
+int sgcpu_remove_points(void *h, int64_t n, const int64_t *keys)
+{
+    Grid &g = *(Grid *)h;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t c = g.find(keys[i] / g.csz);
+        if (c < g.n()) clear_point(g, c, (int)(keys[i] % g.csz));
+    }
+    return SG_OK;
+}
+
+// removeAddUnpackFinalize (SparseGridGpu.hpp:3492-3499): removePoints (remove_points,
+// SparseGridGpu_kernels.cuh:1301-1346: existing points inside a queued box get mask 0),
+// then the queued inserts are merged
+int sgcpu_remove_add_unpack_finalize(void *h, int nprop, const int32_t *props)
+{
+    Grid &g = *(Grid *)h;
+    const int nb = (int)(g.rem_lo.size() / g.dim);
+    for (int64_t c = 0; c < g.n(); ++c)
+        for (int o = 0; o < g.csz; ++o) {
+            if (!(g.mask[c * g.csz + o] & SG_EXIST)) continue;
+            int64_t pc[3];
+            g.inv_lin(g.keys[c] * g.csz + o, pc);
+            for (int k = 0; k < nb; ++k)
+                if (inside(g, pc, &g.rem_lo[k * g.dim], &g.rem_hi[k * g.dim])) {
+                    clear_point(g, c, o);
+                    break;
+                }
+        }
+    g.rem_lo.clear();
+    g.rem_hi.clear();
+    std::vector<int32_t> ops(g.np, SG_OP_RIGHT);
+    for (int q = 0; q < nprop; ++q) ops[props[q]] = SG_OP_LEFT;
+    return flush(g, ops.data());
+}
+
+int sgcpu_remove_add_unpack_reset(void *h)
+{
+    Grid &g = *(Grid *)h;
+    g.rem_lo.clear();
+    g.rem_hi.clear();
+    clear_inserts(g);
     return SG_OK;
 }
 

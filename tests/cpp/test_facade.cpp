@@ -4,8 +4,13 @@
 //   testTagBoundaries           src/SparseGridGpu/tests/SparseGridGpu_tests.cu:284-370
 //   testStencilHeat             src/SparseGridGpu/tests/SparseGridGpu_tests.cu:472-514
 //   test_convolution_3x3x3      src/SparseGridGpu/tests/SparseGridGpu_tests.cu:1012-1080
+//   test_pack_request_with_iterator  src/SparseGridGpu/tests/SparseGridGpu_tests.cu:1379-1472
+//   sparsegridgpu_remove_test   src/SparseGridGpu/tests/SparseGridGpu_tests.cu:1474-1551
+//   sparsegridgpu_pack_unpack   src/SparseGridGpu/tests/SparseGridGpu_tests.cu:1553-1934
+//                               (aggregate<float, float[3]> as four float properties)
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <set>
 #include <vector>
 
@@ -161,12 +166,193 @@ static void testConv3x3x3()
     REQUIRE(sg.countExistingElements() == keys.size());
 }
 
+// The chunks insertSphere3D_radius(V) writes (tests/utils/SparseGridGpu_testKernels.cuh:202-318):
+// blocks 64..95 per axis of a 1000^3 grid with 4^3 chunks, active when 64 > r > 56 about 320.
+template <typename SG> static void insertSphere(SG &sg, bool vector)
+{
+    std::vector<long> ids;
+    std::vector<uint8_t> masks;
+    std::vector<float> v[4];
+    const int e = 4, bs = 250;
+    for (int bz = 64; bz < 96; ++bz)
+        for (int by = 64; by < 96; ++by)
+            for (int bx = 64; bx < 96; ++bx) {
+                uint8_t m[64];
+                float val[4][64];
+                bool any = false;
+                for (int o = 0; o < 64; ++o) {
+                    const long x = bx * e + o % e - 320, y = by * e + (o / e) % e - 320, z = bz * e + o / 16 - 320;
+                    const float r = std::sqrt((float)(x * x + y * y + z * z));
+                    const bool act = r < 64.0f && r > 56.0f;
+                    m[o] = act ? 1 : 0;
+                    val[0][o] = act ? (float)(x + y + z) : 0.0f;
+                    val[1][o] = act ? (float)x : 0.0f;
+                    val[2][o] = act ? (float)y : 0.0f;
+                    val[3][o] = act ? (float)z : 0.0f;
+                    any = any || act;
+                }
+                if (!any) continue;
+                ids.push_back(((long)bz * bs + by) * bs + bx);
+                masks.insert(masks.end(), m, m + 64);
+                for (int q = 0; q < 4; ++q) v[q].insert(v[q].end(), val[q], val[q] + 64);
+            }
+    const void *vals[4] = {v[0].data(), v[1].data(), v[2].data(), v[3].data()};
+    sg.insertBlocks(ids.data(), ids.size(), masks.data(), vals);
+    gpu::ofp_context_t ctx;
+    if (vector) sg.template flush<smax_<0>, smax_<1>, smax_<2>, smax_<3>>(ctx, FLUSH_ON_DEVICE);
+    else sg.template flush<smax_<0>>(ctx, FLUSH_ON_DEVICE);
+}
+
+static size_t align8(size_t v) { return (v + 7) / 8 * 8; }
+
+static void testPackRequestWithIterator()
+{
+    const size_t sz[3] = {1000, 1000, 1000};
+    SparseGridGpu<3, aggregate<float>, 4, 64, long int> sg(sz);
+    gpu::ofp_context_t ctx;
+    sg.setBackgroundValue<0>(0);
+    insertSphere(sg, false);
+    size_t req = 0;
+    sg.packReset();
+    auto it1 = sg.getIterator(grid_key_dx<3>({0, 0, 0}), grid_key_dx<3>({321, 999, 999}));
+    sg.packRequest<0>(it1, req);
+    auto it2 = sg.getIterator(grid_key_dx<3>({322, 0, 0}), grid_key_dx<3>({999, 999, 999}));
+    sg.packRequest<0>(it2, req);
+    sg.packCalculate<0>(req, ctx);
+    const size_t tot = 2 * 8 + 2 * 2 * 3 * 4 + align8(4685 * 8) + align8(4475 * 8) + align8(4686 * 4) +
+                       align8(4476 * 4) + align8(185807 * 4) + align8(176787 * 4) + align8(185807 * 2) +
+                       align8(176787 * 2) + align8(185807 * 1) + align8(176787 * 1);
+    REQUIRE(req == tot);
+}
+
+template <typename SG> static std::vector<grid_key_dx<3, int>> existingPoints(const SG &sg)
+{
+    std::vector<grid_key_dx<3, int>> out;
+    const std::vector<long> ids = sg.private_get_index_array();
+    const std::vector<uint8_t> m = sg.private_get_mask_array();
+    for (size_t c = 0; c < ids.size(); ++c)
+        for (unsigned int o = 0; o < sg.getBlockSize(); ++o)
+            if (m[c * sg.getBlockSize() + o] & 1) out.push_back(sg.getCoord(ids[c] * sg.getBlockSize() + o));
+    return out;
+}
+
+static void testRemove()
+{
+    const size_t sz[3] = {1000, 1000, 1000};
+    SparseGridGpu<3, aggregate<float>, 4, 64, long int> sg(sz);
+    gpu::ofp_context_t ctx;
+    sg.setBackgroundValue<0>(0);
+    insertSphere(sg, false);
+    Box<3, unsigned int> s1({310, 0, 0}, {330, 999, 999}), s2({0, 310, 0}, {999, 330, 999}),
+        s3({0, 0, 310}, {999, 999, 330});
+    sg.remove(s1);
+    sg.remove(s2);
+    sg.remove(s3);
+    sg.removeAddUnpackFinalize<>(ctx, 0);
+    bool match = true;
+    for (const auto &p : existingPoints(sg)) {
+        const float r = std::sqrt((float)((p.get(0) - 320) * (p.get(0) - 320) + (p.get(1) - 320) * (p.get(1) - 320) +
+                                          (p.get(2) - 320) * (p.get(2) - 320)));
+        match = match && !(r < 55.99f || r > 64.01f) && !s1.isInside(p) && !s2.isInside(p) && !s3.isInside(p);
+    }
+    REQUIRE(match);
+}
+
+typedef SparseGridGpu<3, aggregate<float, float, float, float>, 4, 64, long int> SGV;
+
+static void packUnpack(SGV &dst, SGV &src, const Box<3, size_t> (&dbox)[4], bool test_pack)
+{
+    const Box<3, size_t> sbox[4] = {Box<3, size_t>({256, 256, 256}, {273, 390, 390}),
+                                    Box<3, size_t>({320, 256, 256}, {337, 390, 390}),
+                                    Box<3, size_t>({256, 256, 256}, {273, 320, 390}),
+                                    Box<3, size_t>({320, 256, 256}, {337, 320, 390})};
+    gpu::ofp_context_t ctx;
+    src.packReset();
+    size_t req = 0;
+    for (const auto &b : sbox) {
+        auto it = src.getIterator(grid_key_dx<3, size_t>({b.getLow(0), b.getLow(1), b.getLow(2)}),
+                                  grid_key_dx<3, size_t>({b.getHigh(0), b.getHigh(1), b.getHigh(2)}));
+        src.packRequest<0, 1, 2, 3>(it, req);
+    }
+    src.packCalculate<0, 1, 2, 3>(req, ctx);
+    PackMemory mem;
+    mem.resize(req);
+    Pack_stat sts;
+    src.packFinalize<0, 1, 2, 3>(mem, sts);
+    if (test_pack) {
+        const size_t expect[4][2] = {{1107, 41003}, {1420, 54276}, {610, 20828}, {739, 27283}};
+        size_t at = 0;
+        for (int k = 0; k < 4; ++k) {
+            size_t ncnk;
+            std::memcpy(&ncnk, mem.getPointer() + at, 8);
+            REQUIRE(ncnk == expect[k][0]);
+            size_t off = at + ncnk * 8 + 8 + 2 * 3 * 4;
+            unsigned int npnt;
+            std::memcpy(&npnt, mem.getPointer() + off + ncnk * 4, 4);
+            REQUIRE(npnt == expect[k][1]);
+            at = off + align8((ncnk + 1) * 4) + align8(npnt * 16) + align8(npnt * 2) + align8(npnt);
+        }
+    }
+    Unpack_stat ps;
+    dst.removeAddUnpackReset();
+    for (const auto &b : dbox) {
+        auto sub = dst.getIterator(grid_key_dx<3, size_t>({b.getLow(0), b.getLow(1), b.getLow(2)}),
+                                   grid_key_dx<3, size_t>({b.getHigh(0), b.getHigh(1), b.getHigh(2)}));
+        dst.remove(b);
+        dst.unpack<0, 1, 2, 3>(mem, sub, ps, ctx);
+    }
+    dst.removeAddUnpackFinalize<0, 1, 2, 3>(ctx, 0);
+}
+
+static void countDst(const SGV &dst, const Box<3, size_t> (&dbox)[4], const size_t (&expect)[4])
+{
+    const long shift[4] = {0, -46, 44, 0};
+    size_t cnt[4] = {0, 0, 0, 0};
+    bool match = true;
+    for (const auto &p : existingPoints(dst))
+        for (int k = 0; k < 4; ++k)
+            if (dbox[k].isInside(p)) {
+                ++cnt[k];
+                const long x = p.get(0) - (320 + shift[k]), y = p.get(1) - 320, z = p.get(2) - 320;
+                const float r = std::sqrt((float)(x * x + y * y + z * z));
+                match = match && r < 64.0f && r > 56.0f;
+                break;
+            }
+    REQUIRE(match);
+    for (int k = 0; k < 4; ++k) REQUIRE(cnt[k] == expect[k]);
+}
+
+static void testPackUnpack()
+{
+    const size_t sz[3] = {1000, 1000, 1000};
+    const Box<3, size_t> dbox[4] = {Box<3, size_t>({256, 256, 256}, {273, 390, 390}),
+                                    Box<3, size_t>({274, 256, 256}, {291, 390, 390}),
+                                    Box<3, size_t>({300, 256, 256}, {317, 320, 390}),
+                                    Box<3, size_t>({320, 256, 256}, {337, 320, 390})};
+    SGV src(sz), dst(sz);
+    src.setBackgroundValue<0>(0);
+    dst.setBackgroundValue<0>(0);
+    insertSphere(src, true);
+    packUnpack(dst, src, dbox, true);
+    countDst(dst, dbox, {41003, 54276, 20828, 27283});
+    // remove the even points of the source (removeSphere3D_even_radiusV) and repeat
+    std::vector<long> even;
+    for (const auto &p : existingPoints(src))
+        if ((p.get(0) + p.get(1) + p.get(2)) % 2 == 0) even.push_back(src.getLinId(p));
+    src.removePoints(even.data(), even.size());
+    packUnpack(dst, src, dbox, false);
+    countDst(dst, dbox, {20520, 27152, 10423, 13649});
+}
+
 int main()
 {
     testInsert();
     testTagBoundaries();
     testStencilHeat();
     testConv3x3x3();
+    testPackRequestWithIterator();
+    testRemove();
+    testPackUnpack();
     if (failures) {
         std::printf("%d failure(s)\n", failures);
         return 1;

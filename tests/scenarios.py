@@ -242,3 +242,101 @@ def exist_dense(g):
     crd = g.inv_lin_id(keys)
     out[tuple(crd[:, d] for d in range(g.dim - 1, -1, -1))] = mm
     return out
+
+
+def sphere_shell_chunks(vector=False, edge=4, size=1000, start=256, nblk=32, r1=64.0, r2=56.0):
+    """Chunks written by the reference's insertSphere3D_radius(V) test kernel
+    (tests/utils/SparseGridGpu_testKernels.cuh:202-318): blocks start/edge + [0, nblk)^3,
+    a point is active when r1 > |p - c| > r2 (c = start + nblk/2*edge, float32 radius),
+    a block is inserted when it holds an active point.  Values: x+y+z (relative to c),
+    and with vector=True also x, y, z (the reference's float[3] property)."""
+    b0 = start // edge
+    bs = size // edge
+    cs = edge ** 3
+    loc = np.arange(cs)
+    lx, ly, lz = loc % edge, (loc // edge) % edge, loc // (edge * edge)
+    bz, by, bx = np.meshgrid(np.arange(nblk), np.arange(nblk), np.arange(nblk), indexing="ij")
+    bx, by, bz = (b.reshape(-1) + b0 for b in (bx, by, bz))
+    centre = start + nblk // 2 * edge
+    x = (bx[:, None] * edge + lx[None, :]).astype(np.int64) - centre
+    y = (by[:, None] * edge + ly[None, :]).astype(np.int64) - centre
+    z = (bz[:, None] * edge + lz[None, :]).astype(np.int64) - centre
+    r = np.sqrt((x * x + y * y + z * z).astype(np.float32))
+    act = (r < np.float32(r1)) & (r > np.float32(r2))
+    keep = act.any(axis=1)
+    ids = ((bz * bs + by) * bs + bx)[keep].astype(np.int64)
+    order = np.argsort(ids)
+    act = act[keep][order]
+    masks = act.astype(np.uint8)
+    vals = [np.where(act, (x + y + z)[keep][order], 0).astype(np.float32)]
+    if vector:
+        for comp in (x, y, z):
+            vals.append(np.where(act, comp[keep][order], 0).astype(np.float32))
+    return {"size": (size,) * 3, "edge": edge, "chunk_ids": ids[order], "masks": masks, "values": vals,
+            "centre": centre}
+
+
+def sphere_grid(G, vector=False):
+    s = sphere_shell_chunks(vector)
+    n = len(s["values"])
+    g = G(3, s["size"], [np.float32] * n, block_edge=s["edge"])
+    g.insert_block(s["chunk_ids"], s["masks"], s["values"])
+    g.flush(["smax"] * n)
+    return g, s
+
+
+def existing_points(g):
+    """Coordinates [k, dim] of every existing point."""
+    ids = g.chunk_ids()
+    c, o = np.nonzero(g.masks() & 1)
+    return g.inv_lin_id(ids[c] * g.chunk_size + o)
+
+
+def in_box(p, lo, hi):
+    return np.all((p >= np.asarray(lo)) & (p <= np.asarray(hi)), axis=1)
+
+
+def pack_section_counts(buf, nprop_bytes, index_bytes=8, dim=3):
+    """(n_cnk, n_pnt) of each packed box section of buf."""
+    out = []
+    at = 0
+    A = index_bytes
+    al = lambda v: (v + A - 1) // A * A  # noqa: E731
+    while at < len(buf):
+        nc = int(np.frombuffer(buf[at:at + 8].tobytes(), np.uint64)[0])
+        q = at + 8 + 2 * dim * 4 + A * nc
+        npt = int(np.frombuffer(buf[q + 4 * nc:q + 4 * nc + 4].tobytes(), np.uint32)[0])
+        out.append((nc, npt))
+        at = q + al((nc + 1) * 4) + al(npt * nprop_bytes) + al(npt * 2) + al(npt)
+    return out
+
+
+# the reference's pack/unpack boxes (SparseGridGpu_tests.cu:1556-1706)
+PACK_SRC_BOXES = [((256, 256, 256), (273, 390, 390)), ((320, 256, 256), (337, 390, 390)),
+                  ((256, 256, 256), (273, 320, 390)), ((320, 256, 256), (337, 320, 390))]
+PACK_DST_BOXES = [((256, 256, 256), (273, 390, 390)), ((274, 256, 256), (291, 390, 390)),
+                  ((300, 256, 256), (317, 320, 390)), ((320, 256, 256), (337, 320, 390))]
+
+
+def pack_unpack_round(dst, src, props=(0, 1, 2, 3)):
+    """pack_unpack_test (SparseGridGpu_tests.cu:1556-1681): pack four boxes of src, then
+    per destination box remove it and unpack the matching packed box there."""
+    buf = src.pack(PACK_SRC_BOXES, list(props))
+    dst.remove_add_unpack_reset()
+    at = 0
+    for lo, hi in PACK_DST_BOXES:
+        dst.remove_box(lo, hi)
+        at += dst.unpack(buf, lo, list(props), offset=at)
+    dst.remove_add_unpack_finalize(list(props))
+    return buf
+
+
+def shell_in_dst_box(p, centre, k):
+    """The reference's check on unpacked points of destination box k: the shell about the
+    centre moved by the box shift (-46 for box 2, +44 for box 3)."""
+    shift = [0, -46, 44, 0][k]
+    x = p[:, 0].astype(np.int64) - (centre + shift)
+    y = p[:, 1].astype(np.int64) - centre
+    z = p[:, 2].astype(np.int64) - centre
+    r = np.sqrt((x * x + y * y + z * z).astype(np.float32))
+    return (r < np.float32(64)) & (r > np.float32(56))

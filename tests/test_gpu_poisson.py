@@ -135,3 +135,17 @@ def test_alone_and_in_mix_identical(P):
     assert done[3]["sweeps"] == r_alone["sweeps"]
     np.testing.assert_array_equal(u_mix, u_alone)
     b.close()
+
+
+@pytest.mark.parametrize("waves,max_wgs", [(1, 0), (2, 7), (8, 0), (4, 33)])
+def test_launch_shapes_identical(P, waves, max_wgs):
+    """Waves per workgroup and looping waves (a workgroup cap) change only which wave
+    sweeps which tile: solutions bitwise equal to the default launch shape."""
+    specs = _small_grids(4, 51)
+    ga = [P.poisson_grid(s["size"], s["chunk_ids"], s["f"]) for s in specs]
+    gb = [P.poisson_grid(s["size"], s["chunk_ids"], s["f"]) for s in specs]
+    ra = P.solve(ga, P.make_params(tol=1e-10))
+    rb = P.solve(gb, P.make_params(tol=1e-10), tuning={C.TUNE_BATCH_WAVES: waves, C.TUNE_BATCH_MAX_WGS: max_wgs})
+    for a, b, x, y in zip(ga, gb, ra, rb):
+        assert x["sweeps"] == y["sweeps"] and y["converged"]
+        np.testing.assert_array_equal(a.prop(x["solution_prop"]), b.prop(y["solution_prop"]))
