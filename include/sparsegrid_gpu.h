@@ -112,6 +112,14 @@ typedef enum {
     SG_ST_RESIDUAL = 5
 } sg_stencil_kind;
 
+/* how a stencil runs (StencilMode, src/SparseGridGpu/SparseGridGpu.hpp:150-154):
+ * INPLACE stages chunk + ghost layer in LDS (applyStencilInPlace, :954-1048);
+ * INPLACE_NO_SHARED runs one thread per existing point of the point list, reading
+ * neighbours through the chunk table (applyStencilInPlaceNoShared, :1051-1075), so it
+ * writes only existing (EXIST) points.  Both give the same values. */
+#define SG_STENCIL_MODE_INPLACE 1
+#define SG_STENCIL_MODE_INPLACE_NO_SHARED 3
+
 /* which points a stencil writes */
 #define SG_APPLY_INTERIOR 0  /* EXIST && !PADDING  (conv/conv2 functors, SparseGridGpu_kernels.cuh:281) */
 #define SG_APPLY_EXISTING 1  /* mask != 0          (Conv3x3x3 called with bool(mask)) */
@@ -124,7 +132,7 @@ typedef struct {
     int32_t use_box;       /* 0: whole grid; 1: restrict to [box_lo, box_hi] inclusive */
     int32_t box_lo[3];
     int32_t box_hi[3];
-    int32_t pad_;
+    int32_t mode;          /* SG_STENCIL_MODE_* (0 = SG_STENCIL_MODE_INPLACE) */
     double coef[27];       /* floating coefficients (see kinds) */
     int64_t icoef[27];     /* integer coefficients / integer parameters */
 } sg_stencil_desc;
@@ -178,8 +186,15 @@ int sg_flush(sg_grid *g, const int32_t *ops);
 int sg_find_neighbours(sg_grid *g, int nn_type);
 /* setNNType<NN>() (src/SparseGridGpu/SparseGridGpu.hpp:1825-1829) */
 int sg_set_nn_type(sg_grid *g, int nn_type);
-/* tagBoundaries<NN>(ctx, No_check(), opt) (src/SparseGridGpu/SparseGridGpu.hpp:2290-2394) */
+/* tagBoundaries<NN>(ctx, No_check(), opt) (src/SparseGridGpu/SparseGridGpu.hpp:2290-2394);
+ * opt = SG_CALCULATE_EXISTING_POINTS also builds the existing-point list (e_points,
+ * :2354-2391) that SG_STENCIL_MODE_INPLACE_NO_SHARED runs over */
 int sg_tag_boundaries(sg_grid *g, int nn_type, int opt);
+/* size and contents of the existing-point list (getPointBuffer, SparseGridGpu_ker.cuh:296-327):
+ * data positions chunk * chunk_size + offset of the EXIST points, in chunk order and
+ * increasing offset; rebuilt here when a flush or remove made it stale */
+int sg_existing_points(sg_grid *g, int64_t *n);
+int sg_get_existing_points(sg_grid *g, int64_t *out, int mem);
 /* applyStencils / conv / conv2 (src/SparseGridGpu/SparseGridGpu.hpp:2568-2734) */
 int sg_apply_stencil(sg_grid *g, const sg_stencil_desc *desc);
 

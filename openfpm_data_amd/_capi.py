@@ -52,6 +52,8 @@ CALCULATE_EXISTING_POINTS = 1
 
 # stencil kinds (sg_stencil_kind) and apply modes
 ST_LAP, ST_HEAT, ST_GRAY_SCOTT, ST_BOX, ST_RBSOR, ST_RESIDUAL = range(6)
+STENCIL_MODE_INPLACE = 1
+STENCIL_MODE_INPLACE_NO_SHARED = 3
 APPLY_INTERIOR = 0
 APPLY_EXISTING = 1
 
@@ -66,16 +68,17 @@ class StencilDesc(ctypes.Structure):
         ("use_box", ctypes.c_int32),
         ("box_lo", ctypes.c_int32 * 3),
         ("box_hi", ctypes.c_int32 * 3),
-        ("pad_", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
         ("coef", ctypes.c_double * 27),
         ("icoef", ctypes.c_int64 * 27),
     ]
 
 
-def make_desc(kind, src, dst=(), apply=APPLY_INTERIOR, box=None, coef=(), icoef=()):
+def make_desc(kind, src, dst=(), apply=APPLY_INTERIOR, box=None, coef=(), icoef=(), mode=STENCIL_MODE_INPLACE):
     d = StencilDesc()
     d.kind = kind
     d.apply = apply
+    d.mode = mode
     for i, s in enumerate(src):
         d.src[i] = s
     for i, s in enumerate(dst):
@@ -147,6 +150,8 @@ _SIGS = {
     "sg_set_nn_type": (_INT, [_P, _INT]),
     "sg_tag_boundaries": (_INT, [_P, _INT, _INT]),
     "sg_apply_stencil": (_INT, [_P, ctypes.POINTER(StencilDesc)]),
+    "sg_existing_points": (_INT, [_P, ctypes.POINTER(_I64)]),
+    "sg_get_existing_points": (_INT, [_P, _P, _INT]),
     "sg_set_tuning": (_INT, [_P, _INT, _I64]),
     "sg_num_chunks": (_I64, [_P]),
     "sg_chunk_size": (_INT, [_P]),

@@ -278,7 +278,33 @@ int sg_tag_boundaries(sg_grid *g, int nn_type, int opt)
     if (!g || (nn_type != SG_NN_STAR && nn_type != SG_NN_FULL)) return fail(SG_E_ARG, "bad grid or nn type");
     if (opt != SG_NO_CALCULATE_EXISTING_POINTS && opt != SG_CALCULATE_EXISTING_POINTS)
         return fail(SG_E_ARG, "bad option");
-    return tag_boundaries_impl(g, nn_type);
+    int rc = tag_boundaries_impl(g, nn_type);
+    if (rc == SG_OK && opt == SG_CALCULATE_EXISTING_POINTS) rc = existing_points_impl(g);
+    return rc;
+}
+
+int sg_existing_points(sg_grid *g, int64_t *n)
+{
+    if (!g || !n) return fail(SG_E_ARG, "null argument");
+    if (!g->ep_valid) {
+        int rc = existing_points_impl(g);
+        if (rc) return rc;
+    }
+    *n = g->ep_n;
+    return SG_OK;
+}
+
+int sg_get_existing_points(sg_grid *g, int64_t *out, int mem)
+{
+    int64_t n = 0;
+    int rc = sg_existing_points(g, &n);
+    if (rc) return rc;
+    if (n == 0) return SG_OK;
+    if (!out) return fail(SG_E_ARG, "null output");
+    SG_HIP(hipMemcpyAsync(out, g->d_ep, (size_t)n * 8, mem == SG_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                                            : hipMemcpyDeviceToHost, g->stream));
+    SG_HIP(hipStreamSynchronize(g->stream));
+    return SG_OK;
 }
 
 int sg_apply_stencil(sg_grid *g, const sg_stencil_desc *desc)

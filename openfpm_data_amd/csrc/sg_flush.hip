@@ -948,6 +948,7 @@ int flush_chunks_impl(sg_grid *g, const int32_t *ops)
     SG_HIP(hipGetLastError());
     g->cins_n = 0;
     g->nn_valid[0] = g->nn_valid[1] = false;
+    g->ep_valid = false;
     return SG_OK;
 }
 
@@ -1079,7 +1080,9 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     SG_HIP(hipGetLastError());
 
     g->ins_n = 0;
-    g->nn_valid[0] = g->nn_valid[1] = false;  // flush marks the neighbour table stale (SparseGridGpu.hpp:778)
+    // flush marks the neighbour table stale (SparseGridGpu.hpp:778), and the point list
+    g->nn_valid[0] = g->nn_valid[1] = false;
+    g->ep_valid = false;
     return SG_OK;
 }
 

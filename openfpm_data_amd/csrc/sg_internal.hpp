@@ -122,6 +122,12 @@ struct sg_grid {
     // read the mask first and skip loading 8-point rows without points
     int fast_lskip = 0;
 
+    // existing points (data positions chunk * csz + offset) for the point-list stencil
+    // mode; rebuilt when stale (tagBoundaries with CALCULATE_EXISTING_POINTS builds it)
+    int64_t *d_ep = nullptr;
+    int64_t ep_n = 0, ep_cap = 0;
+    bool ep_valid = false;
+
     // boxes queued by sg_remove_box, [k][dim] inclusive corners
     std::vector<int32_t> rem_lo, rem_hi;
 
@@ -149,6 +155,8 @@ int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t
                   const void *const *values, int mem);
 int find_neighbours_impl(sg_grid *g, int type);
 int tag_boundaries_impl(sg_grid *g, int type);
+// existing-point list of the point-list stencil mode (d_ep, ep_n)
+int existing_points_impl(sg_grid *g);
 int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d);
 int get_points_impl(sg_grid *g, int prop, int64_t n, const int64_t *keys, void *out, int mem);
 int fill_background_chunk(sg_grid *g, int prop);

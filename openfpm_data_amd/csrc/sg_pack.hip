@@ -625,6 +625,7 @@ int sg_remove_points(sg_grid *g, int64_t n, const int64_t *keys, int mem)
     hipLaunchKernelGGL(remove_points_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_keys, g->nchunks,
                        g->d_mask, dk, n, g->csz, grid_props(g));
     SG_HIP(hipGetLastError());
+    g->ep_valid = false;
     if (mem != SG_MEM_DEVICE) SG_HIP(hipStreamSynchronize(g->stream));  // the host keys are the caller's
     return SG_OK;
 }
@@ -646,6 +647,7 @@ int sg_remove_add_unpack_finalize(sg_grid *g, int nprop, const int32_t *props)
         hipLaunchKernelGGL(remove_boxes_kernel, dim3((unsigned)g->nchunks), dim3(kWG), 0, g->stream, g->d_keys,
                            g->nchunks, g->d_mask, geom_of(g), B, grid_props(g));
         SG_HIP(hipGetLastError());
+        g->ep_valid = false;
         SG_HIP(hipStreamSynchronize(g->stream));  // the box vectors are cleared next
     }
     g->rem_lo.clear();

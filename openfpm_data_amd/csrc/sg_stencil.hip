@@ -295,6 +295,179 @@ __global__ __launch_bounds__(kWG) void stencil_generic_kernel(const int64_t *key
     }
 }
 
+// Point-list stencil (applyStencilInPlaceNoShared, SparseGridGpu_kernels.cuh:1096-1149):
+// one thread per existing point of the list `ep` (data positions chunk * csz + offset),
+// neighbours read straight from the chunk arrays through the neighbour table (no LDS
+// tile).  The arithmetic is stencil_generic_kernel's, term for term, so both modes give
+// the same bits; a missing neighbour chunk is the background chunk.
+template <typename T>
+__device__ inline T point_fetch(const T *src, const int32_t *nnrow, int type, int64_t c, const int *l, int dx, int dy,
+                                int dz, int dim, int edge, int csz)
+{
+    const int dd[3] = {dx, dy, dz};
+    int disp[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+    bool moved = false;
+    for (int d = 0; d < dim; ++d) {
+        int v = l[d] + dd[d];
+        disp[d] = (v < 0) ? -1 : (v >= edge ? 1 : 0);
+        w[d] = v - disp[d] * edge;
+        moved = moved || disp[d] != 0;
+    }
+    int64_t nb = c;
+    if (moved) {
+        if (type == SG_NN_STAR) {
+            int k = 0;
+            for (int d = 0; d < dim; ++d)
+                if (disp[d]) k = 2 * d + (disp[d] > 0 ? 1 : 0);
+            nb = nnrow[k];
+        } else {
+            int k = 0, m = 1;
+            for (int d = 0; d < dim; ++d) {
+                k += (disp[d] + 1) * m;
+                m *= 3;
+            }
+            nb = nnrow[k];
+        }
+    }
+    int off = w[dim - 1];
+    for (int d = dim - 2; d >= 0; --d) off = off * edge + w[d];
+    return src[nb * csz + off];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void stencil_points_kernel(const int64_t *ep, int64_t ne, const int64_t *keys,
+                                                            GeomArgs ga, const int32_t *nn, int nnn, int type,
+                                                            const uint8_t *mask, const T *src0, const T *src1,
+                                                            T *dst0, T *dst1, StencilArgs sa)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne) return;
+    const int dim = ga.dim, edge = ga.edge, csz = ga.csz;
+    const int64_t id = ep[i], c = id / csz;
+    const int o = (int)(id % csz);
+    const int64_t bs[3] = {ga.bsz0, ga.bsz1, ga.bsz2};
+    int l[3] = {0, 0, 0};
+    int64_t pc[3] = {0, 0, 0}, par = 0;
+    {
+        int64_t b = keys[c];
+        int t = o;
+        for (int d = 0; d < dim; ++d) {
+            l[d] = t % edge;
+            t /= edge;
+            pc[d] = (b % bs[d]) * edge + l[d];
+            b /= bs[d];
+            par += pc[d];
+        }
+    }
+    uint8_t cm = mask[id];
+    if (sa.use_box)
+        for (int d = 0; d < dim; ++d)
+            if (pc[d] < sa.lo[d] || pc[d] > sa.hi[d]) cm = 0;
+    bool active = (sa.apply == SG_APPLY_EXISTING) ? (cm != 0) : ((cm & SG_EXIST) && !(cm & SG_PADDING));
+    const int kind = sa.kind;
+    if (kind == SG_ST_RBSOR && (int)(par & 1) != (int)sa.icoef[0]) active = false;
+    if (!active) return;
+    const int32_t *nnrow = nn + c * nnn;
+    auto at = [&](const T *src, int dx, int dy, int dz) {
+        return point_fetch<T>(src, nnrow, type, c, l, dx, dy, dz, dim, edge, csz);
+    };
+    const T cur = src0[id];
+    T r0 = (T)0, r1 = (T)0;
+    if constexpr (!std::is_integral<T>::value) {
+        if (kind == SG_ST_LAP || kind == SG_ST_HEAT || kind == SG_ST_RESIDUAL) {
+            T a;
+            if (kind == SG_ST_LAP) a = (T)sa.coef[0] * cur;
+            else if (kind == SG_ST_HEAT) a = (T)(-2.0 * dim) * cur;
+            else a = (T)(2.0 * dim) * cur;
+            for (int d = 0; d < dim; ++d) {
+                const T pair = at(src0, -(d == 0), -(d == 1), -(d == 2)) + at(src0, d == 0, d == 1, d == 2);
+                if (kind == SG_ST_LAP) a = a + (T)sa.coef[1] * pair;
+                else if (kind == SG_ST_HEAT) a = a + pair;
+                else a = a - pair;
+            }
+            if (kind == SG_ST_LAP) r0 = a;
+            else if (kind == SG_ST_HEAT) r0 = cur + (T)sa.coef[0] * a;
+            else r0 = src1[id] - a;
+        } else if (kind == SG_ST_GRAY_SCOTT) {
+            const T cu = cur, cv = src1[id];
+            T lu = (T)(-2.0 * dim) * cu, lv = (T)(-2.0 * dim) * cv;
+            for (int d = 0; d < dim; ++d) {
+                lu = lu + (at(src0, -(d == 0), -(d == 1), -(d == 2)) + at(src0, d == 0, d == 1, d == 2));
+                lv = lv + (at(src1, -(d == 0), -(d == 1), -(d == 2)) + at(src1, d == 0, d == 1, d == 2));
+            }
+            const T Du = (T)sa.coef[0], Dv = (T)sa.coef[1], F = (T)sa.coef[2], k = (T)sa.coef[3], dt = (T)sa.coef[4];
+            const T uvv = (cu * cv) * cv;
+            r0 = cu + dt * ((Du * lu - uvv) + F * ((T)1 - cu));
+            r1 = cv + dt * ((Dv * lv + uvv) - (F + k) * cv);
+        } else if (kind == SG_ST_RBSOR) {
+            T nbs = 0;
+            for (int d = 0; d < dim; ++d)
+                nbs = nbs + (at(src0, -(d == 0), -(d == 1), -(d == 2)) + at(src0, d == 0, d == 1, d == 2));
+            const T w = (T)sa.coef[0];
+            const T gs = (src1[id] + nbs) / (T)(2 * dim);
+            r0 = cur + w * (gs - cur);
+        }
+    }
+    if (kind == SG_ST_BOX) {
+        T tot = 0;
+        const int nk = (dim == 3) ? 3 : 1;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int k = 0; k < nk; ++k) {
+                    const int ci = (dim == 3) ? (a * 9 + b * 3 + k) : (a * 3 + b);
+                    const T cf = std::is_integral<T>::value ? (T)sa.icoef[ci] : (T)sa.coef[ci];
+                    tot = dev_add<T>(tot, dev_mul<T>(at(src0, a - 1, b - 1, (dim == 3) ? k - 1 : 0), cf));
+                }
+        r0 = tot;
+    }
+    if (dst0) dst0[id] = r0;
+    if (dst1) dst1[id] = r1;
+}
+
+// existing points per chunk, then their data positions in chunk order and increasing
+// offset (calc_exist_points + fill_e_points, SparseGridGpu_kernels.cuh:1150-1233)
+__global__ __launch_bounds__(kWG) void count_exist_kernel(const uint8_t *mask, int csz, uint32_t *cnt)
+{
+    __shared__ uint32_t s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    uint32_t c = 0;
+    for (int o = threadIdx.x; o < csz; o += kWG) c += (mask[(int64_t)blockIdx.x * csz + o] & SG_EXIST) ? 1u : 0u;
+    if (c) atomicAdd(&s, c);
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kWG) void fill_exist_kernel(const uint8_t *mask, int csz, const uint32_t *start,
+                                                        int64_t *out)
+{
+    __shared__ uint32_t s_wave[kWG / 64];
+    __shared__ uint32_t s_run;
+    const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_run = 0;
+    __syncthreads();
+    for (int o0 = 0; o0 < csz; o0 += kWG) {
+        const int o = o0 + threadIdx.x;
+        const bool has = o < csz && (mask[c * csz + o] & SG_EXIST);
+        const uint64_t bal = __ballot(has);
+        if (lane == 0) s_wave[w] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t pre = s_run, tot = 0;
+        for (int q = 0; q < kWG / 64; ++q) {
+            if (q < w) pre += s_wave[q];
+            tot += s_wave[q];
+        }
+        if (has) {
+            const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+            out[start[c] + pre + (uint32_t)__popcll(bal & lt)] = c * csz + o;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_run += tot;
+        __syncthreads();
+    }
+}
+
 // tagBoundaries (SparseGridGpu_kernels.cuh:667-737): padding = an existing point with
 // a star (NNStar::isPadding) or full (NNfull_is_padding_impl) neighbour that does not
 // exist; the enlarged mask block takes missing chunks from the background (mask 0).
@@ -914,6 +1087,39 @@ int tag_boundaries_impl(sg_grid *g, int type)
     return SG_OK;
 }
 
+int existing_points_impl(sg_grid *g)
+{
+    const int64_t n = g->nchunks;
+    g->ep_n = 0;
+    g->ep_valid = false;
+    if (n == 0) {
+        g->ep_valid = true;
+        return SG_OK;
+    }
+    DevBuf &b = g->scratch[9];
+    int rc = ensure(g->pool, b, (size_t)(n + 1) * 8 + 64);
+    if (rc) return rc;
+    uint32_t *cnt = (uint32_t *)b.p, *start = cnt + (n + 1);
+    hipLaunchKernelGGL(count_exist_kernel, dim3((unsigned)n), dim3(kWG), 0, g->stream, g->d_mask, g->csz, cnt);
+    SG_HIP(hipGetLastError());
+    uint32_t tot = 0;
+    if ((rc = exclusive_scan_u32(g, cnt, start, n, &tot))) return rc;
+    if ((int64_t)tot > g->ep_cap) {
+        g->pool.free(g->d_ep);
+        g->d_ep = (int64_t *)g->pool.alloc((size_t)tot * 8);
+        if (!g->d_ep) {
+            g->ep_cap = 0;
+            return fail(SG_E_OOM, "existing-point list allocation failed");
+        }
+        g->ep_cap = tot;
+    }
+    hipLaunchKernelGGL(fill_exist_kernel, dim3((unsigned)n), dim3(kWG), 0, g->stream, g->d_mask, g->csz, start, g->d_ep);
+    SG_HIP(hipGetLastError());
+    g->ep_n = tot;
+    g->ep_valid = true;
+    return SG_OK;
+}
+
 int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
 {
     const int kind = d->kind;
@@ -932,6 +1138,8 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
         if (g->dt[d->dst[q]] != dt) return fail(SG_E_ARG, "stencil properties must share one type");
     if (kind != SG_ST_BOX && !dtype_is_float(dt)) return fail(SG_E_ARG, "this stencil needs f32 or f64 properties");
     if (d->apply != SG_APPLY_INTERIOR && d->apply != SG_APPLY_EXISTING) return fail(SG_E_ARG, "bad apply mode");
+    const bool no_shared = d->mode == SG_STENCIL_MODE_INPLACE_NO_SHARED;
+    if (!no_shared && d->mode != 0 && d->mode != SG_STENCIL_MODE_INPLACE) return fail(SG_E_ARG, "bad stencil mode");
     const int64_t n = g->nchunks;
     if (n == 0) return SG_OK;
     const int type = (kind == SG_ST_BOX) ? SG_NN_FULL : SG_NN_STAR;
@@ -992,6 +1200,35 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
     // aliasing src q (or no aliasing) only
     const bool alias_ok = (ndst == 1) ? (alias[0] == 0 || alias[0] == 1)
                                       : ((alias[0] == 0 && alias[1] == 0) || (alias[0] == 1 && alias[1] == 2));
+    if (no_shared) {
+        // the existing points (built by tagBoundaries with CALCULATE_EXISTING_POINTS, or
+        // here when stale); aliased outputs start as copies of the old arrays, since only
+        // the listed active points are written
+        if (!g->ep_valid && (rc = existing_points_impl(g))) return rc;
+        for (int q = 0; q < ndst; ++q)
+            if (alias[q])
+                SG_HIP(hipMemcpyAsync(outp[q], g->d_prop[d->dst[q]], arr_bytes, hipMemcpyDeviceToDevice, g->stream));
+        if (g->ep_n > 0) {
+            dispatch_type(dt, [&](auto tag) {
+                typedef decltype(tag) T;
+                const T *s0 = (const T *)g->d_prop[d->src[0]];
+                const T *s1 = nsrc > 1 ? (const T *)g->d_prop[d->src[1]] : nullptr;
+                T *o0 = (T *)(ndst > 0 ? outp[0] : (kind == SG_ST_RBSOR ? g->d_prop[d->src[0]] : nullptr));
+                T *o1 = (T *)(ndst > 1 ? outp[1] : nullptr);
+                hipLaunchKernelGGL(stencil_points_kernel<T>, dim3(grid_1d(g->ep_n, kWG)), dim3(kWG), 0, g->stream,
+                                   g->d_ep, g->ep_n, g->d_keys, geom(g), g->d_nn[type], g->nnn(type), type, g->d_mask,
+                                   s0, s1, o0, o1, sa);
+            });
+            SG_HIP(hipGetLastError());
+        }
+        for (int q = 0; q < ndst; ++q) {
+            if (!alias[q]) continue;
+            const int p = d->dst[q];
+            g->pool.free(g->d_prop[p]);
+            g->d_prop[p] = outp[q];
+        }
+        return SG_OK;
+    }
     const bool fast = g->dim == 3 && g->edge == 8 && type == SG_NN_STAR &&
                       (kind == SG_ST_LAP || kind == SG_ST_HEAT || kind == SG_ST_GRAY_SCOTT) && alias_ok;
     dispatch_type(dt, [&](auto tag) {

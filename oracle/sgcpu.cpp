@@ -707,6 +707,10 @@ int apply_stencil_t(Grid &g, const sg_stencil_desc *ds)
                         if (pc[d] < ds->box_lo[d] || pc[d] > ds->box_hi[d]) cm = 0;
                 bool active = (ds->apply == SG_APPLY_EXISTING) ? (cm != 0)
                                                                 : ((cm & SG_EXIST) && !(cm & SG_PADDING));
+                // applyStencilInPlaceNoShared runs over e_points, the EXIST points only
+                // (SparseGridGpu.hpp:1051-1075, fill_e_points SparseGridGpu_kernels.cuh:1150-1185)
+                if (ds->mode == SG_STENCIL_MODE_INPLACE_NO_SHARED && !(g.mask[c * g.csz + o] & SG_EXIST))
+                    active = false;
                 if (!active) continue;
                 const T *u = eb[0].data();
                 T cur = u[lin];

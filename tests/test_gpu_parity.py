@@ -1,6 +1,8 @@
 """GPU parity: libsg_hip.so (through its C ABI) against the oracle on the same seeded
 inputs.  Integer/index/mask results bit-exact; FP results bit-exact as well (both sides
 evaluate the same operation order without FMA contraction)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -306,3 +308,69 @@ def test_blobs1024_full_flush_tag_box(G, oracle):
         gs.append(g)
     assert gs[0].num_chunks() == gs[1].num_chunks() == 317793
     S.assert_same_state(gs[0], gs[1])
+
+
+def _no_shared_descs(rng, dim, size, kinds):
+    out = []
+    for kind in kinds:
+        lo = [int(rng.integers(0, s // 2)) for s in size]
+        hi = [int(rng.integers(l, s)) for l, s in zip(lo, size)]
+        box = (lo, hi) if rng.random() < 0.5 else None
+        apply = int(rng.choice([C.APPLY_INTERIOR, C.APPLY_EXISTING]))
+        if kind == C.ST_LAP:
+            out.append(C.make_desc(kind, [0], [1], apply, box, [-6.0, 1.0]))
+        elif kind == C.ST_HEAT:
+            out.append(C.make_desc(kind, [1], [1], apply, box, [0.125]))  # in place (Jacobi)
+        elif kind == C.ST_GRAY_SCOTT:
+            out.append(C.make_desc(kind, [0, 1], [2, 1], apply, box, [0.2, 0.1, 0.04, 0.06, 1.0]))
+        elif kind == C.ST_RBSOR:
+            out.append(C.make_desc(kind, [2, 0], [], apply, box, [1.3], [int(rng.integers(0, 2))]))
+        elif kind == C.ST_RESIDUAL:
+            out.append(C.make_desc(kind, [2, 0], [1], apply, box))
+        else:
+            out.append(C.make_desc(kind, [0], [1], apply, box, (), [int(v) for v in rng.integers(-3, 4, 27)]))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_no_shared_mode(G, oracle, seed):
+    """STENCIL_MODE_INPLACE_NO_SHARED (one thread per existing point, no LDS tile): same
+    state as the oracle in that mode (which writes only EXIST points), and the same values
+    as the LDS path on every existing point.  Random 2D/3D grids with padding and
+    padding-only points, boxes, both apply modes, in-place outputs."""
+    rng = np.random.default_rng(700 + seed)
+    dim = 3 if seed % 2 else 2
+    edge = [2, 4, 8][seed % 3]
+    size = [int(rng.integers(2 * edge, 6 * edge)) for _ in range(dim)]
+    integer = seed % 4 == 3
+    dtypes = [np.int32] * 3 if integer else [[np.float64] * 3, [np.float32] * 3][seed % 2]
+    kinds = [C.ST_BOX] if integer else [C.ST_LAP, C.ST_HEAT, C.ST_GRAY_SCOTT, C.ST_RBSOR, C.ST_RESIDUAL]
+    steps = S.random_flush_program(seed, dim, edge, size, dtypes, nflush=2, nrec=250,
+                                   ops=["smax"] * len(dtypes))
+    descs = _no_shared_descs(rng, dim, size, kinds)
+    grids = []
+    for cls in (G, oracle.SparseGridCpu, G):
+        g = S.run_flush_program(cls, dim, edge, size, dtypes, steps, background=[1, 2, 3])
+        g.find_neighbours(C.NN_STAR)
+        g.tag_boundaries(C.NN_STAR, C.CALCULATE_EXISTING_POINTS)
+        grids.append(g)
+    a, b, lds = grids
+    # the point list: EXIST points in chunk order, increasing offset
+    n = ctypes.c_int64()
+    C.check(a._lib.sg_existing_points(a._h, ctypes.byref(n)))
+    ep = np.zeros(max(n.value, 1), np.int64)
+    C.check(a._lib.sg_get_existing_points(a._h, ep.ctypes.data, C.SG_MEM_HOST))
+    c, o = np.nonzero(a.masks() & 1)
+    np.testing.assert_array_equal(ep[:n.value], c * a.chunk_size + o)
+    for d in descs:
+        # the LDS path starts from the same state (it also writes padding-only points
+        # under APPLY_EXISTING, which later stencils read)
+        for p in range(len(dtypes)):
+            lds.set_prop(p, a.prop(p))
+        d.mode = C.STENCIL_MODE_INPLACE_NO_SHARED
+        a.apply_stencil(d)
+        b.apply_stencil(d)
+        d.mode = C.STENCIL_MODE_INPLACE
+        lds.apply_stencil(d)
+        S.assert_same_state(a, b, exist_only=False)
+        S.assert_same_state(a, lds)

@@ -355,3 +355,29 @@ def test_rbsor_colours_and_residual(oracle):
     h.flush(["smax"] * 3)
     h.residual(0, 1, 2)
     assert h.get(2, h.lin_id([[5, 5, 5]]))[0] == 0.0
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_no_shared_mode_is_shared_on_existing_points(oracle, seed):
+    """The point-list mode (applyStencilInPlaceNoShared) runs the same stencil over the EXIST
+    points only: equal to the LDS mode there, and non-EXIST slots keep their values."""
+    rng = np.random.default_rng(40 + seed)
+    dim, edge = 3, [2, 4, 8][seed]
+    size = [int(rng.integers(2 * edge, 5 * edge)) for _ in range(dim)]
+    dtypes = [np.float64] * 3
+    steps = S.random_flush_program(seed, dim, edge, size, dtypes, nflush=2, nrec=200, ops=["smax"] * 3)
+    for kind, src, dst, coef in [(C.ST_LAP, [0], [1], [-6.0, 1.0]), (C.ST_HEAT, [0], [1], [0.1]),
+                                 (C.ST_GRAY_SCOTT, [0, 1], [2, 1], [0.2, 0.1, 0.04, 0.06, 1.0])]:
+        for apply in (C.APPLY_INTERIOR, C.APPLY_EXISTING):
+            gs = []
+            for mode in (C.STENCIL_MODE_INPLACE, C.STENCIL_MODE_INPLACE_NO_SHARED):
+                g = S.run_flush_program(oracle.SparseGridCpu, dim, edge, size, dtypes, steps)
+                g.tag_boundaries(C.NN_STAR)
+                before = [g.prop(p) for p in range(3)]
+                g.apply_stencil(C.make_desc(kind, src, dst, apply, None, coef, mode=mode))
+                gs.append((g, before))
+            (a, _), (b, before) = gs
+            ex = (b.masks() & 1) != 0
+            for p in range(3):
+                np.testing.assert_array_equal(a.prop(p)[ex], b.prop(p)[ex])
+                np.testing.assert_array_equal(b.prop(p)[~ex], before[p][~ex])
