@@ -146,6 +146,13 @@ typedef struct {
 int sg_create(int dim, const int64_t *size, int block_edge, int nprops,
               const int32_t *dtypes, int stencil_radius, sg_grid **out);
 int sg_destroy(sg_grid *g);
+/* chunk linearizer: SG_LIN_SMB = grid_smb, row-major chunk ids (SparseGridGpu, the
+ * default); SG_LIN_ZMB = grid_zmb, Morton chunk ids (SparseGridGpu_z,
+ * src/SparseGridGpu/SparseGridGpu.hpp:3947-3972, src/Grid/Geometry/grid_zmb.hpp).  Only on
+ * an empty grid (no chunks, nothing pending). */
+#define SG_LIN_SMB 0
+#define SG_LIN_ZMB 1
+int sg_set_linearizer(sg_grid *g, int lin);
 const char *sg_last_error(void);
 /* set the HIP stream (hipStream_t as void*) all later work goes to; NULL = default stream */
 int sg_set_stream(sg_grid *g, void *stream);

@@ -137,6 +137,7 @@ _INT = ctypes.c_int
 _SIGS = {
     "sg_create": (_INT, [_INT, ctypes.POINTER(_I64), _INT, _INT, ctypes.POINTER(_I32), _INT, ctypes.POINTER(_P)]),
     "sg_destroy": (_INT, [_P]),
+    "sg_set_linearizer": (_INT, [_P, _INT]),
     "sg_last_error": (ctypes.c_char_p, []),
     "sg_set_stream": (_INT, [_P, _P]),
     "sg_synchronize": (_INT, [_P]),

@@ -196,6 +196,27 @@ int sg_set_stream(sg_grid *g, void *stream)
     return SG_OK;
 }
 
+int sg_set_linearizer(sg_grid *g, int lin)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    if (lin != SG_LIN_SMB && lin != SG_LIN_ZMB) return fail(SG_E_ARG, "unknown linearizer");
+    if (g->nchunks != 0 || g->ins_n != 0 || g->cins_n != 0)
+        return fail(SG_E_STATE, "the linearizer can only change on an empty grid");
+    if (lin == SG_LIN_ZMB) {
+        // Morton ids of the domain's chunks: the largest must leave room for point keys
+        const int64_t limit_bits = 62;
+        int64_t maxb = 0;
+        for (int d = 0; d < g->dim; ++d) maxb = std::max(maxb, g->bsz[d] - 1);
+        int bits = 0;
+        while (bits < 63 && (((int64_t)1 << bits) <= maxb)) ++bits;
+        int cbits = 0;
+        while (((int64_t)1 << cbits) < g->csz) ++cbits;
+        if (bits * g->dim + cbits > limit_bits) return fail(SG_E_RANGE, "domain too large for Morton chunk ids");
+    }
+    g->lin = lin;
+    return SG_OK;
+}
+
 int sg_synchronize(sg_grid *g)
 {
     if (!g) return fail(SG_E_ARG, "null grid");

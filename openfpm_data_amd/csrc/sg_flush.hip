@@ -76,7 +76,7 @@ template <typename T> __global__ void fill_kernel(T *p, T v, int64_t n)
 
 // coordinates -> grid_smb linear ids (grid_smb.hpp:175-189); out-of-range -> flag
 __global__ void coords_to_keys_kernel(const int32_t *coords, int64_t *keys, int64_t n, int dim, int edge, int64_t sz0,
-                                      int64_t sz1, int64_t sz2, int64_t bs0, int64_t bs1, int csz, int *bad)
+                                      int64_t sz1, int64_t sz2, ChunkLin L, int csz, int *bad)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -92,15 +92,12 @@ __global__ void coords_to_keys_kernel(const int32_t *coords, int64_t *keys, int6
         keys[i] = 0;
         return;
     }
-    int64_t b, l;
-    if (dim == 3) {
-        b = ((c[2] / edge) * bs1 + c[1] / edge) * bs0 + c[0] / edge;
-        l = ((c[2] % edge) * edge + c[1] % edge) * edge + c[0] % edge;
-    } else {
-        b = (c[1] / edge) * bs0 + c[0] / edge;
-        l = (c[1] % edge) * edge + c[0] % edge;
+    int64_t bc[3] = {0, 0, 0}, l = 0;
+    for (int d = dim - 1; d >= 0; --d) {
+        bc[d] = c[d] / edge;
+        l = l * edge + c[d] % edge;
     }
-    keys[i] = b * csz + l;
+    keys[i] = chunk_id(L, bc) * csz + l;
 }
 
 __global__ void check_keys_kernel(const int64_t *keys, int64_t n, int64_t limit, int *bad)
@@ -477,11 +474,13 @@ int grow_insert_buffer(sg_grid *g, int64_t need)
 
 hipMemcpyKind kind_from(int mem) { return mem == SG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice; }
 
+// one past the largest point key of the domain (chunk ids grow with every chunk
+// coordinate under both linearizers, so the last chunk has the largest id)
 int64_t key_limit(const sg_grid *g)
 {
-    int64_t nb = 1;
-    for (int d = 0; d < g->dim; ++d) nb *= g->bsz[d];
-    return nb * g->csz;
+    int64_t last[3] = {0, 0, 0};
+    for (int d = 0; d < g->dim; ++d) last[d] = g->bsz[d] - 1;
+    return (chunk_id(chunk_lin(g), last) + 1) * g->csz;
 }
 
 // append masks and values of n records at insert position `at`
@@ -565,8 +564,8 @@ int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *m
     bad = (int *)(((uintptr_t)bad + 7) & ~(uintptr_t)7);
     SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
     hipLaunchKernelGGL(coords_to_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, dc,
-                       g->d_ins_key + at, n, g->dim, g->edge, g->sz[0], g->sz[1], g->sz[2], g->bsz[0], g->bsz[1],
-                       g->csz, bad);
+                       g->d_ins_key + at, n, g->dim, g->edge, g->sz[0], g->sz[1], g->sz[2], chunk_lin(g), g->csz,
+                       bad);
     SG_HIP(hipGetLastError());
     if (mem == SG_MEM_DEVICE) {
         int hbad = 0;

@@ -128,6 +128,9 @@ struct sg_grid {
     int64_t ep_n = 0, ep_cap = 0;
     bool ep_valid = false;
 
+    // chunk linearizer: SG_LIN_SMB (row-major) or SG_LIN_ZMB (Morton)
+    int lin = SG_LIN_SMB;
+
     // boxes queued by sg_remove_box, [k][dim] inclusive corners
     std::vector<int32_t> rem_lo, rem_hi;
 
@@ -140,6 +143,60 @@ struct sg_grid {
 };
 
 namespace sg {
+
+// ---- chunk linearizers -----------------------------------------------------------
+// grid_smb (src/Grid/Geometry/grid_smb.hpp:175-297): chunk id row-major, x fastest.
+// grid_zmb (src/Grid/Geometry/grid_zmb.hpp:65-106, util/zmorton.hpp): chunk id = Morton
+// code, bit i of the chunk coordinate d at bit dim*i + d.  Inside a chunk the local id is
+// x fastest for both.
+struct ChunkLin {
+    int dim, zorder;
+    int64_t bsz[3];
+};
+
+__host__ __device__ inline uint64_t zspread(uint64_t v, int dim)
+{
+    uint64_t r = 0;
+    const int nbits = 64 / dim;
+    for (int i = 0; i < nbits; ++i) r |= ((v >> i) & 1ull) << (dim * i);
+    return r;
+}
+
+__host__ __device__ inline uint64_t zcompact(uint64_t v, int dim)
+{
+    uint64_t r = 0;
+    const int nbits = 64 / dim;
+    for (int i = 0; i < nbits; ++i) r |= ((v >> (dim * i)) & 1ull) << i;
+    return r;
+}
+
+// chunk coordinates of chunk id `id`
+__host__ __device__ inline void chunk_coords(const ChunkLin &L, int64_t id, int64_t *bc)
+{
+    if (L.zorder) {
+        for (int d = 0; d < L.dim; ++d) bc[d] = (int64_t)zcompact((uint64_t)id >> d, L.dim);
+        return;
+    }
+    for (int d = 0; d < L.dim; ++d) {
+        bc[d] = id % L.bsz[d];
+        id /= L.bsz[d];
+    }
+}
+
+// chunk id of chunk coordinates inside the domain
+__host__ __device__ inline int64_t chunk_id(const ChunkLin &L, const int64_t *bc)
+{
+    if (L.zorder) {
+        uint64_t r = 0;
+        for (int d = 0; d < L.dim; ++d) r |= zspread((uint64_t)bc[d], L.dim) << d;
+        return (int64_t)r;
+    }
+    int64_t v = bc[L.dim - 1];
+    for (int d = L.dim - 2; d >= 0; --d) v = v * L.bsz[d] + bc[d];
+    return v;
+}
+
+inline ChunkLin chunk_lin(const sg_grid *g);
 
 // host launchers implemented in the .hip translation units
 int flush_impl(sg_grid *g, const int32_t *ops);
@@ -175,5 +232,14 @@ int radix_sort_pairs_u32(sg_grid *g, uint32_t *keys, uint32_t *vals, uint32_t *k
                          uint32_t **vals_out);
 
 inline unsigned grid_1d(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+inline ChunkLin chunk_lin(const sg_grid *g)
+{
+    ChunkLin L;
+    L.dim = g->dim;
+    L.zorder = g->lin == SG_LIN_ZMB;
+    for (int d = 0; d < 3; ++d) L.bsz[d] = g->bsz[d];
+    return L;
+}
 
 }  // namespace sg

@@ -32,15 +32,13 @@ constexpr int kWG = 256;
 
 struct Geom {
     int dim, edge, csz;
-    int64_t bsz[3];
+    ChunkLin L;
 };
 
 __device__ inline void chunk_origin(const Geom &G, int64_t id, int64_t *o)
 {
-    for (int d = 0; d < G.dim; ++d) {
-        o[d] = (id % G.bsz[d]) * G.edge;
-        id /= G.bsz[d];
-    }
+    chunk_coords(G.L, id, o);
+    for (int d = 0; d < G.dim; ++d) o[d] *= G.edge;
 }
 
 __device__ inline void local_coord(const Geom &G, int off, int64_t *l)
@@ -224,11 +222,11 @@ __global__ __launch_bounds__(kWG) void unpack_records_kernel(UnpackIn U, int64_t
         copy_scalar((unsigned char *)&k4, U.ids + i * 4, 4);
         id = k4;
     }
+    // chunk coordinates in the packing grid (same linearizer, its own chunk counts)
+    ChunkLin PL = G.L;
+    for (int d = 0; d < 3; ++d) PL.bsz[d] = U.pbsz[d];
     int64_t bc[3] = {0, 0, 0};
-    for (int d = 0; d < G.dim; ++d) {
-        bc[d] = id % U.pbsz[d];
-        id /= U.pbsz[d];
-    }
+    chunk_coords(PL, id, bc);
     const int64_t sz[3] = {sz0, sz1, sz2};
     // a malformed section (scan not increasing or past n_pnt) is refused, not followed
     const int64_t j0 = U.scan[i], j1 = U.scan[i + 1];
@@ -251,12 +249,12 @@ __global__ __launch_bounds__(kWG) void unpack_records_kernel(UnpackIn U, int64_t
             rmask[j] = 0;
             continue;
         }
-        int64_t b = c[G.dim - 1] / G.edge, lo = c[G.dim - 1] % G.edge;
-        for (int d = G.dim - 2; d >= 0; --d) {
-            b = b * G.bsz[d] + c[d] / G.edge;
+        int64_t cb[3] = {0, 0, 0}, lo = 0;
+        for (int d = G.dim - 1; d >= 0; --d) {
+            cb[d] = c[d] / G.edge;
             lo = lo * G.edge + c[d] % G.edge;
         }
-        rkey[j] = b * G.csz + lo;
+        rkey[j] = chunk_id(G.L, cb) * G.csz + lo;
         rmask[j] = SG_EXIST;
         for (int q = 0; q < P.np; ++q) {
             unsigned char *dst = (unsigned char *)P.p[q] + (int64_t)j * P.ts[q];
@@ -320,7 +318,7 @@ Geom geom_of(const sg_grid *g)
     G.dim = g->dim;
     G.edge = g->edge;
     G.csz = g->csz;
-    for (int d = 0; d < 3; ++d) G.bsz[d] = g->bsz[d];
+    G.L = chunk_lin(g);
     return G;
 }
 

@@ -47,7 +47,19 @@ template <typename T> __device__ inline T dev_mul(T a, T b)
 struct GeomArgs {
     int dim, edge, csz;
     int64_t bsz0, bsz1, bsz2;
+    int zorder;  // chunk ids are Morton codes (grid_zmb)
 };
+
+__device__ inline ChunkLin lin_of(const GeomArgs &ga)
+{
+    ChunkLin L;
+    L.dim = ga.dim;
+    L.zorder = ga.zorder;
+    L.bsz[0] = ga.bsz0;
+    L.bsz[1] = ga.bsz1;
+    L.bsz[2] = ga.bsz2;
+    return L;
+}
 
 __device__ inline int64_t lower_bound_dev(const int64_t *a, int64_t n, int64_t x)
 {
@@ -82,12 +94,12 @@ __global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs 
     const int dim = ga.dim;
     int64_t bs[3] = {ga.bsz0, ga.bsz1, ga.bsz2};
     int64_t bc[3] = {0, 0, 0};
-    int64_t b = keys[c];
-    for (int d = 0; d < dim; ++d) {
-        bc[d] = b % bs[d];
-        b /= bs[d];
-    }
+    const ChunkLin L = lin_of(ga);
+    chunk_coords(L, keys[c], bc);
     auto block_lin_id = [&](const int64_t *q) -> int64_t {
+        // grid_zmb::BlockLinId is lin_zid without range checks: coordinates outside the
+        // domain give ids of chunks that cannot exist (a -1 spreads to a huge code)
+        if (L.zorder) return chunk_id(L, q);
         int64_t v = q[dim - 1];
         if (v >= bs[dim - 1]) return -1;
         for (int d = dim - 2; d >= 0; --d) {
@@ -174,6 +186,7 @@ struct StencilArgs {
     double bgd[2];  // destination backgrounds
     double bgs[2];  // source backgrounds
     int lskip;      // 8^3 fast path: read the mask first and skip loading rows without points
+    int zorder;     // chunk ids are Morton codes (grid_zmb)
     double coef[27];
     int64_t icoef[27];
 };
@@ -199,15 +212,9 @@ __global__ __launch_bounds__(kWG) void stencil_generic_kernel(const int64_t *key
     load_enlarged<T>(eb0, src0, nnrow, type, c, n, dim, edge, csz, bg0);
     if (src1) load_enlarged<T>(eb1, src1, nnrow, type, c, n, dim, edge, csz, bg1);
     __syncthreads();
-    int64_t bs[3] = {ga.bsz0, ga.bsz1, ga.bsz2};
     int64_t base[3] = {0, 0, 0};
-    {
-        int64_t b = keys[c];
-        for (int d = 0; d < dim; ++d) {
-            base[d] = (b % bs[d]) * edge;
-            b /= bs[d];
-        }
-    }
+    chunk_coords(lin_of(ga), keys[c], base);
+    for (int d = 0; d < dim; ++d) base[d] *= edge;
     const int kind = sa.kind;
     for (int o = threadIdx.x; o < csz; o += blockDim.x) {
         int t = o, lin = 0, acc = 1;
@@ -345,17 +352,15 @@ __global__ __launch_bounds__(kWG) void stencil_points_kernel(const int64_t *ep, 
     const int dim = ga.dim, edge = ga.edge, csz = ga.csz;
     const int64_t id = ep[i], c = id / csz;
     const int o = (int)(id % csz);
-    const int64_t bs[3] = {ga.bsz0, ga.bsz1, ga.bsz2};
     int l[3] = {0, 0, 0};
     int64_t pc[3] = {0, 0, 0}, par = 0;
+    chunk_coords(lin_of(ga), keys[c], pc);
     {
-        int64_t b = keys[c];
         int t = o;
         for (int d = 0; d < dim; ++d) {
             l[d] = t % edge;
             t /= edge;
-            pc[d] = (b % bs[d]) * edge + l[d];
-            b /= bs[d];
+            pc[d] = pc[d] * edge + l[d];
             par += pc[d];
         }
     }
@@ -748,11 +753,15 @@ __device__ inline void process_chunk(const ChunkIn<T, KIND> &in, int64_t c, T *t
 #pragma unroll
         for (int x = 0; x < 8; ++x) inbox[x] = true;
         if (sa.use_box) {
-            int64_t kb = keys[c];
-            const int64_t bx = kb % bsz0;
-            kb /= bsz0;
-            const int64_t by = kb % bsz1;
-            const int64_t bz = kb / bsz1;
+            ChunkLin L;
+            L.dim = 3;
+            L.zorder = sa.zorder;
+            L.bsz[0] = bsz0;
+            L.bsz[1] = bsz1;
+            L.bsz[2] = 1 << 30;
+            int64_t bcd[3];
+            chunk_coords(L, keys[c], bcd);
+            const int64_t bx = bcd[0], by = bcd[1], bz = bcd[2];
             const int64_t py = by * 8 + (lane & 7), pz = bz * 8 + (lane >> 3);
             const bool yz = py >= sa.lo[1] && py <= sa.hi[1] && pz >= sa.lo[2] && pz <= sa.hi[2];
 #pragma unroll
@@ -1042,6 +1051,7 @@ GeomArgs geom(const sg_grid *g)
     ga.bsz0 = g->bsz[0];
     ga.bsz1 = g->bsz[1];
     ga.bsz2 = g->bsz[2];
+    ga.zorder = g->lin == SG_LIN_ZMB;
     return ga;
 }
 
@@ -1154,6 +1164,7 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
     sa.bgd[0] = sa.bgd[1] = 0.0;
     sa.bgs[0] = sa.bgs[1] = 0.0;
     sa.lskip = g->fast_lskip;
+    sa.zorder = g->lin == SG_LIN_ZMB;
     for (int q = 0; q < nsrc && q < 2; ++q) {
         if (g->dt[d->src[q]] == SG_F32) {
             float f;

@@ -13,6 +13,32 @@ import numpy as np
 from . import _capi as C
 
 
+LINEARIZERS = {"smb": 0, "zmb": 1}
+
+
+def morton_encode(bc, dim):
+    """grid_zmb chunk ids (lin_zid, util/zmorton.hpp): bit i of chunk coordinate d goes to
+    bit dim*i + d.  bc: [n, dim] integer array."""
+    bc = np.asarray(bc, dtype=np.int64).reshape(-1, dim).astype(np.uint64)
+    out = np.zeros(bc.shape[0], dtype=np.uint64)
+    for d in range(dim):
+        for i in range(64 // dim):
+            out |= ((bc[:, d] >> np.uint64(i)) & np.uint64(1)) << np.uint64(dim * i + d)
+    return out.astype(np.int64)
+
+
+def morton_decode(ids, dim):
+    """invlin_zid: chunk ids -> [n, dim] chunk coordinates."""
+    v = np.asarray(ids, dtype=np.int64).reshape(-1).astype(np.uint64)
+    out = np.zeros((v.size, dim), dtype=np.int64)
+    for d in range(dim):
+        acc = np.zeros(v.size, dtype=np.uint64)
+        for i in range(64 // dim):
+            acc |= ((v >> np.uint64(dim * i + d)) & np.uint64(1)) << np.uint64(i)
+        out[:, d] = acc.astype(np.int64)
+    return out
+
+
 def _is_device(x):
     return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
 
@@ -42,7 +68,7 @@ class SparseGridGpu:
     SparseGridGpu.hpp:59-64).
     """
 
-    def __init__(self, dim, size, props, block_edge=8, stencil_radius=1):
+    def __init__(self, dim, size, props, block_edge=8, stencil_radius=1, linearizer="smb"):
         self._lib = C.load()
         self.dim = int(dim)
         self.size = tuple(int(s) for s in size)
@@ -56,6 +82,10 @@ class SparseGridGpu:
         C.check(self._lib.sg_create(self.dim, sz, self.block_edge, len(self.dtypes), codes,
                                     int(stencil_radius), ctypes.byref(h)))
         self._h = h
+        # grid_smb (SparseGridGpu) or grid_zmb (SparseGridGpu_z) chunk ids
+        self.linearizer = linearizer
+        if linearizer != "smb":
+            C.check(self._lib.sg_set_linearizer(h, LINEARIZERS[linearizer]))
 
     # -- lifetime ---------------------------------------------------------------------
     def close(self):
@@ -90,16 +120,20 @@ class SparseGridGpu:
         for d in range(self.dim - 2, -1, -1):
             b = b * self.block_sizes[d] + c[:, d] // e
             l = l * e + c[:, d] % e
+        if self.linearizer == "zmb":
+            b = morton_encode(c // e, self.dim)
         return b * self.chunk_size + l
 
     def inv_lin_id(self, keys):
         """grid_smb::InvLinId: point keys -> [n, dim] coordinates."""
-        k = np.asarray(keys, dtype=np.int64)
+        k = np.asarray(keys, dtype=np.int64).reshape(-1)
         b = k // self.chunk_size
         l = k % self.chunk_size
         out = np.empty((k.size, self.dim), dtype=np.int64)
+        bc = morton_decode(b, self.dim) if self.linearizer == "zmb" else None
         for d in range(self.dim):
-            out[:, d] = (b % self.block_sizes[d]) * self.block_edge + l % self.block_edge
+            cb = bc[:, d] if bc is not None else b % self.block_sizes[d]
+            out[:, d] = cb * self.block_edge + l % self.block_edge
             b //= self.block_sizes[d]
             l //= self.block_edge
         return out

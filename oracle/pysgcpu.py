@@ -45,6 +45,8 @@ _SIGS = {
     "sgcpu_lin_id": (_I64, [_P, ctypes.POINTER(_I64)]),
     "sgcpu_inv_lin_id": (_INT, [_P, _I64, ctypes.POINTER(_I64)]),
     "sgcpu_set_threads": (_INT, [_INT]),
+    "sgcpu_set_linearizer": (_INT, [_P, _INT]),
+    "sgcpu_morton": (ctypes.c_uint64, [_INT, ctypes.POINTER(_I64)]),
     "sgcpu_pack_size": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),
     "sgcpu_pack": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, _P, _I64]),
     "sgcpu_unpack": (_INT, [_P, _P, _I64, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),
@@ -91,7 +93,7 @@ def _boxes(boxes, dim):
 
 
 class SparseGridCpu:
-    def __init__(self, dim, size, props, block_edge=8, stencil_radius=1):
+    def __init__(self, dim, size, props, block_edge=8, stencil_radius=1, linearizer="smb"):
         self._lib = load()
         self.dim = int(dim)
         self.size = tuple(int(s) for s in size)
@@ -105,6 +107,9 @@ class SparseGridCpu:
         _check(self._lib.sgcpu_create(self.dim, sz, self.block_edge, len(self.dtypes), codes, int(stencil_radius),
                                       ctypes.byref(h)))
         self._h = h
+        self.linearizer = linearizer
+        if linearizer != "smb":
+            _check(self._lib.sgcpu_set_linearizer(h, {"smb": 0, "zmb": 1}[linearizer]))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -129,8 +134,16 @@ class SparseGridCpu:
         return out
 
     def inv_lin_id(self, keys):
-        """grid_smb::InvLinId (numpy form of the oracle's restatement)."""
+        """grid_smb::InvLinId (numpy form of the oracle's restatement); grid_zmb through the
+        oracle's C restatement."""
         k = np.asarray(keys, dtype=np.int64).reshape(-1)
+        if self.linearizer != "smb":
+            out = np.empty((k.size, self.dim), dtype=np.int64)
+            buf = (ctypes.c_int64 * 3)()
+            for i in range(k.size):
+                _check(self._lib.sgcpu_inv_lin_id(self._h, int(k[i]), buf))
+                out[i] = buf[:self.dim]
+            return out
         b = k // self.chunk_size
         l = k % self.chunk_size
         out = np.empty((k.size, self.dim), dtype=np.int64)
@@ -311,3 +324,9 @@ class SparseGridCpu:
 
     def remove_add_unpack_reset(self):
         _check(self._lib.sgcpu_remove_add_unpack_reset(self._h))
+
+
+def morton(dim, coord):
+    """lin_zid of one chunk coordinate (the oracle's restatement of util/zmorton.hpp)."""
+    c = (ctypes.c_int64 * 3)(*(list(int(v) for v in coord) + [0] * (3 - dim)))
+    return int(load().sgcpu_morton(dim, c))

@@ -148,10 +148,40 @@ struct Grid {
 
     unsigned char *val(int p, int64_t chunk, int off) { return &data[p][(chunk * csz + off) * ts[p]]; }
 
+    // chunk linearizer: false = grid_smb (row-major), true = grid_zmb (Morton)
+    bool zorder = false;
+
+    // ---- grid_zmb (src/Grid/Geometry/grid_zmb.hpp:65-106; util/zmorton.hpp) -----
+    // lin_zid: bit i of chunk coordinate d goes to bit dim*i + d (64/dim bits per axis)
+    static uint64_t morton(const int64_t *bc, int dim)
+    {
+        uint64_t r = 0;
+        for (int d = 0; d < dim; ++d)
+            for (int i = 0; i < 64 / dim; ++i) r |= (((uint64_t)bc[d] >> i) & 1ull) << (dim * i + d);
+        return r;
+    }
+    // invlin_zid
+    static void inv_morton(uint64_t lin, int64_t *bc, int dim)
+    {
+        for (int d = 0; d < dim; ++d) {
+            uint64_t v = 0;
+            for (int i = 0; i < 64 / dim; ++i) v |= ((lin >> (dim * i + d)) & 1ull) << i;
+            bc[d] = (int64_t)v;
+        }
+    }
+
     // ---- grid_smb (src/Grid/Geometry/grid_smb.hpp) ----------------------------
-    // LinId, grid_smb.hpp:175-189
+    // LinId, grid_smb.hpp:175-189 (grid_zmb.hpp:65-78: Morton chunk id, same local id)
     int64_t lin_id(const int64_t *c) const
     {
+        if (zorder) {
+            int64_t bc[3] = {0, 0, 0}, l = 0;
+            for (int d = dim - 1; d >= 0; --d) {
+                bc[d] = c[d] / edge;
+                l = l * edge + c[d] % edge;
+            }
+            return (int64_t)morton(bc, dim) * csz + l;
+        }
         int64_t b = c[dim - 1] / edge, l = c[dim - 1] % edge;
         for (int d = dim - 2; d >= 0; --d) {
             b = b * bsz[d] + c[d] / edge;
@@ -159,20 +189,23 @@ struct Grid {
         }
         return b * csz + l;
     }
-    // InvLinId, grid_smb.hpp:240-255
+    // InvLinId, grid_smb.hpp:240-255 (grid_zmb.hpp:80-97)
     void inv_lin(int64_t lin, int64_t *c) const
     {
         int64_t b = lin / csz, l = lin % csz;
+        int64_t bc[3] = {0, 0, 0};
+        block_inv(b, bc);
         for (int d = 0; d < dim; ++d) {
-            c[d] = (b % bsz[d]) * edge + l % edge;
-            b /= bsz[d];
+            c[d] = bc[d] * edge + l % edge;
             l /= edge;
         }
     }
     // BlockLinId, grid_smb.hpp:258-276: -1 for a coordinate >= blockSz, no check
-    // for negative coordinates.
+    // for negative coordinates.  grid_zmb::BlockLinId (grid_zmb.hpp:99-103) is
+    // lin_zid with no check at all.
     int64_t block_lin_id(const int64_t *bc) const
     {
+        if (zorder) return (int64_t)morton(bc, dim);
         int64_t b = bc[dim - 1];
         if (b >= bsz[dim - 1]) return -1;
         for (int d = dim - 2; d >= 0; --d) {
@@ -182,9 +215,13 @@ struct Grid {
         }
         return b;
     }
-    // BlockInvLinId, grid_smb.hpp:313-323
+    // BlockInvLinId, grid_smb.hpp:313-323 (grid_zmb.hpp:105-111)
     void block_inv(int64_t b, int64_t *bc) const
     {
+        if (zorder) {
+            inv_morton((uint64_t)b, bc, dim);
+            return;
+        }
         for (int d = 0; d < dim; ++d) {
             bc[d] = b % bsz[d];
             b /= bsz[d];
@@ -1222,10 +1259,12 @@ int sgcpu_unpack(void *h, const void *buf, int64_t avail, const int32_t *dst_lo,
             id = id4;
         }
         int64_t bc[3], b = id;
-        for (int d = 0; d < g.dim; ++d) {
-            bc[d] = b % pbsz[d];
-            b /= pbsz[d];
-        }
+        if (g.zorder) Grid::inv_morton((uint64_t)id, bc, g.dim);
+        else
+            for (int d = 0; d < g.dim; ++d) {
+                bc[d] = b % pbsz[d];
+                b /= pbsz[d];
+            }
         for (uint32_t j = sc[i]; j < sc[i + 1]; ++j) {
             int16_t o16;
             memcpy(&o16, s0 + off_at + j * 2, 2);
@@ -1314,6 +1353,18 @@ int sgcpu_remove_add_unpack_reset(void *h)
     clear_inserts(g);
     return SG_OK;
 }
+
+int sgcpu_set_linearizer(void *h, int lin)
+{
+    Grid &g = *(Grid *)h;
+    if (lin != SG_LIN_SMB && lin != SG_LIN_ZMB) return fail(SG_E_ARG, "unknown linearizer");
+    if (g.n() != 0 || !g.s_key.empty()) return fail(SG_E_STATE, "the linearizer can only change on an empty grid");
+    g.zorder = lin == SG_LIN_ZMB;
+    return SG_OK;
+}
+
+// lin_zid of one key (known-answer tests of zmorton_unit_tests.cpp)
+uint64_t sgcpu_morton(int dim, const int64_t *coord) { return Grid::morton(coord, dim); }
 
 int64_t sgcpu_lin_id(void *h, const int64_t *coord) { return ((Grid *)h)->lin_id(coord); }
 

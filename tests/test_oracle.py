@@ -381,3 +381,58 @@ def test_no_shared_mode_is_shared_on_existing_points(oracle, seed):
             for p in range(3):
                 np.testing.assert_array_equal(a.prop(p)[ex], b.prop(p)[ex])
                 np.testing.assert_array_equal(b.prop(p)[~ex], before[p][~ex])
+
+
+def test_morton_known_answers(oracle):
+    """lin_zid / invlin_zid known answers (util/test/zmorton_unit_tests.cpp:95-300)."""
+    m = oracle.morton
+    assert [m(2, k) for k in [(0, 0), (2, 2), (3, 2), (0, 3), (3, 0)]] == [0, 12, 13, 10, 5]
+    assert m(2, (0xF, 0xF)) == 0xFF and m(2, (0xFF, 0xFF)) == 0xFFFF and m(2, (0xFFFF, 0xFFFF)) == 0xFFFFFFFF
+    assert m(2, (0xFFFFFFFF, 0xFFFFFFFF)) == 0xFFFFFFFFFFFFFFFF
+    assert [m(3, (v, v, v)) for v in (0, 2, 4, 8)] == [0, 56, 448, 3584]
+    for k in range(4, 21):
+        assert m(3, (1 << k,) * 3) == 7 << (3 * k)
+    # 2D key (165, 347): bit i of x at bit 2i, of y at bit 2i+1
+    r = m(2, (165, 347))
+    assert all(((165 >> i) & 1) == ((r >> (2 * i)) & 1) and ((347 >> i) & 1) == ((r >> (2 * i + 1)) & 1)
+               for i in range(16))
+
+
+def test_morton_python_layer_matches_oracle(oracle):
+    from openfpm_data_amd.sparsegrid import morton_decode, morton_encode
+    rng = np.random.default_rng(5)
+    for dim in (2, 3):
+        bc = rng.integers(0, 1 << (20 if dim == 3 else 30), (200, dim))
+        ids = morton_encode(bc, dim)
+        assert [int(i) for i in ids] == [oracle.morton(dim, b) for b in bc]
+        np.testing.assert_array_equal(morton_decode(ids, dim), bc)
+
+
+@pytest.mark.parametrize("dim,edge", [(2, 4), (3, 2), (3, 8)])
+def test_zmb_grid_equals_smb_grid(oracle, dim, edge):
+    """The same points in a Morton (SparseGridGpu_z) and a row-major grid: the same field
+    after flush, NNStar tagBoundaries and Laplacian sweeps (chunk order differs only)."""
+    rng = np.random.default_rng(60 + dim * edge)
+    size = [int(rng.integers(3 * edge, 7 * edge)) for _ in range(dim)]
+    n = 400
+    coords = np.stack([rng.integers(0, s, n) for s in size], axis=1).astype(np.int32)
+    vals = rng.integers(-20, 20, n).astype(np.float64)
+    dense = []
+    for lin in ("smb", "zmb"):
+        g = oracle.SparseGridCpu(dim, size, [np.float64, np.float64], block_edge=edge, linearizer=lin)
+        g.insert_coords(coords, [vals, vals])
+        g.flush(["sadd", "smax"])
+        g.find_neighbours(C.NN_STAR)
+        g.tag_boundaries(C.NN_STAR)
+        g.conv_lap(0, 1)
+        g.conv_lap(1, 0, apply=C.APPLY_EXISTING)
+        m = g.masks().reshape(-1)
+        keys = (g.chunk_ids()[:, None] * g.chunk_size + np.arange(g.chunk_size)[None, :]).reshape(-1)[m != 0]
+        md = np.zeros(size[::-1], np.uint8)
+        crd = g.inv_lin_id(keys)
+        md[tuple(crd[:, d] for d in range(dim - 1, -1, -1))] = m[m != 0]
+        dense.append((S.dense_of(g, 0), S.dense_of(g, 1), md))
+        keys = g.lin_id(coords)
+        np.testing.assert_array_equal(g.inv_lin_id(keys), coords)
+    for a, b in zip(dense[0], dense[1]):
+        np.testing.assert_array_equal(a, b)
