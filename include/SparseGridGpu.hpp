@@ -286,8 +286,9 @@ typedef Pack_stat Unpack_stat;
 
 template <unsigned int dim> struct default_edge { static constexpr unsigned int value = dim == 3 ? 8 : (dim == 2 ? 16 : 256); };
 
+// zorder = true: chunk ids are Morton codes (grid_zmb), the reference's SparseGridGpu_z
 template <unsigned int dim, typename AggregateT, unsigned int blockEdgeSize = default_edge<dim>::value,
-          unsigned int threadBlockSize = 512, typename indexT = long int>
+          unsigned int threadBlockSize = 512, typename indexT = long int, bool zorder = false>
 class SparseGridGpu
 {
 public:
@@ -315,6 +316,7 @@ public:
         int32_t dts[nprops > 0 ? nprops : 1];
         fill_dtypes(dts, std::make_index_sequence<nprops>());
         sg::check(sg_create((int)dim, sz, (int)blockEdgeSize, (int)nprops, dts, (int)stencilSupportRadius, &h_));
+        if (zorder) sg::check(sg_set_linearizer(h_, SG_LIN_ZMB));
     }
     ~SparseGridGpu() { sg_destroy(h_); }
     SparseGridGpu(const SparseGridGpu &) = delete;
@@ -341,6 +343,14 @@ public:
             b = b * (indexT)bsz_[d] + c.get(d) / blockEdgeSize;
             l = l * blockEdgeSize + c.get(d) % blockEdgeSize;
         }
+        if (zorder) {
+            // lin_zid: bit i of chunk coordinate d at bit dim*i + d
+            uint64_t z = 0;
+            for (unsigned int d = 0; d < dim; ++d)
+                for (unsigned int i = 0; i < 64 / dim; ++i)
+                    z |= ((((uint64_t)(c.get(d) / blockEdgeSize)) >> i) & 1ull) << (dim * i + d);
+            b = (indexT)z;
+        }
         return b * (indexT)chunk_size() + l;
     }
     grid_key_dx<dim, int> getCoord(indexT lin) const
@@ -348,7 +358,13 @@ public:
         grid_key_dx<dim, int> c;
         indexT b = lin / chunk_size(), l = lin % chunk_size();
         for (unsigned int d = 0; d < dim; ++d) {
-            c.set_d(d, (int)((b % (indexT)bsz_[d]) * blockEdgeSize + l % blockEdgeSize));
+            indexT cb = b % (indexT)bsz_[d];
+            if (zorder) {
+                uint64_t v = 0;
+                for (unsigned int i = 0; i < 64 / dim; ++i) v |= ((((uint64_t)(lin / chunk_size())) >> (dim * i + d)) & 1ull) << i;
+                cb = (indexT)v;
+            }
+            c.set_d(d, (int)(cb * blockEdgeSize + l % blockEdgeSize));
             b /= (indexT)bsz_[d];
             l /= blockEdgeSize;
         }
@@ -593,5 +609,10 @@ private:
     size_t size_[dim];
     size_t bsz_[dim];
 };
+
+// SparseGridGpu_z (src/SparseGridGpu/SparseGridGpu.hpp:3947-3972): Morton chunk ids
+template <unsigned int dim, typename AggregateT, unsigned int blockEdgeSize = default_edge<dim>::value,
+          unsigned int threadBlockSize = 512, typename indexT = long int>
+using SparseGridGpu_z = SparseGridGpu<dim, AggregateT, blockEdgeSize, threadBlockSize, indexT, true>;
 
 #endif  // SPARSEGRIDGPU_FACADE_HPP

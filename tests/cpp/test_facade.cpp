@@ -344,6 +344,38 @@ static void testPackUnpack()
     countDst(dst, dbox, {20520, 27152, 10423, 13649});
 }
 
+// testInsert on a Morton grid: the same known answers through SparseGridGpu_z
+static void testInsertZ()
+{
+    const size_t sz[3] = {64, 64, 64};
+    SparseGridGpu_z<3, aggregate<float>, 8> sg(sz);
+    gpu::ofp_context_t ctx;
+    sg.setBackgroundValue<0>(666);
+    std::vector<long> keys;
+    std::vector<float> vals;
+    for (int z = 0; z < 64; z += 7)
+        for (int y = 0; y < 64; y += 5)
+            for (int x = 0; x < 64; x += 3) {
+                keys.push_back(sg.getLinId(grid_key_dx<3>({x, y, z})));
+                vals.push_back((float)(x + 100 * y + 10000 * z));
+            }
+    const void *v[1] = {vals.data()};
+    sg.insertPoints(keys.data(), keys.size(), v);
+    sg.flush<smax_<0>>(ctx, FLUSH_ON_DEVICE);
+    bool match = true;
+    for (int z = 0; z < 64; ++z)
+        for (int y = 0; y < 64; ++y)
+            for (int x = 0; x < 64; ++x) {
+                const grid_key_dx<3> k({x, y, z});
+                const bool on = x % 3 == 0 && y % 5 == 0 && z % 7 == 0;
+                const float e = on ? (float)(x + 100 * y + 10000 * z) : 666.0f;
+                match = match && sg.get<0>(k) == e;
+                const auto c = sg.getCoord(sg.getLinId(k));
+                match = match && c.get(0) == x && c.get(1) == y && c.get(2) == z;
+            }
+    REQUIRE(match);
+}
+
 int main()
 {
     testInsert();
@@ -353,6 +385,7 @@ int main()
     testPackRequestWithIterator();
     testRemove();
     testPackUnpack();
+    testInsertZ();
     if (failures) {
         std::printf("%d failure(s)\n", failures);
         return 1;
