@@ -518,6 +518,215 @@ __global__ __launch_bounds__(kWG) void tag_boundaries_kernel(int64_t n, GeomArgs
     }
 }
 
+// ---- 3D 8^3 tagBoundaries, bit-parallel ---------------------------------------------
+// One wave per chunk, lane = x-row (y = l & 7, z = l >> 3).  Each wave builds a 10 x 10
+// tile of 10-bit EXIST words, one per row (z, y) in [-1, 8]^2, bit j <-> x = j - 1, from
+// the chunk, its face / edge / corner neighbours (through the neighbour table; a missing
+// chunk is the background chunk, mask 0).  A point is padding when one of its star (6)
+// or full (26) neighbours does not exist: AND of shifted words.  Same result as the
+// generic kernel point by point; masks are read and written as 8-byte rows.
+__device__ inline uint32_t exist_bits8(uint64_t row)
+{
+    uint32_t b = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b |= (uint32_t)((row >> (8 * i)) & 1u) << i;
+    return b;
+}
+
+// ghost row i (0..35) of the 10 x 10 tile: (yy, zz) in [-1, 8]^2 outside [0, 7]^2
+__device__ inline void ghost_row_yz(int i, int &yy, int &zz)
+{
+    if (i < 10) { zz = -1; yy = i - 1; }
+    else if (i < 20) { zz = 8; yy = i - 11; }
+    else if (i < 28) { yy = -1; zz = i - 20; }
+    else { yy = 8; zz = i - 28; }
+}
+
+// neighbour-table entry of chunk displacement (dx, dy, dz): star order -x +x -y +y -z +z,
+// full index (dx+1) + 3 (dy+1) + 9 (dz+1); -1 when a star table has no such entry
+__device__ inline int nn_entry(int type, int dx, int dy, int dz)
+{
+    if (type == SG_NN_FULL) return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+    const int nz = (dx != 0) + (dy != 0) + (dz != 0);
+    if (nz == 0) return -2;  // the chunk itself
+    if (nz > 1) return -1;
+    if (dx) return dx < 0 ? 0 : 1;
+    if (dy) return dy < 0 ? 2 : 3;
+    return dz < 0 ? 4 : 5;
+}
+
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_kernel(int64_t n, const int32_t *__restrict__ nn,
+                                                                    int nnn, int type, uint8_t *__restrict__ mask)
+{
+    __shared__ uint16_t tile[WAVES][100];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * WAVES + w;
+    if (c >= n) return;  // waves are independent: no workgroup barrier below
+    const int32_t *row_nn = nn + c * nnn;
+    const int y = lane & 7, z = lane >> 3;
+    auto chunk_of = [&](int dx, int dy, int dz) -> int64_t {
+        const int k = nn_entry(type, dx, dy, dz);
+        if (k == -2) return c;
+        return (k < 0) ? -1 : (int64_t)row_nn[k];
+    };
+    // x ghost bit of row (yy, zz) in chunk displacement (dx, dy, dz)
+    auto xbit = [&](int dx, int dy, int dz, int ly, int lz, int lx) -> uint32_t {
+        const int64_t nb = chunk_of(dx, dy, dz);
+        if (nb < 0) return 0u;
+        return mask[nb * 512 + (lz * 8 + ly) * 8 + lx] & 1u;
+    };
+    // own row with its x ghosts
+    const uint64_t own = *(const uint64_t *)(mask + c * 512 + lane * 8);
+    {
+        const uint32_t xm = xbit(-1, 0, 0, y, z, 7), xp = xbit(1, 0, 0, y, z, 0);
+        tile[w][(z + 1) * 10 + (y + 1)] = (uint16_t)(xm | (exist_bits8(own) << 1) | (xp << 9));
+    }
+    if (lane < 36) {
+        int yy, zz;
+        ghost_row_yz(lane, yy, zz);
+        const int dy = (yy < 0) ? -1 : (yy > 7 ? 1 : 0), dz = (zz < 0) ? -1 : (zz > 7 ? 1 : 0);
+        const int ly = yy - dy * 8, lz = zz - dz * 8;
+        uint32_t word = 0;
+        const int64_t nb = chunk_of(0, dy, dz);
+        if (nb >= 0) word = exist_bits8(*(const uint64_t *)(mask + nb * 512 + (lz * 8 + ly) * 8)) << 1;
+        if (type == SG_NN_FULL) word |= xbit(-1, dy, dz, ly, lz, 7) | (xbit(1, dy, dz, ly, lz, 0) << 9);
+        tile[w][(zz + 1) * 10 + (yy + 1)] = (uint16_t)word;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint16_t *t = tile[w];
+    const uint32_t self = t[(z + 1) * 10 + (y + 1)];
+    uint32_t all;  // bit j: every neighbour of x = j - 1 exists
+    if (type == SG_NN_FULL) {
+        all = 0x3FFu;
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const uint32_t r = t[(z + dz) * 10 + (y + dy)];
+                all &= r & (r << 1) & (r >> 1);
+            }
+    } else {
+        all = (self << 1) & (self >> 1) & t[(z + 1) * 10 + y] & t[(z + 1) * 10 + y + 2] & t[z * 10 + y + 1] &
+              t[(z + 2) * 10 + y + 1];
+    }
+    const uint32_t pad = (self >> 1) & ~(all >> 1) & 0xFFu;  // existing and not surrounded
+    uint64_t out = own;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t m = (own >> (8 * i)) & 0xFFull;
+        if (!(m & 1)) continue;
+        const uint64_t nm = ((pad >> i) & 1u) ? (m | 2ull) : (m & ~2ull);
+        out = (out & ~(0xFFull << (8 * i))) | (nm << (8 * i));
+    }
+    if (out != own) *(uint64_t *)(mask + c * 512 + lane * 8) = out;
+}
+
+// ---- 3D 8^3 27-point box ----------------------------------------------------------
+// One wave per chunk, lane = x-row.  The wave stages a 10 x 10 x 10 tile of the source
+// (rows (z, y) in [-1, 8]^2, 10 values with the x ghosts) from the chunk and its 26
+// neighbours (full neighbour table; a missing chunk is the background chunk), then each
+// lane sums its 8 points over the 3 x 3 x 3 box in stencil_generic_kernel's order (x offset
+// outermost, z innermost).  Rows whose points are all active or empty are stored whole
+// (empty points take the destination background); other rows point by point.
+template <typename T, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32_t *__restrict__ nn,
+                                                         const uint8_t *__restrict__ mask, const T *__restrict__ src,
+                                                         T *__restrict__ dst, const int64_t *__restrict__ keys,
+                                                         int64_t bsz0, int64_t bsz1, T bg_dst, StencilArgs sa)
+{
+    __shared__ T tile[WAVES][100 * 10];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * WAVES + w;
+    if (c >= n) return;  // waves are independent
+    const int32_t *row_nn = nn + c * 27;
+    const int y = lane & 7, z = lane >> 3;
+    T *tw = tile[w];
+    auto chunk_of = [&](int dx, int dy, int dz) -> int64_t {
+        return (dx == 0 && dy == 0 && dz == 0) ? c : (int64_t)row_nn[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+    };
+    // own row + x ghosts
+    {
+        T *dstrow = tw + ((z + 1) * 10 + (y + 1)) * 10;
+        const T *sr = src + c * 512 + lane * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dstrow[i + 1] = sr[i];
+        dstrow[0] = src[chunk_of(-1, 0, 0) * 512 + lane * 8 + 7];
+        dstrow[9] = src[chunk_of(1, 0, 0) * 512 + lane * 8 + 0];
+    }
+    if (lane < 36) {
+        int yy, zz;
+        ghost_row_yz(lane, yy, zz);
+        const int dy = (yy < 0) ? -1 : (yy > 7 ? 1 : 0), dz = (zz < 0) ? -1 : (zz > 7 ? 1 : 0);
+        const int lrow = (zz - dz * 8) * 8 + (yy - dy * 8);
+        T *dstrow = tw + ((zz + 1) * 10 + (yy + 1)) * 10;
+        const T *sr = src + chunk_of(0, dy, dz) * 512 + lrow * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dstrow[i + 1] = sr[i];
+        dstrow[0] = src[chunk_of(-1, dy, dz) * 512 + lrow * 8 + 7];
+        dstrow[9] = src[chunk_of(1, dy, dz) * 512 + lrow * 8 + 0];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint64_t mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
+    if (mrow == 0) return;  // no point in this row: the destination already holds the background
+    bool inbox[8];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) inbox[x] = true;
+    if (sa.use_box) {
+        ChunkLin L;
+        L.dim = 3;
+        L.zorder = sa.zorder;
+        L.bsz[0] = bsz0;
+        L.bsz[1] = bsz1;
+        L.bsz[2] = 1 << 30;
+        int64_t bc[3];
+        chunk_coords(L, keys[c], bc);
+        const int64_t py = bc[1] * 8 + y, pz = bc[2] * 8 + z;
+        const bool yz = py >= sa.lo[1] && py <= sa.hi[1] && pz >= sa.lo[2] && pz <= sa.hi[2];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            const int64_t px = bc[0] * 8 + x;
+            inbox[x] = yz && px >= sa.lo[0] && px <= sa.hi[0];
+        }
+    }
+    T cf[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) cf[q] = std::is_integral<T>::value ? (T)sa.icoef[q] : (T)sa.coef[q];
+    T out[8];
+    bool act[8];
+    bool whole = true;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        uint8_t cm = (uint8_t)(mrow >> (8 * x));
+        if (!inbox[x]) cm = 0;
+        act[x] = (sa.apply == SG_APPLY_EXISTING) ? (cm != 0) : ((cm & SG_EXIST) && !(cm & SG_PADDING));
+        const uint8_t raw = (uint8_t)(mrow >> (8 * x));
+        whole = whole && (act[x] || raw == 0);
+        T tot = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    tot = dev_add<T>(tot, dev_mul<T>(tw[((z + k) * 10 + (y + j)) * 10 + (x + i)], cf[i * 9 + j * 3 + k]));
+        out[x] = act[x] ? tot : bg_dst;
+    }
+    T *d = dst + c * 512 + lane * 8;
+    if (whole) {
+#pragma unroll
+        for (int x = 0; x < 8; ++x) d[x] = out[x];
+    } else {
+#pragma unroll
+        for (int x = 0; x < 8; ++x)
+            if (act[x]) d[x] = out[x];
+    }
+}
+
 // ---- 3D 8^3 star fast path ------------------------------------------------------
 // LDS tile per property: rows (z, y) with z, y in [-1, 8] -> index (z+1)*10 + (y+1),
 // each row 8 values.
@@ -1089,6 +1298,14 @@ int tag_boundaries_impl(sg_grid *g, int type)
     if (n == 0) return SG_OK;
     int rc = ensure_nn(g, type);
     if (rc) return rc;
+    if (g->dim == 3 && g->edge == 8) {
+        // the tile kernel writes masks in place: rows are read by other waves only through
+        // ghost words (EXIST bits), which this kernel never changes
+        hipLaunchKernelGGL(tag_boundaries8_kernel<4>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, g->stream, n,
+                           g->d_nn[type], g->nnn(type), type, g->d_mask);
+        SG_HIP(hipGetLastError());
+        return SG_OK;
+    }
     const int e = g->edge + 2;
     const size_t ebs = (g->dim == 3) ? (size_t)e * e * e : (size_t)e * e;
     hipLaunchKernelGGL(tag_boundaries_kernel, dim3((unsigned)n), dim3(kWG), ebs, g->stream, n, geom(g),
@@ -1239,6 +1456,24 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
             g->d_prop[p] = outp[q];
         }
         return SG_OK;
+    }
+    if (kind == SG_ST_BOX && g->dim == 3 && g->edge == 8 && alias[0] == 0) {
+        bool done = false;
+        dispatch_type(dt, [&](auto tag) {
+            typedef decltype(tag) T;
+            if constexpr (sizeof(T) >= 4) {
+                T bgd;
+                memcpy(&bgd, g->bg[d->dst[0]], sizeof(T));
+                hipLaunchKernelGGL((box8_kernel<T, 4>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, g->stream, n,
+                                   g->d_nn[type], g->d_mask, (const T *)g->d_prop[d->src[0]],
+                                   (T *)g->d_prop[d->dst[0]], g->d_keys, g->bsz[0], g->bsz[1], bgd, sa);
+                done = true;
+            }
+        });
+        if (done) {
+            SG_HIP(hipGetLastError());
+            return SG_OK;
+        }
     }
     const bool fast = g->dim == 3 && g->edge == 8 && type == SG_NN_STAR &&
                       (kind == SG_ST_LAP || kind == SG_ST_HEAT || kind == SG_ST_GRAY_SCOTT) && alias_ok;

@@ -436,3 +436,37 @@ def test_zmb_grid_parity(G, oracle, dim, edge):
         g.unpack(buf, [1] * dim, [0, 3])
         g.remove_add_unpack_finalize([0, 3])
     S.assert_same_state(a, b)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tag_and_box_8cubed_random(G, oracle, seed):
+    """The 8^3 tile kernels of tagBoundaries (bit-parallel, star and full) and of the
+    27-point box (all scalar widths >= 4 bytes, box restriction, both apply modes) on random
+    3D grids whose masks hold every bit pattern, against the oracle."""
+    rng = np.random.default_rng(1000 + seed)
+    size = [int(rng.integers(24, 60)) for _ in range(3)]
+    dt = [np.float32, np.float64, np.int32, np.uint32, np.int64, np.uint64][seed]
+    dtypes = [dt, dt]
+    steps = S.random_flush_program(seed, 3, 8, size, dtypes, nflush=2, nrec=1500, ops=["smax", "smax"])
+    nn = [C.NN_STAR, C.NN_FULL][seed % 2]
+    lo = [int(rng.integers(0, s // 2)) for s in size]
+    hi = [int(rng.integers(l, s)) for l, s in zip(lo, size)]
+    integer = np.issubdtype(np.dtype(dt), np.integer)
+    coef = rng.integers(-3, 4, 27)
+    grids = []
+    for cls in (G, oracle.SparseGridCpu):
+        g = S.run_flush_program(cls, 3, 8, size, dtypes, steps, background=[1, 2])
+        g.set_nn_type(C.NN_FULL)
+        g.tag_boundaries(nn)
+        grids.append(g)
+    a, b = grids
+    S.assert_same_state(a, b)  # masks after tagging
+    for p in range(2):
+        b.set_prop(p, a.prop(p))  # PADDING-only slots have no specified value after a flush
+    for apply, box in [(C.APPLY_EXISTING, None), (C.APPLY_INTERIOR, (lo, hi)), (C.APPLY_EXISTING, (lo, hi))]:
+        for g in grids:
+            if integer:
+                g.apply_stencil(C.make_desc(C.ST_BOX, [0], [1], apply, box, (), [int(v) for v in coef]))
+            else:
+                g.apply_stencil(C.make_desc(C.ST_BOX, [0], [1], apply, box, [float(v) for v in coef]))
+        S.assert_same_state(a, b, exist_only=False)
