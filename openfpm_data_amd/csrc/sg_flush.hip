@@ -17,8 +17,9 @@
 //
 // Steps: stable LSD radix sort of (point key, record id) -> run heads + scan ->
 // unique points -> chunk heads + scan -> unique chunks -> binary-search merge with
-// the old chunk list (new chunks get positions by a merge-path count) -> one
-// thread per unique point folds its records and the old value in place.
+// the old chunk list (new chunks get positions by a merge-path count) -> one thread
+// per unique point folds its records and the old value in place, every property and
+// then the mask.
 //
 // Whole-chunk inserts (insertBlock) stay chunk records while no point record is
 // pending: only the chunk ids are sorted, and one workgroup per touched chunk folds
@@ -292,34 +293,79 @@ __global__ __launch_bounds__(kWG) void move_chunks_kernel(const V *src, V *dst, 
     for (int i = threadIdx.x; i < vec_per_chunk; i += kWG) dst[d * vec_per_chunk + i] = src[c * vec_per_chunk + i];
 }
 
+// fill chunks with one value (16-byte stores when the chunk size allows)
 template <typename T>
 __global__ __launch_bounds__(kWG) void init_chunks_kernel(T *dst, const int64_t *dpos, int64_t nb, int csz, T v)
 {
     const int64_t b = blockIdx.x;
     if (b >= nb) return;
     const int64_t d = dpos ? dpos[b] : b;
+    if (((size_t)csz * sizeof(T)) % 16 == 0) {
+        union {
+            T t[16 / sizeof(T)];
+            uint4 q;
+        } pat;
+#pragma unroll
+        for (int i = 0; i < (int)(16 / sizeof(T)); ++i) pat.t[i] = v;
+        uint4 *p = (uint4 *)(dst + d * csz);
+        const int nq = (int)((size_t)csz * sizeof(T) / 16);
+        for (int i = threadIdx.x; i < nq; i += kWG) p[i] = pat.q;
+        return;
+    }
     for (int o = threadIdx.x; o < csz; o += kWG) dst[d * csz + o] = v;
 }
 
-// fold the records of each unique point, then the old value, into the chunk arrays
-// (segreduce_total_with_mask + applyOp, BlockMapGpu_kernels.cuh:326-477,
-//  BlockMapGpu_dimensionalityWrappers.cuh:214-226)
-template <typename T>
-__global__ void fold_points_kernel(const int64_t *ukey, const uint32_t *rstart, const uint32_t *perm,
-                                   const uint32_t *uchunk, const int64_t *cpos, int64_t nu, int csz,
-                                   const uint8_t *rmask, const T *rval, T *out, const uint8_t *omask, int op, T bgv)
+// ---- fold of every property and the mask ------------------------------------------
+struct FoldProps {
+    int np;
+    int dt[kMaxProps];
+    int ts[kMaxProps];
+    int op[kMaxProps];
+    const unsigned char *rval[kMaxProps];  // record values (insert buffer, one array per property)
+    unsigned char *out[kMaxProps];         // chunk arrays
+    uint64_t bg[kMaxProps];
+    // the records interleaved for the fold: record r at rec + r * stride, property q at
+    // byte off[q], mask byte at moff (one cache line per record for the random gathers)
+    const unsigned char *rec;
+    int stride, moff;
+    int off[kMaxProps];
+};
+
+// insert buffer (one array per property + masks) -> interleaved records, in order
+__global__ __launch_bounds__(kWG) void interleave_records_kernel(int64_t n, const uint8_t *rmask, FoldProps F,
+                                                                unsigned char *rec)
 {
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= nu) return;
-    const uint32_t r0 = rstart[u], r1 = rstart[u + 1];
-    uint32_t rec = perm[r0];
-    T a = rval[rec];
-    uint8_t am = rmask[rec];
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    unsigned char *d = rec + r * F.stride;
+    for (int q = 0; q < F.np; ++q) {
+        const unsigned char *s = F.rval[q] + r * F.ts[q];
+        unsigned char *t = d + F.off[q];
+        switch (F.ts[q]) {
+        case 8: *(uint64_t *)t = *(const uint64_t *)s; break;
+        case 4: *(uint32_t *)t = *(const uint32_t *)s; break;
+        case 2: *(uint16_t *)t = *(const uint16_t *)s; break;
+        default: *t = *s; break;
+        }
+    }
+    d[F.moff] = rmask[r];
+}
+
+// The fold of one unique point, every property (segreduce_total_with_mask + applyOp,
+// BlockMapGpu_kernels.cuh:326-477, BlockMapGpu_dimensionalityWrappers.cuh:214-226): the
+// point's records in insertion order, a = op(a, b) when both exist, a = b when only b
+// exists; then the old value as b; mask = OR of everything.
+template <typename T>
+__device__ inline T fold_records(int op, uint32_t r0, uint32_t r1, const uint32_t *perm, const FoldProps &F, int q)
+{
+    const unsigned char *p = F.rec + (size_t)perm[r0] * F.stride;
+    T a = *(const T *)(p + F.off[q]);
+    uint8_t am = p[F.moff];
     for (uint32_t i = r0 + 1; i < r1; ++i) {
-        rec = perm[i];
-        const uint8_t bm = rmask[rec];
+        p = F.rec + (size_t)perm[i] * F.stride;
+        const uint8_t bm = p[F.moff];
         const bool aE = am & 1, bE = bm & 1;
-        const T b = rval[rec];
+        const T b = *(const T *)(p + F.off[q]);
         if (op == SG_OP_NONE) {
             if (bE) a = b;
         } else if (aE && bE) {
@@ -329,29 +375,56 @@ __global__ void fold_points_kernel(const int64_t *ukey, const uint32_t *rstart, 
         }
         am |= bm;
     }
-    const int64_t slot = cpos[uchunk[u]] * csz + (ukey[u] % csz);
-    const uint8_t om = omask[slot];
-    const bool aE = am & 1, oE = om & 1;
-    if (op == SG_OP_NONE) {
-        if (!aE && oE) a = out[slot];
-    } else if (aE && oE) {
-        a = dev_reduce<T>(op, a, out[slot]);
-    } else if (oE) {
-        a = out[slot];
-    }
-    out[slot] = ((am | om) == 0) ? bgv : a;
+    return a;
 }
 
-__global__ void fold_masks_kernel(const int64_t *ukey, const uint32_t *rstart, const uint32_t *perm,
-                                  const uint32_t *uchunk, const int64_t *cpos, int64_t nu, int csz,
-                                  const uint8_t *rmask, uint8_t *omask)
+template <typename T> __device__ inline void fold_old(int op, T a, uint8_t am, T *slot, uint8_t om, T bgv)
 {
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool aE = am & 1, oE = om & 1;
+    if (op == SG_OP_NONE) {
+        if (!aE && oE) a = *slot;
+    } else if (aE && oE) {
+        a = dev_reduce<T>(op, a, *slot);
+    } else if (oE) {
+        a = *slot;
+    }
+    *slot = ((am | om) == 0) ? bgv : a;
+}
+
+#define SG_FOLD_TYPES(X)                                                                                       \
+    X(SG_F32, float) X(SG_F64, double) X(SG_I32, int32_t) X(SG_U32, uint32_t) X(SG_I64, int64_t)               \
+    X(SG_U64, uint64_t) X(SG_I8, int8_t) X(SG_U8, uint8_t) X(SG_I16, int16_t) X(SG_U16, uint16_t)
+
+// one thread per unique point: every property and then the mask, in place in the chunk
+// arrays (the records' mask and order are read once per property, from cache after the
+// first)
+__global__ __launch_bounds__(kWG) void fold_points_kernel(const int64_t *ukey, const uint32_t *rstart,
+                                                         const uint32_t *perm, const uint32_t *uchunk,
+                                                         const int64_t *cpos, int64_t nu, int csz, FoldProps F,
+                                                         uint8_t *omask)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nu) return;
-    uint8_t am = 0;
-    for (uint32_t i = rstart[u]; i < rstart[u + 1]; ++i) am |= rmask[perm[i]];
+    const uint32_t r0 = rstart[u], r1 = rstart[u + 1];
     const int64_t slot = cpos[uchunk[u]] * csz + (ukey[u] % csz);
-    omask[slot] |= am;
+    uint8_t am = 0;
+    for (uint32_t i = r0; i < r1; ++i) am |= F.rec[(size_t)perm[i] * F.stride + F.moff];
+    const uint8_t om = omask[slot];
+    for (int q = 0; q < F.np; ++q) {
+        switch (F.dt[q]) {
+#define SG_CASE(CODE, T)                                                                                        \
+    case CODE: {                                                                                                \
+        T bgv;                                                                                                  \
+        __builtin_memcpy(&bgv, &F.bg[q], sizeof(T));                                                            \
+        const T a = fold_records<T>(F.op[q], r0, r1, perm, F, q);                                              \
+        fold_old<T>(F.op[q], a, am, (T *)F.out[q] + slot, om, bgv);                                            \
+        break;                                                                                                  \
+    }
+            SG_FOLD_TYPES(SG_CASE)
+#undef SG_CASE
+        }
+    }
+    omask[slot] = (uint8_t)(om | am);
 }
 
 __global__ void uchunk_index_kernel(const uint32_t *hscan, const uint32_t *head, int64_t nu, uint32_t *uchunk)
@@ -1048,9 +1121,6 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     uint32_t nn32 = 0;
     if ((rc = exclusive_scan_u32(g, chead, cscan, nu, &nn32))) return rc;
     const int64_t nnew = nn32;
-    uint32_t *uchunk = rstart + (n + 1);
-    hipLaunchKernelGGL(uchunk_index_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, cscan, chead, nu, uchunk);
-    SG_HIP(hipGetLastError());
 
     // unique chunk ids, then their positions in the (grown) chunk map
     if ((rc = ensure(g->pool, g->scratch[12], (size_t)nnew * 8 + 64))) return rc;
@@ -1058,26 +1128,44 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     hipLaunchKernelGGL(unique_chunks_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, chead, cscan, nu,
                        csz, nkey);
     SG_HIP(hipGetLastError());
-    int64_t *cpos = nullptr;
-    if ((rc = merge_chunk_set(g, nkey, nnew, &cpos))) return rc;
 
-    // fold: values of every property (they read the old masks), then the masks
+    // fold every property and the mask of each unique point in place
+    FoldProps F;
+    F.np = g->np;
     for (int p = 0; p < g->np; ++p) {
-        const int op = ops ? ops[p] : SG_OP_NONE;
-        dispatch_type(g->dt[p], [&](auto tag) {
-            typedef decltype(tag) T;
-            T v;
-            memcpy(&v, g->bg[p], sizeof(T));
-            hipLaunchKernelGGL(fold_points_kernel<T>, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart,
-                               perm, uchunk, cpos, nu, csz, g->d_ins_mask, (const T *)g->d_ins_val[p],
-                               (T *)g->d_prop[p], g->d_mask, op, v);
-        });
+        F.dt[p] = g->dt[p];
+        F.ts[p] = g->ts[p];
+        F.op[p] = ops ? ops[p] : SG_OP_NONE;
+        F.rval[p] = (const unsigned char *)g->d_ins_val[p];
+        F.bg[p] = 0;
+        memcpy(&F.bg[p], g->bg[p], g->ts[p]);
+    }
+    // interleaved records: properties at their natural alignment, then the mask byte
+    {
+        int at = 0, align = 4;
+        for (int p = 0; p < g->np; ++p) {
+            at = (at + g->ts[p] - 1) / g->ts[p] * g->ts[p];
+            F.off[p] = at;
+            at += g->ts[p];
+            align = std::max(align, g->ts[p]);
+        }
+        F.moff = at;
+        F.stride = (at + 1 + align - 1) / align * align;
+        if ((rc = ensure(g->pool, g->scratch[7], (size_t)n * F.stride + 64))) return rc;
+        F.rec = (const unsigned char *)g->scratch[7].p;
+        hipLaunchKernelGGL(interleave_records_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, n,
+                           g->d_ins_mask, F, (unsigned char *)g->scratch[7].p);
         SG_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(fold_masks_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart, perm, uchunk,
-                       cpos, nu, csz, g->d_ins_mask, g->d_mask);
+    uint32_t *uchunk = rstart + (n + 1);
+    hipLaunchKernelGGL(uchunk_index_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, cscan, chead, nu, uchunk);
     SG_HIP(hipGetLastError());
-
+    int64_t *cpos = nullptr;
+    if ((rc = merge_chunk_set(g, nkey, nnew, &cpos))) return rc;
+    for (int p = 0; p < g->np; ++p) F.out[p] = (unsigned char *)g->d_prop[p];
+    hipLaunchKernelGGL(fold_points_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart, perm, uchunk,
+                       cpos, nu, csz, F, g->d_mask);
+    SG_HIP(hipGetLastError());
     g->ins_n = 0;
     // flush marks the neighbour table stale (SparseGridGpu.hpp:778), and the point list
     g->nn_valid[0] = g->nn_valid[1] = false;

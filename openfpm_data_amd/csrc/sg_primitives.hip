@@ -6,10 +6,11 @@
 //
 // Radix sort: 8-bit digits, one tile of 4096 keys per 256-thread workgroup.
 //   pass = histogram (LDS atomics) -> device scan of the digit-major count table
-//          -> scatter with a stable in-tile rank computed by wave64 ballot matching
-//             (for each item, the lanes holding the same digit are found with 8
-//             ballots; rank = popcount of lower peer lanes + counts of earlier waves
-//             and earlier items of the tile).
+//          -> scatter: a stable in-tile rank computed by wave64 ballot matching (for
+//             each item, the lanes holding the same digit are found with 8 ballots;
+//             rank = popcount of lower peer lanes + counts of earlier waves and earlier
+//             items of the tile) places the tile in LDS in digit order, which is then
+//             written out so that each digit's run is contiguous in global memory.
 // Stability: items are ranked in index order inside a tile and tiles are ranked in
 // tile order by the scanned table, so equal keys keep their input order (as
 // CUB SortPairs does), which the flush relies on for its merge order.
@@ -118,15 +119,26 @@ __global__ __launch_bounds__(kSortWG) void radix_hist_kernel(const K *keys, uint
     counts[(int64_t)t * ntiles + blockIdx.x] = s_hist[t];
 }
 
+// Scatter of one pass: the tile's items are first placed in LDS in (digit, index) order
+// (stable in-tile ranks from wave64 ballot matching), then written out in that order, so
+// each digit's run of the tile goes to consecutive global addresses.
 template <typename K>
 __global__ __launch_bounds__(kSortWG) void radix_scatter_kernel(const K *keys, const uint32_t *vals, K *okeys,
-                                                                uint32_t *ovals, const uint32_t *offsets, int64_t n,
-                                                                int shift, int ntiles)
+                                                                uint32_t *ovals, const uint32_t *offsets,
+                                                                const uint32_t *counts, int64_t n, int shift,
+                                                                int ntiles)
 {
-    __shared__ uint32_t s_base[256];
+    __shared__ K s_key[kSortTile];
+    __shared__ uint32_t s_val[kSortTile];
+    __shared__ uint32_t s_gbase[256];  // global start of digit d for this tile
+    __shared__ uint32_t s_lstart[256]; // start of digit d inside the tile
+    __shared__ uint32_t s_run[256];    // items of digit d placed so far
     __shared__ uint32_t s_wcnt[kSortWG / 64][256];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    s_base[t] = offsets[(int64_t)t * ntiles + blockIdx.x];
+    s_gbase[t] = offsets[(int64_t)t * ntiles + blockIdx.x];
+    uint32_t tot;
+    s_lstart[t] = block_exclusive_scan(counts[(int64_t)t * ntiles + blockIdx.x], &tot);
+    s_run[t] = 0;
 #pragma unroll
     for (int q = 0; q < kSortWG / 64; ++q) s_wcnt[q][t] = 0;
     __syncthreads();
@@ -135,8 +147,8 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_kernel(const K *keys, c
     for (int j = 0; j < kSortIPT; ++j) {
         const int64_t i = base + (int64_t)j * kSortWG + t;
         const bool valid = i < n;
-        K key = valid ? keys[i] : (K)0;
-        uint32_t val = valid ? vals[i] : 0u;
+        const K key = valid ? keys[i] : (K)0;
+        const uint32_t val = valid ? vals[i] : 0u;
         const uint32_t d = (uint32_t)(key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -149,10 +161,10 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_kernel(const K *keys, c
         if (valid && rank == 0) s_wcnt[w][d] = (uint32_t)__popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t pos = s_base[d] + rank;
+            uint32_t pos = s_lstart[d] + s_run[d] + rank;
             for (int q = 0; q < w; ++q) pos += s_wcnt[q][d];
-            okeys[pos] = key;
-            ovals[pos] = val;
+            s_key[pos] = key;
+            s_val[pos] = val;
         }
         __syncthreads();
         uint32_t sum = 0;
@@ -161,8 +173,16 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_kernel(const K *keys, c
             sum += s_wcnt[q][t];
             s_wcnt[q][t] = 0;
         }
-        s_base[t] += sum;
+        s_run[t] += sum;
         __syncthreads();
+    }
+    const int m = (int)((n - base < kSortTile) ? (n - base) : kSortTile);
+    for (int k = t; k < m; k += kSortWG) {
+        const K key = s_key[k];
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const uint32_t pos = s_gbase[d] + (uint32_t)k - s_lstart[d];
+        okeys[pos] = key;
+        ovals[pos] = s_val[k];
     }
 }
 
@@ -189,7 +209,7 @@ int radix_sort_pairs(sg_grid *g, K *keys, uint32_t *vals, K *keys_tmp, uint32_t 
         rc = exclusive_scan_u32(g, counts, offs, (int64_t)256 * ntiles, nullptr);
         if (rc) return rc;
         hipLaunchKernelGGL(radix_scatter_kernel<K>, dim3(ntiles), dim3(kSortWG), 0, g->stream, src_k, src_v, dst_k,
-                           dst_v, offs, n, shift, ntiles);
+                           dst_v, offs, counts, n, shift, ntiles);
         SG_HIP(hipGetLastError());
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);
