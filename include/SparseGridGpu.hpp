@@ -10,6 +10,10 @@
  *   (:2290), applyStencils<Stencil>(box, mode, args...) (:2704), get<p>(coord) (:1897),
  *   insertFlush<p>(coord) (:2832), getLinId / getCoord (:1849-1857), getBox (:2690),
  *   countExistingElements / countBoundaryElements / measureBlockOccupancy (:2422-2552),
+ *   deviceToHost / hostToDevice (BlockMapGpu.hpp:230-241), getIterator() / getIterator(start,
+ *   stop) over the host copy (:3751-3764, Iterators/), packRequest / packCalculate / pack /
+ *   packFinalize / unpack / remove / removeAddUnpackReset / removeAddUnpackFinalize (:3040-3733),
+ *   copy_to / removeCopyToFinalize (:3550, :3314),
  *   reduction tags smax_ / smin_ / sadd_ / sRight_ / sLeft_ / sBitwiseOr_
  *   (src/Vector/cuda/map_vector_sparse_cuda_kernels.cuh), NNStar / NNFull, the test
  *   stencils HeatStencil and Conv3x3x3 (src/SparseGridGpu/tests/utils/SparseGridGpu_util_test.cuh).
@@ -23,6 +27,7 @@
 #ifndef SPARSEGRIDGPU_FACADE_HPP
 #define SPARSEGRIDGPU_FACADE_HPP
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -116,6 +121,9 @@ enum tag_boundaries { NO_CALCULATE_EXISTING_POINTS = SG_NO_CALCULATE_EXISTING_PO
                       CALCULATE_EXISTING_POINTS = SG_CALCULATE_EXISTING_POINTS };
 enum StencilMode { STENCIL_MODE_INPLACE = 1, STENCIL_MODE_INPLACE_NO_SHARED = 3 };
 enum mask_sparse { NOT_EXIST = 0, EXIST = 1, PADDING = 2, EXIST_AND_PADDING = 3 };
+// removeCopyToFinalize options (Grid/grid_common.hpp:180-188)
+enum rem_copy_opt { PHASE1 = SG_COPY_PHASE1, PHASE2 = SG_COPY_PHASE2, PHASE3 = SG_COPY_PHASE3,
+                    KEEP_GEOMETRY = SG_COPY_KEEP_GEOMETRY };
 
 struct No_check {};
 
@@ -243,18 +251,150 @@ template <unsigned int dim, unsigned int p_src, unsigned int p_dst> struct Conv3
 
 // ---- the grid ---------------------------------------------------------------------------
 
-// Sub-grid iterator of the reference (SparseGridGpu_iterator_sub, getIterator(start, stop)),
-// here only the inclusive box it spans: pack/unpack take their boxes from it.
-template <unsigned int dim> class SparseGridGpu_iterator_sub
+// One point of a grid's host copy: chunk position + offset in the chunk
+// (sparse_grid_gpu_index, Iterators/SparseGridGpu_iterator.hpp:16-160).
+template <typename SparseGridType> class sparse_grid_gpu_index
 {
 public:
-    SparseGridGpu_iterator_sub(const grid_key_dx<dim, int> &start, const grid_key_dx<dim, int> &stop)
-        : start_(start), stop_(stop) {}
-    const grid_key_dx<dim, int> &getStart() const { return start_; }
-    const grid_key_dx<dim, int> &getStop() const { return stop_; }
+    static constexpr unsigned int dims = SparseGridType::dims;
+    sparse_grid_gpu_index(const SparseGridType &g, int cnk_pos_id, int data_id)
+        : g_(&g), cnk_(cnk_pos_id), data_(data_id) {}
+    int get_cnk_pos_id() const { return cnk_; }
+    int get_data_id() const { return data_; }
+    void set_cnk_pos_id(int c) { cnk_ = c; }
+    void set_data_id(int d) { data_ = d; }
+    // grid coordinates of the point
+    grid_key_dx<dims, int> getKey() const
+    {
+        return g_->getCoord(g_->host_chunk_id(cnk_) * g_->getBlockSize() + data_);
+    }
+    int get(unsigned int i) const { return getKey().get(i); }
 
 private:
+    const SparseGridType *g_;
+    int cnk_, data_;
+};
+
+// Iterator over every point of the host copy whose mask is not 0, chunk by chunk in the
+// order of the chunk list (SparseGridGpu_iterator, Iterators/SparseGridGpu_iterator.hpp:162-306).
+template <unsigned int dim, typename SparseGridType> class SparseGridGpu_iterator
+{
+public:
+    explicit SparseGridGpu_iterator(const SparseGridType &g) : g_(&g) { settle(); }
+    bool isNext() const { return chunk_ < g_->host_chunks(); }
+    SparseGridGpu_iterator &operator++()
+    {
+        if (++pnt_ >= g_->getBlockSize()) {
+            ++chunk_;
+            pnt_ = 0;
+        }
+        settle();
+        return *this;
+    }
+    sparse_grid_gpu_index<SparseGridType> get() const { return {*g_, (int)chunk_, (int)pnt_}; }
+
+private:
+    void settle()
+    {
+        const size_t cs = g_->getBlockSize(), nc = g_->host_chunks();
+        const uint8_t *m = g_->host_mask();
+        while (chunk_ < nc) {
+            while (pnt_ < cs && m[chunk_ * cs + pnt_] == 0) ++pnt_;
+            if (pnt_ < cs) return;
+            ++chunk_;
+            pnt_ = 0;
+        }
+    }
+    const SparseGridType *g_;
+    size_t chunk_ = 0, pnt_ = 0;
+};
+
+// Iterator over the EXIST points of the host copy inside the inclusive box [start, stop]:
+// chunks in list order, points x-fastest within each chunk's part of the box
+// (SparseGridGpu_iterator_sub, Iterators/SparseGridGpu_iterator_sub.hpp:13-260). pack,
+// unpack and copy sections take their boxes from it; walking it needs the host copy
+// (deviceToHost), which is looked at on the first isNext / get.
+template <unsigned int dim, typename SparseGridType> class SparseGridGpu_iterator_sub
+{
+public:
+    SparseGridGpu_iterator_sub(const SparseGridType &g, const grid_key_dx<dim, int> &start,
+                               const grid_key_dx<dim, int> &stop)
+        : g_(&g), start_(start), stop_(stop) {}
+    const grid_key_dx<dim, int> &getStart() const { return start_; }
+    const grid_key_dx<dim, int> &getStop() const { return stop_; }
+    bool isNext() const
+    {
+        begin();
+        return chunk_ < g_->host_chunks();
+    }
+    SparseGridGpu_iterator_sub &operator++()
+    {
+        begin();
+        settle(true);
+        return *this;
+    }
+    sparse_grid_gpu_index<SparseGridType> get() const
+    {
+        begin();
+        return {*g_, (int)chunk_, local()};
+    }
+
+private:
+    static constexpr int E = SparseGridType::blockEdgeSize_;
+    void begin() const
+    {
+        if (started_) return;
+        g_->require_host_copy();
+        started_ = true;
+        settle(false);
+    }
+    // chunk c's part of the box in local coordinates; false when it is empty
+    bool enter(size_t c) const
+    {
+        const auto base = g_->getCoord(g_->host_chunk_id(c) * g_->getBlockSize());
+        for (unsigned int d = 0; d < dim; ++d) {
+            lo_[d] = std::max(start_.get(d), base.get(d)) - base.get(d);
+            hi_[d] = std::min(stop_.get(d), base.get(d) + E - 1) - base.get(d);
+            if (lo_[d] > hi_[d]) return false;
+            cur_[d] = lo_[d];
+        }
+        return true;
+    }
+    bool step() const
+    {
+        for (unsigned int d = 0; d < dim; ++d) {
+            if (++cur_[d] <= hi_[d]) return true;
+            cur_[d] = lo_[d];
+        }
+        return false;
+    }
+    int local() const
+    {
+        int l = cur_[dim - 1];
+        for (int d = (int)dim - 2; d >= 0; --d) l = l * E + cur_[d];
+        return l;
+    }
+    void settle(bool advance) const
+    {
+        const size_t nc = g_->host_chunks(), cs = g_->getBlockSize();
+        const uint8_t *m = g_->host_mask();
+        while (chunk_ < nc) {
+            if (!inside_) inside_ = enter(chunk_);
+            else if (advance && !step()) inside_ = false;
+            advance = false;
+            if (inside_) {
+                if (m[chunk_ * cs + local()] & 1) return;
+                advance = true;
+                continue;
+            }
+            ++chunk_;
+        }
+    }
+    const SparseGridType *g_;
     grid_key_dx<dim, int> start_, stop_;
+    mutable bool started_ = false, inside_ = false;
+    mutable size_t chunk_ = 0;
+    mutable int lo_[dim] = {}, hi_[dim] = {}, cur_[dim] = {};
 };
 
 // Host pack memory (the reference's ExtPreAlloc<CudaMemory> over a preallocated buffer):
@@ -321,12 +461,19 @@ public:
     ~SparseGridGpu() { sg_destroy(h_); }
     SparseGridGpu(const SparseGridGpu &) = delete;
     SparseGridGpu &operator=(const SparseGridGpu &) = delete;
-    SparseGridGpu(SparseGridGpu &&o) noexcept : h_(o.h_)
+    SparseGridGpu(SparseGridGpu &&o) noexcept
+        : pack_lo_(std::move(o.pack_lo_)), pack_hi_(std::move(o.pack_hi_)), host_copy_(o.host_copy_),
+          host_ids_(std::move(o.host_ids_)), host_mask_(std::move(o.host_mask_)), h_(o.h_)
     {
         o.h_ = nullptr;
+        o.host_copy_ = false;
         for (unsigned int i = 0; i < dim; ++i) {
             size_[i] = o.size_[i];
             bsz_[i] = o.bsz_[i];
+        }
+        for (unsigned int q = 0; q < (nprops > 0 ? nprops : 1); ++q) {
+            host_prop_[q] = std::move(o.host_prop_[q]);
+            host_has_[q] = o.host_has_[q];
         }
     }
 
@@ -475,7 +622,63 @@ public:
     {
         sg::check(sg_set_prop(h_, (int)p, v.data(), SG_MEM_HOST));
     }
-    template <unsigned int... p> void deviceToHost() { sg::check(sg_synchronize(h_)); }
+
+    // -- host copy (deviceToHost / hostToDevice, BlockMapGpu.hpp:230-241) --
+    // deviceToHost<p...>() copies the chunk list, the masks and properties p... (every
+    // property for an empty list) to host memory. The host iterators, get<p> and getFlag
+    // of a sparse_grid_gpu_index read it, get<p> of an index writes it, and
+    // hostToDevice<p...>() sends properties p... (every copied one for an empty list) back.
+    template <unsigned int... p> void deviceToHost()
+    {
+        sg::check(sg_synchronize(h_));
+        host_ids_ = private_get_index_array();
+        host_mask_ = private_get_mask_array();
+        for (unsigned int q = 0; q < nprops; ++q)
+            if (sizeof...(p) == 0 || listed<p...>(q)) {
+                host_prop_[q].resize(host_mask_.size() * prop_bytes(q) + 1);
+                if (!host_mask_.empty()) sg::check(sg_get_prop(h_, (int)q, host_prop_[q].data(), SG_MEM_HOST));
+                host_has_[q] = true;
+            }
+        host_copy_ = true;
+    }
+    template <unsigned int... p> void hostToDevice()
+    {
+        require_host_copy();
+        if ((size_t)sg_num_chunks(h_) != host_ids_.size())
+            throw sg::error("hostToDevice: the grid's chunks changed since deviceToHost");
+        for (unsigned int q = 0; q < nprops; ++q)
+            if ((sizeof...(p) == 0 && host_has_[q]) || (sizeof...(p) != 0 && listed<p...>(q))) {
+                if (!host_has_[q]) throw sg::error("hostToDevice: property not in the host copy");
+                if (!host_mask_.empty()) sg::check(sg_set_prop(h_, (int)q, host_prop_[q].data(), SG_MEM_HOST));
+            }
+    }
+    template <unsigned int p> prop_type<p> &get(const sparse_grid_gpu_index<SparseGridGpu> &k)
+    {
+        return host_value<p>(k);
+    }
+    template <unsigned int p> const prop_type<p> &get(const sparse_grid_gpu_index<SparseGridGpu> &k) const
+    {
+        return const_cast<SparseGridGpu *>(this)->host_value<p>(k);
+    }
+    unsigned char getFlag(const sparse_grid_gpu_index<SparseGridGpu> &k) const
+    {
+        require_host_copy();
+        return host_mask_[(size_t)k.get_cnk_pos_id() * chunk_size() + k.get_data_id()];
+    }
+    typedef SparseGridGpu_iterator<dim, SparseGridGpu> iterator;
+    iterator getIterator() const
+    {
+        require_host_copy();
+        return iterator(*this);
+    }
+    // host-copy accessors of the iterators
+    size_t host_chunks() const { return host_ids_.size(); }
+    const uint8_t *host_mask() const { return host_mask_.data(); }
+    indexT host_chunk_id(size_t c) const { return host_ids_[c]; }
+    void require_host_copy() const
+    {
+        if (!host_copy_) throw sg::error("no host copy of the grid: call deviceToHost() first");
+    }
 
     size_t countExistingElements() const
     {
@@ -509,10 +712,10 @@ public:
     }
 
     // -- pack / unpack / remove (SparseGridGpu.hpp:3040-3733) --
-    typedef SparseGridGpu_iterator_sub<dim> sub_iterator;
+    typedef SparseGridGpu_iterator_sub<dim, SparseGridGpu> sub_iterator;
     sub_iterator getIterator(const grid_key_dx<dim, int> &start, const grid_key_dx<dim, int> &stop) const
     {
-        return sub_iterator(start, stop);
+        return sub_iterator(*this, start, stop);
     }
     template <typename CoordT>
     sub_iterator getIterator(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop) const
@@ -522,7 +725,7 @@ public:
             a.set_d(d, (int)start.get(d));
             b.set_d(d, (int)stop.get(d));
         }
-        return sub_iterator(a, b);
+        return sub_iterator(*this, a, b);
     }
     void packReset()
     {
@@ -593,10 +796,56 @@ public:
         const int32_t props[] = {(int32_t)prp..., 0};
         sg::check(sg_remove_add_unpack_finalize(h_, (int)sizeof...(prp), props));
     }
+    // copy_to (SparseGridGpu.hpp:3550-3559): queue, on grid_src, a copy of its existing
+    // points in box_src into this grid, box_src's low corner landing at box_dst's
+    template <typename CoordT>
+    void copy_to(SparseGridGpu &grid_src, const Box<dim, CoordT> &box_src, const Box<dim, CoordT> &box_dst)
+    {
+        int32_t lo[dim], hi[dim], dl[dim];
+        for (unsigned int d = 0; d < dim; ++d) {
+            lo[d] = (int32_t)box_src.getLow(d);
+            hi[d] = (int32_t)box_src.getHigh(d);
+            dl[d] = (int32_t)box_dst.getLow(d);
+        }
+        sg::check(sg_copy_to(h_, grid_src.h_, lo, hi, dl));
+    }
+    // removeCopyToFinalize (SparseGridGpu.hpp:3314-3329): PHASE1 on every grid taking part,
+    // then PHASE2 on every grid, then PHASE3 on every grid
+    template <unsigned int... prp> void removeCopyToFinalize(gpu::ofp_context_t &, int opt)
+    {
+        const int32_t props[] = {(int32_t)prp..., 0};
+        sg::check(sg_remove_copy_to_finalize(h_, opt, (int)sizeof...(prp), props));
+    }
 
 private:
     int npack() const { return (int)(pack_lo_.size() / dim); }
     std::vector<int32_t> pack_lo_, pack_hi_;
+
+    template <unsigned int... p> static bool listed(unsigned int q)
+    {
+        const unsigned int l[] = {p..., ~0u};
+        for (unsigned int i = 0; i < sizeof...(p); ++i)
+            if (l[i] == q) return true;
+        return false;
+    }
+    template <size_t... I> static size_t prop_bytes_(unsigned int q, std::index_sequence<I...>)
+    {
+        const size_t b[] = {sizeof(typename std::tuple_element<I, props_tuple>::type)..., 0};
+        return b[q];
+    }
+    static size_t prop_bytes(unsigned int q) { return prop_bytes_(q, std::make_index_sequence<nprops>()); }
+    template <unsigned int p> prop_type<p> &host_value(const sparse_grid_gpu_index<SparseGridGpu> &k)
+    {
+        require_host_copy();
+        if (!host_has_[p]) throw sg::error("property not in the host copy: deviceToHost<p>() first");
+        const size_t i = (size_t)k.get_cnk_pos_id() * chunk_size() + k.get_data_id();
+        return reinterpret_cast<prop_type<p> *>(host_prop_[p].data())[i];
+    }
+    bool host_copy_ = false;
+    std::vector<indexT> host_ids_;
+    std::vector<uint8_t> host_mask_;
+    std::vector<unsigned char> host_prop_[nprops > 0 ? nprops : 1];
+    bool host_has_[nprops > 0 ? nprops : 1] = {};
 
     template <size_t... I> static void fill_dtypes(int32_t *dts, std::index_sequence<I...>)
     {

@@ -335,8 +335,29 @@ int sg_remove_points(sg_grid *g, int64_t n, const int64_t *keys, int mem);
  * boxes, then merge the pending records: the listed (unpacked) properties take the new
  * values, the others keep theirs, mask |= EXIST */
 int sg_remove_add_unpack_finalize(sg_grid *g, int nprop, const int32_t *props);
-/* removeAddUnpackReset (SparseGridGpu.hpp:3362-3383): drop queued boxes and pending records */
+/* removeAddUnpackReset (SparseGridGpu.hpp:3362-3383): drop queued boxes, queued copies
+ * and pending records */
 int sg_remove_add_unpack_reset(sg_grid *g);
+
+/* copy_to (SparseGridGpu.hpp:3550-3559, sparse_grid_section :467-480): queue, on the
+ * source grid, a copy of the existing points of its inclusive box [src_lo, src_hi] into
+ * dst, the box's low corner landing at dst_lo.  The grids must have the same dimension
+ * and, for the copied properties, the same types; src may be dst. */
+int sg_copy_to(sg_grid *dst, sg_grid *src, const int32_t *src_lo, const int32_t *src_hi, const int32_t *dst_lo);
+/* removeCopyToFinalize (SparseGridGpu.hpp:3314-3329, phases :1116-1290; rem_copy_opt,
+ * Grid/grid_common.hpp:180-188).  Call PHASE1 on every grid taking part, then PHASE2 on
+ * every grid, then PHASE3 on every grid:
+ *   PHASE1  remove the points of the boxes queued by sg_remove_box;
+ *   PHASE2  pack this grid's queued copies (on the device) and queue their points in the
+ *           destinations;
+ *   PHASE3  merge the points queued in this grid: the listed properties take the copied
+ *           values, the others keep theirs, mask |= EXIST.
+ * The KEEP_GEOMETRY bit (4) of the reference is accepted and has no effect. */
+#define SG_COPY_PHASE1 0
+#define SG_COPY_PHASE2 1
+#define SG_COPY_PHASE3 2
+#define SG_COPY_KEEP_GEOMETRY 4
+int sg_remove_copy_to_finalize(sg_grid *g, int phase, int nprop, const int32_t *props);
 
 /* ---- device memory (no counterpart in the reference: its vectors keep their
  *      capacity; a server admitting grids against an HBM budget needs both) ---- */

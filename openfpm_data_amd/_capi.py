@@ -172,6 +172,8 @@ _SIGS = {
     "sg_remove_points": (_INT, [_P, _I64, _P, _INT]),
     "sg_remove_add_unpack_finalize": (_INT, [_P, _INT, _P]),
     "sg_remove_add_unpack_reset": (_INT, [_P]),
+    "sg_copy_to": (_INT, [_P, _P, _P, _P, _P]),
+    "sg_remove_copy_to_finalize": (_INT, [_P, _INT, _INT, _P]),
     "sg_trim": (_INT, [_P]),
     "sg_device_bytes": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_batch_create": (_INT, [ctypes.POINTER(PoissonParams), _P, ctypes.POINTER(_P)]),

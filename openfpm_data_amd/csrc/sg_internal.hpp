@@ -133,6 +133,13 @@ struct sg_grid {
 
     // boxes queued by sg_remove_box, [k][dim] inclusive corners
     std::vector<int32_t> rem_lo, rem_hi;
+    // copies queued on this grid as the source by sg_copy_to: the points of [lo, hi]
+    // go to `dst` with lo at dst_lo
+    struct copy_section {
+        sg_grid *dst;
+        int32_t lo[3], hi[3], dst_lo[3];
+    };
+    std::vector<copy_section> copies;
 
     hipStream_t stream = nullptr;
     sg::DevicePool pool;

@@ -628,11 +628,13 @@ int sg_remove_points(sg_grid *g, int64_t n, const int64_t *keys, int mem)
     return SG_OK;
 }
 
-int sg_remove_add_unpack_finalize(sg_grid *g, int nprop, const int32_t *props)
+}  // extern "C"
+
+namespace {
+
+// clear the points of the boxes queued by sg_remove_box (mask 0, value background)
+int remove_queued_boxes(sg_grid *g)
 {
-    if (!g || (nprop > 0 && !props)) return fail(SG_E_ARG, "null argument");
-    for (int q = 0; q < nprop; ++q)
-        if (props[q] < 0 || props[q] >= g->np) return fail(SG_E_ARG, "bad property");
     const int nb = (int)(g->rem_lo.size() / std::max(1, g->dim));
     if (nb > 0 && g->nchunks > 0) {
         const size_t bb = (size_t)nb * g->dim * 4;
@@ -650,11 +652,109 @@ int sg_remove_add_unpack_finalize(sg_grid *g, int nprop, const int32_t *props)
     }
     g->rem_lo.clear();
     g->rem_hi.clear();
-    // unpacked properties: the unpacked value (left operand) wins over the old one; the
-    // others keep the old value (right operand), new points get the background
+    return SG_OK;
+}
+
+int check_props(sg_grid *g, int nprop, const int32_t *props)
+{
+    if (nprop > 0 && !props) return fail(SG_E_ARG, "null argument");
+    for (int q = 0; q < nprop; ++q)
+        if (props[q] < 0 || props[q] >= g->np) return fail(SG_E_ARG, "bad property");
+    return SG_OK;
+}
+
+// unpacked properties: the unpacked value (left operand) wins over the old one; the
+// others keep the old value (right operand), new points get the background
+int flush_unpacked(sg_grid *g, int nprop, const int32_t *props)
+{
     std::vector<int32_t> ops(g->np, SG_OP_RIGHT);
     for (int q = 0; q < nprop; ++q) ops[props[q]] = SG_OP_LEFT;
     return flush_impl(g, ops.data());
+}
+
+// phase 2 of removeCopyToFinalize: pack every queued section of g into one device buffer,
+// then unpack each into its destination's insert buffer
+int copy_sections_out(sg_grid *g, int nprop, const int32_t *props)
+{
+    if (g->copies.empty()) return SG_OK;
+    const int n = (int)g->copies.size(), D = g->dim;
+    for (const auto &c : g->copies) {
+        sg_grid *d = c.dst;
+        if (d->dim != D) return fail(SG_E_ARG, "copy_to: grids of different dimension");
+        for (int q = 0; q < nprop; ++q)
+            if (props[q] >= d->np || d->dt[props[q]] != g->dt[props[q]])
+                return fail(SG_E_ARG, "copy_to: property types differ between the grids");
+    }
+    std::vector<int32_t> lo((size_t)n * D), hi((size_t)n * D);
+    for (int k = 0; k < n; ++k)
+        for (int a = 0; a < D; ++a) {
+            lo[(size_t)k * D + a] = g->copies[k].lo[a];
+            hi[(size_t)k * D + a] = g->copies[k].hi[a];
+        }
+    std::vector<int64_t> bytes(n);
+    int64_t tot = 0;
+    for (int k = 0; k < n; ++k) {
+        int rc = sg_pack_size(g, 1, &lo[(size_t)k * D], &hi[(size_t)k * D], nprop, props, 8, &bytes[k]);
+        if (rc) return rc;
+        tot += bytes[k];
+    }
+    DevBuf &cb = g->scratch[3];
+    int rc = ensure(g->pool, cb, (size_t)tot + 64);
+    if (rc) return rc;
+    rc = sg_pack(g, n, lo.data(), hi.data(), nprop, props, 8, cb.p, tot, SG_MEM_DEVICE);
+    if (rc) return rc;
+    SG_HIP(hipStreamSynchronize(g->stream));  // the destinations read it on their streams
+    int64_t at = 0;
+    for (int k = 0; k < n; ++k) {
+        int64_t used = 0;
+        rc = sg_unpack(g->copies[k].dst, (const unsigned char *)cb.p + at, tot - at, SG_MEM_DEVICE,
+                       g->copies[k].dst_lo, nprop, props, 8, &used);
+        if (rc) return rc;
+        at += used;
+    }
+    g->copies.clear();
+    return SG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sg_remove_add_unpack_finalize(sg_grid *g, int nprop, const int32_t *props)
+{
+    if (!g) return fail(SG_E_ARG, "null argument");
+    int rc = check_props(g, nprop, props);
+    if (rc) return rc;
+    if ((rc = remove_queued_boxes(g))) return rc;
+    return flush_unpacked(g, nprop, props);
+}
+
+int sg_copy_to(sg_grid *dst, sg_grid *src, const int32_t *src_lo, const int32_t *src_hi, const int32_t *dst_lo)
+{
+    if (!dst || !src || !src_lo || !src_hi || !dst_lo) return fail(SG_E_ARG, "null argument");
+    if (dst->dim != src->dim) return fail(SG_E_ARG, "copy_to: grids of different dimension");
+    sg_grid::copy_section c{};
+    c.dst = dst;
+    for (int d = 0; d < 3; ++d) {
+        c.lo[d] = d < src->dim ? src_lo[d] : 0;
+        c.hi[d] = d < src->dim ? src_hi[d] : 0;
+        c.dst_lo[d] = d < src->dim ? dst_lo[d] : 0;
+    }
+    src->copies.push_back(c);
+    return SG_OK;
+}
+
+int sg_remove_copy_to_finalize(sg_grid *g, int phase, int nprop, const int32_t *props)
+{
+    if (!g) return fail(SG_E_ARG, "null argument");
+    int rc = check_props(g, nprop, props);
+    if (rc) return rc;
+    switch (phase & 3) {
+    case SG_COPY_PHASE1: return remove_queued_boxes(g);
+    case SG_COPY_PHASE2: return copy_sections_out(g, nprop, props);
+    case SG_COPY_PHASE3: return flush_unpacked(g, nprop, props);
+    default: return fail(SG_E_ARG, "bad phase");
+    }
 }
 
 int sg_remove_add_unpack_reset(sg_grid *g)
@@ -662,6 +762,7 @@ int sg_remove_add_unpack_reset(sg_grid *g)
     if (!g) return fail(SG_E_ARG, "null grid");
     g->rem_lo.clear();
     g->rem_hi.clear();
+    g->copies.clear();
     g->ins_n = 0;
     g->cins_n = 0;
     return SG_OK;

@@ -370,3 +370,18 @@ class SparseGridGpu:
 
     def remove_add_unpack_reset(self):
         C.check(self._lib.sg_remove_add_unpack_reset(self._h))
+
+    def copy_to(self, src, box_src, box_dst):
+        """copy_to (SparseGridGpu.hpp:3550-3559): queue, on `src`, a copy of the existing
+        points of the inclusive box_src = (lo, hi) of `src` into this grid, the box's low
+        corner landing at box_dst's low corner.  Done by remove_copy_to_finalize."""
+        lo = np.ascontiguousarray(np.asarray(box_src[0], np.int32))
+        hi = np.ascontiguousarray(np.asarray(box_src[1], np.int32))
+        dl = np.ascontiguousarray(np.asarray(box_dst[0], np.int32))
+        C.check(self._lib.sg_copy_to(self._h, src._h, lo.ctypes.data, hi.ctypes.data, dl.ctypes.data))
+
+    def remove_copy_to_finalize(self, props, phase):
+        """removeCopyToFinalize (SparseGridGpu.hpp:3314-3329): phase 0 on every grid
+        taking part, then phase 1 on every grid, then phase 2 on every grid."""
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        C.check(self._lib.sg_remove_copy_to_finalize(self._h, int(phase), pr.size, pr.ctypes.data))

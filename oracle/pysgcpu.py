@@ -54,6 +54,8 @@ _SIGS = {
     "sgcpu_remove_points": (_INT, [_P, _I64, _P]),
     "sgcpu_remove_add_unpack_finalize": (_INT, [_P, _INT, _P]),
     "sgcpu_remove_add_unpack_reset": (_INT, [_P]),
+    "sgcpu_copy_to": (_INT, [_P, _P, _P, _P, _P]),
+    "sgcpu_remove_copy_to_finalize": (_INT, [_P, _INT, _INT, _P]),
 }
 _lib = None
 
@@ -324,6 +326,16 @@ class SparseGridCpu:
 
     def remove_add_unpack_reset(self):
         _check(self._lib.sgcpu_remove_add_unpack_reset(self._h))
+
+    def copy_to(self, src, box_src, box_dst):
+        lo = np.ascontiguousarray(np.asarray(box_src[0], np.int32))
+        hi = np.ascontiguousarray(np.asarray(box_src[1], np.int32))
+        dl = np.ascontiguousarray(np.asarray(box_dst[0], np.int32))
+        _check(self._lib.sgcpu_copy_to(self._h, src._h, lo.ctypes.data, hi.ctypes.data, dl.ctypes.data))
+
+    def remove_copy_to_finalize(self, props, phase):
+        pr = np.ascontiguousarray(np.asarray(props, np.int32))
+        _check(self._lib.sgcpu_remove_copy_to_finalize(self._h, int(phase), pr.size, pr.ctypes.data))
 
 
 def morton(dim, coord):

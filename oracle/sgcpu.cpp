@@ -139,6 +139,12 @@ struct Grid {
 
     // boxes queued by remove(Box) (rem_sects, SparseGridGpu.hpp:3528-3531), [k][dim]
     std::vector<int32_t> rem_lo, rem_hi;
+    // copies queued by copy_to on this grid as the source (copySect, SparseGridGpu.hpp:3550-3559)
+    struct Section {
+        void *dst;
+        int32_t lo[3], hi[3], dst_lo[3];
+    };
+    std::vector<Section> copies;
     std::vector<int> ghost_lin, ghost_nb;
 
     int64_t n() const { return (int64_t)keys.size(); }
@@ -1323,9 +1329,8 @@ int sgcpu_remove_points(void *h, int64_t n, const int64_t *keys)
 // removeAddUnpackFinalize (SparseGridGpu.hpp:3492-3499): removePoints (remove_points,
 // SparseGridGpu_kernels.cuh:1301-1346: existing points inside a queued box get mask 0),
 // then the queued inserts are merged
-int sgcpu_remove_add_unpack_finalize(void *h, int nprop, const int32_t *props)
+static void remove_queued(Grid &g)
 {
-    Grid &g = *(Grid *)h;
     const int nb = (int)(g.rem_lo.size() / g.dim);
     for (int64_t c = 0; c < g.n(); ++c)
         for (int o = 0; o < g.csz; ++o) {
@@ -1340,9 +1345,77 @@ int sgcpu_remove_add_unpack_finalize(void *h, int nprop, const int32_t *props)
         }
     g.rem_lo.clear();
     g.rem_hi.clear();
+}
+
+static int flush_unpacked(Grid &g, int nprop, const int32_t *props)
+{
     std::vector<int32_t> ops(g.np, SG_OP_RIGHT);
     for (int q = 0; q < nprop; ++q) ops[props[q]] = SG_OP_LEFT;
     return flush(g, ops.data());
+}
+
+int sgcpu_remove_add_unpack_finalize(void *h, int nprop, const int32_t *props)
+{
+    Grid &g = *(Grid *)h;
+    remove_queued(g);
+    return flush_unpacked(g, nprop, props);
+}
+
+// copy_to (SparseGridGpu.hpp:3550-3559): the section is queued on the source grid
+int sgcpu_copy_to(void *dst, void *src, const int32_t *src_lo, const int32_t *src_hi, const int32_t *dst_lo)
+{
+    Grid &s = *(Grid *)src;
+    Grid::Section c{};
+    c.dst = dst;
+    for (int d = 0; d < s.dim; ++d) {
+        c.lo[d] = src_lo[d];
+        c.hi[d] = src_hi[d];
+        c.dst_lo[d] = dst_lo[d];
+    }
+    s.copies.push_back(c);
+    return SG_OK;
+}
+
+// removeCopyToFinalize (SparseGridGpu.hpp:3314-3329).  PHASE1 = removeCopyToFinalize_phase1
+// (:1116-1121, removePoints); PHASE2 = _phase2 (:1123-1174): pack every queued section of
+// this grid (packRequest / packCalculate / pack / packFinalize), then preUnpack
+// (:1095-1113) converts each packed box into its destination's insert buffer at the
+// destination box's low corner; PHASE3 = _phase3 (:1176-1290): flush with sLeft_ on the
+// copied properties, then copy_packed_data_to_chunks sets mask |= EXIST.
+int sgcpu_remove_copy_to_finalize(void *h, int phase, int nprop, const int32_t *props)
+{
+    Grid &g = *(Grid *)h;
+    switch (phase & 3) {
+    case SG_COPY_PHASE1:
+        remove_queued(g);
+        return SG_OK;
+    case SG_COPY_PHASE2: {
+        const int n = (int)g.copies.size(), D = g.dim;
+        if (n == 0) return SG_OK;
+        std::vector<int32_t> lo((size_t)n * D), hi((size_t)n * D);
+        for (int k = 0; k < n; ++k)
+            for (int a = 0; a < D; ++a) {
+                lo[(size_t)k * D + a] = g.copies[k].lo[a];
+                hi[(size_t)k * D + a] = g.copies[k].hi[a];
+            }
+        int64_t tot = 0;
+        int rc = sgcpu_pack_size(h, n, lo.data(), hi.data(), nprop, props, 8, &tot);
+        if (rc) return rc;
+        std::vector<unsigned char> buf((size_t)tot + 8);
+        if ((rc = sgcpu_pack(h, n, lo.data(), hi.data(), nprop, props, 8, buf.data(), tot))) return rc;
+        int64_t at = 0;
+        for (int k = 0; k < n; ++k) {
+            int64_t used = 0;
+            rc = sgcpu_unpack(g.copies[k].dst, buf.data() + at, tot - at, g.copies[k].dst_lo, nprop, props, 8, &used);
+            if (rc) return rc;
+            at += used;
+        }
+        g.copies.clear();
+        return SG_OK;
+    }
+    case SG_COPY_PHASE3: return flush_unpacked(g, nprop, props);
+    default: return fail(SG_E_ARG, "bad phase");
+    }
 }
 
 int sgcpu_remove_add_unpack_reset(void *h)
@@ -1350,6 +1423,7 @@ int sgcpu_remove_add_unpack_reset(void *h)
     Grid &g = *(Grid *)h;
     g.rem_lo.clear();
     g.rem_hi.clear();
+    g.copies.clear();
     clear_inserts(g);
     return SG_OK;
 }

@@ -8,6 +8,10 @@
 //   sparsegridgpu_remove_test   src/SparseGridGpu/tests/SparseGridGpu_tests.cu:1474-1551
 //   sparsegridgpu_pack_unpack   src/SparseGridGpu/tests/SparseGridGpu_tests.cu:1553-1934
 //                               (aggregate<float, float[3]> as four float properties)
+//   test_sparse_grid_iterator_sub_host  src/SparseGridGpu/tests/SparseGridGpu_tests.cu:1179-1229
+//   test_sparse_grid_iterator_host      src/SparseGridGpu/tests/SparseGridGpu_tests.cu:1231-1282
+//   testCopyTo: copy_to + removeCopyToFinalize (SparseGridGpu.hpp:3314-3329, :3550-3559) of
+//   the ball of the sub-iterator test into an empty grid
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -344,6 +348,129 @@ static void testPackUnpack()
     countDst(dst, dbox, {20520, 27152, 10423, 13649});
 }
 
+// The chunks insertSphere3D writes (tests/utils/SparseGridGpu_testKernels.cuh:143-200):
+// 32^3 blocks from start / edge, points active when r1 > r >= r2 about start + 16 edges,
+// value 1 on the active points of every block that has one.
+template <typename SG> static void insertSphere3D(SG &sg, int start, float r1, float r2)
+{
+    const int e = SG::blockEdgeSize_, cs = e * e * e;
+    const int c0 = start + 16 * e;
+    std::vector<long> ids;
+    std::vector<uint8_t> masks;
+    std::vector<float> v;
+    for (int bz = 0; bz < 32; ++bz)
+        for (int by = 0; by < 32; ++by)
+            for (int bx = 0; bx < 32; ++bx) {
+                std::vector<uint8_t> m(cs);
+                std::vector<float> val(cs, 0.0f);
+                bool any = false;
+                for (int o = 0; o < cs; ++o) {
+                    const long x = start + bx * e + o % e - c0, y = start + by * e + (o / e) % e - c0,
+                               z = start + bz * e + o / (e * e) - c0;
+                    const float r = std::sqrt((float)(x * x + y * y + z * z));
+                    const bool act = r < r1 && r >= r2;
+                    m[o] = act ? 1 : 0;
+                    val[o] = act ? 1.0f : 0.0f;
+                    any = any || act;
+                }
+                if (!any) continue;
+                grid_key_dx<3, int> b0({start + bx * e, start + by * e, start + bz * e});
+                ids.push_back(sg.getLinId(b0) / cs);
+                masks.insert(masks.end(), m.begin(), m.end());
+                v.insert(v.end(), val.begin(), val.end());
+            }
+    const void *vals[2] = {v.data(), nullptr};
+    sg.insertBlocks(ids.data(), ids.size(), masks.data(), vals);
+    gpu::ofp_context_t ctx;
+    sg.template flush<smax_<0>>(ctx, FLUSH_ON_DEVICE);
+}
+
+static void testIteratorSubHost()
+{
+    const size_t sz[3] = {768, 768, 768};
+    SparseGridGpu<3, aggregate<float, float>, 4, 64, long int> sg(sz);
+    sg.setBackgroundValue<0>(0);
+    insertSphere3D(sg, 256, 32.0f, 0.0f);
+    sg.deviceToHost<0>();
+    bool match = true;
+    int count = 0;
+    auto it = sg.getIterator(grid_key_dx<3>({303, 303, 303}), grid_key_dx<3>({337, 337, 337}));
+    std::set<long> seen;
+    while (it.isNext()) {
+        auto key = it.get();
+        match = match && sg.get<0>(key) == 1.0f;
+        const auto c = key.getKey();
+        match = match && c.get(0) >= 303 && c.get(0) <= 337 && c.get(1) >= 303 && c.get(1) <= 337 &&
+                c.get(2) >= 303 && c.get(2) <= 337;
+        seen.insert(sg.getLinId(c));
+        sg.get<0>(key) = 5.0f;
+        ++count;
+        ++it;
+    }
+    REQUIRE(match);
+    REQUIRE(count == 42875);
+    REQUIRE(seen.size() == 42875u);
+    // the written host values travel back with hostToDevice
+    sg.hostToDevice<0>();
+    REQUIRE(sg.get<0>(grid_key_dx<3>({303, 337, 320})) == 5.0f);
+    REQUIRE(sg.get<0>(grid_key_dx<3>({302, 320, 320})) == 1.0f);
+}
+
+static void testIteratorHost()
+{
+    const size_t sz[3] = {512, 512, 512};
+    SparseGridGpu<3, aggregate<float, float>, 4, 64, long int> sg(sz);
+    sg.setBackgroundValue<0>(0);
+    insertSphere3D(sg, 256, 64.0f, 32.0f);
+    bool threw = false;
+    try {
+        sg.getIterator();
+    } catch (const sg::error &) {
+        threw = true;
+    }
+    REQUIRE(threw);  // no host copy yet
+    sg.deviceToHost<0>();
+    bool match = true;
+    size_t count = 0;
+    auto it = sg.getIterator();
+    while (it.isNext()) {
+        auto key = it.get();
+        match = match && sg.get<0>(key) == 1.0f && (sg.getFlag(key) & 1);
+        ++count;
+        ++it;
+    }
+    REQUIRE(sg.countExistingElements() != 0);
+    REQUIRE(sg.countExistingElements() == count);
+    REQUIRE(match);
+}
+
+static void testCopyTo()
+{
+    const size_t sz[3] = {768, 768, 768};
+    typedef SparseGridGpu<3, aggregate<float, float>, 4, 64, long int> SG;
+    SG a(sz), b(sz);
+    gpu::ofp_context_t ctx;
+    a.setBackgroundValue<0>(0);
+    b.setBackgroundValue<0>(0);
+    insertSphere3D(a, 256, 32.0f, 0.0f);
+    const size_t na = a.countExistingElements();
+    b.copy_to(a, Box<3, size_t>({303, 303, 303}, {337, 337, 337}), Box<3, size_t>({10, 20, 30}, {44, 54, 64}));
+    a.remove(Box<3, int>({0, 0, 0}, {319, 767, 767}));
+    for (int ph : {PHASE1, PHASE2, PHASE3}) {
+        a.removeCopyToFinalize<0>(ctx, ph);
+        b.removeCopyToFinalize<0>(ctx, ph);
+    }
+    // the removal comes first, so only the x >= 320 part of the box is copied
+    REQUIRE(b.countExistingElements() == 18 * 35 * 35);
+    REQUIRE(b.get<0>(grid_key_dx<3>({44, 54, 64})) == 1.0f);
+    REQUIRE(b.get<0>(grid_key_dx<3>({27, 20, 30})) == 1.0f);
+    REQUIRE(b.get<0>(grid_key_dx<3>({26, 20, 30})) == 0.0f);
+    REQUIRE(a.countExistingElements() < na);
+    size_t left = 0;
+    for (const auto &p : existingPoints(a)) left += p.get(0) >= 320;
+    REQUIRE(left == a.countExistingElements());
+}
+
 // testInsert on a Morton grid: the same known answers through SparseGridGpu_z
 static void testInsertZ()
 {
@@ -386,6 +513,9 @@ int main()
     testRemove();
     testPackUnpack();
     testInsertZ();
+    testIteratorSubHost();
+    testIteratorHost();
+    testCopyTo();
     if (failures) {
         std::printf("%d failure(s)\n", failures);
         return 1;

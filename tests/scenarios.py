@@ -340,3 +340,85 @@ def shell_in_dst_box(p, centre, k):
     z = p[:, 2].astype(np.int64) - centre
     r = np.sqrt((x * x + y * y + z * z).astype(np.float32))
     return (r < np.float32(64)) & (r > np.float32(56))
+
+
+# ---- copy_to between three grids (removeCopyToFinalize phases) ----------------------
+
+COPY_SIZE = (40, 36, 44)
+
+
+def copy_ring(G, seed, props=(0,)):
+    """Three grids of random EXIST points (float64, int32 properties). Grid k copies a box
+    of grid k+1 (mod 3) into itself; grid 0 also copies a box of itself; grid 1 removes a
+    box. Then removeCopyToFinalize phase 0, 1, 2 on every grid, in grid order.
+    Returns (grids, states before, sections, removes)."""
+    rng = np.random.default_rng(seed)
+    size = np.array(COPY_SIZE)
+    grids, before = [], []
+    for _ in range(3):
+        g = G(3, COPY_SIZE, [np.float64, np.int32], block_edge=4)
+        g.set_background_value(0, -1.5)
+        g.set_background_value(1, 7)
+        pts = rng.integers(0, size, size=(12000, 3)).astype(np.int32)
+        g.insert_coords(pts, [rng.standard_normal(len(pts)), rng.integers(-1000, 1000, len(pts)).astype(np.int32)])
+        g.flush(["smax", "smax"])
+        grids.append(g)
+        before.append(point_state(g))
+    ext = np.array([11, 13, 9])
+    sections = {0: [], 1: [], 2: []}
+    for k in range(3):
+        src = (k + 1) % 3
+        lo = rng.integers(0, size - ext)
+        dlo = np.array([int(rng.integers(0, 20 - ext[0])), int(rng.integers(0, size[1] - ext[1])),
+                        int(rng.integers(0, size[2] - ext[2]))])
+        sections[src].append((k, lo, lo + ext - 1, dlo))
+    lo = rng.integers(0, size - ext)
+    dlo = np.array([int(rng.integers(20, size[0] - ext[0])), 3, 5])
+    sections[0].append((0, lo, lo + ext - 1, dlo))
+    rlo = rng.integers(0, size - 12)
+    removes = [(1, rlo, rlo + 11)]
+    for src in range(3):
+        for dst, lo, hi, dl in sections[src]:
+            grids[dst].copy_to(grids[src], (lo, hi), (dl, dl + ext - 1))
+    for k, lo, hi in removes:
+        grids[k].remove_box(lo, hi)
+    for phase in range(3):
+        for g in grids:
+            g.remove_copy_to_finalize(list(props), phase)
+    return grids, before, sections, removes
+
+
+def point_state(g):
+    """{coordinate: (mask, values)} of every slot whose mask is not 0."""
+    ids = g.chunk_ids()
+    m = g.masks().reshape(-1)
+    vals = [g.prop(p).reshape(-1) for p in range(len(g.dtypes))]
+    flat = np.nonzero(m)[0]
+    keys = ids[flat // g.chunk_size] * g.chunk_size + flat % g.chunk_size
+    crd = g.inv_lin_id(keys)
+    return {tuple(int(v) for v in crd[i]): (int(m[f]), tuple(v[f].item() for v in vals)) for i, f in enumerate(flat)}
+
+
+def model_copy_ring(before, sections, removes, props, bg):
+    """The phases on point dictionaries: removes first, then every section reads its source
+    as left by the removes, then each destination takes the copied values on the listed
+    properties (others: old value where the point existed, else the background), mask |= EXIST."""
+    st = [dict(s) for s in before]
+    for k, lo, hi in removes:
+        for key in list(st[k]):
+            if st[k][key][0] & 1 and all(lo[d] <= key[d] <= hi[d] for d in range(3)):
+                del st[k][key]
+    incoming = [dict() for _ in st]
+    for src in range(len(st)):
+        for dst, lo, hi, dl in sections[src]:
+            for key, (m, v) in st[src].items():
+                if m & 1 and all(lo[d] <= key[d] <= hi[d] for d in range(3)):
+                    nk = tuple(int(key[d] - lo[d] + dl[d]) for d in range(3))
+                    assert nk not in incoming[dst]
+                    incoming[dst][nk] = v
+    for d in range(len(st)):
+        for nk, v in incoming[d].items():
+            om, ov = st[d].get(nk, (0, None))
+            nv = tuple(v[p] if p in props else (ov[p] if om & 1 else bg[p]) for p in range(len(v)))
+            st[d][nk] = (om | 1, nv)
+    return st

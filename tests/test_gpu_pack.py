@@ -107,6 +107,17 @@ def test_pack_unpack_random(G, oracle, seed):
     S.assert_same_state(res[0][1], res[1][1])
 
 
+@pytest.mark.parametrize("seed,props", [(0, (0,)), (1, (0, 1)), (2, (1,))])
+def test_copy_to_ring(G, oracle, seed, props):
+    """copy_to between three grids (one copying into itself, one removing a box), phases
+    0, 1, 2 on every grid: the same grids as the oracle's."""
+    res = [S.copy_ring(cls, seed, props)[0] for cls in (G, oracle.SparseGridCpu)]
+    for a, b in zip(*res):
+        S.assert_same_state(a, b, exist_only=False)
+        a.close()
+        b.close()
+
+
 def test_unpack_outside_grid_fails(G):
     from openfpm_data_amd._capi import SgError
     g = G(3, (32, 32, 32), [np.float32], block_edge=8)

@@ -3,6 +3,7 @@
 each packed box, point counts after unpacking into shifted boxes and after removing the
 even points, and the removal of box sections."""
 import numpy as np
+import pytest
 
 from tests import scenarios as S
 
@@ -80,3 +81,15 @@ def test_pack_unpack_known_counts(oracle):
     assert counts == [20520, 27152, 10423, 13649]
     src.close()
     dst.close()
+
+
+@pytest.mark.parametrize("seed,props", [(0, (0,)), (1, (0, 1)), (2, (1,))])
+def test_copy_to_ring_matches_point_model(oracle, seed, props):
+    """copy_to + removeCopyToFinalize (SparseGridGpu.hpp:1116-1290, :3314-3329, :3550-3559):
+    removes before every copy, copies read their sources as the removes left them, copied
+    properties take the source values, the others keep theirs, mask |= EXIST."""
+    grids, before, sections, removes = S.copy_ring(oracle.SparseGridCpu, seed, props)
+    model = S.model_copy_ring(before, sections, removes, props, (-1.5, 7))
+    for g, m in zip(grids, model):
+        assert S.point_state(g) == m
+        g.close()
