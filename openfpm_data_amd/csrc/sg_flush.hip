@@ -262,24 +262,62 @@ __global__ void chunk_pos_kernel(const int64_t *nkey, const int64_t *oldpos, con
 }
 
 // scatter the old chunks into the enlarged arrays (copy_old_ker, BlockMapGpu_kernels.cuh:489-513)
+// src_of[p] = the old chunk that lands at p, or -1 for a new chunk
 __global__ void move_old_keys_kernel(const int64_t *okeys, int64_t m, const int64_t *bn, int64_t nb, int64_t *nkeys,
-                                     int64_t *opos)
+                                     int64_t *opos, int64_t *src_of)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     int64_t p = i + lower_bound_dev(bn, nb, okeys[i]);
     nkeys[p] = okeys[i];
     opos[i] = p;
+    src_of[p] = i;
 }
 
 __global__ void place_new_keys_kernel(const int64_t *bn, int64_t nb, const int64_t *okeys, int64_t m, int64_t *nkeys,
-                                      int64_t *bpos)
+                                      int64_t *bpos, int64_t *src_of)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
     int64_t p = b + lower_bound_dev(okeys, m, bn[b]);
     nkeys[p] = bn[b];
     bpos[b] = p;
+    src_of[p] = -1;
+}
+
+// The grown chunk arrays in one pass: for every array (mask, properties) and every output
+// chunk position j < nout, chunk j is the old chunk src_of[j] or, for a new chunk and the
+// background chunk at nout - 1, the array's fill pattern. 16-byte vectors in output order,
+// so every store stream is sequential.
+struct UnionArrays {
+    int na;
+    const uint4 *src[kMaxProps + 1];
+    uint4 *dst[kMaxProps + 1];
+    uint64_t pat[kMaxProps + 1];  // fill value replicated over 8 bytes
+    int vshift[kMaxProps + 1];    // log2 of the 16-byte vectors per chunk
+};
+
+__global__ __launch_bounds__(kWG) void build_union_kernel(UnionArrays U, const int64_t *src_of, int64_t nout)
+{
+    const int64_t stride = (int64_t)gridDim.x * kWG;
+    // constant indices into the kernel argument (a runtime index would copy it to scratch)
+#pragma unroll
+    for (int a = 0; a < kMaxProps + 1; ++a) {
+        if (a >= U.na) break;
+        const int sh = U.vshift[a];
+        const int64_t nv = nout << sh, lane_mask = ((int64_t)1 << sh) - 1;
+        const uint4 *src = U.src[a];
+        uint4 *dst = U.dst[a];
+        const uint4 pat = make_uint4((uint32_t)U.pat[a], (uint32_t)(U.pat[a] >> 32), (uint32_t)U.pat[a],
+                                     (uint32_t)(U.pat[a] >> 32));
+        for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < nv; i += stride) {
+            const int64_t j = i >> sh;
+            const int64_t c = (j < nout - 1) ? src_of[j] : -1;
+            uint4 v = pat;
+            if (c >= 0) v = src[(c << sh) + (i & lane_mask)];
+            dst[i] = v;
+        }
+    }
 }
 
 // one workgroup per chunk: copy chunk `src` of an array to chunk `dst` of another
@@ -866,17 +904,43 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
             if (!nprop[p]) return fail(SG_E_OOM, "chunk array allocation failed");
         }
         DevBuf &b_o = g->scratch[6];
-        if ((rc = ensure(g->pool, b_o, (size_t)(m + nb) * 8 + 128))) return rc;
+        if ((rc = ensure(g->pool, b_o, (size_t)(m + nb) * 8 * 2 + 128))) return rc;
         int64_t *opos = (int64_t *)b_o.p;
         int64_t *bpos = opos + m;
+        int64_t *src_of = bpos + nb;
         if (m > 0) {
             hipLaunchKernelGGL(move_old_keys_kernel, dim3(grid_1d(m, kWG)), dim3(kWG), 0, g->stream, g->d_keys, m, bn,
-                               nb, nkeys, opos);
+                               nb, nkeys, opos, src_of);
             SG_HIP(hipGetLastError());
         }
         hipLaunchKernelGGL(place_new_keys_kernel, dim3(grid_1d(nb, kWG)), dim3(kWG), 0, g->stream, bn, nb, g->d_keys,
-                           m, nkeys, bpos);
+                           m, nkeys, bpos, src_of);
         SG_HIP(hipGetLastError());
+        // one streaming pass when every array's chunk is a power-of-two count of 16-byte vectors
+        UnionArrays U;
+        U.na = 1 + g->np;
+        bool flat = true;
+        for (int a = 0; a < U.na && flat; ++a) {
+            const size_t cb = (size_t)csz * (a == 0 ? 1 : g->ts[a - 1]);
+            flat = cb % 16 == 0 && ((cb / 16) & (cb / 16 - 1)) == 0;
+            if (!flat) break;
+            int sh = 0;
+            while (((size_t)1 << sh) < cb / 16) ++sh;
+            U.vshift[a] = sh;
+            unsigned char pat[8];
+            const int ts = (a == 0) ? 1 : g->ts[a - 1];
+            for (int b = 0; b < 8; b += ts) {
+                if (a == 0) pat[b] = 0;
+                else memcpy(pat + b, g->bg[a - 1], ts);
+            }
+            memcpy(&U.pat[a], pat, 8);
+            U.src[a] = (const uint4 *)(a == 0 ? (const void *)g->d_mask : g->d_prop[a - 1]);
+            U.dst[a] = (uint4 *)(a == 0 ? (void *)nmask : nprop[a - 1]);
+        }
+        if (flat) {
+            hipLaunchKernelGGL(build_union_kernel, dim3(2048), dim3(kWG), 0, g->stream, U, src_of, total + 1);
+            SG_HIP(hipGetLastError());
+        }
         // chunk data: copy as 16-byte vectors when the chunk size allows
         auto move = [&](const void *src, void *dst, size_t chunk_bytes) -> int {
             if (m == 0) return SG_OK;
@@ -889,14 +953,16 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
             SG_HIP(hipGetLastError());
             return SG_OK;
         };
-        if ((rc = move(g->d_mask, nmask, csz))) return rc;
-        hipLaunchKernelGGL(init_chunks_kernel<uint8_t>, dim3((unsigned)nb), dim3(kWG), 0, g->stream, nmask, bpos, nb,
-                           csz, (uint8_t)0);
-        SG_HIP(hipGetLastError());
-        hipLaunchKernelGGL(init_chunks_kernel<uint8_t>, dim3(1), dim3(kWG), 0, g->stream, nmask + total * csz,
-                           (const int64_t *)nullptr, (int64_t)1, csz, (uint8_t)0);
-        SG_HIP(hipGetLastError());
-        for (int p = 0; p < g->np; ++p) {
+        if (!flat && (rc = move(g->d_mask, nmask, csz))) return rc;
+        if (!flat) {
+            hipLaunchKernelGGL(init_chunks_kernel<uint8_t>, dim3((unsigned)nb), dim3(kWG), 0, g->stream, nmask, bpos,
+                               nb, csz, (uint8_t)0);
+            SG_HIP(hipGetLastError());
+            hipLaunchKernelGGL(init_chunks_kernel<uint8_t>, dim3(1), dim3(kWG), 0, g->stream, nmask + total * csz,
+                               (const int64_t *)nullptr, (int64_t)1, csz, (uint8_t)0);
+            SG_HIP(hipGetLastError());
+        }
+        for (int p = 0; p < g->np && !flat; ++p) {
             if ((rc = move(g->d_prop[p], nprop[p], (size_t)csz * g->ts[p]))) return rc;
             dispatch_type(g->dt[p], [&](auto tag) {
                 typedef decltype(tag) T;

@@ -60,41 +60,87 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *total)
     return pre + inc - v;
 }
 
+// Scan tiles are 4096 items read as four striped passes: pass j covers items
+// [j * 1024, (j + 1) * 1024) of the tile, thread t holding items 4t .. 4t + 3 of it, so each
+// load instruction of a wave reads 1 KB contiguous (one 16-byte vector per lane when the
+// arrays are 16-byte aligned).
+constexpr int kScanVec = 4;
+constexpr int kScanPass = kScanIPT / kScanVec;  // 4 passes of 1024 items
+
+template <bool VEC> __device__ inline void scan_load(const uint32_t *in, int64_t e, int64_t n, uint32_t (&v)[kScanVec])
+{
+    if (VEC && e + kScanVec <= n) {
+        const uint4 q = *(const uint4 *)(in + e);
+        v[0] = q.x;
+        v[1] = q.y;
+        v[2] = q.z;
+        v[3] = q.w;
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < kScanVec; ++i) v[i] = (e + i < n) ? in[e + i] : 0u;
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(kScanWG) void scan_reduce_kernel(const uint32_t *in, uint32_t *block_sums, int64_t n)
 {
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanVec;
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanIPT; ++k) {
-        int64_t i = base + k;
-        if (i < n) s += in[i];
+    for (int j = 0; j < kScanPass; ++j) {
+        uint32_t v[kScanVec];
+        scan_load<VEC>(in, base + (int64_t)j * (kScanWG * kScanVec), n, v);
+        s += v[0] + v[1] + v[2] + v[3];
     }
     uint32_t tot;
     block_exclusive_scan(s, &tot);
     if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(kScanWG) void scan_down_kernel(const uint32_t *in, uint32_t *out, const uint32_t *block_offsets,
-                                                            int64_t n, uint32_t *total_out)
+// exclusive scan of the tile (in place allowed), offset by block_offsets[tile]
+template <bool VEC>
+__global__ __launch_bounds__(kScanWG) void scan_down_kernel(const uint32_t *in, uint32_t *out,
+                                                            const uint32_t *block_offsets, int64_t n,
+                                                            uint32_t *total_out)
 {
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanIPT;
-    uint32_t v[kScanIPT];
-    uint32_t s = 0;
+    __shared__ uint32_t s_wave[kScanPass][kScanWG / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanVec;
+    uint32_t v[kScanPass][kScanVec], inc[kScanPass];
 #pragma unroll
-    for (int k = 0; k < kScanIPT; ++k) {
-        int64_t i = base + k;
-        v[k] = (i < n) ? in[i] : 0u;
-        s += v[k];
+    for (int j = 0; j < kScanPass; ++j) {
+        scan_load<VEC>(in, base + (int64_t)j * (kScanWG * kScanVec), n, v[j]);
+        inc[j] = wave_inclusive_scan(v[j][0] + v[j][1] + v[j][2] + v[j][3]);
+        if (lane == 63) s_wave[j][w] = inc[j];
     }
-    uint32_t tot;
-    uint32_t pre = block_exclusive_scan(s, &tot);
-    pre += block_offsets ? block_offsets[blockIdx.x] : 0u;
+    __syncthreads();
+    uint32_t off = block_offsets ? block_offsets[blockIdx.x] : 0u;
 #pragma unroll
-    for (int k = 0; k < kScanIPT; ++k) {
-        int64_t i = base + k;
-        if (i < n) out[i] = pre;
-        pre += v[k];
-        if (total_out && i == n - 1) *total_out = pre;
+    for (int j = 0; j < kScanPass; ++j) {
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int i = 0; i < kScanWG / 64; ++i) {
+            const uint32_t sw = s_wave[j][i];
+            if (i < w) pre += sw;
+            tot += sw;
+        }
+        uint32_t run = off + pre + inc[j] - (v[j][0] + v[j][1] + v[j][2] + v[j][3]);
+        const int64_t e = base + (int64_t)j * (kScanWG * kScanVec);
+        uint32_t o[kScanVec];
+#pragma unroll
+        for (int i = 0; i < kScanVec; ++i) {
+            o[i] = run;
+            run += v[j][i];
+            if (total_out && e + i == n - 1) *total_out = run;
+        }
+        if (VEC && e + kScanVec <= n) {
+            *(uint4 *)(out + e) = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < kScanVec; ++i)
+                if (e + i < n) out[e + i] = o[i];
+        }
+        off += tot;
     }
 }
 
@@ -247,23 +293,33 @@ int exclusive_scan_u32(sg_grid *g, const uint32_t *in, uint32_t *out, int64_t n,
         at += sizes[l];
     }
     uint32_t *d_total = pool + at;
+    auto aligned = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
     const uint32_t *lin = in;
     for (size_t l = 0; l + 1 < sizes.size(); ++l) {
-        hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)sizes[l + 1]), dim3(kScanWG), 0, g->stream, lin,
-                           sums[l + 1], sizes[l]);
+        if (aligned(lin))
+            hipLaunchKernelGGL(scan_reduce_kernel<true>, dim3((unsigned)sizes[l + 1]), dim3(kScanWG), 0, g->stream,
+                               lin, sums[l + 1], sizes[l]);
+        else
+            hipLaunchKernelGGL(scan_reduce_kernel<false>, dim3((unsigned)sizes[l + 1]), dim3(kScanWG), 0, g->stream,
+                               lin, sums[l + 1], sizes[l]);
         SG_HIP(hipGetLastError());
         lin = sums[l + 1];
     }
+    auto down = [&](const uint32_t *src, uint32_t *dst, const uint32_t *bo, int64_t len, uint32_t *tot) {
+        const dim3 grid((unsigned)((len + kScanTile - 1) / kScanTile));
+        if (aligned(src) && aligned(dst))
+            hipLaunchKernelGGL(scan_down_kernel<true>, grid, dim3(kScanWG), 0, g->stream, src, dst, bo, len, tot);
+        else
+            hipLaunchKernelGGL(scan_down_kernel<false>, grid, dim3(kScanWG), 0, g->stream, src, dst, bo, len, tot);
+    };
     for (size_t l = sizes.size() - 1; l >= 1; --l) {
         // scan level l (its single top tile or using offsets from level l+1)
         const uint32_t *bo = (l + 1 < sizes.size()) ? offs[l + 1] : nullptr;
-        hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)((sizes[l] + kScanTile - 1) / kScanTile)), dim3(kScanWG),
-                           0, g->stream, sums[l], offs[l], bo, sizes[l], (uint32_t *)nullptr);
+        down(sums[l], offs[l], bo, sizes[l], (uint32_t *)nullptr);
         SG_HIP(hipGetLastError());
     }
     const uint32_t *bo0 = (sizes.size() > 1) ? offs[1] : nullptr;
-    hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)((n + kScanTile - 1) / kScanTile)), dim3(kScanWG), 0,
-                       g->stream, in, out, bo0, n, total ? d_total : (uint32_t *)nullptr);
+    down(in, out, bo0, n, total ? d_total : (uint32_t *)nullptr);
     SG_HIP(hipGetLastError());
     if (total) {
         // the inclusive total is written by the thread owning item n-1 (works in place)

@@ -534,12 +534,14 @@ __device__ inline uint32_t exist_bits8(uint64_t row)
 }
 
 // ghost row i (0..35) of the 10 x 10 tile: (yy, zz) in [-1, 8]^2 outside [0, 7]^2
+// (selects, not branches: branchy stores through the references end up in scratch)
 __device__ inline void ghost_row_yz(int i, int &yy, int &zz)
 {
-    if (i < 10) { zz = -1; yy = i - 1; }
-    else if (i < 20) { zz = 8; yy = i - 11; }
-    else if (i < 28) { yy = -1; zz = i - 20; }
-    else { yy = 8; zz = i - 28; }
+    const bool zface = i < 20;  // rows 0-19: z = -1 / 8 faces; rows 20-35: y = -1 / 8 faces
+    const int along = zface ? (i < 10 ? i - 1 : i - 11) : (i < 28 ? i - 20 : i - 28);
+    const int face = zface ? (i < 10 ? -1 : 8) : (i < 28 ? -1 : 8);
+    yy = zface ? along : face;
+    zz = zface ? face : along;
 }
 
 // neighbour-table entry of chunk displacement (dx, dy, dz): star order -x +x -y +y -z +z,
