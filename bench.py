@@ -156,10 +156,20 @@ def run_flush_secondary(cpu_threads, with_cpu):
         nch, npts = g.num_chunks(), g.count_existing_elements()
         g.close()
     t = np.median(np.array(times[1:]), axis=0)  # first repetition warms the allocator
+    # algorithmic bytes: insert records in (8-byte key + 12 value bytes + mask byte), every
+    # slot of the resulting chunks written once (3 x 4-byte values + mask); tagBoundaries
+    # reads and writes each mask byte; the box reads one uint32 and the mask and writes one
+    # uint32 per slot; both read 26 neighbour-table entries per chunk
+    slots = nch * 512
+    rl = lambda b, ms: {"bound": "hbm", "achieved": round(b / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,  # noqa: E731
+                        "unit": "GB/s", "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     out = {"workload": "blobs1024_flush", "inserts": n, "chunks": nch, "points": npts,
            "insert_flush_ms": round(float(t[0]), 3), "inserts_per_s": round(n / (t[0] * 1e-3) / 1e9, 4),
            "unit": "G inserts/s", "find_neighbours_full_ms": round(float(t[1]), 3),
-           "tag_boundaries_full_ms": round(float(t[2]), 3), "box27_uint32_ms": round(float(t[3]), 3)}
+           "tag_boundaries_full_ms": round(float(t[2]), 3), "box27_uint32_ms": round(float(t[3]), 3),
+           "roofline": {"insert_flush": rl(n * 21 + slots * 13, t[0]),
+                        "tag_boundaries_full": rl(slots * 2 + nch * 26 * 4, t[2]),
+                        "box27_uint32": rl(slots * 9 + nch * 26 * 4, t[3])}}
     if with_cpu:
         from oracle import pysgcpu
         pysgcpu.set_threads(cpu_threads)
@@ -287,6 +297,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-secondary", action="store_true", help="skip the insertion/flush workload")
+    ap.add_argument("--only", choices=["flush", "poisson", "serving"],
+                    help="run one secondary workload alone and print its JSON (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 20 device copies of 1 GiB (reference for the FETCH/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -305,6 +317,15 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (the one-GPU "
               "figure is the kernel at its per-GPU shape); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        if args.only == "flush":
+            print(json.dumps(run_flush_secondary(threads, not args.no_cpu)))
+        elif args.only == "poisson":
+            print(json.dumps(run_poisson_secondary()))
+        else:
+            print(json.dumps(run_serving_secondary()))
+        return
 
     from openfpm_data_amd import workloads as W
     from openfpm_data_amd import autotune
