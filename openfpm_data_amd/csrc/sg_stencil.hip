@@ -562,7 +562,8 @@ __global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_kernel(int64_t n, 
                                                                     int nnn, int type, uint8_t *__restrict__ mask)
 {
     __shared__ uint16_t tile[WAVES][100];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // uniform wave index: the neighbour-table row is read by scalar loads
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t c = (int64_t)blockIdx.x * WAVES + w;
     if (c >= n) return;  // waves are independent: no workgroup barrier below
     const int32_t *row_nn = nn + c * nnn;
@@ -640,7 +641,8 @@ __global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32
                                                          int64_t bsz0, int64_t bsz1, T bg_dst, StencilArgs sa)
 {
     __shared__ T tile[WAVES][100 * 10];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // uniform wave index: the neighbour-table row is read by scalar loads
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t c = (int64_t)blockIdx.x * WAVES + w;
     if (c >= n) return;  // waves are independent
     const int32_t *row_nn = nn + c * 27;
@@ -1147,23 +1149,24 @@ __global__ __launch_bounds__(64) void stencil_wave_kernel(const int32_t *__restr
 // are usually x-neighbours, so each wave publishes its x = 0 and x = 7 columns in LDS
 // and takes its x-ghosts from the neighbouring wave when the neighbour table says the
 // -x / +x chunk is the previous / next chunk of the group; only the other x-ghosts are
-// loaded from memory.  Group g runs in XCD g's slot (g & 7 of the launch order).
+// loaded from memory.  One group: chunks grp * W .. grp * W + W - 1.
+template <typename T, int KIND, int W> struct GroupLds {
+    static constexpr bool kTwo = (KIND == SG_ST_GRAY_SCOTT);
+    static constexpr int NP = kTwo ? 2 : 1;
+    __attribute__((aligned(16))) T tiles0[W][100 * 8];
+    __attribute__((aligned(16))) T tiles1[W][kTwo ? 100 * 8 : 8];
+    T cols[W][NP][2][64];  // [wave][property][x = 0 | x = 7][row]
+};
+
 template <typename T, int KIND, int W>
-__global__ __launch_bounds__(64 * W) void stencil_group_kernel(const int32_t *__restrict__ nn, int64_t n,
-                                                               const uint8_t *__restrict__ mask,
-                                                               const T *__restrict__ src0, const T *__restrict__ src1,
-                                                               T *__restrict__ dst0, T *__restrict__ dst1, int alias,
-                                                               StencilArgs sa, const int64_t *__restrict__ keys,
-                                                               int64_t bsz0, int64_t bsz1, int64_t groups_per_xcd)
+__device__ inline void group_step(GroupLds<T, KIND, W> &L, int64_t grp, int w, int lane,
+                                  const int32_t *__restrict__ nn, int64_t n, const uint8_t *__restrict__ mask,
+                                  const T *__restrict__ src0, const T *__restrict__ src1, T *__restrict__ dst0,
+                                  T *__restrict__ dst1, int alias, const StencilArgs &sa,
+                                  const int64_t *__restrict__ keys, int64_t bsz0, int64_t bsz1)
 {
-    constexpr bool kTwo = (KIND == SG_ST_GRAY_SCOTT);
-    constexpr int NP = kTwo ? 2 : 1;
-    __shared__ __attribute__((aligned(16))) T tiles0[W][100 * 8];
-    __shared__ __attribute__((aligned(16))) T tiles1[W][kTwo ? 100 * 8 : 8];
-    __shared__ T cols[W][NP][2][64];  // [wave][property][x = 0 | x = 7][row]
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t b = blockIdx.x;
-    const int64_t grp = (b & 7) * groups_per_xcd + (b >> 3);
+    constexpr bool kTwo = GroupLds<T, KIND, W>::kTwo;
+    constexpr int NP = GroupLds<T, KIND, W>::NP;
     const int64_t c = grp * W + w;
     const bool valid = c < n;  // every wave reaches the barrier
     ChunkIn<T, KIND> in;
@@ -1177,24 +1180,43 @@ __global__ __launch_bounds__(64 * W) void stencil_group_kernel(const int32_t *__
         xp_grp = (w < W - 1) && (c + 1 < n) && nbp[1] == c + 1;
         issue_ghosts<T>(in.u, src0, nbp, lane, !xm_grp, !xp_grp);
         if constexpr (kTwo) issue_ghosts<T>(in.v, src1, nbp, lane, !xm_grp, !xp_grp);
-        cols[w][0][0][lane] = in.u.own.v[0];
-        cols[w][0][1][lane] = in.u.own.v[7];
+        L.cols[w][0][0][lane] = in.u.own.v[0];
+        L.cols[w][0][1][lane] = in.u.own.v[7];
         if constexpr (kTwo) {
-            cols[w][NP - 1][0][lane] = in.v.own.v[0];
-            cols[w][NP - 1][1][lane] = in.v.own.v[7];
+            L.cols[w][NP - 1][0][lane] = in.v.own.v[0];
+            L.cols[w][NP - 1][1][lane] = in.v.own.v[7];
         }
     }
     __syncthreads();
     if (!valid) return;
     if (xm_grp) {
-        in.u.xm = cols[w - 1][0][1][lane];
-        if constexpr (kTwo) in.v.xm = cols[w - 1][NP - 1][1][lane];
+        in.u.xm = L.cols[w - 1][0][1][lane];
+        if constexpr (kTwo) in.v.xm = L.cols[w - 1][NP - 1][1][lane];
     }
     if (xp_grp) {
-        in.u.xp = cols[w + 1][0][0][lane];
-        if constexpr (kTwo) in.v.xp = cols[w + 1][NP - 1][0][lane];
+        in.u.xp = L.cols[w + 1][0][0][lane];
+        if constexpr (kTwo) in.v.xp = L.cols[w + 1][NP - 1][0][lane];
     }
-    process_chunk<T, KIND>(in, c, tiles0[w], tiles1[w], dst0, dst1, alias, sa, keys, bsz0, bsz1, lane);
+    process_chunk<T, KIND>(in, c, L.tiles0[w], L.tiles1[w], dst0, dst1, alias, sa, keys, bsz0, bsz1, lane);
+}
+
+// one group per workgroup; group g runs in XCD g's slot (g & 7 of the launch order)
+template <typename T, int KIND, int W>
+__global__ __launch_bounds__(64 * W) void stencil_group_kernel(const int32_t *__restrict__ nn, int64_t n,
+                                                               const uint8_t *__restrict__ mask,
+                                                               const T *__restrict__ src0, const T *__restrict__ src1,
+                                                               T *__restrict__ dst0, T *__restrict__ dst1, int alias,
+                                                               StencilArgs sa, const int64_t *__restrict__ keys,
+                                                               int64_t bsz0, int64_t bsz1, int64_t groups_per_xcd)
+{
+    __shared__ GroupLds<T, KIND, W> L;
+    // the wave index is uniform: with it in a scalar register the neighbour-table row is
+    // read by scalar loads, which the ghost loads wait for without waiting for the own
+    // rows (vector loads complete in order)
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t b = blockIdx.x;
+    const int64_t grp = (b & 7) * groups_per_xcd + (b >> 3);
+    group_step<T, KIND, W>(L, grp, w, lane, nn, n, mask, src0, src1, dst0, dst1, alias, sa, keys, bsz0, bsz1);
 }
 
 template <typename T>
