@@ -271,7 +271,9 @@ int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, s
  * 3 / 4 / 5 workgroups of 4 / 8 / 2 consecutive chunks sharing x faces in LDS */
 #define SG_TUNE_STENCIL_VARIANT 2
 /* 8^3 star-stencil stores: 0 active points only, 1 whole 8-point rows where every point
- * is active or absent (absent points keep the background), 2 as 1, non-temporal */
+ * is active or absent (absent points keep the background), 2 as 1, non-temporal, 3 as 2
+ * with the rows passed through LDS so that each store instruction writes 1 KB of the
+ * chunk contiguously */
 #define SG_TUNE_STENCIL_STORES 3
 /* 8^3 star-stencil loads: 1 = read the mask first and skip rows without points */
 #define SG_TUNE_STENCIL_SKIP_EMPTY_ROWS 4

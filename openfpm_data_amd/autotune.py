@@ -30,11 +30,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PACKAGED = os.path.join(_HERE, "tuned.json")
 
 STENCIL_CANDIDATES = [
-    {"variant": 0, "stores": 2},
-    {"variant": 1, "wg_per_xcd": 64, "stores": 2},
-    {"variant": 1, "wg_per_xcd": 160, "stores": 2},
-    {"variant": 1, "wg_per_xcd": 256, "stores": 2},
-    {"variant": 5, "stores": 2},
+    {"variant": 0, "stores": 3},
+    {"variant": 1, "wg_per_xcd": 160, "stores": 3},
+    {"variant": 5, "stores": 3},
+    {"variant": 3, "stores": 3},
+    {"variant": 4, "stores": 3},
     {"variant": 3, "stores": 2},
     {"variant": 4, "stores": 2},
     {"variant": 4, "stores": 1},

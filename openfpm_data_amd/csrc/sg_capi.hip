@@ -348,7 +348,7 @@ int sg_set_tuning(sg_grid *g, int key, int64_t value)
         return SG_OK;
     }
     if (key == SG_TUNE_STENCIL_STORES) {
-        if (value < 0 || value > 2) return fail(SG_E_RANGE, "store mode must be 0, 1 or 2");
+        if (value < 0 || value > 3) return fail(SG_E_RANGE, "store mode must be 0, 1, 2 or 3");
         g->fast_wmode = (int)value;
         return SG_OK;
     }

@@ -114,11 +114,13 @@ struct sg_grid {
     // per XCD; set through sg_set_tuning (the autotuner picks it per workload)
     int64_t fast_wg_per_xcd = 160;
     // 0: one chunk per 64-lane workgroup; 1: persistent + pipelined; 2: persistent +
-    // pipelined with registers capped for 4 waves per SIMD
-    int fast_variant = 4;
+    // pipelined with registers capped for 4 waves per SIMD; 3 / 4 / 5: workgroups of
+    // 4 / 8 / 2 consecutive chunks sharing x faces in LDS
+    int fast_variant = 3;
     // stores of the 8^3 star-stencil kernel: 0 active points only, 1 whole rows where
-    // every slot is active or empty, 2 as 1 with non-temporal stores
-    int fast_wmode = 2;
+    // every slot is active or empty, 2 as 1 with non-temporal stores, 3 as 2 through LDS
+    // as lane-contiguous 16-byte vectors
+    int fast_wmode = 3;
     // read the mask first and skip loading 8-point rows without points
     int fast_lskip = 0;
 

@@ -181,7 +181,8 @@ __device__ void load_enlarged(T *eb, const T *arr, const int32_t *nnrow, int typ
 
 struct StencilArgs {
     int kind, apply, use_box;
-    int wmode;  // 8^3 fast path stores: 0 active points only, 1 whole rows, 2 whole rows non-temporal
+    int wmode;  // 8^3 fast path stores: 0 active points only, 1 whole rows, 2 whole rows non-temporal,
+                // 3 as 2 through LDS as lane-contiguous 16-byte vectors
     int lo[3], hi[3];
     double bgd[2];  // destination backgrounds
     double bgs[2];  // source backgrounds
@@ -794,6 +795,14 @@ template <typename T> __device__ inline void store_row_nt(T *p, const Row8<T> &r
     }
 }
 
+// 16 bytes from LDS to global memory, non-temporal (raw bits)
+template <typename T> __device__ inline void copy16_nt(T *dst, const T *lds)
+{
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u a = *(const v4u *)lds;
+    __builtin_nontemporal_store(a, (v4u *)dst);
+}
+
 __device__ inline void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1044,6 +1053,52 @@ __device__ inline void process_chunk(const ChunkIn<T, KIND> &in, int64_t c, T *t
         // a slot without a point holds the destination's background (library invariant),
         // so rewriting it with the background changes nothing
         whole = whole && (act[x] || (uint8_t)(in.mrow >> (8 * x)) == 0);
+    }
+    if (sa.wmode == 3 && !alias) {
+        // The rows go through this wave's LDS tile so that every store instruction writes
+        // 1 KB of the chunk contiguously (16 bytes per lane), non-temporal.  Row flags:
+        // 0 no point (nothing stored: the row holds the background), 1 whole row (absent
+        // points take the background), 2 active points only (bits 8-15: which).
+        uint32_t fl = 0;
+        if (in.mrow != 0) {
+            if (whole) {
+                fl = 1;
+                const T b0 = (T)sa.bgd[0], b1 = (T)sa.bgd[1];
+#pragma unroll
+                for (int x = 0; x < 8; ++x) {
+                    r0.v[x] = act[x] ? r0.v[x] : b0;
+                    if constexpr (kTwo) r1.v[x] = act[x] ? r1.v[x] : b1;
+                }
+            } else {
+                fl = 2;
+#pragma unroll
+                for (int x = 0; x < 8; ++x) fl |= (uint32_t)act[x] << (8 + x);
+            }
+        }
+        store_row(tile0 + lane * 8, r0);
+        if constexpr (kTwo) store_row(tile1 + lane * 8, r1);
+        wave_lds_sync();
+        constexpr int VPR = (int)sizeof(T) / 2;   // 16-byte vectors per 8-point row
+        constexpr int PPV = 16 / (int)sizeof(T);  // points per vector
+#pragma unroll
+        for (int k = 0; k < VPR; ++k) {
+            const int vi = k * 64 + lane, row = vi / VPR, part = vi % VPR;
+            const uint32_t f = (uint32_t)__shfl((int)fl, row, 64);
+            const int off = row * 8 + part * PPV;
+            if ((f & 3) == 1) {
+                copy16_nt(dst0 + c * 512 + off, tile0 + off);
+                if constexpr (kTwo) copy16_nt(dst1 + c * 512 + off, tile1 + off);
+            } else if (f & 2) {
+#pragma unroll
+                for (int x = 0; x < PPV; ++x)
+                    if ((f >> (8 + part * PPV + x)) & 1u) {
+                        dst0[c * 512 + off + x] = tile0[off + x];
+                        if constexpr (kTwo) dst1[c * 512 + off + x] = tile1[off + x];
+                    }
+            }
+        }
+        wave_lds_sync();  // the tiles are rewritten by the next chunk of this wave
+        return;
     }
     T *o0 = dst0 + c * 512 + lane * 8;
     T *o1 = kTwo ? dst1 + c * 512 + lane * 8 : nullptr;

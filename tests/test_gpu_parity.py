@@ -261,7 +261,8 @@ def test_blobs_flush_subsample(G, oracle):
 
 
 @pytest.mark.parametrize("variant,stores,lskip", [(0, 0, 0), (0, 1, 1), (1, 2, 0), (2, 1, 0), (3, 2, 1),
-                                                  (4, 2, 0), (4, 0, 1), (5, 1, 0)])
+                                                  (4, 2, 0), (4, 0, 1), (5, 1, 0), (4, 3, 0), (3, 3, 1),
+                                                  (0, 3, 0), (1, 3, 1)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_fast_stencil_variants(G, oracle, variant, stores, lskip, dtype):
     """Every selectable launch shape of the 8^3 star-stencil kernel gives the oracle's
