@@ -153,6 +153,39 @@ struct sg_grid {
 
 namespace sg {
 
+// ---- chunk write-back --------------------------------------------------------------
+// 16 bytes from LDS to global memory, non-temporal (raw bits)
+template <typename T> __device__ inline void copy16_nt(T *dst, const T *lds)
+{
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const v4u a = *(const v4u *)lds;
+    __builtin_nontemporal_store(a, (v4u *)dst);
+}
+
+// Write back one property of an 8^3 chunk from a wave's LDS staging area `lds` (row r of
+// the chunk at lds + 8 r, written there by the lane owning the row) as lane-contiguous
+// 16-byte non-temporal stores, 1 KB per instruction.  fl is the calling lane's row flag:
+// 0 nothing to store, 1 whole row, 2 active points only (bits 8-15: which); the lanes
+// storing a row read its flag by a cross-lane shuffle.
+template <typename T> __device__ inline void store_chunk_contig(T *dst, const T *lds, uint32_t fl, int lane)
+{
+    constexpr int VPR = (int)sizeof(T) / 2;   // 16-byte vectors per 8-point row
+    constexpr int PPV = 16 / (int)sizeof(T);  // points per vector
+#pragma unroll
+    for (int k = 0; k < VPR; ++k) {
+        const int vi = k * 64 + lane, row = vi / VPR, part = vi % VPR;
+        const uint32_t f = (uint32_t)__shfl((int)fl, row, 64);
+        const int off = row * 8 + part * PPV;
+        if ((f & 3) == 1) {
+            copy16_nt(dst + off, lds + off);
+        } else if (f & 2) {
+#pragma unroll
+            for (int x = 0; x < PPV; ++x)
+                if ((f >> (8 + part * PPV + x)) & 1u) dst[off + x] = lds[off + x];
+        }
+    }
+}
+
 // ---- chunk linearizers -----------------------------------------------------------
 // grid_smb (src/Grid/Geometry/grid_smb.hpp:175-297): chunk id row-major, x fastest.
 // grid_zmb (src/Grid/Geometry/grid_zmb.hpp:65-106, util/zmorton.hpp): chunk id = Morton

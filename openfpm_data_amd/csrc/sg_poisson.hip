@@ -177,7 +177,12 @@ __device__ inline void sweep_tile(const PGrid &g, int64_t c, T *tile, int lane, 
         const T nv = cur + wgt * (gs - cur);
         out.v[i] = (active && mine) ? nv : cur;
     }
-    if (mrow != 0) st8(x + c * 512 + lane * 8, out);
+    // rows with a point are written whole (both colours: the other colour keeps its
+    // value): staged in the LDS tile, then written back as lane-contiguous 16-byte stores
+    wave_sync_lds();  // every lane has read its neighbour rows
+    st8(tile + lane * 8, out);
+    wave_sync_lds();
+    store_chunk_contig(x + c * 512, tile, mrow != 0 ? 1u : 0u, lane);
     // the next tile of this wave restages the LDS tile: finish reading it first
     wave_sync_lds();
 }
@@ -194,7 +199,9 @@ __global__ __launch_bounds__(64 * WAVES) void rbsor_batch_kernel(const PGrid *__
                                                                 int zero_x, int sweep)
 {
     __shared__ __attribute__((aligned(16))) T tiles[WAVES][100 * 8];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // uniform wave index: the tile's grid descriptor and neighbour-table row come by
+    // scalar loads
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t nact = act[sweep];
     const int64_t nwaves = (int64_t)gridDim.x * WAVES;
     for (int64_t t = (int64_t)blockIdx.x * WAVES + w; t < nact; t += nwaves) {
@@ -220,7 +227,7 @@ __global__ __launch_bounds__(256) void defect_batch_kernel(const PGrid *__restri
                                                            double *__restrict__ partial)
 {
     __shared__ __attribute__((aligned(16))) double tiles[4][100 * 8];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t t = (int64_t)blockIdx.x * 4 + w;
     if (t >= ntiles) return;
     const int gi = tile_grid[t];

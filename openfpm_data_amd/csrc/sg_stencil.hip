@@ -677,59 +677,74 @@ __global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const uint64_t mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
-    if (mrow == 0) return;  // no point in this row: the destination already holds the background
-    bool inbox[8];
+    // row flag: 0 no point (the destination row already holds the background), 1 whole
+    // row, 2 active points only (bits 8-15: which)
+    uint32_t fl = 0;
+    T out[8];
+    if (mrow != 0) {
+        bool inbox[8];
 #pragma unroll
-    for (int x = 0; x < 8; ++x) inbox[x] = true;
-    if (sa.use_box) {
-        ChunkLin L;
-        L.dim = 3;
-        L.zorder = sa.zorder;
-        L.bsz[0] = bsz0;
-        L.bsz[1] = bsz1;
-        L.bsz[2] = 1 << 30;
-        int64_t bc[3];
-        chunk_coords(L, keys[c], bc);
-        const int64_t py = bc[1] * 8 + y, pz = bc[2] * 8 + z;
-        const bool yz = py >= sa.lo[1] && py <= sa.hi[1] && pz >= sa.lo[2] && pz <= sa.hi[2];
+        for (int x = 0; x < 8; ++x) inbox[x] = true;
+        if (sa.use_box) {
+            ChunkLin L;
+            L.dim = 3;
+            L.zorder = sa.zorder;
+            L.bsz[0] = bsz0;
+            L.bsz[1] = bsz1;
+            L.bsz[2] = 1 << 30;
+            int64_t bc[3];
+            chunk_coords(L, keys[c], bc);
+            const int64_t py = bc[1] * 8 + y, pz = bc[2] * 8 + z;
+            const bool yz = py >= sa.lo[1] && py <= sa.hi[1] && pz >= sa.lo[2] && pz <= sa.hi[2];
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const int64_t px = bc[0] * 8 + x;
+                inbox[x] = yz && px >= sa.lo[0] && px <= sa.hi[0];
+            }
+        }
+        T cf[27];
+#pragma unroll
+        for (int q = 0; q < 27; ++q) cf[q] = std::is_integral<T>::value ? (T)sa.icoef[q] : (T)sa.coef[q];
+        bool act[8];
+        bool whole = true;
 #pragma unroll
         for (int x = 0; x < 8; ++x) {
-            const int64_t px = bc[0] * 8 + x;
-            inbox[x] = yz && px >= sa.lo[0] && px <= sa.hi[0];
+            uint8_t cm = (uint8_t)(mrow >> (8 * x));
+            if (!inbox[x]) cm = 0;
+            act[x] = (sa.apply == SG_APPLY_EXISTING) ? (cm != 0) : ((cm & SG_EXIST) && !(cm & SG_PADDING));
+            const uint8_t raw = (uint8_t)(mrow >> (8 * x));
+            whole = whole && (act[x] || raw == 0);
+            T tot = 0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        tot = dev_add<T>(tot, dev_mul<T>(tw[((z + k) * 10 + (y + j)) * 10 + (x + i)], cf[i * 9 + j * 3 + k]));
+            out[x] = act[x] ? tot : bg_dst;
+        }
+        if (whole) {
+            fl = 1;
+        } else {
+            fl = 2;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) fl |= (uint32_t)act[x] << (8 + x);
         }
     }
-    T cf[27];
+    // the tile is free once every lane has summed its points: stage the rows there and
+    // write the chunk back contiguously
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (fl != 0) {
 #pragma unroll
-    for (int q = 0; q < 27; ++q) cf[q] = std::is_integral<T>::value ? (T)sa.icoef[q] : (T)sa.coef[q];
-    T out[8];
-    bool act[8];
-    bool whole = true;
-#pragma unroll
-    for (int x = 0; x < 8; ++x) {
-        uint8_t cm = (uint8_t)(mrow >> (8 * x));
-        if (!inbox[x]) cm = 0;
-        act[x] = (sa.apply == SG_APPLY_EXISTING) ? (cm != 0) : ((cm & SG_EXIST) && !(cm & SG_PADDING));
-        const uint8_t raw = (uint8_t)(mrow >> (8 * x));
-        whole = whole && (act[x] || raw == 0);
-        T tot = 0;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    tot = dev_add<T>(tot, dev_mul<T>(tw[((z + k) * 10 + (y + j)) * 10 + (x + i)], cf[i * 9 + j * 3 + k]));
-        out[x] = act[x] ? tot : bg_dst;
+        for (int x = 0; x < 8; ++x) tw[lane * 8 + x] = out[x];
     }
-    T *d = dst + c * 512 + lane * 8;
-    if (whole) {
-#pragma unroll
-        for (int x = 0; x < 8; ++x) d[x] = out[x];
-    } else {
-#pragma unroll
-        for (int x = 0; x < 8; ++x)
-            if (act[x]) d[x] = out[x];
-    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    store_chunk_contig(dst + c * 512, tw, fl, lane);
 }
 
 // ---- 3D 8^3 star fast path ------------------------------------------------------
@@ -793,14 +808,6 @@ template <typename T> __device__ inline void store_row_nt(T *p, const Row8<T> &r
             __builtin_nontemporal_store(a, (v2d *)p + i);
         }
     }
-}
-
-// 16 bytes from LDS to global memory, non-temporal (raw bits)
-template <typename T> __device__ inline void copy16_nt(T *dst, const T *lds)
-{
-    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    const v4u a = *(const v4u *)lds;
-    __builtin_nontemporal_store(a, (v4u *)dst);
 }
 
 __device__ inline void wave_lds_sync()
@@ -1078,25 +1085,8 @@ __device__ inline void process_chunk(const ChunkIn<T, KIND> &in, int64_t c, T *t
         store_row(tile0 + lane * 8, r0);
         if constexpr (kTwo) store_row(tile1 + lane * 8, r1);
         wave_lds_sync();
-        constexpr int VPR = (int)sizeof(T) / 2;   // 16-byte vectors per 8-point row
-        constexpr int PPV = 16 / (int)sizeof(T);  // points per vector
-#pragma unroll
-        for (int k = 0; k < VPR; ++k) {
-            const int vi = k * 64 + lane, row = vi / VPR, part = vi % VPR;
-            const uint32_t f = (uint32_t)__shfl((int)fl, row, 64);
-            const int off = row * 8 + part * PPV;
-            if ((f & 3) == 1) {
-                copy16_nt(dst0 + c * 512 + off, tile0 + off);
-                if constexpr (kTwo) copy16_nt(dst1 + c * 512 + off, tile1 + off);
-            } else if (f & 2) {
-#pragma unroll
-                for (int x = 0; x < PPV; ++x)
-                    if ((f >> (8 + part * PPV + x)) & 1u) {
-                        dst0[c * 512 + off + x] = tile0[off + x];
-                        if constexpr (kTwo) dst1[c * 512 + off + x] = tile1[off + x];
-                    }
-            }
-        }
+        store_chunk_contig(dst0 + c * 512, tile0, fl, lane);
+        if constexpr (kTwo) store_chunk_contig(dst1 + c * 512, tile1, fl, lane);
         wave_lds_sync();  // the tiles are rewritten by the next chunk of this wave
         return;
     }
