@@ -362,8 +362,8 @@ struct FoldProps {
     const unsigned char *rval[kMaxProps];  // record values (insert buffer, one array per property)
     unsigned char *out[kMaxProps];         // chunk arrays
     uint64_t bg[kMaxProps];
-    // the records interleaved for the fold: record r at rec + r * stride, property q at
-    // byte off[q], mask byte at moff (one cache line per record for the random gathers)
+    // the records interleaved and in key order for the fold: the i-th record of the sort
+    // at rec + i * stride, property q at byte off[q], mask byte at moff
     const unsigned char *rec;
     int stride, moff;
     int off[kMaxProps];
@@ -389,18 +389,29 @@ __global__ __launch_bounds__(kWG) void interleave_records_kernel(int64_t n, cons
     d[F.moff] = rmask[r];
 }
 
+// records in key order: out[i] = rec[perm[i]], in 16-byte vectors (stride = vecs * 16);
+// one thread per vector, so the random gathers are all in flight together
+__global__ __launch_bounds__(kWG) void permute_records_kernel(const uint4 *__restrict__ rec,
+                                                             const uint32_t *__restrict__ perm, int64_t n, int vecs,
+                                                             uint4 *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * vecs) return;
+    const int64_t r = i / vecs, k = i - r * vecs;
+    out[i] = rec[(int64_t)perm[r] * vecs + k];
+}
+
 // The fold of one unique point, every property (segreduce_total_with_mask + applyOp,
 // BlockMapGpu_kernels.cuh:326-477, BlockMapGpu_dimensionalityWrappers.cuh:214-226): the
 // point's records in insertion order, a = op(a, b) when both exist, a = b when only b
 // exists; then the old value as b; mask = OR of everything.
-template <typename T>
-__device__ inline T fold_records(int op, uint32_t r0, uint32_t r1, const uint32_t *perm, const FoldProps &F, int q)
+template <typename T> __device__ inline T fold_records(int op, uint32_t r0, uint32_t r1, const FoldProps &F, int q)
 {
-    const unsigned char *p = F.rec + (size_t)perm[r0] * F.stride;
+    const unsigned char *p = F.rec + (size_t)r0 * F.stride;
     T a = *(const T *)(p + F.off[q]);
     uint8_t am = p[F.moff];
     for (uint32_t i = r0 + 1; i < r1; ++i) {
-        p = F.rec + (size_t)perm[i] * F.stride;
+        p = F.rec + (size_t)i * F.stride;
         const uint8_t bm = p[F.moff];
         const bool aE = am & 1, bE = bm & 1;
         const T b = *(const T *)(p + F.off[q]);
@@ -437,16 +448,15 @@ template <typename T> __device__ inline void fold_old(int op, T a, uint8_t am, T
 // arrays (the records' mask and order are read once per property, from cache after the
 // first)
 __global__ __launch_bounds__(kWG) void fold_points_kernel(const int64_t *ukey, const uint32_t *rstart,
-                                                         const uint32_t *perm, const uint32_t *uchunk,
-                                                         const int64_t *cpos, int64_t nu, int csz, FoldProps F,
-                                                         uint8_t *omask)
+                                                         const uint32_t *uchunk, const int64_t *cpos, int64_t nu,
+                                                         int csz, FoldProps F, uint8_t *omask)
 {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nu) return;
     const uint32_t r0 = rstart[u], r1 = rstart[u + 1];
     const int64_t slot = cpos[uchunk[u]] * csz + (ukey[u] % csz);
     uint8_t am = 0;
-    for (uint32_t i = r0; i < r1; ++i) am |= F.rec[(size_t)perm[i] * F.stride + F.moff];
+    for (uint32_t i = r0; i < r1; ++i) am |= F.rec[(size_t)i * F.stride + F.moff];
     const uint8_t om = omask[slot];
     for (int q = 0; q < F.np; ++q) {
         switch (F.dt[q]) {
@@ -454,7 +464,7 @@ __global__ __launch_bounds__(kWG) void fold_points_kernel(const int64_t *ukey, c
     case CODE: {                                                                                                \
         T bgv;                                                                                                  \
         __builtin_memcpy(&bgv, &F.bg[q], sizeof(T));                                                            \
-        const T a = fold_records<T>(F.op[q], r0, r1, perm, F, q);                                              \
+        const T a = fold_records<T>(F.op[q], r0, r1, F, q);                                                   \
         fold_old<T>(F.op[q], a, am, (T *)F.out[q] + slot, om, bgv);                                            \
         break;                                                                                                  \
     }
@@ -1206,22 +1216,28 @@ int flush_impl(sg_grid *g, const int32_t *ops)
         F.bg[p] = 0;
         memcpy(&F.bg[p], g->bg[p], g->ts[p]);
     }
-    // interleaved records: properties at their natural alignment, then the mask byte
+    // interleaved records (properties at their natural alignment, then the mask byte),
+    // then put in key order
     {
-        int at = 0, align = 4;
+        int at = 0;
         for (int p = 0; p < g->np; ++p) {
             at = (at + g->ts[p] - 1) / g->ts[p] * g->ts[p];
             F.off[p] = at;
             at += g->ts[p];
-            align = std::max(align, g->ts[p]);
         }
         F.moff = at;
-        F.stride = (at + 1 + align - 1) / align * align;
-        if ((rc = ensure(g->pool, g->scratch[7], (size_t)n * F.stride + 64))) return rc;
-        F.rec = (const unsigned char *)g->scratch[7].p;
+        F.stride = (at + 1 + 15) / 16 * 16;  // whole 16-byte vectors for the permutation
+        if ((rc = ensure(g->pool, g->scratch[7], (size_t)n * F.stride + 64)) ||
+            (rc = ensure(g->pool, g->scratch[13], (size_t)n * F.stride + 64)))
+            return rc;
         hipLaunchKernelGGL(interleave_records_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, n,
                            g->d_ins_mask, F, (unsigned char *)g->scratch[7].p);
         SG_HIP(hipGetLastError());
+        const int vecs = F.stride / 16;
+        hipLaunchKernelGGL(permute_records_kernel, dim3(grid_1d(n * vecs, kWG)), dim3(kWG), 0, g->stream,
+                           (const uint4 *)g->scratch[7].p, perm, n, vecs, (uint4 *)g->scratch[13].p);
+        SG_HIP(hipGetLastError());
+        F.rec = (const unsigned char *)g->scratch[13].p;
     }
     uint32_t *uchunk = rstart + (n + 1);
     hipLaunchKernelGGL(uchunk_index_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, cscan, chead, nu, uchunk);
@@ -1229,8 +1245,8 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     int64_t *cpos = nullptr;
     if ((rc = merge_chunk_set(g, nkey, nnew, &cpos))) return rc;
     for (int p = 0; p < g->np; ++p) F.out[p] = (unsigned char *)g->d_prop[p];
-    hipLaunchKernelGGL(fold_points_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart, perm, uchunk,
-                       cpos, nu, csz, F, g->d_mask);
+    hipLaunchKernelGGL(fold_points_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart, uchunk, cpos,
+                       nu, csz, F, g->d_mask);
     SG_HIP(hipGetLastError());
     g->ins_n = 0;
     // flush marks the neighbour table stale (SparseGridGpu.hpp:778), and the point list

@@ -145,7 +145,7 @@ struct sg_grid {
 
     hipStream_t stream = nullptr;
     sg::DevicePool pool;
-    sg::DevBuf scratch[13];  // flush / sort / scan temporaries
+    sg::DevBuf scratch[14];  // flush / sort / scan temporaries
     sg::DevBuf hostlike;     // small device staging
 
     int nnn(int type) const { return type == SG_NN_STAR ? (1 << dim) : (dim == 2 ? 9 : 27); }
