@@ -211,11 +211,16 @@ __global__ void chunk_head_kernel(const int64_t *ukey, uint32_t *head, int64_t n
     if (u < nu) head[u] = (u == 0 || ukey[u - 1] / csz != ukey[u] / csz) ? 1u : 0u;
 }
 
+// unique chunk ids, and (ustart != nullptr) the first unique point of each, ustart[nn] = nu
 __global__ void unique_chunks_kernel(const int64_t *ukey, const uint32_t *head, const uint32_t *hscan, int64_t nu,
-                                     int csz, int64_t *nkey)
+                                     int csz, int64_t *nkey, uint32_t *ustart, uint32_t nn)
 {
     int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u < nu && head[u]) nkey[hscan[u]] = ukey[u] / csz;
+    if (u < nu && head[u]) {
+        nkey[hscan[u]] = ukey[u] / csz;
+        if (ustart) ustart[hscan[u]] = (uint32_t)u;
+    }
+    if (ustart && u == nu - 1) ustart[nn] = (uint32_t)nu;
 }
 
 __device__ inline int64_t lower_bound_dev(const int64_t *a, int64_t n, int64_t x)
@@ -297,7 +302,9 @@ struct UnionArrays {
     int vshift[kMaxProps + 1];    // log2 of the 16-byte vectors per chunk
 };
 
-__global__ __launch_bounds__(kWG) void build_union_kernel(UnionArrays U, const int64_t *src_of, int64_t nout)
+// skip (optional): output chunks written by someone else (the fused fold)
+__global__ __launch_bounds__(kWG) void build_union_kernel(UnionArrays U, const int64_t *src_of, int64_t nout,
+                                                         const uint8_t *skip)
 {
     const int64_t stride = (int64_t)gridDim.x * kWG;
     // constant indices into the kernel argument (a runtime index would copy it to scratch)
@@ -312,6 +319,7 @@ __global__ __launch_bounds__(kWG) void build_union_kernel(UnionArrays U, const i
                                      (uint32_t)(U.pat[a] >> 32));
         for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < nv; i += stride) {
             const int64_t j = i >> sh;
+            if (skip && j < nout - 1 && skip[j]) continue;
             const int64_t c = (j < nout - 1) ? src_of[j] : -1;
             uint4 v = pat;
             if (c >= 0) v = src[(c << sh) + (i & lane_mask)];
@@ -473,6 +481,133 @@ __global__ __launch_bounds__(kWG) void fold_points_kernel(const int64_t *ukey, c
         }
     }
     omask[slot] = (uint8_t)(om | am);
+}
+
+__global__ void mark_positions_kernel(const int64_t *pos, int64_t n, uint8_t *flag)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[pos[i]] = 1;
+}
+
+// LDS image of one chunk for the fused fold: property q at byte off[q], mask at moff,
+// `bytes` per wave; src_* are the arrays touched old chunks are read from
+struct FusedImg {
+    int off[kMaxProps];
+    int moff, bytes;
+    uint64_t pat[kMaxProps];  // background replicated over 8 bytes
+    const unsigned char *src[kMaxProps];
+    const uint8_t *smask;
+};
+
+// 16-byte copies between global memory and a wave's LDS image (bytes % 16 == 0)
+__device__ inline void image_in(unsigned char *img, const unsigned char *src, int bytes, uint64_t pat, int lane)
+{
+    uint4 *d = (uint4 *)img;
+    const int nv = bytes >> 4;
+    if (src) {
+        const uint4 *s = (const uint4 *)src;
+        for (int i = lane; i < nv; i += 64) d[i] = s[i];
+    } else {
+        const uint4 v = make_uint4((uint32_t)pat, (uint32_t)(pat >> 32), (uint32_t)pat, (uint32_t)(pat >> 32));
+        for (int i = lane; i < nv; i += 64) d[i] = v;
+    }
+}
+
+__device__ inline void image_out(unsigned char *dst, const unsigned char *img, int bytes, int lane)
+{
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    const int nv = bytes >> 4;
+    for (int i = lane; i < nv; i += 64) __builtin_nontemporal_store(((const v4u *)img)[i], (v4u *)dst + i);
+}
+
+// The fold of every unique point (same rules as fold_points_kernel), written to a compact
+// array in key order instead of into the chunks: result u at res + u * F.stride, in the
+// record layout (property q at byte F.off[q], mask at F.moff), so the stores are
+// coalesced.  The old value comes from the chunk's old contents (I.src / I.smask at its
+// old position) or is the background for a brand-new chunk.
+__global__ __launch_bounds__(kWG) void fold_points_compact_kernel(const int64_t *__restrict__ ukey,
+                                                                  const uint32_t *__restrict__ rstart,
+                                                                  const uint32_t *__restrict__ uchunk,
+                                                                  const int64_t *__restrict__ oldpos,
+                                                                  const uint32_t *__restrict__ isnew, int64_t nu,
+                                                                  int csz, FoldProps F, FusedImg I,
+                                                                  unsigned char *__restrict__ res)
+{
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const uint32_t r0 = rstart[u], r1 = rstart[u + 1];
+    const uint32_t j = uchunk[u];
+    const bool fresh = isnew[j] != 0;
+    const int64_t slot = fresh ? 0 : oldpos[j] * csz + (ukey[u] % csz);
+    uint8_t am = 0;
+    for (uint32_t i = r0; i < r1; ++i) am |= F.rec[(size_t)i * F.stride + F.moff];
+    const uint8_t om = fresh ? 0 : I.smask[slot];
+    unsigned char *out = res + (size_t)u * F.stride;
+    for (int q = 0; q < F.np; ++q) {
+        switch (F.dt[q]) {
+#define SG_CASE(CODE, T)                                                                                        \
+    case CODE: {                                                                                                \
+        T bgv;                                                                                                  \
+        __builtin_memcpy(&bgv, &F.bg[q], sizeof(T));                                                            \
+        const T a = fold_records<T>(F.op[q], r0, r1, F, q);                                                   \
+        T v = fresh ? bgv : ((const T *)I.src[q])[slot];                                                        \
+        fold_old<T>(F.op[q], a, am, &v, om, bgv);                                                               \
+        *(T *)(out + F.off[q]) = v;                                                                             \
+        break;                                                                                                  \
+    }
+            SG_FOLD_TYPES(SG_CASE)
+#undef SG_CASE
+        }
+    }
+    out[F.moff] = (uint8_t)(om | am);
+}
+
+// One wave per touched chunk j: the chunk is built in LDS (old contents, or mask 0 /
+// background when new), the folded results of its unique points are placed at their
+// slots, and the whole chunk is written once, contiguously, at its final position.
+__global__ __launch_bounds__(256) void write_chunks_kernel(const int64_t *__restrict__ ukey,
+                                                           const uint32_t *__restrict__ ustart,
+                                                           const int64_t *__restrict__ cpos,
+                                                           const int64_t *__restrict__ oldpos,
+                                                           const uint32_t *__restrict__ isnew, int64_t nnew, int csz,
+                                                           int waves, FoldProps F, FusedImg I,
+                                                           const unsigned char *__restrict__ res,
+                                                           uint8_t *__restrict__ omask)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_img[];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * waves + w;
+    if (w >= waves || j >= nnew) return;  // waves are independent: no workgroup barrier
+    unsigned char *img = smem_img + (size_t)w * I.bytes;
+    const bool fresh = isnew[j] != 0;
+    const int64_t src = fresh ? 0 : oldpos[j];
+    for (int q = 0; q < F.np; ++q)
+        image_in(img + I.off[q], fresh ? nullptr : I.src[q] + src * csz * F.ts[q], csz * F.ts[q], I.pat[q], lane);
+    image_in(img + I.moff, fresh ? nullptr : I.smask + src * csz, csz, 0, lane);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t u1 = ustart[j + 1];
+    for (uint32_t u = ustart[j] + lane; u < u1; u += 64) {
+        const int slot = (int)(ukey[u] % csz);
+        const unsigned char *r = res + (size_t)u * F.stride;
+        for (int q = 0; q < F.np; ++q) {
+            unsigned char *d = img + I.off[q] + slot * F.ts[q];
+            switch (F.ts[q]) {
+            case 8: *(uint64_t *)d = *(const uint64_t *)(r + F.off[q]); break;
+            case 4: *(uint32_t *)d = *(const uint32_t *)(r + F.off[q]); break;
+            case 2: *(uint16_t *)d = *(const uint16_t *)(r + F.off[q]); break;
+            default: *d = r[F.off[q]]; break;
+            }
+        }
+        img[I.moff + slot] = r[F.moff];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int64_t dst = cpos[j];
+    for (int q = 0; q < F.np; ++q) image_out(F.out[q] + dst * csz * F.ts[q], img + I.off[q], csz * F.ts[q], lane);
+    image_out(omask + dst * csz, img + I.moff, csz, lane);
 }
 
 __global__ void uchunk_index_kernel(const uint32_t *hscan, const uint32_t *head, int64_t nu, uint32_t *uchunk)
@@ -871,10 +1006,30 @@ int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t
 
 namespace {
 
-// Merge the sorted unique chunk ids `nkey` [nnew] touched by a flush into the chunk map:
+// Deferred merge: the caller writes every touched chunk whole (the fused fold) from the
+// arrays in src_* (the old arrays when the set grew); those are freed by finish_merge.
+struct MergeDefer {
+    const int64_t *oldpos = nullptr;  // [nnew] old position of each touched chunk
+    const uint32_t *isnew = nullptr;  // [nnew] 1: brand-new chunk
+    const uint8_t *src_mask = nullptr;
+    const void *src_prop[kMaxProps] = {};
+    void *free_keys = nullptr, *free_mask = nullptr, *free_prop[kMaxProps] = {};
+};
+
+void finish_merge(sg_grid *g, MergeDefer &df)
+{
+    // stream order: later allocations reuse these only after the fold has run
+    if (df.free_keys) g->pool.free(df.free_keys);
+    if (df.free_mask) g->pool.free(df.free_mask);
+    for (int p = 0; p < g->np; ++p)
+        if (df.free_prop[p]) g->pool.free(df.free_prop[p]);
+    df = MergeDefer();
+}
+
+
 // brand-new chunks are placed by merge-path rank, the arrays grow (old chunks moved, new
 // ones mask 0 / background), and cpos[j] is the final position of nkey[j].
-int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpos_out)
+int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpos_out, MergeDefer *df)
 {
     const int csz = g->csz;
     const int64_t m = g->nchunks;
@@ -898,6 +1053,12 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
     hipLaunchKernelGGL(chunk_pos_kernel, dim3(grid_1d(nnew, kWG)), dim3(kWG), 0, g->stream, nkey, oldpos, nscan, nnew,
                        g->d_keys, m, bn, nb, cpos);
     SG_HIP(hipGetLastError());
+    if (df) {
+        df->oldpos = oldpos;
+        df->isnew = isnew;
+        df->src_mask = g->d_mask;
+        for (int p = 0; p < g->np; ++p) df->src_prop[p] = g->d_prop[p];
+    }
 
     if (nb > 0) {
         // the chunk set grows: build the union arrays (old chunks moved, new chunks
@@ -914,7 +1075,7 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
             if (!nprop[p]) return fail(SG_E_OOM, "chunk array allocation failed");
         }
         DevBuf &b_o = g->scratch[6];
-        if ((rc = ensure(g->pool, b_o, (size_t)(m + nb) * 8 * 2 + 128))) return rc;
+        if ((rc = ensure(g->pool, b_o, (size_t)(m + nb) * (8 * 2 + 1) + 128))) return rc;
         int64_t *opos = (int64_t *)b_o.p;
         int64_t *bpos = opos + m;
         int64_t *src_of = bpos + nb;
@@ -948,7 +1109,17 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
             U.dst[a] = (uint4 *)(a == 0 ? (void *)nmask : nprop[a - 1]);
         }
         if (flat) {
-            hipLaunchKernelGGL(build_union_kernel, dim3(2048), dim3(kWG), 0, g->stream, U, src_of, total + 1);
+            uint8_t *skip = nullptr;
+            if (df) {
+                // the touched chunks are written whole by the caller's fused fold
+                skip = (uint8_t *)(src_of + (m + nb));
+                hipLaunchKernelGGL(fill_u8_kernel, dim3(grid_1d(total, kWG)), dim3(kWG), 0, g->stream, skip, (uint8_t)0,
+                                   total);
+                hipLaunchKernelGGL(mark_positions_kernel, dim3(grid_1d(nnew, kWG)), dim3(kWG), 0, g->stream, cpos,
+                                   nnew, skip);
+                SG_HIP(hipGetLastError());
+            }
+            hipLaunchKernelGGL(build_union_kernel, dim3(2048), dim3(kWG), 0, g->stream, U, src_of, total + 1, skip);
             SG_HIP(hipGetLastError());
         }
         // chunk data: copy as 16-byte vectors when the chunk size allows
@@ -985,9 +1156,16 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
             });
             SG_HIP(hipGetLastError());
         }
-        g->pool.free(g->d_keys);
-        g->pool.free(g->d_mask);
-        for (int p = 0; p < g->np; ++p) g->pool.free(g->d_prop[p]);
+        if (df) {
+            // the fused fold reads the touched old chunks from the old arrays: freed after it
+            df->free_keys = g->d_keys;
+            df->free_mask = g->d_mask;
+            for (int p = 0; p < g->np; ++p) df->free_prop[p] = g->d_prop[p];
+        } else {
+            g->pool.free(g->d_keys);
+            g->pool.free(g->d_mask);
+            for (int p = 0; p < g->np; ++p) g->pool.free(g->d_prop[p]);
+        }
         g->d_keys = nkeys;
         g->d_mask = nmask;
         for (int p = 0; p < g->np; ++p) g->d_prop[p] = nprop[p];
@@ -1078,7 +1256,7 @@ int flush_chunks_impl(sg_grid *g, const int32_t *ops)
                            (const uint64_t *)sk, head, hscan, n, ukey, rstart, nu32);
     SG_HIP(hipGetLastError());
     int64_t *cpos = nullptr;
-    if ((rc = merge_chunk_set(g, ukey, nu, &cpos))) return rc;
+    if ((rc = merge_chunk_set(g, ukey, nu, &cpos, nullptr))) return rc;
     for (int p = 0; p < g->np; ++p) {
         const int op = ops ? ops[p] : SG_OP_NONE;
         dispatch_type(g->dt[p], [&](auto tag) {
@@ -1199,10 +1377,11 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     const int64_t nnew = nn32;
 
     // unique chunk ids, then their positions in the (grown) chunk map
-    if ((rc = ensure(g->pool, g->scratch[12], (size_t)nnew * 8 + 64))) return rc;
+    if ((rc = ensure(g->pool, g->scratch[12], (size_t)nnew * 8 + (size_t)(nnew + 1) * 4 + 64))) return rc;
     int64_t *nkey = (int64_t *)g->scratch[12].p;
+    uint32_t *ustart = (uint32_t *)(nkey + nnew);
     hipLaunchKernelGGL(unique_chunks_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, chead, cscan, nu,
-                       csz, nkey);
+                       csz, nkey, ustart, nn32);
     SG_HIP(hipGetLastError());
 
     // fold every property and the mask of each unique point in place
@@ -1242,12 +1421,49 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     uint32_t *uchunk = rstart + (n + 1);
     hipLaunchKernelGGL(uchunk_index_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, cscan, chead, nu, uchunk);
     SG_HIP(hipGetLastError());
+    // fold into a compact array, then one wave builds each touched chunk in LDS and writes
+    // it once, when every array's chunk is whole 16-byte vectors and a chunk image fits
+    // 64 KB of LDS
+    FusedImg I;
+    bool fused = csz % 16 == 0;
+    {
+        int at = 0;
+        for (int p = 0; p < g->np; ++p) {
+            fused = fused && ((size_t)csz * g->ts[p]) % 16 == 0;
+            I.off[p] = at;
+            at += csz * g->ts[p];
+            unsigned char pat[8];
+            for (int b = 0; b < 8; b += g->ts[p]) memcpy(pat + b, g->bg[p], g->ts[p]);
+            memcpy(&I.pat[p], pat, 8);
+        }
+        I.moff = at;
+        I.bytes = (at + csz + 15) / 16 * 16;
+        fused = fused && I.bytes <= 65536;
+    }
+    MergeDefer df;
     int64_t *cpos = nullptr;
-    if ((rc = merge_chunk_set(g, nkey, nnew, &cpos))) return rc;
+    if ((rc = merge_chunk_set(g, nkey, nnew, &cpos, fused ? &df : nullptr))) return rc;
     for (int p = 0; p < g->np; ++p) F.out[p] = (unsigned char *)g->d_prop[p];
-    hipLaunchKernelGGL(fold_points_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart, uchunk, cpos,
-                       nu, csz, F, g->d_mask);
-    SG_HIP(hipGetLastError());
+    if (fused) {
+        for (int p = 0; p < g->np; ++p) I.src[p] = (const unsigned char *)df.src_prop[p];
+        I.smask = df.src_mask;
+        // the folded points, compact in key order (the permuted records are no longer needed)
+        if ((rc = ensure(g->pool, g->scratch[7], (size_t)nu * F.stride + 64))) return rc;
+        unsigned char *res = (unsigned char *)g->scratch[7].p;
+        hipLaunchKernelGGL(fold_points_compact_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart,
+                           uchunk, df.oldpos, df.isnew, nu, csz, F, I, res);
+        SG_HIP(hipGetLastError());
+        const int waves = std::max(1, std::min(4, 65536 / I.bytes));
+        hipLaunchKernelGGL(write_chunks_kernel, dim3((unsigned)((nnew + waves - 1) / waves)), dim3(64 * waves),
+                           (size_t)waves * I.bytes, g->stream, ukey, ustart, cpos, df.oldpos, df.isnew, nnew, csz,
+                           waves, F, I, (const unsigned char *)res, g->d_mask);
+        SG_HIP(hipGetLastError());
+        finish_merge(g, df);
+    } else {
+        hipLaunchKernelGGL(fold_points_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart, uchunk,
+                           cpos, nu, csz, F, g->d_mask);
+        SG_HIP(hipGetLastError());
+    }
     g->ins_n = 0;
     // flush marks the neighbour table stale (SparseGridGpu.hpp:778), and the point list
     g->nn_valid[0] = g->nn_valid[1] = false;
