@@ -578,19 +578,29 @@ __global__ __launch_bounds__(256) void write_chunks_kernel(const int64_t *__rest
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t j = (int64_t)blockIdx.x * waves + w;
     if (w >= waves || j >= nnew) return;  // waves are independent: no workgroup barrier
-    unsigned char *img = smem_img + (size_t)w * I.bytes;
+    unsigned char *img = smem_img + (size_t)w * (I.bytes + 1024);
+    unsigned char *stage = img + I.bytes;  // 16-byte results of the lanes' first points
     const bool fresh = isnew[j] != 0;
     const int64_t src = fresh ? 0 : oldpos[j];
+    const uint32_t u0 = ustart[j], u1 = ustart[j + 1];
+    // the lane's first point (most chunks have fewer than 64): its key and, for 16-byte
+    // results, the result are loaded before the image is initialised, so their latency
+    // overlaps it
+    const uint32_t ua = u0 + lane;
+    const bool has = ua < u1, r16 = F.stride == 16;
+    const int64_t keya = has ? ukey[ua] : 0;
+    uint4 ra = make_uint4(0, 0, 0, 0);
+    if (has && r16) ra = *(const uint4 *)(res + (size_t)ua * 16);
     for (int q = 0; q < F.np; ++q)
         image_in(img + I.off[q], fresh ? nullptr : I.src[q] + src * csz * F.ts[q], csz * F.ts[q], I.pat[q], lane);
     image_in(img + I.moff, fresh ? nullptr : I.smask + src * csz, csz, 0, lane);
+    if (r16) *(uint4 *)(stage + lane * 16) = ra;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t u1 = ustart[j + 1];
-    for (uint32_t u = ustart[j] + lane; u < u1; u += 64) {
-        const int slot = (int)(ukey[u] % csz);
-        const unsigned char *r = res + (size_t)u * F.stride;
+    for (uint32_t u = ua; u < u1; u += 64) {
+        const int slot = (int)(((u == ua) ? keya : ukey[u]) % csz);
+        const unsigned char *r = (u == ua && r16) ? stage + lane * 16 : res + (size_t)u * F.stride;
         for (int q = 0; q < F.np; ++q) {
             unsigned char *d = img + I.off[q] + slot * F.ts[q];
             switch (F.ts[q]) {
@@ -1438,7 +1448,7 @@ int flush_impl(sg_grid *g, const int32_t *ops)
         }
         I.moff = at;
         I.bytes = (at + csz + 15) / 16 * 16;
-        fused = fused && I.bytes <= 65536;
+        fused = fused && I.bytes + 1024 <= 65536;
     }
     MergeDefer df;
     int64_t *cpos = nullptr;
@@ -1453,9 +1463,9 @@ int flush_impl(sg_grid *g, const int32_t *ops)
         hipLaunchKernelGGL(fold_points_compact_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart,
                            uchunk, df.oldpos, df.isnew, nu, csz, F, I, res);
         SG_HIP(hipGetLastError());
-        const int waves = std::max(1, std::min(4, 65536 / I.bytes));
+        const int waves = std::max(1, std::min(4, 65536 / (I.bytes + 1024)));
         hipLaunchKernelGGL(write_chunks_kernel, dim3((unsigned)((nnew + waves - 1) / waves)), dim3(64 * waves),
-                           (size_t)waves * I.bytes, g->stream, ukey, ustart, cpos, df.oldpos, df.isnew, nnew, csz,
+                           (size_t)waves * (I.bytes + 1024), g->stream, ukey, ustart, cpos, df.oldpos, df.isnew, nnew, csz,
                            waves, F, I, (const unsigned char *)res, g->d_mask);
         SG_HIP(hipGetLastError());
         finish_merge(g, df);
