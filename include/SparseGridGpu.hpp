@@ -13,7 +13,8 @@
  *   deviceToHost / hostToDevice (BlockMapGpu.hpp:230-241), getIterator() / getIterator(start,
  *   stop) over the host copy (:3751-3764, Iterators/), packRequest / packCalculate / pack /
  *   packFinalize / unpack / remove / removeAddUnpackReset / removeAddUnpackFinalize (:3040-3733),
- *   copy_to / removeCopyToFinalize (:3550, :3314),
+ *   copy_to / removeCopyToFinalize (:3550, :3314), construct_link_dw / construct_link_up and
+ *   getDownLinks / getDownLinksOffsets / getUpLinks / getUpLinksOffsets (:2026-2262),
  *   reduction tags smax_ / smin_ / sadd_ / sRight_ / sLeft_ / sBitwiseOr_
  *   (src/Vector/cuda/map_vector_sparse_cuda_kernels.cuh), NNStar / NNFull, the test
  *   stencils HeatStencil and Conv3x3x3 (src/SparseGridGpu/tests/utils/SparseGridGpu_util_test.cuh).
@@ -809,6 +810,42 @@ public:
         }
         sg::check(sg_copy_to(h_, grid_src.h_, lo, hi, dl));
     }
+    // -- links between levels (construct_link_dw / construct_link_up, SparseGridGpu.hpp:2090-2262) --
+    // padding points of this grid inside db (chunk by chunk, by offset inside a chunk);
+    // down: their existing children 2 x + p_dw + ((j >> k) & 1)_k, j < 2 dim, in grid_dw;
+    // up: their existing parent (x - p_up) / 2 in grid_up
+    struct link_entry {
+        int chunk;        // chunk position in the other grid
+        short int offset; // offset inside the chunk
+    };
+    template <typename PointT>
+    void construct_link_dw(SparseGridGpu &grid_dw, const Box<dim, int> &db, const PointT &p_dw, gpu::ofp_context_t &)
+    {
+        construct_link_(grid_dw, SG_LINK_DOWN, db, p_dw);
+    }
+    template <typename PointT>
+    void construct_link_up(SparseGridGpu &grid_up, const Box<dim, int> &db, const PointT &p_up, gpu::ofp_context_t &)
+    {
+        construct_link_(grid_up, SG_LINK_UP, db, p_up);
+    }
+    // per padding point the first of its links (getDownLinksOffsets / getUpLinksOffsets)
+    std::vector<unsigned int> getDownLinksOffsets() const { return link_offsets_(SG_LINK_DOWN); }
+    std::vector<unsigned int> getUpLinksOffsets() const { return link_offsets_(SG_LINK_UP); }
+    std::vector<link_entry> getDownLinks() const { return links_(SG_LINK_DOWN); }
+    std::vector<link_entry> getUpLinks() const { return links_(SG_LINK_UP); }
+    // the padding points the links start from: (chunk position, offset) in this grid
+    std::vector<link_entry> getLinkPaddingPoints(int which) const
+    {
+        int64_t np = 0, nl = 0;
+        sg::check(sg_link_sizes(h_, which, &np, &nl));
+        std::vector<int32_t> c((size_t)np);
+        std::vector<int16_t> o((size_t)np);
+        sg::check(sg_get_links(h_, which, c.data(), o.data(), nullptr, nullptr, nullptr, SG_MEM_HOST));
+        std::vector<link_entry> out((size_t)np);
+        for (size_t i = 0; i < out.size(); ++i) out[i] = {c[i], o[i]};
+        return out;
+    }
+
     // removeCopyToFinalize (SparseGridGpu.hpp:3314-3329): PHASE1 on every grid taking part,
     // then PHASE2 on every grid, then PHASE3 on every grid
     template <unsigned int... prp> void removeCopyToFinalize(gpu::ofp_context_t &, int opt)
@@ -820,6 +857,36 @@ public:
 private:
     int npack() const { return (int)(pack_lo_.size() / dim); }
     std::vector<int32_t> pack_lo_, pack_hi_;
+
+    template <typename PointT> void construct_link_(SparseGridGpu &other, int which, const Box<dim, int> &db, const PointT &p)
+    {
+        int32_t lo[dim], hi[dim], pp[dim];
+        for (unsigned int d = 0; d < dim; ++d) {
+            lo[d] = db.getLow(d);
+            hi[d] = db.getHigh(d);
+            pp[d] = (int32_t)p.get(d);
+        }
+        sg::check(sg_construct_link(h_, other.h_, which, lo, hi, pp));
+    }
+    std::vector<unsigned int> link_offsets_(int which) const
+    {
+        int64_t np = 0, nl = 0;
+        sg::check(sg_link_sizes(h_, which, &np, &nl));
+        std::vector<uint32_t> sc((size_t)np + 1);
+        sg::check(sg_get_links(h_, which, nullptr, nullptr, sc.data(), nullptr, nullptr, SG_MEM_HOST));
+        return std::vector<unsigned int>(sc.begin(), sc.end() - 1);  // the reference drops the total
+    }
+    std::vector<link_entry> links_(int which) const
+    {
+        int64_t np = 0, nl = 0;
+        sg::check(sg_link_sizes(h_, which, &np, &nl));
+        std::vector<int32_t> c((size_t)nl);
+        std::vector<int16_t> o((size_t)nl);
+        sg::check(sg_get_links(h_, which, nullptr, nullptr, nullptr, c.data(), o.data(), SG_MEM_HOST));
+        std::vector<link_entry> out((size_t)nl);
+        for (size_t i = 0; i < out.size(); ++i) out[i] = {c[i], o[i]};
+        return out;
+    }
 
     template <unsigned int... p> static bool listed(unsigned int q)
     {

@@ -361,6 +361,26 @@ int sg_copy_to(sg_grid *dst, sg_grid *src, const int32_t *src_lo, const int32_t 
 #define SG_COPY_KEEP_GEOMETRY 4
 int sg_remove_copy_to_finalize(sg_grid *g, int phase, int nprop, const int32_t *props);
 
+/* ---- links between levels (construct_link_dw / construct_link_up,
+ *      SparseGridGpu.hpp:2090-2262; kernels SparseGridGpu_kernels.cuh:775-1000) ----
+ * For the padding points (PADDING bit) of g inside the inclusive box [lo, hi] -- chunk by
+ * chunk in chunk-list order, by increasing offset inside a chunk -- the linked points of
+ * `other`: SG_LINK_DOWN, the existing points 2 x + p + ((j >> k) & 1)_k for j < 2 dim
+ * (the reference's loop bound: 6 of the 8 children in 3D); SG_LINK_UP, the existing
+ * point (x - p) / 2 (C division).  A point outside `other`'s domain does not exist.
+ * The result is kept in g per direction until the next call of that direction. */
+#define SG_LINK_DOWN 0
+#define SG_LINK_UP 1
+int sg_construct_link(sg_grid *g, sg_grid *other, int which, const int32_t *lo, const int32_t *hi, const int32_t *p);
+/* number of padding points and of links of the last sg_construct_link of `which` */
+int sg_link_sizes(sg_grid *g, int which, int64_t *n_pad, int64_t *n_links);
+/* getDownLinks / getDownLinksOffsets / getUpLinks / getUpLinksOffsets: padding points
+ * (chunk position, offset in chunk) [n_pad], offsets of each padding point's links
+ * [n_pad + 1] (exclusive scan, last = n_links), links (chunk position, offset in chunk
+ * of `other`) [n_links]; null outputs are skipped */
+int sg_get_links(sg_grid *g, int which, int32_t *pad_chunk, int16_t *pad_off, uint32_t *scan, int32_t *link_chunk,
+                 int16_t *link_off, int mem);
+
 /* ---- device memory (no counterpart in the reference: its vectors keep their
  *      capacity; a server admitting grids against an HBM budget needs both) ---- */
 /* wait for the handle's stream, then free the flush temporaries and cached blocks; the

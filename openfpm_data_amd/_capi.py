@@ -42,6 +42,8 @@ TUNE_STENCIL_STORES = 3
 TUNE_STENCIL_SKIP_EMPTY_ROWS = 4
 TUNE_BATCH_WAVES = 5
 TUNE_BATCH_MAX_WGS = 6
+LINK_DOWN = 0
+LINK_UP = 1
 
 EXIST = 1
 PADDING = 2
@@ -173,6 +175,9 @@ _SIGS = {
     "sg_remove_add_unpack_finalize": (_INT, [_P, _INT, _P]),
     "sg_remove_add_unpack_reset": (_INT, [_P]),
     "sg_copy_to": (_INT, [_P, _P, _P, _P, _P]),
+    "sg_construct_link": (_INT, [_P, _P, _INT, _P, _P, _P]),
+    "sg_link_sizes": (_INT, [_P, _INT, _P, _P]),
+    "sg_get_links": (_INT, [_P, _INT, _P, _P, _P, _P, _P, _INT]),
     "sg_remove_copy_to_finalize": (_INT, [_P, _INT, _INT, _P]),
     "sg_trim": (_INT, [_P]),
     "sg_device_bytes": (_INT, [_P, ctypes.POINTER(_I64)]),

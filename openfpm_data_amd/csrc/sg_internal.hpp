@@ -135,6 +135,19 @@ struct sg_grid {
 
     // boxes queued by sg_remove_box, [k][dim] inclusive corners
     std::vector<int32_t> rem_lo, rem_hi;
+    // links to the finer (SG_LINK_DOWN) and coarser (SG_LINK_UP) level built by
+    // sg_construct_link: padding points of this grid, per padding point the first link
+    // (n_pad + 1 entries), and the linked (chunk position, offset) of the other grid
+    struct link_set {
+        int32_t *pad_chunk = nullptr;
+        int16_t *pad_off = nullptr;
+        uint32_t *scan = nullptr;
+        int32_t *link_chunk = nullptr;
+        int16_t *link_off = nullptr;
+        int64_t n_pad = 0, n_links = 0;
+    };
+    link_set links[2];
+
     // copies queued on this grid as the source by sg_copy_to: the points of [lo, hi]
     // go to `dst` with lo at dst_lo
     struct copy_section {

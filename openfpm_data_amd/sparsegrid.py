@@ -371,6 +371,28 @@ class SparseGridGpu:
     def remove_add_unpack_reset(self):
         C.check(self._lib.sg_remove_add_unpack_reset(self._h))
 
+    def construct_link(self, other, which, box, p):
+        """construct_link_dw (which = 0, `other` finer) / construct_link_up (which = 1,
+        `other` coarser) for the padding points in the inclusive box = (lo, hi)."""
+        lo = np.ascontiguousarray(np.asarray(box[0], np.int32))
+        hi = np.ascontiguousarray(np.asarray(box[1], np.int32))
+        pp = np.ascontiguousarray(np.asarray(p, np.int32))
+        C.check(self._lib.sg_construct_link(self._h, other._h, int(which), lo.ctypes.data, hi.ctypes.data, pp.ctypes.data))
+
+    def links(self, which):
+        """(pad_chunk, pad_off, offsets [n_pad + 1], link_chunk, link_off) of the last
+        construct_link of that direction (getDownLinks / getDownLinksOffsets, ...)."""
+        npad, nl = ctypes.c_int64(), ctypes.c_int64()
+        C.check(self._lib.sg_link_sizes(self._h, int(which), ctypes.byref(npad), ctypes.byref(nl)))
+        pc = np.zeros(npad.value, np.int32)
+        po = np.zeros(npad.value, np.int16)
+        sc = np.zeros(npad.value + 1, np.uint32)
+        lc = np.zeros(nl.value, np.int32)
+        lo = np.zeros(nl.value, np.int16)
+        C.check(self._lib.sg_get_links(self._h, int(which), pc.ctypes.data, po.ctypes.data, sc.ctypes.data,
+                                        lc.ctypes.data, lo.ctypes.data, C.SG_MEM_HOST))
+        return pc, po, sc, lc, lo
+
     def copy_to(self, src, box_src, box_dst):
         """copy_to (SparseGridGpu.hpp:3550-3559): queue, on `src`, a copy of the existing
         points of the inclusive box_src = (lo, hi) of `src` into this grid, the box's low

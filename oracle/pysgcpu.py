@@ -55,6 +55,9 @@ _SIGS = {
     "sgcpu_remove_add_unpack_finalize": (_INT, [_P, _INT, _P]),
     "sgcpu_remove_add_unpack_reset": (_INT, [_P]),
     "sgcpu_copy_to": (_INT, [_P, _P, _P, _P, _P]),
+    "sgcpu_construct_link": (_INT, [_P, _P, _INT, _P, _P, _P]),
+    "sgcpu_link_sizes": (_INT, [_P, _INT, _P, _P]),
+    "sgcpu_get_links": (_INT, [_P, _INT, _P, _P, _P, _P, _P]),
     "sgcpu_remove_copy_to_finalize": (_INT, [_P, _INT, _INT, _P]),
 }
 _lib = None
@@ -326,6 +329,28 @@ class SparseGridCpu:
 
     def remove_add_unpack_reset(self):
         _check(self._lib.sgcpu_remove_add_unpack_reset(self._h))
+
+    def construct_link(self, other, which, box, p):
+        """construct_link_dw (which = 0, `other` finer) / construct_link_up (which = 1,
+        `other` coarser) for the padding points in the inclusive box = (lo, hi)."""
+        lo = np.ascontiguousarray(np.asarray(box[0], np.int32))
+        hi = np.ascontiguousarray(np.asarray(box[1], np.int32))
+        pp = np.ascontiguousarray(np.asarray(p, np.int32))
+        _check(self._lib.sgcpu_construct_link(self._h, other._h, int(which), lo.ctypes.data, hi.ctypes.data, pp.ctypes.data))
+
+    def links(self, which):
+        """(pad_chunk, pad_off, offsets [n_pad + 1], link_chunk, link_off) of the last
+        construct_link of that direction (getDownLinks / getDownLinksOffsets, ...)."""
+        npad, nl = ctypes.c_int64(), ctypes.c_int64()
+        _check(self._lib.sgcpu_link_sizes(self._h, int(which), ctypes.byref(npad), ctypes.byref(nl)))
+        pc = np.zeros(npad.value, np.int32)
+        po = np.zeros(npad.value, np.int16)
+        sc = np.zeros(npad.value + 1, np.uint32)
+        lc = np.zeros(nl.value, np.int32)
+        lo = np.zeros(nl.value, np.int16)
+        _check(self._lib.sgcpu_get_links(self._h, int(which), pc.ctypes.data, po.ctypes.data, sc.ctypes.data,
+                                        lc.ctypes.data, lo.ctypes.data))
+        return pc, po, sc, lc, lo
 
     def copy_to(self, src, box_src, box_dst):
         lo = np.ascontiguousarray(np.asarray(box_src[0], np.int32))

@@ -139,6 +139,14 @@ struct Grid {
 
     // boxes queued by remove(Box) (rem_sects, SparseGridGpu.hpp:3528-3531), [k][dim]
     std::vector<int32_t> rem_lo, rem_hi;
+    // links to the finer (0) and coarser (1) level (link_dw_scan / link_dw, link_up_scan /
+    // link_up, SparseGridGpu.hpp:2090-2262), with the padding points they start from
+    struct Links {
+        std::vector<int32_t> pc, lc;
+        std::vector<int16_t> po, lo;
+        std::vector<uint32_t> scan{0};
+    };
+    Links links[2];
     // copies queued by copy_to on this grid as the source (copySect, SparseGridGpu.hpp:3550-3559)
     struct Section {
         void *dst;
@@ -1425,6 +1433,91 @@ int sgcpu_remove_add_unpack_reset(void *h)
     g.rem_hi.clear();
     g.copies.clear();
     clear_inserts(g);
+    return SG_OK;
+}
+
+// construct_link_dw / construct_link_up (SparseGridGpu.hpp:2090-2262).  count_paddings /
+// collect_paddings (SparseGridGpu_kernels.cuh:775-838): the PADDING points inside the box,
+// chunk by chunk (here by increasing offset inside a chunk).  link_construct_dw_count /
+// link_construct_insert_dw (:846-966): for j < 2 * dim (the reference's bound), the point
+// 2 x + p_dw + ((j >> k) & 1)_k of the finer grid, if its EXIST bit is set.
+// link_construct_up_count / link_construct_insert_up (:886-1000): the point (x - p_up) / 2
+// (C division) of the coarser grid, if it exists.  Points outside the other grid's domain
+// do not exist.
+static bool link_target(const Grid &o, const int64_t *q, int32_t *pos, int16_t *off)
+{
+    for (int d = 0; d < o.dim; ++d)
+        if (q[d] < 0 || q[d] >= o.sz[d]) return false;
+    const int64_t key = o.lin_id(q);
+    const int64_t c = o.find(key / o.csz);
+    if (c >= o.n()) return false;
+    const int l = (int)(key % o.csz);
+    if (!(o.mask[c * o.csz + l] & SG_EXIST)) return false;
+    *pos = (int32_t)c;
+    *off = (int16_t)l;
+    return true;
+}
+
+int sgcpu_construct_link(void *h, void *other, int which, const int32_t *lo, const int32_t *hi, const int32_t *p)
+{
+    Grid &g = *(Grid *)h;
+    const Grid &o = *(const Grid *)other;
+    if (which != SG_LINK_DOWN && which != SG_LINK_UP) return fail(SG_E_ARG, "link direction must be down or up");
+    Grid::Links L;
+    for (int64_t c = 0; c < g.n(); ++c)
+        for (int l = 0; l < g.csz; ++l) {
+            if (!(g.mask[c * g.csz + l] & SG_PADDING)) continue;
+            int64_t x[3] = {0, 0, 0};
+            g.inv_lin(g.keys[c] * g.csz + l, x);
+            bool in = true;
+            for (int d = 0; d < g.dim; ++d) in = in && x[d] >= lo[d] && x[d] <= hi[d];
+            if (!in) continue;
+            L.pc.push_back((int32_t)c);
+            L.po.push_back((int16_t)l);
+            if (which == SG_LINK_DOWN) {
+                for (int j = 0; j < 2 * g.dim; ++j) {
+                    int64_t q[3] = {0, 0, 0};
+                    for (int d = 0; d < g.dim; ++d) q[d] = x[d] * 2 + p[d] + ((j >> d) & 1);
+                    int32_t pos;
+                    int16_t off;
+                    if (link_target(o, q, &pos, &off)) {
+                        L.lc.push_back(pos);
+                        L.lo.push_back(off);
+                    }
+                }
+            } else {
+                int64_t q[3] = {0, 0, 0};
+                for (int d = 0; d < g.dim; ++d) q[d] = (x[d] - p[d]) / 2;
+                int32_t pos;
+                int16_t off;
+                if (link_target(o, q, &pos, &off)) {
+                    L.lc.push_back(pos);
+                    L.lo.push_back(off);
+                }
+            }
+            L.scan.push_back((uint32_t)L.lc.size());  // scan[i + 1]: end of padding point i's links
+        }
+    g.links[which] = L;
+    return SG_OK;
+}
+
+int sgcpu_link_sizes(void *h, int which, int64_t *n_pad, int64_t *n_links)
+{
+    const Grid &g = *(const Grid *)h;
+    *n_pad = (int64_t)g.links[which].pc.size();
+    *n_links = (int64_t)g.links[which].lc.size();
+    return SG_OK;
+}
+
+int sgcpu_get_links(void *h, int which, int32_t *pad_chunk, int16_t *pad_off, uint32_t *scan, int32_t *link_chunk,
+                    int16_t *link_off)
+{
+    const Grid::Links &L = ((const Grid *)h)->links[which];
+    if (pad_chunk) std::copy(L.pc.begin(), L.pc.end(), pad_chunk);
+    if (pad_off) std::copy(L.po.begin(), L.po.end(), pad_off);
+    if (scan) std::copy(L.scan.begin(), L.scan.end(), scan);
+    if (link_chunk) std::copy(L.lc.begin(), L.lc.end(), link_chunk);
+    if (link_off) std::copy(L.lo.begin(), L.lo.end(), link_off);
     return SG_OK;
 }
 

@@ -12,6 +12,8 @@
 //   test_sparse_grid_iterator_host      src/SparseGridGpu/tests/SparseGridGpu_tests.cu:1231-1282
 //   testCopyTo: copy_to + removeCopyToFinalize (SparseGridGpu.hpp:3314-3329, :3550-3559) of
 //   the ball of the sub-iterator test into an empty grid
+//   testLinks: construct_link_dw / construct_link_up (SparseGridGpu.hpp:2090-2262) on a coarse
+//   ball and a fine ball: every link is a child / the parent of its padding point
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -471,6 +473,66 @@ static void testCopyTo()
     REQUIRE(left == a.countExistingElements());
 }
 
+static void testLinks()
+{
+    typedef SparseGridGpu<3, aggregate<float>, 4, 64, long int> SG;
+    const size_t szc[3] = {32, 32, 32}, szf[3] = {64, 64, 64};
+    SG c(szc), f(szf), f2(szf);
+    gpu::ofp_context_t ctx;
+    auto ball = [](SG &g, int n, int r) {
+        std::vector<long> keys;
+        for (int z = 0; z < n; ++z)
+            for (int y = 0; y < n; ++y)
+                for (int x = 0; x < n; ++x)
+                    if ((x - n / 2) * (x - n / 2) + (y - n / 2) * (y - n / 2) + (z - n / 2) * (z - n / 2) < r * r)
+                        keys.push_back(g.getLinId(grid_key_dx<3>({x, y, z})));
+        std::vector<float> v(keys.size(), 1.0f);
+        const void *vals[1] = {v.data()};
+        g.insertPoints(keys.data(), keys.size(), vals);
+        gpu::ofp_context_t cx;
+        g.flush<smax_<0>>(cx, FLUSH_ON_DEVICE);
+    };
+    // the children of the coarse boundary (radius ~12 -> ~24) lie in f (radius 27); the
+    // parents of the boundary of f2 (radius ~20 -> ~10) lie in the coarse ball
+    ball(c, 32, 12);
+    ball(f, 64, 27);
+    ball(f2, 64, 20);
+    c.tagBoundaries(ctx);
+    f2.tagBoundaries(ctx);
+    c.construct_link_dw(f, Box<3, int>({0, 0, 0}, {31, 31, 31}), grid_key_dx<3>({0, 0, 0}), ctx);
+    f2.construct_link_up(c, Box<3, int>({0, 0, 0}, {63, 63, 63}), grid_key_dx<3>({0, 0, 0}), ctx);
+    const auto pad = c.getLinkPaddingPoints(SG_LINK_DOWN);
+    const auto off = c.getDownLinksOffsets();
+    const auto dw = c.getDownLinks();
+    const std::vector<long> cid = c.private_get_index_array(), fid = f.private_get_index_array();
+    REQUIRE(!pad.empty() && off.size() == pad.size() && !dw.empty());
+    bool ok = true;
+    for (size_t i = 0; i < pad.size(); ++i) {
+        const auto x = c.getCoord(cid[pad[i].chunk] * 64 + pad[i].offset);
+        const size_t e = (i + 1 < off.size()) ? off[i + 1] : dw.size();
+        for (size_t k = off[i]; k < e; ++k) {
+            const auto q = f.getCoord(fid[dw[k].chunk] * 64 + dw[k].offset);
+            for (int d = 0; d < 3; ++d) ok = ok && (q.get(d) - 2 * x.get(d) == 0 || q.get(d) - 2 * x.get(d) == 1);
+        }
+    }
+    REQUIRE(ok);
+    const auto padu = f2.getLinkPaddingPoints(SG_LINK_UP);
+    const auto offu = f2.getUpLinksOffsets();
+    const auto up = f2.getUpLinks();
+    const std::vector<long> f2id = f2.private_get_index_array();
+    REQUIRE(!padu.empty() && offu.size() == padu.size() && up.size() == padu.size());
+    for (size_t i = 0; i < padu.size(); ++i) {
+        const auto x = f2.getCoord(f2id[padu[i].chunk] * 64 + padu[i].offset);
+        const size_t e = (i + 1 < offu.size()) ? offu[i + 1] : up.size();
+        ok = ok && e - offu[i] <= 1;
+        for (size_t k = offu[i]; k < e; ++k) {
+            const auto q = c.getCoord(cid[up[k].chunk] * 64 + up[k].offset);
+            for (int d = 0; d < 3; ++d) ok = ok && q.get(d) == x.get(d) / 2;
+        }
+    }
+    REQUIRE(ok);
+}
+
 // testInsert on a Morton grid: the same known answers through SparseGridGpu_z
 static void testInsertZ()
 {
@@ -516,6 +578,7 @@ int main()
     testIteratorSubHost();
     testIteratorHost();
     testCopyTo();
+    testLinks();
     if (failures) {
         std::printf("%d failure(s)\n", failures);
         return 1;

@@ -422,3 +422,65 @@ def model_copy_ring(before, sections, removes, props, bg):
             nv = tuple(v[p] if p in props else (ov[p] if om & 1 else bg[p]) for p in range(len(v)))
             st[d][nk] = (om | 1, nv)
     return st
+
+
+# ---- links between levels (construct_link_dw / construct_link_up) -------------------
+
+def link_levels(G, seed, dim=3, edge=4, lin="smb"):
+    """A coarse grid (random points, tagBoundaries so that it has padding points) and a
+    finer grid of twice its size with random points. Returns (coarse, fine)."""
+    rng = np.random.default_rng(seed)
+    cs = (20, 24, 16)[:dim]
+    fs = tuple(2 * s for s in cs)
+    c = G(dim, cs, [np.float32], block_edge=edge, linearizer=lin)
+    pts = rng.integers(0, np.array(cs), size=(int(0.3 * np.prod(cs)), dim)).astype(np.int32)
+    c.insert_coords(pts, [np.ones(len(pts), np.float32)])
+    c.flush(["smax"])
+    c.tag_boundaries()
+    f = G(dim, fs, [np.float32], block_edge=edge, linearizer=lin)
+    pts = rng.integers(0, np.array(fs), size=(int(0.35 * np.prod(fs)), dim)).astype(np.int32)
+    f.insert_coords(pts, [np.ones(len(pts), np.float32)])
+    f.flush(["smax"])
+    return c, f
+
+
+def model_links(g, other, which, box, p):
+    """construct_link_dw / _up on coordinates: padding points of g in the box, chunk by
+    chunk, by offset; down: the existing points 2 x + p + ((j >> k) & 1)_k, j < 2 dim, of
+    the finer grid; up: the existing point (x - p) / 2 (C division) of the coarser one."""
+    dim = g.dim
+    ids, m = g.chunk_ids(), g.masks()
+    oids, om = other.chunk_ids(), other.masks()
+    opos = {int(k): i for i, k in enumerate(oids)}
+    size = np.array(other.size)
+
+    def target(q):
+        if any(q[d] < 0 or q[d] >= size[d] for d in range(dim)):
+            return None
+        key = int(other.lin_id(np.array([q], np.int32))[0])
+        c = opos.get(key // other.chunk_size)
+        if c is None or not (om[c, key % other.chunk_size] & 1):
+            return None
+        return c, key % other.chunk_size
+
+    pc, po, sc, lc, lo = [], [], [0], [], []
+    for c in range(len(ids)):
+        for o in range(g.chunk_size):
+            if not (m[c, o] & 2):
+                continue
+            x = g.inv_lin_id(np.array([ids[c] * g.chunk_size + o]))[0]
+            if any(x[d] < box[0][d] or x[d] > box[1][d] for d in range(dim)):
+                continue
+            pc.append(c)
+            po.append(o)
+            if which == 0:
+                qs = [[int(x[d]) * 2 + p[d] + ((j >> d) & 1) for d in range(dim)] for j in range(2 * dim)]
+            else:
+                qs = [[int((int(x[d]) - p[d]) / 2) for d in range(dim)]]
+            for q in qs:
+                t = target(q)
+                if t is not None:
+                    lc.append(t[0])
+                    lo.append(t[1])
+            sc.append(len(lc))
+    return pc, po, sc, lc, lo

@@ -118,6 +118,25 @@ def test_copy_to_ring(G, oracle, seed, props):
         b.close()
 
 
+@pytest.mark.parametrize("dim,edge,lin", [(3, 4, "smb"), (3, 8, "zmb"), (2, 4, "smb")])
+def test_links_match_oracle(G, oracle, dim, edge, lin):
+    """construct_link_dw / construct_link_up: padding points, offsets and links equal the
+    oracle's, both directions."""
+    res = []
+    for cls in (G, oracle.SparseGridCpu):
+        c, f = S.link_levels(cls, 11, dim, edge, lin)
+        box = ((0, 1, 0)[:dim], (19, 22, 15)[:dim])
+        c.construct_link(f, 0, box, (1, 0, 1)[:dim])
+        f.tag_boundaries()
+        f.construct_link(c, 1, tuple(tuple(2 * v for v in b) for b in box), (0, 1, 1)[:dim])
+        res.append((c.links(0), f.links(1)))
+        c.close()
+        f.close()
+    for a, b in zip(res[0][0] + res[0][1], res[1][0] + res[1][1]):
+        np.testing.assert_array_equal(a, b)
+    assert len(res[0][0][3]) > 0 and len(res[0][1][3]) > 0
+
+
 def test_unpack_outside_grid_fails(G):
     from openfpm_data_amd._capi import SgError
     g = G(3, (32, 32, 32), [np.float32], block_edge=8)

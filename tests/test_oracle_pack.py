@@ -93,3 +93,27 @@ def test_copy_to_ring_matches_point_model(oracle, seed, props):
     for g, m in zip(grids, model):
         assert S.point_state(g) == m
         g.close()
+
+
+@pytest.mark.parametrize("dim,which,p", [(3, 0, (1, 0, 1)), (3, 0, (0, 0, 0)), (2, 0, (1, 1)), (3, 1, (1, 0, 0)),
+                                         (2, 1, (0, 1))])
+def test_links_match_coordinate_model(oracle, dim, which, p):
+    """construct_link_dw / construct_link_up (SparseGridGpu.hpp:2090-2262) against the
+    model on coordinates: padding points in the box, their children / parent that exist."""
+    c, f = S.link_levels(oracle.SparseGridCpu, 5 + dim + which, dim)
+    box = ((1, 2, 1)[:dim], (17, 20, 14)[:dim])
+    if which == 0:
+        c.construct_link(f, 0, box, p)
+        got = c.links(0)
+        want = S.model_links(c, f, 0, box, p)
+    else:
+        fbox = tuple(tuple(2 * v for v in b) for b in box)
+        f.tag_boundaries()
+        f.construct_link(c, 1, fbox, p)
+        got = f.links(1)
+        want = S.model_links(f, c, 1, fbox, p)
+    assert len(want[0]) > 20 and len(want[3]) > 10
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, np.asarray(b, a.dtype))
+    c.close()
+    f.close()
