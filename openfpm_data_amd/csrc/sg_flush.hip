@@ -1301,7 +1301,10 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     const int64_t limit = key_limit(g);
     int rc;
 
-    // validate device-side keys
+    // validate device-side keys: the flag comes back with the first scan total below (the
+    // grid is not touched before it is read; out-of-range keys only reorder scratch data)
+    uint32_t *hw = host_words(g);
+    if (!hw) return fail(SG_E_OOM, "pinned host allocation failed");
     {
         rc = ensure(g->pool, g->scratch[6], 64);
         if (rc) return rc;
@@ -1310,13 +1313,7 @@ int flush_impl(sg_grid *g, const int32_t *ops)
         hipLaunchKernelGGL(check_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key, n, limit,
                            bad);
         SG_HIP(hipGetLastError());
-        int hbad = 0;
-        SG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
-        SG_HIP(hipStreamSynchronize(g->stream));
-        if (hbad) {
-            g->ins_n = 0;
-            return fail(SG_E_RANGE, "point key outside the grid in the insert buffer");
-        }
+        SG_HIP(hipMemcpyAsync(hw + 8, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
     }
     int bits = 0;
     while (bits < 63 && (((int64_t)1 << bits) < limit)) ++bits;
@@ -1366,6 +1363,10 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     SG_HIP(hipGetLastError());
     uint32_t nu32 = 0;
     if ((rc = exclusive_scan_u32(g, head, hscan, n, &nu32))) return rc;
+    if (hw[8]) {  // the key check above, complete now
+        g->ins_n = 0;
+        return fail(SG_E_RANGE, "point key outside the grid in the insert buffer");
+    }
     const int64_t nu = nu32;
     int64_t *ukey = (int64_t *)((char *)b_u.p + (((size_t)n * 4 + 15) & ~(size_t)15));
     uint32_t *rstart = (uint32_t *)(ukey + n);

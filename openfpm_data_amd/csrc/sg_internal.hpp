@@ -160,6 +160,13 @@ struct sg_grid {
     sg::DevicePool pool;
     sg::DevBuf scratch[14];  // flush / sort / scan temporaries
     sg::DevBuf hostlike;     // small device staging
+    // pinned host words for device -> host copies of flags and totals: queued without
+    // blocking the host, read after the next stream synchronisation
+    uint32_t *h_words = nullptr;
+    ~sg_grid()
+    {
+        if (h_words) (void)hipHostFree(h_words);
+    }
 
     int nnn(int type) const { return type == SG_NN_STAR ? (1 << dim) : (dim == 2 ? 9 : 27); }
 };
@@ -278,6 +285,8 @@ int count_mask_impl(sg_grid *g, int which, int64_t *out);
 // device-wide primitives (sg_primitives.hip)
 // exclusive scan of n uint32 values; *total (host) receives the sum when non-null
 int exclusive_scan_u32(sg_grid *g, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total);
+// pinned host words of g (16 of them), allocated on first use; nullptr on failure
+uint32_t *host_words(sg_grid *g);
 // stable LSD radix sort of (key, value) pairs on the low `bits` bits of the key
 int radix_sort_pairs_u64(sg_grid *g, uint64_t *keys, uint32_t *vals, uint64_t *keys_tmp,
                          uint32_t *vals_tmp, int64_t n, int bits, uint64_t **keys_out,

@@ -322,11 +322,22 @@ int exclusive_scan_u32(sg_grid *g, const uint32_t *in, uint32_t *out, int64_t n,
     down(in, out, bo0, n, total ? d_total : (uint32_t *)nullptr);
     SG_HIP(hipGetLastError());
     if (total) {
-        // the inclusive total is written by the thread owning item n-1 (works in place)
-        SG_HIP(hipMemcpyAsync(total, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+        // the inclusive total is written by the thread owning item n-1 (works in place);
+        // through a pinned word, so that copies queued before it need no wait of their own
+        uint32_t *hw = host_words(g);
+        if (!hw) return fail(SG_E_OOM, "pinned host allocation failed");
+        SG_HIP(hipMemcpyAsync(hw, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
         SG_HIP(hipStreamSynchronize(g->stream));
+        *total = hw[0];
     }
     return SG_OK;
+}
+
+uint32_t *host_words(sg_grid *g)
+{
+    if (!g->h_words && hipHostMalloc((void **)&g->h_words, 16 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+        g->h_words = nullptr;
+    return g->h_words;
 }
 
 int radix_sort_pairs_u64(sg_grid *g, uint64_t *keys, uint32_t *vals, uint64_t *keys_tmp, uint32_t *vals_tmp,

@@ -471,3 +471,23 @@ def test_tag_and_box_8cubed_random(G, oracle, seed):
             else:
                 g.apply_stencil(C.make_desc(C.ST_BOX, [0], [1], apply, box, [float(v) for v in coef]))
         S.assert_same_state(a, b, exist_only=False)
+
+
+def test_flush_rejects_device_keys_outside_grid(G):
+    """Point keys handed over in device memory are checked by the flush: a key beyond the
+    grid fails it (SgError), drops the pending records and leaves the grid unchanged."""
+    import torch
+    from openfpm_data_amd._capi import SgError
+    g = G(3, (32, 32, 32), [np.float32], block_edge=8)
+    g.insert_coords([[1, 2, 3]], [np.array([1.0], np.float32)])
+    g.flush(["smax"])
+    before = (g.chunk_ids().copy(), g.masks().copy(), g.prop(0).copy())
+    keys = torch.tensor([5, 32 * 32 * 32 + 7, 9], dtype=torch.int64, device="cuda")
+    vals = torch.ones(3, dtype=torch.float32, device="cuda")
+    g.insert(keys, [vals])
+    with pytest.raises(SgError):
+        g.flush(["smax"])
+    assert g.pending_inserts() == 0
+    for a, b in zip(before, (g.chunk_ids(), g.masks(), g.prop(0))):
+        np.testing.assert_array_equal(a, b)
+    g.close()
