@@ -293,6 +293,36 @@ def test_fast_stencil_variants(G, oracle, variant, stores, lskip, dtype):
     S.assert_same_state(gs[0], gs[1])
 
 
+def test_grouped_stencil_variants_agree_on_a_large_shell(G, oracle):
+    """Groups of 2 / 4 / 8 chunks with every store mode, on a shell of several thousand
+    chunks (many workgroups in flight, so that a missing wait between the waves of a group
+    shows), give bitwise the same Gray-Scott steps as each other and as the oracle."""
+    rng = np.random.default_rng(73)
+    pts = W.shell_points(176, 60.0, 84.0)
+    u0 = rng.uniform(0.3, 1.0, len(pts)).astype(np.float32)
+    v0 = rng.uniform(0.0, 0.4, len(pts)).astype(np.float32)
+
+    def run(cls, variant=None, stores=None):
+        g = cls(3, (176, 176, 176), [np.float32] * 4, block_edge=8)
+        if variant is not None:
+            g.set_tuning(C.TUNE_STENCIL_VARIANT, variant)
+            g.set_tuning(C.TUNE_STENCIL_STORES, stores)
+        g.insert_coords(pts, [u0, v0, np.zeros_like(u0), np.zeros_like(u0)])
+        g.flush(["smax"] * 4)
+        for _ in range(3):
+            g.conv2_gray_scott(0, 1, 2, 3, **W.GS_PARAMS)
+            g.conv2_gray_scott(2, 3, 0, 1, **W.GS_PARAMS)
+        return g
+
+    ref = run(oracle.SparseGridCpu)
+    for variant in (3, 4, 5):
+        for stores in (1, 2, 3):
+            g = run(G, variant, stores)
+            S.assert_same_state(g, ref)
+            g.close()
+    ref.close()
+
+
 def test_blobs1024_full_flush_tag_box(G, oracle):
     """The full insertion workload: 2^24 inserts into 1024^3 from 64 Gaussian blobs (30%
     repeats), int32 smax + uint32 sadd, NNFull tagBoundaries, 27-point uint32 box sum:
