@@ -85,6 +85,31 @@ __device__ inline int64_t find_chunk(const int64_t *keys, int64_t n, int64_t key
 // BlockLinId (grid_smb.hpp:258-276) returns -1 for a coordinate >= blockSz and does not
 // check for -1: NNStar guards dPos < 0 itself, NNFull does not, so at the low x / y
 // faces its linear id aliases another chunk; this table reproduces both.
+// Position of chunk id `key` near chunk c (id kc), or n: in a sorted list of distinct ids
+// the chunk with id kc + d is at most d positions after c (before c for d < 0), so the
+// binary search runs in that window only (a few steps for the x and y neighbours).
+__device__ inline int64_t find_chunk_near(const int64_t *keys, int64_t n, int64_t c, int64_t kc, int64_t key)
+{
+    int64_t lo, hi;
+    if (key > kc) {
+        const uint64_t d = (uint64_t)key - (uint64_t)kc;
+        lo = c + 1;
+        hi = (d >= (uint64_t)(n - c)) ? n : c + 1 + (int64_t)d;
+    } else if (key < kc) {
+        const uint64_t d = (uint64_t)kc - (uint64_t)key;
+        hi = c;
+        lo = (d >= (uint64_t)c) ? 0 : c - (int64_t)d;
+    } else {
+        return c;
+    }
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && keys[lo] == key) ? lo : n;
+}
+
 __global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs ga, int type, int nnn, int32_t *nn)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,7 +143,7 @@ __global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs 
             bc[d] = dPos;
             int64_t bl = block_lin_id(bc);
             bl = (dPos < 0) ? -1 : bl;
-            pos = find_chunk(keys, n, bl);
+            pos = find_chunk_near(keys, n, c, keys[c], bl);
         }
     } else {
         pos = n;
@@ -128,7 +153,7 @@ __global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs 
                 bc[d] += cnt % 3 - 1;
                 cnt /= 3;
             }
-            pos = find_chunk(keys, n, block_lin_id(bc));
+            pos = find_chunk_near(keys, n, c, keys[c], block_lin_id(bc));
         }
     }
     nn[t] = (int32_t)pos;
@@ -1288,6 +1313,13 @@ template <typename T> __global__ void normalize_kernel(T *p, const uint8_t *mask
     if (i < n && mask[i] == 0) p[i] = v;
 }
 
+// countExistingElements (which 0: EXIST bits) / countBoundaryElements (which 1: EXIST and
+// PADDING), SparseGridGpu.hpp:2422-2470: 16 mask bytes per load, four bytes per popcount
+__device__ inline uint32_t count_word(uint32_t w, int which)
+{
+    return (uint32_t)__popc((which == 0 ? w : (w & (w >> 1))) & 0x01010101u);
+}
+
 __global__ __launch_bounds__(kWG) void count_mask_kernel(const uint8_t *mask, int64_t n, int which,
                                                         unsigned long long *out)
 {
@@ -1295,13 +1327,21 @@ __global__ __launch_bounds__(kWG) void count_mask_kernel(const uint8_t *mask, in
     if (threadIdx.x == 0) s = 0;
     __syncthreads();
     unsigned long long c = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t nv = n >> 4, stride = (int64_t)gridDim.x * blockDim.x;
+    const uint4 *v = (const uint4 *)mask;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+        const uint4 q = v[i];
+        c += count_word(q.x, which) + count_word(q.y, which) + count_word(q.z, which) + count_word(q.w, which);
+    }
+    for (int64_t i = (nv << 4) + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint8_t m = mask[i];
         c += (which == 0) ? (m & 1) : ((m & 3) == 3);
     }
-    atomicAdd(&s, c);
+    // one shared atomic per wave
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s, c);
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(out, s);
+    if (threadIdx.x == 0 && s) atomicAdd(out, s);
 }
 
 template <typename F> void dispatch_type(int dt, F &&f)
@@ -1689,7 +1729,7 @@ int count_mask_impl(sg_grid *g, int which, int64_t *out)
     SG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), g->stream));
     const int64_t total = g->nchunks * g->csz;
     if (total > 0) {
-        unsigned blocks = (unsigned)std::min<int64_t>(grid_1d(total, kWG), 4096);
+        unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_1d(total / 16, kWG), 2048));
         hipLaunchKernelGGL(count_mask_kernel, dim3(blocks), dim3(kWG), 0, g->stream, g->d_mask, total, which, d);
         SG_HIP(hipGetLastError());
     }
