@@ -233,13 +233,21 @@ __host__ __device__ inline uint64_t zcompact(uint64_t v, int dim)
 }
 
 // chunk coordinates of chunk id `id`
+// (loops run to the constant 3 and stop at dim, so that callers' coordinate arrays stay
+// in registers)
 __host__ __device__ inline void chunk_coords(const ChunkLin &L, int64_t id, int64_t *bc)
 {
     if (L.zorder) {
-        for (int d = 0; d < L.dim; ++d) bc[d] = (int64_t)zcompact((uint64_t)id >> d, L.dim);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d >= L.dim) break;
+            bc[d] = (int64_t)zcompact((uint64_t)id >> d, L.dim);
+        }
         return;
     }
-    for (int d = 0; d < L.dim; ++d) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= L.dim) break;
         bc[d] = id % L.bsz[d];
         id /= L.bsz[d];
     }
@@ -250,11 +258,19 @@ __host__ __device__ inline int64_t chunk_id(const ChunkLin &L, const int64_t *bc
 {
     if (L.zorder) {
         uint64_t r = 0;
-        for (int d = 0; d < L.dim; ++d) r |= zspread((uint64_t)bc[d], L.dim) << d;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d >= L.dim) break;
+            r |= zspread((uint64_t)bc[d], L.dim) << d;
+        }
         return (int64_t)r;
     }
-    int64_t v = bc[L.dim - 1];
-    for (int d = L.dim - 2; d >= 0; --d) v = v * L.bsz[d] + bc[d];
+    int64_t v = 0;
+#pragma unroll
+    for (int d = 2; d >= 0; --d) {
+        if (d >= L.dim) continue;
+        v = v * L.bsz[d] + bc[d];
+    }
     return v;
 }
 

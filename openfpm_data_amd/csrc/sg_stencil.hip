@@ -240,12 +240,16 @@ __global__ __launch_bounds__(kWG) void stencil_generic_kernel(const int64_t *key
     __syncthreads();
     int64_t base[3] = {0, 0, 0};
     chunk_coords(lin_of(ga), keys[c], base);
-    for (int d = 0; d < dim; ++d) base[d] *= edge;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) base[d] *= edge;
     const int kind = sa.kind;
     for (int o = threadIdx.x; o < csz; o += blockDim.x) {
         int t = o, lin = 0, acc = 1;
         int64_t pc[3] = {0, 0, 0}, par = 0;
-        for (int d = 0; d < dim; ++d) {
+        // loops to the constant 3, stopping at dim: the small arrays stay in registers
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d >= dim) break;
             const int lc = t % edge;
             pc[d] = base[d] + lc;
             par += pc[d];
@@ -254,9 +258,11 @@ __global__ __launch_bounds__(kWG) void stencil_generic_kernel(const int64_t *key
             acc *= e;
         }
         uint8_t cm = mask[c * csz + o];
-        if (sa.use_box)
-            for (int d = 0; d < dim; ++d)
-                if (pc[d] < sa.lo[d] || pc[d] > sa.hi[d]) cm = 0;
+        if (sa.use_box) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (d < dim && (pc[d] < sa.lo[d] || pc[d] > sa.hi[d])) cm = 0;
+        }
         bool active = (sa.apply == SG_APPLY_EXISTING) ? (cm != 0) : ((cm & SG_EXIST) && !(cm & SG_PADDING));
         const T cur = eb0[lin];
         T r0 = alias0 ? eb0[lin] : (T)0, r1 = alias1 ? eb1[lin] : (T)0;
@@ -1646,7 +1652,9 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
         memcpy(&bg1, g->bg[nsrc > 1 ? d->src[1] : d->src[0]], sizeof(T));
         const int e = g->edge + 2;
         const size_t ebs = (g->dim == 3) ? (size_t)e * e * e : (size_t)e * e;
-        hipLaunchKernelGGL(stencil_generic_kernel<T>, dim3((unsigned)n), dim3(kWG), ebs * sizeof(T) * nsrc, g->stream,
+        // one thread per slot up to kWG (small chunks leave no idle waves)
+        const int wg = std::max(64, std::min(kWG, (g->csz + 63) / 64 * 64));
+        hipLaunchKernelGGL(stencil_generic_kernel<T>, dim3((unsigned)n), dim3(wg), ebs * sizeof(T) * nsrc, g->stream,
                            g->d_keys, n, geom(g), g->d_nn[type], g->nnn(type), type, g->d_mask, s0, s1, o0, o1,
                            alias[0], alias[1], bg0, bg1, sa);
     });
