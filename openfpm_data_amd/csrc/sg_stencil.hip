@@ -1423,7 +1423,8 @@ int tag_boundaries_impl(sg_grid *g, int type)
     }
     const int e = g->edge + 2;
     const size_t ebs = (g->dim == 3) ? (size_t)e * e * e : (size_t)e * e;
-    hipLaunchKernelGGL(tag_boundaries_kernel, dim3((unsigned)n), dim3(kWG), ebs, g->stream, n, geom(g),
+    const int wg = std::max(64, std::min(kWG, (g->csz + 63) / 64 * 64));  // one thread per slot up to kWG
+    hipLaunchKernelGGL(tag_boundaries_kernel, dim3((unsigned)n), dim3(wg), ebs, g->stream, n, geom(g),
                        g->d_nn[type], g->nnn(type), type, g->d_mask);
     SG_HIP(hipGetLastError());
     return SG_OK;
