@@ -521,3 +521,30 @@ def test_flush_rejects_device_keys_outside_grid(G):
     for a, b in zip(before, (g.chunk_ids(), g.masks(), g.prop(0))):
         np.testing.assert_array_equal(a, b)
     g.close()
+
+
+def test_operations_on_an_empty_grid(G, oracle):
+    """Every operation on a grid without chunks: nothing happens, nothing fails, and the
+    results equal the oracle's (counts 0, empty packs, empty links, no copies)."""
+    for cls in (G, oracle.SparseGridCpu):
+        g = cls(3, (32, 32, 32), [np.float32, np.float32, np.float32, np.float32], block_edge=8)
+        h = cls(3, (64, 64, 64), [np.float32, np.float32, np.float32, np.float32], block_edge=8)
+        g.flush(["smax"] * 4)
+        g.find_neighbours()
+        g.tag_boundaries()
+        g.conv2_gray_scott(0, 1, 2, 3, **W.GS_PARAMS)
+        g.conv_lap(0, 1)
+        assert g.num_chunks() == 0 and g.count_existing_elements() == 0
+        assert g.pack_size([((0, 0, 0), (31, 31, 31))], [0]) == g.pack([((0, 0, 0), (31, 31, 31))], [0]).size
+        g.remove_box((0, 0, 0), (31, 31, 31))
+        g.remove_add_unpack_finalize([0])
+        g.construct_link(h, 0, ((0, 0, 0), (31, 31, 31)), (0, 0, 0))
+        pc, po, sc, lc, lo = g.links(0)
+        assert len(pc) == 0 and len(lc) == 0 and list(sc) == [0]
+        h.copy_to(g, ((0, 0, 0), (31, 31, 31)), ((0, 0, 0), (31, 31, 31)))
+        for phase in range(3):
+            g.remove_copy_to_finalize([0], phase)
+            h.remove_copy_to_finalize([0], phase)
+        assert h.num_chunks() == 0
+        g.close()
+        h.close()
