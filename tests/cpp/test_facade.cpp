@@ -126,12 +126,13 @@ static void testStencilHeat()
     REQUIRE(match);
 }
 
-static void testConv3x3x3()
+// run on SparseGridGpu and on SparseGridGpu_z, as the reference does (:1170-1177)
+template <typename SG> static void testConv3x3x3()
 {
     const size_t sz[3] = {64, 64, 64};
-    SparseGridGpu<3, aggregate<float, float>> sg(sz);
+    SG sg(sz);
     gpu::ofp_context_t ctx;
-    sg.setBackgroundValue<0>(0);
+    sg.template setBackgroundValue<0>(0);
     std::vector<long> keys;
     std::vector<float> one, zero;
     for (int z = 0; z < 64; ++z)
@@ -146,17 +147,17 @@ static void testConv3x3x3()
             }
     const void *v[] = {one.data(), zero.data()};
     sg.insertPoints(keys.data(), keys.size(), v);
-    sg.flush<smax_<0>, smax_<1>>(ctx, FLUSH_ON_DEVICE);
-    sg.findNeighbours<NNFull<3>>();
-    sg.setNNType<NNFull<3>>();
-    sg.tagBoundaries<NNFull<3>>(ctx);
+    sg.template flush<smax_<0>, smax_<1>>(ctx, FLUSH_ON_DEVICE);
+    sg.template findNeighbours<NNFull<3>>();
+    sg.template setNNType<NNFull<3>>();
+    sg.template tagBoundaries<NNFull<3>>(ctx);
     conv_coeff cc;
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             for (int k = 0; k < 3; ++k) cc.coeff[k][j][i] = 1.0;
-    sg.applyStencils<Conv3x3x3<3, 0, 1>>(sg.getBox(), STENCIL_MODE_INPLACE, cc);
+    sg.template applyStencils<Conv3x3x3<3, 0, 1>>(sg.getBox(), STENCIL_MODE_INPLACE, cc);
     std::vector<uint8_t> m = sg.private_get_mask_array();
-    std::vector<float> out = sg.private_get_data_array<1>();
+    std::vector<float> out = sg.template private_get_data_array<1>();
     bool match = true;
     size_t n_interior = 0;
     for (size_t i = 0; i < m.size(); ++i)
@@ -570,7 +571,8 @@ int main()
     testInsert();
     testTagBoundaries();
     testStencilHeat();
-    testConv3x3x3();
+    testConv3x3x3<SparseGridGpu<3, aggregate<float, float>>>();
+    testConv3x3x3<SparseGridGpu_z<3, aggregate<float, float>>>();
     testPackRequestWithIterator();
     testRemove();
     testPackUnpack();
