@@ -268,7 +268,8 @@ int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, s
 #define SG_TUNE_STENCIL_WG_PER_XCD 1
 /* 8^3 star-stencil kernel variant: 0 one chunk per 64-lane workgroup, 1 persistent
  * software-pipelined waves, 2 as 1 with registers capped for 4 waves per SIMD,
- * 3 / 4 / 5 workgroups of 4 / 8 / 2 consecutive chunks sharing x faces in LDS */
+ * 3 / 4 / 5 workgroups of 4 / 8 / 2 consecutive chunks sharing x faces in LDS,
+ * 6 / 7 / 8 as 3 / 4 / 5 with one LDS tile per wave for both Gray-Scott properties */
 #define SG_TUNE_STENCIL_VARIANT 2
 /* 8^3 star-stencil stores: 0 active points only, 1 whole 8-point rows where every point
  * is active or absent (absent points keep the background), 2 as 1, non-temporal, 3 as 2

@@ -4,7 +4,8 @@ JSON.
 What is tuned
 * 8^3 star stencils (Laplacian, heat, Gray-Scott): the kernel variant (one chunk per
   workgroup; persistent waves with a workgroups-per-XCD count; groups of 2 / 4 / 8 chunks
-  per workgroup sharing x faces through LDS, whose LDS footprint grows with the group) and
+  per workgroup sharing x faces through LDS, whose LDS footprint grows with the group,
+  with one LDS tile per property or one tile shared by the two Gray-Scott properties) and
   the store mode (whole rows, plain or non-temporal) -- ``tune_stencil``;
 * batched red-black sweeps: waves (chunk tiles) per workgroup, i.e. the LDS tile footprint
   per workgroup, and a workgroup cap that makes waves loop over tiles -- ``tune_batch``;
@@ -38,6 +39,9 @@ STENCIL_CANDIDATES = [
     {"variant": 3, "stores": 2},
     {"variant": 4, "stores": 2},
     {"variant": 4, "stores": 1},
+    {"variant": 6, "stores": 3},
+    {"variant": 7, "stores": 3},
+    {"variant": 8, "stores": 3},
 ]
 
 BATCH_CANDIDATES = [

@@ -343,7 +343,7 @@ int sg_set_tuning(sg_grid *g, int key, int64_t value)
         return SG_OK;
     }
     if (key == SG_TUNE_STENCIL_VARIANT) {
-        if (value < 0 || value > 5) return fail(SG_E_RANGE, "stencil variant must be in [0, 5]");
+        if (value < 0 || value > 8) return fail(SG_E_RANGE, "stencil variant must be in [0, 8]");
         g->fast_variant = (int)value;
         return SG_OK;
     }

@@ -997,14 +997,17 @@ __device__ inline void issue_chunk(ChunkIn<T, KIND> &in, const int32_t *__restri
     if constexpr (ChunkIn<T, KIND>::kTwo) issue_ghosts<T>(in.v, src1, nbp, lane, true, true);
 }
 
-template <typename T, int KIND>
+// ONE: the two Gray-Scott properties pass through one LDS tile (tile0) one after the
+// other (half the LDS per wave, so more workgroups fit a CU); the u Laplacian is reduced
+// to one value per point before v is staged.
+template <typename T, int KIND, bool ONE = false>
 __device__ inline void process_chunk(const ChunkIn<T, KIND> &in, int64_t c, T *tile0, T *tile1, T *__restrict__ dst0,
                                      T *__restrict__ dst1, int alias, const StencilArgs &sa,
                                      const int64_t *__restrict__ keys, int64_t bsz0, int64_t bsz1, int lane)
 {
     constexpr bool kTwo = ChunkIn<T, KIND>::kTwo;
     stage_lds<T>(in.u, tile0, lane);
-    if constexpr (kTwo) stage_lds<T>(in.v, tile1, lane);
+    if constexpr (kTwo && !ONE) stage_lds<T>(in.v, tile1, lane);
     wave_lds_sync();
 
     bool act[8];
@@ -1061,6 +1064,34 @@ __device__ inline void process_chunk(const ChunkIn<T, KIND> &in, int64_t c, T *t
             a = a + pz[x];
             r0.v[x] = in.u.own.v[x] + dt * a;
         }
+    } else if constexpr (ONE) {
+        const T Du = (T)sa.coef[0], Dv = (T)sa.coef[1], F = (T)sa.coef[2], k = (T)sa.coef[3], dt = (T)sa.coef[4];
+        T lu[8], lv[8];
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            T a = (T)(-6.0) * in.u.own.v[x];
+            a = a + px[x];
+            a = a + py[x];
+            lu[x] = a + pz[x];
+        }
+        wave_lds_sync();  // the u rows are read: the tile takes v
+        stage_lds<T>(in.v, tile0, lane);
+        wave_lds_sync();
+        lap_terms<T>(in.v, tile0, lane, px, py, pz);
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            T a = (T)(-6.0) * in.v.own.v[x];
+            a = a + px[x];
+            a = a + py[x];
+            lv[x] = a + pz[x];
+        }
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            const T cu = in.u.own.v[x], cv = in.v.own.v[x];
+            const T uvv = (cu * cv) * cv;
+            r0.v[x] = cu + dt * ((Du * lu[x] - uvv) + F * ((T)1 - cu));
+            r1.v[x] = cv + dt * ((Dv * lv[x] + uvv) - (F + k) * cv);
+        }
     } else {
         T qx[8], qy[8], qz[8];
         lap_terms<T>(in.v, tile1, lane, qx, qy, qz);
@@ -1112,6 +1143,17 @@ __device__ inline void process_chunk(const ChunkIn<T, KIND> &in, int64_t c, T *t
 #pragma unroll
                 for (int x = 0; x < 8; ++x) fl |= (uint32_t)act[x] << (8 + x);
             }
+        }
+        if constexpr (kTwo && ONE) {
+            store_row(tile0 + lane * 8, r0);
+            wave_lds_sync();
+            store_chunk_contig(dst0 + c * 512, tile0, fl, lane);
+            wave_lds_sync();
+            store_row(tile0 + lane * 8, r1);
+            wave_lds_sync();
+            store_chunk_contig(dst1 + c * 512, tile0, fl, lane);
+            wave_lds_sync();
+            return;
         }
         store_row(tile0 + lane * 8, r0);
         if constexpr (kTwo) store_row(tile1 + lane * 8, r1);
@@ -1226,23 +1268,23 @@ __global__ __launch_bounds__(64) void stencil_wave_kernel(const int32_t *__restr
 // and takes its x-ghosts from the neighbouring wave when the neighbour table says the
 // -x / +x chunk is the previous / next chunk of the group; only the other x-ghosts are
 // loaded from memory.  One group: chunks grp * W .. grp * W + W - 1.
-template <typename T, int KIND, int W> struct GroupLds {
+template <typename T, int KIND, int W, bool ONE = false> struct GroupLds {
     static constexpr bool kTwo = (KIND == SG_ST_GRAY_SCOTT);
     static constexpr int NP = kTwo ? 2 : 1;
     __attribute__((aligned(16))) T tiles0[W][100 * 8];
-    __attribute__((aligned(16))) T tiles1[W][kTwo ? 100 * 8 : 8];
+    __attribute__((aligned(16))) T tiles1[W][(kTwo && !ONE) ? 100 * 8 : 8];
     T cols[W][NP][2][64];  // [wave][property][x = 0 | x = 7][row]
 };
 
-template <typename T, int KIND, int W>
-__device__ inline void group_step(GroupLds<T, KIND, W> &L, int64_t grp, int w, int lane,
+template <typename T, int KIND, int W, bool ONE>
+__device__ inline void group_step(GroupLds<T, KIND, W, ONE> &L, int64_t grp, int w, int lane,
                                   const int32_t *__restrict__ nn, int64_t n, const uint8_t *__restrict__ mask,
                                   const T *__restrict__ src0, const T *__restrict__ src1, T *__restrict__ dst0,
                                   T *__restrict__ dst1, int alias, const StencilArgs &sa,
                                   const int64_t *__restrict__ keys, int64_t bsz0, int64_t bsz1)
 {
-    constexpr bool kTwo = GroupLds<T, KIND, W>::kTwo;
-    constexpr int NP = GroupLds<T, KIND, W>::NP;
+    constexpr bool kTwo = GroupLds<T, KIND, W, ONE>::kTwo;
+    constexpr int NP = GroupLds<T, KIND, W, ONE>::NP;
     const int64_t c = grp * W + w;
     const bool valid = c < n;  // every wave reaches the barrier
     ChunkIn<T, KIND> in;
@@ -1273,11 +1315,11 @@ __device__ inline void group_step(GroupLds<T, KIND, W> &L, int64_t grp, int w, i
         in.u.xp = L.cols[w + 1][0][0][lane];
         if constexpr (kTwo) in.v.xp = L.cols[w + 1][NP - 1][0][lane];
     }
-    process_chunk<T, KIND>(in, c, L.tiles0[w], L.tiles1[w], dst0, dst1, alias, sa, keys, bsz0, bsz1, lane);
+    process_chunk<T, KIND, ONE>(in, c, L.tiles0[w], L.tiles1[w], dst0, dst1, alias, sa, keys, bsz0, bsz1, lane);
 }
 
 // one group per workgroup; group g runs in XCD g's slot (g & 7 of the launch order)
-template <typename T, int KIND, int W>
+template <typename T, int KIND, int W, bool ONE>
 __global__ __launch_bounds__(64 * W) void stencil_group_kernel(const int32_t *__restrict__ nn, int64_t n,
                                                                const uint8_t *__restrict__ mask,
                                                                const T *__restrict__ src0, const T *__restrict__ src1,
@@ -1285,14 +1327,14 @@ __global__ __launch_bounds__(64 * W) void stencil_group_kernel(const int32_t *__
                                                                StencilArgs sa, const int64_t *__restrict__ keys,
                                                                int64_t bsz0, int64_t bsz1, int64_t groups_per_xcd)
 {
-    __shared__ GroupLds<T, KIND, W> L;
+    __shared__ GroupLds<T, KIND, W, ONE> L;
     // the wave index is uniform: with it in a scalar register the neighbour-table row is
     // read by scalar loads, which the ghost loads wait for without waiting for the own
     // rows (vector loads complete in order)
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t b = blockIdx.x;
     const int64_t grp = (b & 7) * groups_per_xcd + (b >> 3);
-    group_step<T, KIND, W>(L, grp, w, lane, nn, n, mask, src0, src1, dst0, dst1, alias, sa, keys, bsz0, bsz1);
+    group_step<T, KIND, W, ONE>(L, grp, w, lane, nn, n, mask, src0, src1, dst0, dst1, alias, sa, keys, bsz0, bsz1);
 }
 
 template <typename T>
@@ -1614,17 +1656,27 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
                     }
                     if (g->fast_variant >= 3) {
                         // groups of W consecutive chunks sharing x faces through LDS
-                        auto grouped = [&](auto wtag) {
+                        // (variants 6 / 7 / 8: the same with one LDS tile per wave for both
+                        // Gray-Scott properties)
+                        auto grouped = [&](auto wtag, auto onetag) {
                             constexpr int GW = decltype(wtag)::value;
+                            constexpr bool ONE = decltype(onetag)::value;
                             const int64_t ngroups = (n + GW - 1) / GW;
                             const int64_t gpx = (ngroups + 7) / 8;
-                            hipLaunchKernelGGL((stencil_group_kernel<T, K, GW>), dim3((unsigned)(gpx * 8)),
+                            hipLaunchKernelGGL((stencil_group_kernel<T, K, GW, ONE>), dim3((unsigned)(gpx * 8)),
                                                dim3(64 * GW), 0, g->stream, g->d_nn[type], n, g->d_mask, s0, s1, o0,
                                                o1, al, sa, g->d_keys, g->bsz[0], g->bsz[1], gpx);
                         };
-                        if (g->fast_variant == 3) grouped(std::integral_constant<int, 4>());
-                        else if (g->fast_variant == 4) grouped(std::integral_constant<int, 8>());
-                        else grouped(std::integral_constant<int, 2>());
+                        const std::false_type two_tiles;
+                        const std::true_type one_tile;
+                        switch (g->fast_variant) {
+                        case 3: grouped(std::integral_constant<int, 4>(), two_tiles); break;
+                        case 4: grouped(std::integral_constant<int, 8>(), two_tiles); break;
+                        case 5: grouped(std::integral_constant<int, 2>(), two_tiles); break;
+                        case 6: grouped(std::integral_constant<int, 4>(), one_tile); break;
+                        case 7: grouped(std::integral_constant<int, 8>(), one_tile); break;
+                        default: grouped(std::integral_constant<int, 2>(), one_tile); break;
+                        }
                         return;
                     }
                     // persistent waves: wg_per_xcd workgroups of 4 waves per XCD, each wave

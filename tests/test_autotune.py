@@ -27,7 +27,7 @@ def test_size_class():
 
 def test_candidates_are_valid_settings():
     for c in A.STENCIL_CANDIDATES:
-        assert 0 <= c["variant"] <= 5 and c["stores"] in (0, 1, 2, 3)
+        assert 0 <= c["variant"] <= 8 and c["stores"] in (0, 1, 2, 3)
     for c in A.BATCH_CANDIDATES:
         assert c["waves"] in (1, 2, 4, 8) and c["max_wgs"] >= 0
 

@@ -262,7 +262,8 @@ def test_blobs_flush_subsample(G, oracle):
 
 @pytest.mark.parametrize("variant,stores,lskip", [(0, 0, 0), (0, 1, 1), (1, 2, 0), (2, 1, 0), (3, 2, 1),
                                                   (4, 2, 0), (4, 0, 1), (5, 1, 0), (4, 3, 0), (3, 3, 1),
-                                                  (0, 3, 0), (1, 3, 1)])
+                                                  (0, 3, 0), (1, 3, 1), (6, 3, 0), (7, 3, 1), (8, 3, 0),
+                                                  (6, 2, 1), (7, 0, 0), (8, 1, 1)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_fast_stencil_variants(G, oracle, variant, stores, lskip, dtype):
     """Every selectable launch shape of the 8^3 star-stencil kernel gives the oracle's
@@ -294,8 +295,9 @@ def test_fast_stencil_variants(G, oracle, variant, stores, lskip, dtype):
 
 
 def test_grouped_stencil_variants_agree_on_a_large_shell(G, oracle):
-    """Groups of 2 / 4 / 8 chunks with every store mode, on a shell of several thousand
-    chunks (many workgroups in flight, so that a missing wait between the waves of a group
+    """Groups of 2 / 4 / 8 chunks (two LDS tiles per wave or one) with every store mode,
+    on a shell of several thousand chunks (many workgroups in flight, so that a missing
+    wait between the waves of a group
     shows), give bitwise the same Gray-Scott steps as each other and as the oracle."""
     rng = np.random.default_rng(73)
     pts = W.shell_points(176, 60.0, 84.0)
@@ -315,7 +317,7 @@ def test_grouped_stencil_variants_agree_on_a_large_shell(G, oracle):
         return g
 
     ref = run(oracle.SparseGridCpu)
-    for variant in (3, 4, 5):
+    for variant in (3, 4, 5, 6, 7, 8):
         for stores in (1, 2, 3):
             g = run(G, variant, stores)
             S.assert_same_state(g, ref)
