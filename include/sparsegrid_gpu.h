@@ -145,6 +145,8 @@ typedef struct {
  * block_edge^dim <= 1024; stencil_radius must be 1). */
 int sg_create(int dim, const int64_t *size, int block_edge, int nprops,
               const int32_t *dtypes, int stencil_radius, sg_grid **out);
+/* waits for the handle's stream; its device blocks are kept, idle, for later handles on
+ * the same device to allocate from (sg_trim on any handle of the device frees them) */
 int sg_destroy(sg_grid *g);
 /* chunk linearizer: SG_LIN_SMB = grid_smb, row-major chunk ids (SparseGridGpu, the
  * default); SG_LIN_ZMB = grid_zmb, Morton chunk ids (SparseGridGpu_z,
@@ -384,7 +386,8 @@ int sg_get_links(sg_grid *g, int which, int32_t *pad_chunk, int16_t *pad_off, ui
 
 /* ---- device memory (no counterpart in the reference: its vectors keep their
  *      capacity; a server admitting grids against an HBM budget needs both) ---- */
-/* wait for the handle's stream, then free the flush temporaries and cached blocks; the
+/* wait for the handle's stream, then free the flush temporaries and cached blocks, and
+ * the device blocks the library keeps from destroyed handles for reuse by new ones; the
  * chunk map, properties, masks and neighbour tables stay */
 int sg_trim(sg_grid *g);
 /* bytes of device memory the handle holds (resident arrays + cached temporaries) */

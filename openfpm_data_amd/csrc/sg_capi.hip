@@ -4,6 +4,7 @@
 #include "sg_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace sg {
 
@@ -17,6 +18,40 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
+namespace {
+
+// idle blocks of destroyed handles: size -> (block, device)
+std::mutex g_idle_mu;
+std::multimap<size_t, std::pair<void *, int>> g_idle;
+
+// an idle block of the device with at least `bytes` and at most twice that
+std::pair<void *, size_t> take_idle(size_t bytes, int dev)
+{
+    std::lock_guard<std::mutex> lk(g_idle_mu);
+    for (auto it = g_idle.lower_bound(bytes); it != g_idle.end() && it->first <= 2 * bytes; ++it) {
+        if (it->second.second != dev) continue;
+        const std::pair<void *, size_t> r(it->second.first, it->first);
+        g_idle.erase(it);
+        return r;
+    }
+    return {nullptr, 0};
+}
+
+void free_idle(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_idle_mu);
+    for (auto it = g_idle.begin(); it != g_idle.end();) {
+        if (it->second.second == dev) {
+            (void)hipFree(it->second.first);
+            it = g_idle.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+}  // namespace
+
 void *DevicePool::alloc(size_t bytes)
 {
     if (bytes == 0) bytes = 256;
@@ -29,21 +64,33 @@ void *DevicePool::alloc(size_t bytes)
         live_ += sizes_[p];
         return p;
     }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const std::pair<void *, size_t> idle = take_idle(bytes, dev);
+    if (idle.first) {
+        sizes_[idle.first] = idle.second;
+        devs_[idle.first] = dev;
+        live_ += idle.second;
+        return idle.first;
+    }
     void *p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) {
         (void)hipGetLastError();
-        // drop the cache and retry once
+        // drop the caches and retry once
         for (auto &kv : cache_) {
             (void)hipFree(kv.second);
             sizes_.erase(kv.second);
+            devs_.erase(kv.second);
         }
         cache_.clear();
+        free_idle(dev);
         if (hipMalloc(&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
             return nullptr;
         }
     }
     sizes_[p] = bytes;
+    devs_[p] = dev;
     live_ += bytes;
     return p;
 }
@@ -59,8 +106,21 @@ void DevicePool::free(void *p)
 
 void DevicePool::release()
 {
+    {
+        std::lock_guard<std::mutex> lk(g_idle_mu);
+        for (auto &kv : sizes_) g_idle.emplace(kv.second, std::make_pair(kv.first, devs_[kv.first]));
+    }
+    sizes_.clear();
+    devs_.clear();
+    cache_.clear();
+    live_ = 0;
+}
+
+void DevicePool::free_all()
+{
     for (auto &kv : sizes_) (void)hipFree(kv.first);
     sizes_.clear();
+    devs_.clear();
     cache_.clear();
     live_ = 0;
 }
@@ -70,8 +130,12 @@ void DevicePool::trim()
     for (auto &kv : cache_) {
         (void)hipFree(kv.second);
         sizes_.erase(kv.second);
+        devs_.erase(kv.second);
     }
     cache_.clear();
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    free_idle(dev);
 }
 
 size_t DevicePool::held_bytes() const

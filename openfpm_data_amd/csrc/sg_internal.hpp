@@ -54,19 +54,26 @@ inline bool dtype_is_float(int dt) { return dt == SG_F32 || dt == SG_F64; }
 // hipMalloc/hipFree synchronise the device; a grid reallocates its arrays on
 // every flush that adds chunks, so freed blocks are kept and reused.  All work of a
 // handle is ordered on the handle's stream, so reuse by the same handle is safe.
+// When a handle is destroyed its blocks go to a process-wide cache of idle blocks (per
+// device), from which later handles allocate before calling hipMalloc: creating and
+// destroying grids over and over then costs no hipMalloc.  release() may only be called
+// once the handle's stream is idle (its owners synchronise the stream first), since the
+// blocks are then reused by other streams without further ordering.
 class DevicePool {
 public:
-    ~DevicePool() { release(); }
+    ~DevicePool() { free_all(); }
     void *alloc(size_t bytes);
     void free(void *p);
-    void release();
-    void trim();  // hipFree the cached (not live) blocks
+    void release();   // every block of this handle to the process-wide idle cache
+    void trim();      // hipFree this handle's cached blocks and the idle cache of the device
     size_t live_bytes() const { return live_; }
     size_t held_bytes() const;
 
 private:
+    void free_all();  // hipFree every block of this handle
     std::multimap<size_t, void *> cache_;
     std::map<void *, size_t> sizes_;
+    std::map<void *, int> devs_;
     size_t live_ = 0;
 };
 
