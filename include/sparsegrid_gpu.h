@@ -280,6 +280,9 @@ int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, s
 #define SG_TUNE_STENCIL_STORES 3
 /* 8^3 star-stencil loads: 1 = read the mask first and skip rows without points */
 #define SG_TUNE_STENCIL_SKIP_EMPTY_ROWS 4
+/* 8^3 tagBoundaries: chunks per wave, 1, 2 (default) or 4; a wave issues the loads of all
+ * its chunks before tagging any */
+#define SG_TUNE_TAG_CHUNKS_PER_WAVE 7
 int sg_set_tuning(sg_grid *g, int key, int64_t value);
 
 /* ---- queries ---------------------------------------------------------------- */

@@ -406,6 +406,11 @@ int sg_set_tuning(sg_grid *g, int key, int64_t value)
         g->fast_wg_per_xcd = value;
         return SG_OK;
     }
+    if (key == SG_TUNE_TAG_CHUNKS_PER_WAVE) {
+        if (value != 1 && value != 2 && value != 4) return fail(SG_E_RANGE, "chunks per wave must be 1, 2 or 4");
+        g->tag_cpw = (int)value;
+        return SG_OK;
+    }
     if (key == SG_TUNE_STENCIL_VARIANT) {
         if (value < 0 || value > 8) return fail(SG_E_RANGE, "stencil variant must be in [0, 8]");
         g->fast_variant = (int)value;

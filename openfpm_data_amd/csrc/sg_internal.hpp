@@ -128,6 +128,8 @@ struct sg_grid {
     // every slot is active or empty, 2 as 1 with non-temporal stores, 3 as 2 through LDS
     // as lane-contiguous 16-byte vectors
     int fast_wmode = 3;
+    // 8^3 tagBoundaries: chunks per wave (1, 2 or 4), all loaded before any is tagged
+    int tag_cpw = 2;
     // read the mask first and skip loading 8-point rows without points
     int fast_lskip = 0;
 

@@ -589,74 +589,101 @@ __device__ inline int nn_entry(int type, int dx, int dy, int dz)
     return dz < 0 ? 4 : 5;
 }
 
-template <int WAVES>
+// One wave per CPW consecutive chunks (lane = x-row of each): every load of the wave's
+// chunks (own mask rows, x-ghost EXIST bits, the 36 ghost rows of the y / z faces and, for
+// the full neighbourhood, their x-ghost bits) is issued before any is used, so a wave
+// keeps CPW chunks of loads in flight; then each chunk's 10 x 10 tile of EXIST words is
+// staged in LDS and its points tagged.
+template <int WAVES, int CPW>
 __global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_kernel(int64_t n, const int32_t *__restrict__ nn,
                                                                     int nnn, int type, uint8_t *__restrict__ mask)
 {
-    __shared__ uint16_t tile[WAVES][100];
-    // uniform wave index: the neighbour-table row is read by scalar loads
+    __shared__ uint16_t tile[WAVES][CPW][100];
+    // uniform wave index: the neighbour-table rows are read by scalar loads
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int64_t c = (int64_t)blockIdx.x * WAVES + w;
-    if (c >= n) return;  // waves are independent: no workgroup barrier below
-    const int32_t *row_nn = nn + c * nnn;
+    const int64_t cb = ((int64_t)blockIdx.x * WAVES + w) * CPW;
+    if (cb >= n) return;  // waves are independent: no workgroup barrier below
     const int y = lane & 7, z = lane >> 3;
-    auto chunk_of = [&](int dx, int dy, int dz) -> int64_t {
-        const int k = nn_entry(type, dx, dy, dz);
-        if (k == -2) return c;
-        return (k < 0) ? -1 : (int64_t)row_nn[k];
-    };
-    // x ghost bit of row (yy, zz) in chunk displacement (dx, dy, dz)
-    auto xbit = [&](int dx, int dy, int dz, int ly, int lz, int lx) -> uint32_t {
-        const int64_t nb = chunk_of(dx, dy, dz);
-        if (nb < 0) return 0u;
-        return mask[nb * 512 + (lz * 8 + ly) * 8 + lx] & 1u;
-    };
-    // own row with its x ghosts
-    const uint64_t own = *(const uint64_t *)(mask + c * 512 + lane * 8);
-    {
-        const uint32_t xm = xbit(-1, 0, 0, y, z, 7), xp = xbit(1, 0, 0, y, z, 0);
-        tile[w][(z + 1) * 10 + (y + 1)] = (uint16_t)(xm | (exist_bits8(own) << 1) | (xp << 9));
+    int gyy, gzz;
+    ghost_row_yz(lane < 36 ? lane : 0, gyy, gzz);
+    const int gdy = (gyy < 0) ? -1 : (gyy > 7 ? 1 : 0), gdz = (gzz < 0) ? -1 : (gzz > 7 ? 1 : 0);
+    const int gly = gyy - gdy * 8, glz = gzz - gdz * 8;
+    uint64_t own[CPW], grow[CPW];
+    uint32_t xm[CPW], xp[CPW], gxm[CPW], gxp[CPW];
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+        const int64_t c = cb + k;
+        own[k] = 0;
+        grow[k] = 0;
+        xm[k] = xp[k] = gxm[k] = gxp[k] = 0;
+        if (c >= n) continue;
+        const int32_t *row_nn = nn + c * nnn;
+        auto chunk_of = [&](int dx, int dy, int dz) -> int64_t {
+            const int e = nn_entry(type, dx, dy, dz);
+            if (e == -2) return c;
+            return (e < 0) ? -1 : (int64_t)row_nn[e];
+        };
+        // EXIST bit of x-row (ly, lz), point lx, of the chunk at displacement (dx, dy, dz)
+        auto xbit = [&](int dx, int dy, int dz, int ly, int lz, int lx) -> uint32_t {
+            const int64_t nb = chunk_of(dx, dy, dz);
+            if (nb < 0) return 0u;
+            return mask[nb * 512 + (lz * 8 + ly) * 8 + lx];
+        };
+        own[k] = *(const uint64_t *)(mask + c * 512 + lane * 8);
+        xm[k] = xbit(-1, 0, 0, y, z, 7);
+        xp[k] = xbit(1, 0, 0, y, z, 0);
+        if (lane < 36) {
+            const int64_t nb = chunk_of(0, gdy, gdz);
+            if (nb >= 0) grow[k] = *(const uint64_t *)(mask + nb * 512 + (glz * 8 + gly) * 8);
+            if (type == SG_NN_FULL) {
+                gxm[k] = xbit(-1, gdy, gdz, gly, glz, 7);
+                gxp[k] = xbit(1, gdy, gdz, gly, glz, 0);
+            }
+        }
     }
-    if (lane < 36) {
-        int yy, zz;
-        ghost_row_yz(lane, yy, zz);
-        const int dy = (yy < 0) ? -1 : (yy > 7 ? 1 : 0), dz = (zz < 0) ? -1 : (zz > 7 ? 1 : 0);
-        const int ly = yy - dy * 8, lz = zz - dz * 8;
-        uint32_t word = 0;
-        const int64_t nb = chunk_of(0, dy, dz);
-        if (nb >= 0) word = exist_bits8(*(const uint64_t *)(mask + nb * 512 + (lz * 8 + ly) * 8)) << 1;
-        if (type == SG_NN_FULL) word |= xbit(-1, dy, dz, ly, lz, 7) | (xbit(1, dy, dz, ly, lz, 0) << 9);
-        tile[w][(zz + 1) * 10 + (yy + 1)] = (uint16_t)word;
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+        if (cb + k >= n) continue;
+        tile[w][k][(z + 1) * 10 + (y + 1)] =
+            (uint16_t)((xm[k] & 1u) | (exist_bits8(own[k]) << 1) | ((xp[k] & 1u) << 9));
+        if (lane < 36)
+            tile[w][k][(gzz + 1) * 10 + (gyy + 1)] =
+                (uint16_t)((exist_bits8(grow[k]) << 1) | (gxm[k] & 1u) | ((gxp[k] & 1u) << 9));
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint16_t *t = tile[w];
-    const uint32_t self = t[(z + 1) * 10 + (y + 1)];
-    uint32_t all;  // bit j: every neighbour of x = j - 1 exists
-    if (type == SG_NN_FULL) {
-        all = 0x3FFu;
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
+    for (int k = 0; k < CPW; ++k) {
+        const int64_t c = cb + k;
+        if (c >= n) continue;
+        const uint16_t *t = tile[w][k];
+        const uint32_t self = t[(z + 1) * 10 + (y + 1)];
+        uint32_t all;  // bit j: every neighbour of x = j - 1 exists
+        if (type == SG_NN_FULL) {
+            all = 0x3FFu;
 #pragma unroll
-            for (int dy = 0; dy < 3; ++dy) {
-                const uint32_t r = t[(z + dz) * 10 + (y + dy)];
-                all &= r & (r << 1) & (r >> 1);
-            }
-    } else {
-        all = (self << 1) & (self >> 1) & t[(z + 1) * 10 + y] & t[(z + 1) * 10 + y + 2] & t[z * 10 + y + 1] &
-              t[(z + 2) * 10 + y + 1];
+            for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) {
+                    const uint32_t r = t[(z + dz) * 10 + (y + dy)];
+                    all &= r & (r << 1) & (r >> 1);
+                }
+        } else {
+            all = (self << 1) & (self >> 1) & t[(z + 1) * 10 + y] & t[(z + 1) * 10 + y + 2] & t[z * 10 + y + 1] &
+                  t[(z + 2) * 10 + y + 1];
+        }
+        const uint32_t pad = (self >> 1) & ~(all >> 1) & 0xFFu;  // existing and not surrounded
+        const uint64_t o = own[k];
+        uint64_t out = o;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint64_t m = (o >> (8 * i)) & 0xFFull;
+            const uint64_t nm = ((pad >> i) & 1u) ? (m | 2ull) : (m & ~2ull);
+            out = (m & 1) ? ((out & ~(0xFFull << (8 * i))) | (nm << (8 * i))) : out;
+        }
+        if (out != o) *(uint64_t *)(mask + c * 512 + lane * 8) = out;
     }
-    const uint32_t pad = (self >> 1) & ~(all >> 1) & 0xFFu;  // existing and not surrounded
-    uint64_t out = own;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint64_t m = (own >> (8 * i)) & 0xFFull;
-        if (!(m & 1)) continue;
-        const uint64_t nm = ((pad >> i) & 1u) ? (m | 2ull) : (m & ~2ull);
-        out = (out & ~(0xFFull << (8 * i))) | (nm << (8 * i));
-    }
-    if (out != own) *(uint64_t *)(mask + c * 512 + lane * 8) = out;
 }
 
 // ---- 3D 8^3 27-point box ----------------------------------------------------------
@@ -1458,8 +1485,15 @@ int tag_boundaries_impl(sg_grid *g, int type)
     if (g->dim == 3 && g->edge == 8) {
         // the tile kernel writes masks in place: rows are read by other waves only through
         // ghost words (EXIST bits), which this kernel never changes
-        hipLaunchKernelGGL(tag_boundaries8_kernel<4>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, g->stream, n,
-                           g->d_nn[type], g->nnn(type), type, g->d_mask);
+        auto launch = [&](auto cpw) {
+            constexpr int CPW = decltype(cpw)::value;
+            const int64_t per_wg = 4 * CPW;
+            hipLaunchKernelGGL((tag_boundaries8_kernel<4, CPW>), dim3((unsigned)((n + per_wg - 1) / per_wg)), dim3(256),
+                               0, g->stream, n, g->d_nn[type], g->nnn(type), type, g->d_mask);
+        };
+        if (g->tag_cpw == 1) launch(std::integral_constant<int, 1>());
+        else if (g->tag_cpw == 2) launch(std::integral_constant<int, 2>());
+        else launch(std::integral_constant<int, 4>());
         SG_HIP(hipGetLastError());
         return SG_OK;
     }

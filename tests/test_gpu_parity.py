@@ -505,6 +505,26 @@ def test_tag_and_box_8cubed_random(G, oracle, seed):
         S.assert_same_state(a, b, exist_only=False)
 
 
+@pytest.mark.parametrize("cpw", [1, 2, 4])
+@pytest.mark.parametrize("nn", [C.NN_STAR, C.NN_FULL])
+def test_tag_boundaries_chunks_per_wave(G, oracle, cpw, nn):
+    """The 8^3 tagBoundaries kernel with 1, 2 or 4 chunks per wave (masks with every bit
+    pattern) tags the same points as the oracle."""
+    rng = np.random.default_rng(1100 + cpw)
+    size = [int(rng.integers(60, 90)) for _ in range(3)]
+    steps = S.random_flush_program(7 + cpw, 3, 8, size, [np.int32], nflush=2, nrec=4000, ops=["smax"])
+    grids = []
+    for cls in (G, oracle.SparseGridCpu):
+        g = S.run_flush_program(cls, 3, 8, size, [np.int32], steps)
+        if cls is G:
+            g.set_tuning(C.TUNE_TAG_CHUNKS_PER_WAVE, cpw)
+        g.set_nn_type(nn)
+        g.tag_boundaries(nn)
+        grids.append(g)
+    S.assert_same_state(grids[0], grids[1])
+    assert grids[0].count_boundary_elements() == grids[1].count_boundary_elements() > 0
+
+
 def test_flush_rejects_device_keys_outside_grid(G):
     """Point keys handed over in device memory are checked by the flush: a key beyond the
     grid fails it (SgError), drops the pending records and leaves the grid unchanged."""
