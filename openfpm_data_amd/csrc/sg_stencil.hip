@@ -710,6 +710,8 @@ __global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32
     auto chunk_of = [&](int dx, int dy, int dz) -> int64_t {
         return (dx == 0 && dy == 0 && dz == 0) ? c : (int64_t)row_nn[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
     };
+    // the mask row is loaded with the source rows, not after the tile is staged
+    const uint64_t mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
     // own row + x ghosts
     {
         T *dstrow = tw + ((z + 1) * 10 + (y + 1)) * 10;
@@ -734,7 +736,6 @@ __global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint64_t mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
     // row flag: 0 no point (the destination row already holds the background), 1 whole
     // row, 2 active points only (bits 8-15: which)
     uint32_t fl = 0;
