@@ -693,49 +693,69 @@ __global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_kernel(int64_t n, 
 // lane sums its 8 points over the 3 x 3 x 3 box in stencil_generic_kernel's order (x offset
 // outermost, z innermost).  Rows whose points are all active or empty are stored whole
 // (empty points take the destination background); other rows point by point.
-template <typename T, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32_t *__restrict__ nn,
-                                                         const uint8_t *__restrict__ mask, const T *__restrict__ src,
-                                                         T *__restrict__ dst, const int64_t *__restrict__ keys,
-                                                         int64_t bsz0, int64_t bsz1, T bg_dst, StencilArgs sa)
+// Per-lane loads of one chunk for the box: the own x-row with its two x-ghost values,
+// one ghost row of the y / z faces (lanes 0-35) with its x ghosts, and the mask row.
+template <typename T> struct BoxIn {
+    T own[8], gr[8];
+    T oxm, oxp, gxm, gxp;
+    uint64_t mrow;
+};
+
+template <typename T>
+__device__ inline void box_load(BoxIn<T> &in, int64_t c, const int32_t *__restrict__ nn,
+                                const uint8_t *__restrict__ mask, const T *__restrict__ src, int lane)
 {
-    __shared__ T tile[WAVES][100 * 10];
-    // uniform wave index: the neighbour-table row is read by scalar loads
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int64_t c = (int64_t)blockIdx.x * WAVES + w;
-    if (c >= n) return;  // waves are independent
     const int32_t *row_nn = nn + c * 27;
-    const int y = lane & 7, z = lane >> 3;
-    T *tw = tile[w];
     auto chunk_of = [&](int dx, int dy, int dz) -> int64_t {
         return (dx == 0 && dy == 0 && dz == 0) ? c : (int64_t)row_nn[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
     };
-    // the mask row is loaded with the source rows, not after the tile is staged
-    const uint64_t mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
-    // own row + x ghosts
-    {
-        T *dstrow = tw + ((z + 1) * 10 + (y + 1)) * 10;
-        const T *sr = src + c * 512 + lane * 8;
+    in.mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
+    const T *sr = src + c * 512 + lane * 8;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) dstrow[i + 1] = sr[i];
-        dstrow[0] = src[chunk_of(-1, 0, 0) * 512 + lane * 8 + 7];
-        dstrow[9] = src[chunk_of(1, 0, 0) * 512 + lane * 8 + 0];
-    }
+    for (int i = 0; i < 8; ++i) in.own[i] = sr[i];
+    in.oxm = src[chunk_of(-1, 0, 0) * 512 + lane * 8 + 7];
+    in.oxp = src[chunk_of(1, 0, 0) * 512 + lane * 8 + 0];
     if (lane < 36) {
         int yy, zz;
         ghost_row_yz(lane, yy, zz);
         const int dy = (yy < 0) ? -1 : (yy > 7 ? 1 : 0), dz = (zz < 0) ? -1 : (zz > 7 ? 1 : 0);
         const int lrow = (zz - dz * 8) * 8 + (yy - dy * 8);
-        T *dstrow = tw + ((zz + 1) * 10 + (yy + 1)) * 10;
-        const T *sr = src + chunk_of(0, dy, dz) * 512 + lrow * 8;
+        const T *gs = src + chunk_of(0, dy, dz) * 512 + lrow * 8;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) dstrow[i + 1] = sr[i];
-        dstrow[0] = src[chunk_of(-1, dy, dz) * 512 + lrow * 8 + 7];
-        dstrow[9] = src[chunk_of(1, dy, dz) * 512 + lrow * 8 + 0];
+        for (int i = 0; i < 8; ++i) in.gr[i] = gs[i];
+        in.gxm = src[chunk_of(-1, dy, dz) * 512 + lrow * 8 + 7];
+        in.gxp = src[chunk_of(1, dy, dz) * 512 + lrow * 8 + 0];
+    }
+}
+
+// Stages chunk c's 10 x 10 x 10 tile from `in`, sums the 3 x 3 x 3 box of each of the
+// lane's 8 points and writes the chunk back through the tile.
+template <typename T>
+__device__ inline void box_finish(const BoxIn<T> &in, int64_t c, T *tw, T *__restrict__ dst,
+                                  const int64_t *__restrict__ keys, int64_t bsz0, int64_t bsz1, T bg_dst,
+                                  const StencilArgs &sa, int lane)
+{
+    const int y = lane & 7, z = lane >> 3;
+    {
+        T *dstrow = tw + ((z + 1) * 10 + (y + 1)) * 10;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dstrow[i + 1] = in.own[i];
+        dstrow[0] = in.oxm;
+        dstrow[9] = in.oxp;
+    }
+    if (lane < 36) {
+        int yy, zz;
+        ghost_row_yz(lane, yy, zz);
+        T *dstrow = tw + ((zz + 1) * 10 + (yy + 1)) * 10;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) dstrow[i + 1] = in.gr[i];
+        dstrow[0] = in.gxm;
+        dstrow[9] = in.gxp;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint64_t mrow = in.mrow;
     // row flag: 0 no point (the destination row already holds the background), 1 whole
     // row, 2 active points only (bits 8-15: which)
     uint32_t fl = 0;
@@ -804,6 +824,29 @@ __global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     store_chunk_contig(dst + c * 512, tw, fl, lane);
+}
+
+// One wave per chunk (lane = x-row).  Every load of the chunk (source rows, ghosts, mask
+// row) is issued into registers before any is used, then the 10 x 10 x 10 tile of the
+// source (rows (z, y) in [-1, 8]^2, 10 values with the x ghosts, from the chunk and its 26
+// neighbours; a missing chunk is the background chunk) is staged in LDS and each lane
+// sums its 8 points over the 3 x 3 x 3 box in stencil_generic_kernel's order (x offset
+// outermost, z innermost).  Rows whose points are all active or empty are stored whole
+// (empty points take the destination background); other rows point by point.
+template <typename T, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32_t *__restrict__ nn,
+                                                         const uint8_t *__restrict__ mask, const T *__restrict__ src,
+                                                         T *__restrict__ dst, const int64_t *__restrict__ keys,
+                                                         int64_t bsz0, int64_t bsz1, T bg_dst, StencilArgs sa)
+{
+    __shared__ T tile[WAVES][100 * 10];
+    // uniform wave index: the neighbour-table row is read by scalar loads
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * WAVES + w;
+    if (c >= n) return;  // waves are independent
+    BoxIn<T> a;
+    box_load<T>(a, c, nn, mask, src, lane);
+    box_finish<T>(a, c, tile[w], dst, keys, bsz0, bsz1, bg_dst, sa, lane);
 }
 
 // ---- 3D 8^3 star fast path ------------------------------------------------------
