@@ -239,9 +239,7 @@ def run_serving_secondary(n_open=384, n_cap=64, budget_gib=32.0, batch_tuning=No
     """Served Poisson requests (edge log-uniform in [32, 384], 5-40% of the chunks active,
     FP32 f, tolerance 1e-10 or 1e-6, seed 4), mixed method, HBM budget `budget_gib`:
     capacity measured closed-loop (first `n_cap` requests all present at t = 0), then
-    `n_open` requests arriving as a Poisson process at 70% of that capacity (seed 5): the
-    closed-loop figure builds every grid before solving them in one batch, while arrivals
-    interleave grid builds with the batched steps, which sustain about 80% of it."""
+    `n_open` requests arriving as a Poisson process at 80% of that capacity (seed 5)."""
     from openfpm_data_amd import serving as S
     from openfpm_data_amd import workloads as W
 
@@ -250,7 +248,7 @@ def run_serving_secondary(n_open=384, n_cap=64, budget_gib=32.0, batch_tuning=No
     srv = S.PoissonServer(budget, tuning=batch_tuning)
     srv.run(reqs[:8], np.zeros(8))  # warm-up
     _, cap = srv.run(reqs[:n_cap], np.zeros(n_cap))
-    rate = 0.7 * cap["solves_per_s"]
+    rate = 0.8 * cap["solves_per_s"]
     _, s = srv.run(reqs, S.poisson_arrivals(n_open, rate, 5))
     srv.close()
     return {"workload": "poisson_serving", "requests": n_open, "budget_gib": budget_gib,

@@ -10,6 +10,8 @@ relative-residual tolerance.  The loop
    starved by smaller ones behind it);
 3. builds each admitted grid on the device (insert of whole chunks, flush, neighbour
    table), frees its build temporaries (``sg_trim``) and books its measured resident bytes;
+   the builds run in a builder thread on a stream of their own, so that they overlap the
+   batched steps of the grids already live (the library's calls release the GIL);
 4. runs one outer iteration of every live grid in shared launches (``sg_batch_step``):
    requests that arrived while others were on the GPU join between two outer iterations;
 5. hands finished grids back: the solution is copied to the host, the grid is freed and
@@ -20,6 +22,8 @@ grid's iterates do not depend on its batch mates (see ``sg_batch_add``), so the 
 request is bitwise the same served alone or in any mix.
 """
 import collections
+import queue
+import threading
 import time
 
 import numpy as np
@@ -106,31 +110,52 @@ class PoissonServer:
 
     params: ``poisson.make_params(...)`` (method and inner-solve settings; each request
     carries its own tolerance).  budget_bytes: HBM budget for resident grids.  tuning:
-    {key: value} of sg_batch_set_tuning (the sweep kernel's launch shape)."""
+    {key: value} of sg_batch_set_tuning (the sweep kernel's launch shape).  build_thread:
+    build admitted grids in a builder thread (on a stream of its own) while the batch
+    steps; False builds them inline, between two steps."""
 
-    def __init__(self, budget_bytes, params=None, keep_solutions=False, tuning=None):
+    def __init__(self, budget_bytes, params=None, keep_solutions=False, tuning=None, build_thread=True):
         import torch
         self.torch = torch
         self.params = params if params is not None else P.make_params()
         self.stream = torch.cuda.Stream()
+        self.build_stream = torch.cuda.Stream()
         self.batch = P.PoissonBatch(self.params, self.stream)
         for k, v in (tuning or {}).items():
             self.batch.set_tuning(k, v)
         self.adm = Admission(budget_bytes)
         self.keep = keep_solutions
+        self.threaded = build_thread
 
     def close(self):
         self.batch.close()
 
-    def _build(self, req):
+    def _build(self, req, stream):
+        """Builds the request's grid on `stream`; returns (grid, resident bytes) once the
+        grid is complete (sg_trim waits for the stream)."""
         torch = self.torch
-        with torch.cuda.stream(self.stream):
+        with torch.cuda.stream(stream):
             f = torch.from_numpy(np.ascontiguousarray(req["f"])).to("cuda", non_blocking=False)
             ids = torch.from_numpy(np.asarray(req["chunk_ids"], np.int64)).to("cuda")
-            g = P.poisson_grid(req["size"], ids, f, self.stream)
+            g = P.poisson_grid(req["size"], ids, f, stream)
             del f, ids
         C.check(C.load().sg_trim(g._h))
-        return g
+        if stream is not self.stream:
+            # later work on the grid runs on the batch's stream (the build stream is idle:
+            # sg_trim waited for it, so the switch does not wait)
+            g.set_stream(self.stream)
+        return g, grid_device_bytes(g)
+
+    def _builder(self, inq, outq):
+        while True:
+            req = inq.get()
+            if req is None:
+                return
+            try:
+                g, nbytes = self._build(req, self.build_stream)
+                outq.put((req, g, nbytes, None))
+            except BaseException as e:  # handed to the serving loop, which raises it
+                outq.put((req, None, 0, e))
 
     def run(self, requests, arrivals):
         """Serve `requests` arriving at times `arrivals` (seconds from the start, sorted).
@@ -146,46 +171,82 @@ class PoissonServer:
         steps = 0
         max_live = 0
         t_build = t_step = t_fetch = 0.0
-        while len(records) < len(reqs):
-            t = now()
-            while nxt < len(reqs) and arr[nxt] <= t:
-                r = reqs[nxt]
-                r["_arrival"] = arr[nxt]
-                self.adm.arrive(r)
-                nxt += 1
-            admitted, rejected = self.adm.admit()
-            for r in rejected:
-                records[r["id"]] = {"id": r["id"], "rejected": True, "arrival": r["_arrival"]}
-            for r in admitted:
-                tb = now()
-                g = self._build(r)
-                self.adm.settle(r["id"], grid_device_bytes(g))
-                self.batch.add(g, r["id"], r["tol"])
-                live[r["id"]] = (r, g, now())
-                t_build += now() - tb
-            max_live = max(max_live, len(live))
-            if self.batch.live() == 0:
-                if nxt < len(reqs) and not self.adm.queue:
-                    time.sleep(max(0.0, arr[nxt] - now()))
-                continue
-            ts = now()
-            self.batch.step()
-            steps += 1
-            t_step += now() - ts
-            for tag, res in self.batch.poll():
-                r, g, t_adm = live.pop(tag)
-                tf = now()
-                u = g.prop(res["solution_prop"])  # solution to the host
-                t_done = now()
-                t_fetch += t_done - tf
-                rec = {"id": tag, "arrival": r["_arrival"], "admitted": t_adm, "done": t_done,
-                       "latency": t_done - r["_arrival"], "chunks": len(r["chunk_ids"]),
-                       "points": len(r["chunk_ids"]) * 512, "tol": r["tol"], **res}
-                if self.keep:
-                    rec["u"] = u
-                records[tag] = rec
-                g.close()
-                self.adm.release(tag)
+        inq, outq = queue.Queue(), queue.Queue()
+        worker = None
+        if self.threaded:
+            worker = threading.Thread(target=self._builder, args=(inq, outq), daemon=True)
+            worker.start()
+        building = 0
+
+        def join(item):
+            # a built grid joins the batch
+            r, g, nbytes, err = item
+            if err is not None:
+                raise err
+            self.adm.settle(r["id"], nbytes)
+            self.batch.add(g, r["id"], r["tol"])
+            live[r["id"]] = (r, g, now())
+
+        try:
+            while len(records) < len(reqs):
+                t = now()
+                while nxt < len(reqs) and arr[nxt] <= t:
+                    r = reqs[nxt]
+                    r["_arrival"] = arr[nxt]
+                    self.adm.arrive(r)
+                    nxt += 1
+                admitted, rejected = self.adm.admit()
+                for r in rejected:
+                    records[r["id"]] = {"id": r["id"], "rejected": True, "arrival": r["_arrival"]}
+                for r in admitted:
+                    if worker is not None:
+                        inq.put(r)
+                        building += 1
+                    else:
+                        tb = now()
+                        g, nbytes = self._build(r, self.stream)
+                        join((r, g, nbytes, None))
+                        t_build += now() - tb
+                while building:
+                    try:
+                        item = outq.get_nowait()
+                    except queue.Empty:
+                        break
+                    building -= 1
+                    join(item)
+                max_live = max(max_live, len(live))
+                if self.batch.live() == 0:
+                    if building:
+                        tb = now()
+                        item = outq.get()  # nothing to step: wait for a build
+                        building -= 1
+                        join(item)
+                        t_build += now() - tb
+                    elif nxt < len(reqs) and not self.adm.queue:
+                        time.sleep(max(0.0, arr[nxt] - now()))
+                    continue
+                ts = now()
+                self.batch.step()
+                steps += 1
+                t_step += now() - ts
+                for tag, res in self.batch.poll():
+                    r, g, t_adm = live.pop(tag)
+                    tf = now()
+                    u = g.prop(res["solution_prop"])  # solution to the host
+                    t_done = now()
+                    t_fetch += t_done - tf
+                    rec = {"id": tag, "arrival": r["_arrival"], "admitted": t_adm, "done": t_done,
+                           "latency": t_done - r["_arrival"], "chunks": len(r["chunk_ids"]),
+                           "points": len(r["chunk_ids"]) * 512, "tol": r["tol"], **res}
+                    if self.keep:
+                        rec["u"] = u
+                    records[tag] = rec
+                    g.close()
+                    self.adm.release(tag)
+        finally:
+            if worker is not None:
+                inq.put(None)
+                worker.join()
         wall = now()
         recs = [records[r["id"]] for r in requests]  # in the caller's order
         done = [x for x in recs if not x.get("rejected")]
