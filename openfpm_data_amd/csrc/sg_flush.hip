@@ -562,15 +562,18 @@ __global__ __launch_bounds__(kWG) void fold_points_compact_kernel(const int64_t 
     out[F.moff] = (uint8_t)(om | am);
 }
 
-// One wave per touched chunk j: the chunk is built in LDS (old contents, or mask 0 /
-// background when new), the folded results of its unique points are placed at their
-// slots, and the whole chunk is written once, contiguously, at its final position.
+// One wave per touched chunk j, one array at a time (each property, then the mask): the
+// array's chunk is built in LDS (old contents, or the background / mask 0 when new), the
+// folded results of the chunk's unique points are placed at their slots, and the array's
+// chunk is written once, contiguously, at its final position.  A wave holds one array's
+// chunk plus a 1 KB stage of its first 64 results in LDS, so more waves fit a CU than
+// with the whole chunk image.
 __global__ __launch_bounds__(256) void write_chunks_kernel(const int64_t *__restrict__ ukey,
                                                            const uint32_t *__restrict__ ustart,
                                                            const int64_t *__restrict__ cpos,
                                                            const int64_t *__restrict__ oldpos,
                                                            const uint32_t *__restrict__ isnew, int64_t nnew, int csz,
-                                                           int waves, FoldProps F, FusedImg I,
+                                                           int waves, int img_bytes, FoldProps F, FusedImg I,
                                                            const unsigned char *__restrict__ res,
                                                            uint8_t *__restrict__ omask)
 {
@@ -578,46 +581,50 @@ __global__ __launch_bounds__(256) void write_chunks_kernel(const int64_t *__rest
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t j = (int64_t)blockIdx.x * waves + w;
     if (w >= waves || j >= nnew) return;  // waves are independent: no workgroup barrier
-    unsigned char *img = smem_img + (size_t)w * (I.bytes + 1024);
-    unsigned char *stage = img + I.bytes;  // 16-byte results of the lanes' first points
+    unsigned char *img = smem_img + (size_t)w * (img_bytes + 1024);
+    unsigned char *stage = img + img_bytes;  // 16-byte results of the lanes' first points
     const bool fresh = isnew[j] != 0;
     const int64_t src = fresh ? 0 : oldpos[j];
+    const int64_t dst = cpos[j];
     const uint32_t u0 = ustart[j], u1 = ustart[j + 1];
     // the lane's first point (most chunks have fewer than 64): its key and, for 16-byte
-    // results, the result are loaded before the image is initialised, so their latency
-    // overlaps it
+    // results, the result are loaded once, before the first array's chunk is initialised
     const uint32_t ua = u0 + lane;
     const bool has = ua < u1, r16 = F.stride == 16;
     const int64_t keya = has ? ukey[ua] : 0;
     uint4 ra = make_uint4(0, 0, 0, 0);
     if (has && r16) ra = *(const uint4 *)(res + (size_t)ua * 16);
-    for (int q = 0; q < F.np; ++q)
-        image_in(img + I.off[q], fresh ? nullptr : I.src[q] + src * csz * F.ts[q], csz * F.ts[q], I.pat[q], lane);
-    image_in(img + I.moff, fresh ? nullptr : I.smask + src * csz, csz, 0, lane);
     if (r16) *(uint4 *)(stage + lane * 16) = ra;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t u = ua; u < u1; u += 64) {
-        const int slot = (int)(((u == ua) ? keya : ukey[u]) % csz);
-        const unsigned char *r = (u == ua && r16) ? stage + lane * 16 : res + (size_t)u * F.stride;
-        for (int q = 0; q < F.np; ++q) {
-            unsigned char *d = img + I.off[q] + slot * F.ts[q];
-            switch (F.ts[q]) {
-            case 8: *(uint64_t *)d = *(const uint64_t *)(r + F.off[q]); break;
-            case 4: *(uint32_t *)d = *(const uint32_t *)(r + F.off[q]); break;
-            case 2: *(uint16_t *)d = *(const uint16_t *)(r + F.off[q]); break;
-            default: *d = r[F.off[q]]; break;
+    for (int a = 0; a <= F.np; ++a) {
+        const bool m = a == F.np;  // the mask
+        const int ts = m ? 1 : F.ts[a];
+        const int bytes = csz * ts;
+        const int off = m ? F.moff : F.off[a];
+        image_in(img, fresh ? nullptr : (m ? I.smask + src * csz : I.src[a] + src * bytes), bytes, m ? 0 : I.pat[a],
+                 lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t u = ua; u < u1; u += 64) {
+            const int slot = (int)(((u == ua) ? keya : ukey[u]) % csz);
+            const unsigned char *r = ((u == ua && r16) ? stage + lane * 16 : res + (size_t)u * F.stride) + off;
+            unsigned char *d = img + slot * ts;
+            switch (ts) {
+            case 8: *(uint64_t *)d = *(const uint64_t *)r; break;
+            case 4: *(uint32_t *)d = *(const uint32_t *)r; break;
+            case 2: *(uint16_t *)d = *(const uint16_t *)r; break;
+            default: *d = *r; break;
             }
         }
-        img[I.moff + slot] = r[F.moff];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        image_out(m ? omask + dst * csz : F.out[a] + dst * bytes, img, bytes, lane);
+        // the image is initialised again for the next array
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const int64_t dst = cpos[j];
-    for (int q = 0; q < F.np; ++q) image_out(F.out[q] + dst * csz * F.ts[q], img + I.off[q], csz * F.ts[q], lane);
-    image_out(omask + dst * csz, img + I.moff, csz, lane);
 }
 
 __global__ void uchunk_index_kernel(const uint32_t *hscan, const uint32_t *head, int64_t nu, uint32_t *uchunk)
@@ -1464,10 +1471,12 @@ int flush_impl(sg_grid *g, const int32_t *ops)
         hipLaunchKernelGGL(fold_points_compact_kernel, dim3(grid_1d(nu, kWG)), dim3(kWG), 0, g->stream, ukey, rstart,
                            uchunk, df.oldpos, df.isnew, nu, csz, F, I, res);
         SG_HIP(hipGetLastError());
-        const int waves = std::max(1, std::min(4, 65536 / (I.bytes + 1024)));
+        int img_bytes = csz;  // the largest array's chunk
+        for (int p = 0; p < g->np; ++p) img_bytes = std::max(img_bytes, csz * g->ts[p]);
+        const int waves = std::max(1, std::min(4, 65536 / (img_bytes + 1024)));
         hipLaunchKernelGGL(write_chunks_kernel, dim3((unsigned)((nnew + waves - 1) / waves)), dim3(64 * waves),
-                           (size_t)waves * (I.bytes + 1024), g->stream, ukey, ustart, cpos, df.oldpos, df.isnew, nnew, csz,
-                           waves, F, I, (const unsigned char *)res, g->d_mask);
+                           (size_t)waves * (img_bytes + 1024), g->stream, ukey, ustart, cpos, df.oldpos, df.isnew,
+                           nnew, csz, waves, img_bytes, F, I, (const unsigned char *)res, g->d_mask);
         SG_HIP(hipGetLastError());
         finish_merge(g, df);
     } else {
