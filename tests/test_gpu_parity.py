@@ -525,6 +525,48 @@ def test_tag_boundaries_chunks_per_wave(G, oracle, cpw, nn):
     assert grids[0].count_boundary_elements() == grids[1].count_boundary_elements() > 0
 
 
+def test_new_grids_reuse_blocks_of_destroyed_ones(G, oracle):
+    """Grids built one after another, each on a stream of its own and destroyed before
+    the next (which then allocates from the blocks the library kept): flush, NNFull
+    tagBoundaries and a Gray-Scott step match the oracle every time; sg_trim then frees
+    the kept blocks."""
+    import torch
+    lib = C.load()
+    for rep in range(3):
+        rng = np.random.default_rng(300 + rep)
+        size = [int(rng.integers(40, 70)) for _ in range(3)]
+        dtypes = [np.float32, np.float32, np.float32, np.float32]
+        steps = S.random_flush_program(30 + rep, 3, 8, size, dtypes, nflush=2, nrec=3000,
+                                       ops=["smax", "smax", "smax", "smax"])
+        grids = []
+        for cls in (G, oracle.SparseGridCpu):
+            g = cls(3, size, dtypes, block_edge=8)
+            if cls is G:
+                g.set_stream(torch.cuda.Stream())
+            for recs, ops in steps:
+                for kind, a, m, v in recs:
+                    if kind == "points":
+                        g.insert(a, v, m)
+                    else:
+                        g.insert_block(a, m, v)
+                g.flush(ops)
+            g.set_nn_type(C.NN_FULL)
+            g.tag_boundaries(C.NN_FULL)
+            grids.append(g)
+        a, b = grids
+        S.assert_same_state(a, b)
+        for p in range(4):
+            b.set_prop(p, a.prop(p))  # PADDING-only slots have no specified value after a flush
+        for g in grids:
+            g.conv2_gray_scott(0, 1, 2, 3, **W.GS_PARAMS)
+        S.assert_same_state(a, b)
+        grids[0].close()
+        grids[1].close()
+    g = G(3, (16, 16, 16), [np.float32], block_edge=8)
+    C.check(lib.sg_trim(g._h))
+    g.close()
+
+
 def test_flush_rejects_device_keys_outside_grid(G):
     """Point keys handed over in device memory are checked by the flush: a key beyond the
     grid fails it (SgError), drops the pending records and leaves the grid unchanged."""

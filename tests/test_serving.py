@@ -64,9 +64,11 @@ def test_request_generator_shapes():
 
 
 @pytest.mark.gpu
-def test_served_results_match_alone():
+@pytest.mark.parametrize("build_thread", [True, False])
+def test_served_results_match_alone(build_thread):
     """Requests served with overlapping lifetimes under a budget that forces queueing give
-    bitwise the same solutions as each solved alone."""
+    bitwise the same solutions as each solved alone (grids built by the builder thread or
+    inline)."""
     import torch
     from openfpm_data_amd import poisson as P
 
@@ -78,7 +80,7 @@ def test_served_results_match_alone():
                      "f": rng.uniform(-1, 1, (ids.size, 512)).astype(np.float32),
                      "tol": 1e-10 if i % 2 == 0 else 1e-6})
     biggest = max(S.resident_bytes(len(r["chunk_ids"])) + S.build_bytes(len(r["chunk_ids"])) for r in reqs)
-    srv = S.PoissonServer(budget_bytes=3 * biggest, keep_solutions=True)
+    srv = S.PoissonServer(budget_bytes=3 * biggest, keep_solutions=True, build_thread=build_thread)
     arrivals = np.arange(8) * 0.002
     recs, summ = srv.run(reqs, arrivals)
     srv.close()
