@@ -42,6 +42,9 @@
 #include <vector>
 
 #include "sparsegrid_gpu.h"
+#if defined(__HIPCC__)
+#include "SparseGridGpu_conv.hpp"
+#endif
 
 namespace sg {
 
@@ -591,6 +594,94 @@ public:
         sg_stencil_desc d = Stencil::desc(box, args...);
         sg::check(sg_apply_stencil(h_, &d));
     }
+
+#if defined(__HIPCC__)
+    // -- conv family with user __device__ lambdas (SparseGridGpu.hpp:2568-2683), for
+    //    translation units compiled by hipcc: include/SparseGridGpu_conv.hpp.  The box
+    //    [start, stop] is inclusive; stencil_size is the ghost width the lambda may reach.
+    // res = func(cpb, i, j[, k])
+    template <unsigned int prop_src, unsigned int prop_dst, unsigned int stencil_size, typename lambda_f,
+              typename CoordT, typename... ArgsT>
+    void conv(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop, lambda_f func, ArgsT...)
+    {
+        static_assert(prop_dst != prop_src, "conv: the destination must not be the source");
+        run_conv<stencil_size, 0, prop_src, prop_src, prop_src, prop_dst, prop_dst, prop_dst>(start, stop, func);
+    }
+    // res = func(cur, cross_stencil) (xm[d] / xp[d]: the face neighbours), extra arguments
+    // appended
+    template <unsigned int prop_src, unsigned int prop_dst, unsigned int stencil_size, typename lambda_f,
+              typename CoordT, typename... ArgsT>
+    void conv_cross(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop, lambda_f func,
+                    ArgsT... args)
+    {
+        static_assert(prop_dst != prop_src, "conv_cross: the destination must not be the source");
+        run_conv<stencil_size, 1, prop_src, prop_src, prop_src, prop_dst, prop_dst, prop_dst>(
+            start, stop, sg::conv_detail::with_args<lambda_f, ArgsT...>{func, {args...}});
+    }
+    // res = func(cpb, block, offset, i, j[, k]); block.template get<p>()[offset] reads the
+    // chunk's own data of any property
+    template <unsigned int prop_src, unsigned int prop_dst, unsigned int stencil_size, typename lambda_f,
+              typename CoordT, typename... ArgsT>
+    void conv_cross_b(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop, lambda_f func,
+                      ArgsT...)
+    {
+        static_assert(prop_dst != prop_src, "conv_cross_b: the destination must not be the source");
+        run_conv<stencil_size, 2, prop_src, prop_src, prop_src, prop_dst, prop_dst, prop_dst>(start, stop, func);
+    }
+    // func(res1, res2, cpb1, cpb2, i, j[, k])
+    template <unsigned int prop_src1, unsigned int prop_src2, unsigned int prop_dst1, unsigned int prop_dst2,
+              unsigned int stencil_size, typename lambda_f, typename CoordT, typename... ArgsT>
+    void conv2(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop, lambda_f func, ArgsT...)
+    {
+        static_assert(prop_dst1 != prop_src1 && prop_dst1 != prop_src2 && prop_dst2 != prop_src1 &&
+                          prop_dst2 != prop_src2 && prop_dst1 != prop_dst2,
+                      "conv2: the destinations must differ from each other and from the sources");
+        run_conv<stencil_size, 3, prop_src1, prop_src2, prop_src2, prop_dst1, prop_dst2, prop_dst2>(start, stop, func);
+    }
+    // func(res1, res2, cpb1, cpb2, block, offset, i, j[, k])
+    template <unsigned int prop_src1, unsigned int prop_src2, unsigned int prop_dst1, unsigned int prop_dst2,
+              unsigned int stencil_size, typename lambda_f, typename CoordT, typename... ArgsT>
+    void conv2_b(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop, lambda_f func,
+                 ArgsT...)
+    {
+        static_assert(prop_dst1 != prop_src1 && prop_dst1 != prop_src2 && prop_dst2 != prop_src1 &&
+                          prop_dst2 != prop_src2 && prop_dst1 != prop_dst2,
+                      "conv2_b: the destinations must differ from each other and from the sources");
+        run_conv<stencil_size, 4, prop_src1, prop_src2, prop_src2, prop_dst1, prop_dst2, prop_dst2>(start, stop, func);
+    }
+    // func(res1, res2, res3, cpb1, cpb2, cpb3, block, offset, i, j[, k])
+    template <unsigned int prop_src1, unsigned int prop_src2, unsigned int prop_src3, unsigned int prop_dst1,
+              unsigned int prop_dst2, unsigned int prop_dst3, unsigned int stencil_size, typename lambda_f,
+              typename CoordT, typename... ArgsT>
+    void conv3_b(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop, lambda_f func,
+                 ArgsT...)
+    {
+        static_assert(prop_dst1 != prop_src1 && prop_dst1 != prop_src2 && prop_dst1 != prop_src3 &&
+                          prop_dst2 != prop_src1 && prop_dst2 != prop_src2 && prop_dst2 != prop_src3 &&
+                          prop_dst3 != prop_src1 && prop_dst3 != prop_src2 && prop_dst3 != prop_src3 &&
+                          prop_dst1 != prop_dst2 && prop_dst1 != prop_dst3 && prop_dst2 != prop_dst3,
+                      "conv3_b: the destinations must differ from each other and from the sources");
+        run_conv<stencil_size, 5, prop_src1, prop_src2, prop_src3, prop_dst1, prop_dst2, prop_dst3>(start, stop, func);
+    }
+
+private:
+    template <unsigned int R, int MODE, unsigned int s1, unsigned int s2, unsigned int s3, unsigned int d1,
+              unsigned int d2, unsigned int d3, typename CoordT, typename F>
+    void run_conv(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop, F f)
+    {
+        static_assert(s1 < nprops && s2 < nprops && s3 < nprops && d1 < nprops && d2 < nprops && d3 < nprops,
+                      "property index out of range");
+        int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (unsigned int d = 0; d < dim; ++d) {
+            lo[d] = (int64_t)start.get(d);
+            hi[d] = (int64_t)stop.get(d);
+        }
+        sg::check(sg::conv_detail::launch<dim, blockEdgeSize, R, MODE, props_tuple, s1, s2, s3, d1, d2, d3>(h_, lo, hi,
+                                                                                                          f));
+    }
+
+public:
+#endif
 
     // -- access --
     template <unsigned int p, typename CoordT> prop_type<p> get(const grid_key_dx<dim, CoordT> &coord) const

@@ -306,6 +306,25 @@ int sg_get_points(sg_grid *g, int prop, int64_t n, const int64_t *keys,
 int sg_get_neighbours(sg_grid *g, int nn_type, int32_t *out, int mem);
 /* device address of property p's array (prop == -1: the mask array) */
 int sg_device_ptr(sg_grid *g, int prop, void **out);
+
+/* SparseGridGpu::toKernel() (src/SparseGridGpu/SparseGridGpu.hpp:1716-1751): the grid's
+ * device arrays, for kernels compiled by the caller (the facade's conv family,
+ * include/SparseGridGpu_conv.hpp).  Builds the nn_type neighbour table when it is stale.
+ * The pointers stay valid until the next call that changes the chunk set or a property
+ * array (flush, remove, unpack, a stencil writing into one of its sources, ...); work on
+ * them belongs on `stream`, the handle's stream. */
+typedef struct sg_kernel_view {
+    const int64_t *keys;   /* sorted chunk ids [nchunks] */
+    uint8_t *mask;         /* [(nchunks + 1) * chunk_size]; chunk nchunks is the background */
+    void *prop[16];        /* property arrays, [(nchunks + 1) * chunk_size] each */
+    const int32_t *nn;     /* neighbour table [nchunks][nnn] (positions; nchunks = missing) */
+    void *stream;          /* hipStream_t of the handle */
+    int64_t nchunks;
+    int64_t size[3];       /* domain size in points */
+    int64_t bsz[3];        /* domain size in chunks */
+    int32_t nnn, dim, edge, chunk_size, zorder, nprops;
+} sg_kernel_view;
+int sg_to_kernel(sg_grid *g, int nn_type, sg_kernel_view *out);
 /* countExistingElements / countBoundaryElements (src/SparseGridGpu/SparseGridGpu.hpp:2422-2490) */
 int sg_count_existing(sg_grid *g, int64_t *out);
 int sg_count_boundary(sg_grid *g, int64_t *out);

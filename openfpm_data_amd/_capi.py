@@ -166,6 +166,7 @@ _SIGS = {
     "sg_get_points": (_INT, [_P, _INT, _I64, _P, _P, _INT]),
     "sg_get_neighbours": (_INT, [_P, _INT, _P, _INT]),
     "sg_device_ptr": (_INT, [_P, _INT, ctypes.POINTER(_P)]),
+    "sg_to_kernel": (_INT, [_P, _INT, _P]),
     "sg_count_existing": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_count_boundary": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_pack_size": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),
@@ -193,6 +194,14 @@ _SIGS = {
 }
 
 _lib = None
+
+
+class KernelView(ctypes.Structure):
+    """sg_kernel_view (the grid's device arrays, filled by sg_to_kernel)."""
+    _fields_ = [("keys", _P), ("mask", _P), ("prop", _P * 16), ("nn", _P), ("stream", _P),
+                ("nchunks", _I64), ("size", _I64 * 3), ("bsz", _I64 * 3),
+                ("nnn", ctypes.c_int32), ("dim", ctypes.c_int32), ("edge", ctypes.c_int32),
+                ("chunk_size", ctypes.c_int32), ("zorder", ctypes.c_int32), ("nprops", ctypes.c_int32)]
 
 
 def exported_symbols():

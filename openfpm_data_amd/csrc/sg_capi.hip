@@ -504,6 +504,34 @@ int sg_device_ptr(sg_grid *g, int prop, void **out)
     return SG_OK;
 }
 
+int sg_to_kernel(sg_grid *g, int nn_type, sg_kernel_view *out)
+{
+    if (!g || !out) return fail(SG_E_ARG, "null argument");
+    if (nn_type != SG_NN_STAR && nn_type != SG_NN_FULL) return fail(SG_E_ARG, "bad nn type");
+    if (!g->nn_valid[nn_type]) {
+        const int rc = find_neighbours_impl(g, nn_type);
+        if (rc) return rc;
+    }
+    memset(out, 0, sizeof(*out));
+    out->keys = g->d_keys;
+    out->mask = g->d_mask;
+    for (int p = 0; p < g->np; ++p) out->prop[p] = g->d_prop[p];
+    out->nn = g->d_nn[nn_type];
+    out->stream = (void *)g->stream;
+    out->nchunks = g->nchunks;
+    for (int d = 0; d < 3; ++d) {
+        out->size[d] = d < g->dim ? g->sz[d] : 1;
+        out->bsz[d] = d < g->dim ? g->bsz[d] : 1;
+    }
+    out->nnn = g->nnn(nn_type);
+    out->dim = g->dim;
+    out->edge = g->edge;
+    out->chunk_size = g->csz;
+    out->zorder = g->lin == SG_LIN_ZMB;
+    out->nprops = g->np;
+    return SG_OK;
+}
+
 int sg_count_existing(sg_grid *g, int64_t *out)
 {
     if (!g || !out) return fail(SG_E_ARG, "null argument");

@@ -41,3 +41,22 @@ def test_stencil_desc_layout(tmp_path):
     assert C.StencilDesc.use_box.offset == ub
     assert C.StencilDesc.coef.offset == co
     assert C.StencilDesc.icoef.offset == ic
+
+
+def test_kernel_view_layout(tmp_path):
+    """The ctypes mirror of sg_kernel_view has the C compiler's size and offsets."""
+    src = tmp_path / "kv.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "sparsegrid_gpu.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(sg_kernel_view),'
+        ' offsetof(sg_kernel_view, nn), offsetof(sg_kernel_view, nchunks),'
+        ' offsetof(sg_kernel_view, nnn), offsetof(sg_kernel_view, nprops)); return 0;}\n')
+    exe = tmp_path / "kv"
+    import subprocess
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    size, nn, nch, nnn, npr = map(int, subprocess.check_output([str(exe)]).split())
+    assert ctypes.sizeof(C.KernelView) == size
+    assert C.KernelView.nn.offset == nn
+    assert C.KernelView.nchunks.offset == nch
+    assert C.KernelView.nnn.offset == nnn
+    assert C.KernelView.nprops.offset == npr

@@ -567,6 +567,23 @@ def test_new_grids_reuse_blocks_of_destroyed_ones(G, oracle):
     g.close()
 
 
+def test_to_kernel_view(G):
+    """sg_to_kernel (toKernel): the device arrays of the grid, the full neighbour table it
+    builds when stale, and the geometry."""
+    pts = W.shell_points(40, 8.0, 14.0)
+    g = G(3, (40, 40, 40), [np.float32, np.int32], block_edge=8)
+    g.insert_coords(pts, [np.ones(len(pts), np.float32), np.ones(len(pts), np.int32)])
+    g.flush(["smax", "smax"])
+    v = C.KernelView()
+    C.check(C.load().sg_to_kernel(g._h, C.NN_FULL, ctypes.byref(v)))
+    assert v.nchunks == g.num_chunks() and v.nnn == 27 and v.dim == 3 and v.edge == 8
+    assert v.chunk_size == 512 and v.nprops == 2 and v.zorder == 0
+    assert list(v.size) == [40, 40, 40] and list(v.bsz) == [5, 5, 5]
+    assert v.prop[0] == g.device_ptr(0) and v.prop[1] == g.device_ptr(1) and v.mask == g.device_ptr(-1)
+    assert v.keys and v.nn
+    g.close()
+
+
 def test_flush_rejects_device_keys_outside_grid(G):
     """Point keys handed over in device memory are checked by the flush: a key beyond the
     grid fails it (SgError), drops the pending records and leaves the grid unchanged."""
