@@ -22,7 +22,10 @@
  *   - insertion is host-batched (insertPoints / insertBlocks) instead of user __global__
  *     kernels calling insert<p>() on toKernel();
  *   - applyStencils takes the named stencils of this header (descriptors evaluated by
- *     library kernels), not arbitrary __device__ functors or lambdas;
+ *     library kernels), not arbitrary __device__ functors; user __device__ lambdas go
+ *     through conv / conv2 / conv_cross / conv_cross_b / conv2_b / conv3_b, available in
+ *     translation units compiled by hipcc (include/SparseGridGpu_conv.hpp, over the
+ *     toKernel view sg_to_kernel);
  *   - failures throw sg::error (the reference prints and continues).
  */
 #ifndef SPARSEGRIDGPU_FACADE_HPP
