@@ -72,11 +72,11 @@ template <typename F, typename... A> struct with_args {
     F f;
     std::tuple<A...> a;
     template <typename T, typename CS, size_t... I>
-    __device__ T call(T cur, const CS &cs, std::index_sequence<I...>) const
+    __device__ auto call(T &cur, CS &cs, std::index_sequence<I...>) const
     {
         return f(cur, cs, std::get<I>(a)...);
     }
-    template <typename T, typename CS> __device__ T operator()(T cur, const CS &cs) const
+    template <typename T, typename CS> __device__ auto operator()(T &cur, CS &cs) const
     {
         return call(cur, cs, std::index_sequence_for<A...>());
     }
@@ -201,7 +201,7 @@ __global__ void conv_kernel(ConvArgs a, F f)
     T1 *o1 = (T1 *)a.prop[d1];
     if constexpr (MODE == 1) {
         if (p.m == 0) return;
-        const T1 cur = (dim == 3) ? cpb1(p.i, p.j, p.k) : cpb1(p.i, p.j);
+        T1 cur = (dim == 3) ? cpb1(p.i, p.j, p.k) : cpb1(p.i, p.j);
         T1 res = cur;
         if (p.active) {
             cross_stencil<dim, T1> cs;
@@ -288,5 +288,13 @@ inline int launch(sg_grid *h, const int64_t (&lo)[3], const int64_t (&hi)[3], F 
 
 }  // namespace conv_detail
 }  // namespace sg
+
+// the reference's names in user code: cross_stencil<dim, T> and
+// GetCpBlockType<grid, prop, stencil_size>::type (the type of the lambdas' block argument)
+using sg::cross_stencil;
+template <typename SG, unsigned int prop, unsigned int stencil_size> struct GetCpBlockType {
+    typedef sg::conv_detail::cp_block<typename SG::template prop_type<prop>, SG::dims, SG::blockEdgeSize_, stencil_size>
+        type;
+};
 
 #endif  // SPARSEGRIDGPU_CONV_HPP

@@ -317,8 +317,142 @@ static void testConv2D()
     REQUIRE(n > 1000);
 }
 
+// ---- the reference's own conv tests (SparseGridGpu_tests.cu:519-792), restated ----
+
+// 16 x 16 domain fully inserted (chunks of 8 x 8), star tagBoundaries; the boundary
+// (padding) points of property q get vmax[q] * x / 15 (BoundaryStencilSetXRescaled), the
+// others 0
+typedef SparseGridGpu<2, aggregate<float, float, float, float>, 8, 64> Grid2;
+static void build_square(Grid2 &sg, const float (&vmax)[4])
+{
+    gpu::ofp_context_t ctx;
+    sg.setBackgroundValue<0>(0);
+    std::vector<long> keys;
+    std::vector<float> zero(256, 0.0f);
+    for (int y = 0; y < 16; ++y)
+        for (int x = 0; x < 16; ++x) keys.push_back(sg.getLinId(grid_key_dx<2, int>({x, y})));
+    const void *v[] = {zero.data(), zero.data(), zero.data(), zero.data()};
+    sg.insertPoints(keys.data(), keys.size(), v);
+    sg.flush<smax_<0>, smax_<1>, smax_<2>, smax_<3>>(ctx, FLUSH_ON_DEVICE);
+    sg.findNeighbours();
+    sg.tagBoundaries(ctx);
+    const std::vector<long> ids = sg.private_get_index_array();
+    const std::vector<uint8_t> m = sg.private_get_mask_array();
+    std::vector<float> u[4];
+    for (int q = 0; q < 4; ++q) u[q].assign(m.size(), 0.0f);
+    for (size_t i = 0; i < m.size(); ++i) {
+        auto c = sg.getCoord(ids[i / 64] * 64 + (long)(i % 64));
+        if (m[i] & PADDING)
+            for (int q = 0; q < 4; ++q) u[q][i] = vmax[q] * c.get(0) / 15.0f;
+    }
+    sg.hostToDevice<0>(u[0]);
+    sg.hostToDevice<1>(u[1]);
+    sg.hostToDevice<2>(u[2]);
+    sg.hostToDevice<3>(u[3]);
+}
+
+// steady state of 1000 Jacobi pairs: linear in x within 1e-2
+static void require_linear(Grid2 &sg, int q, float vmax)
+{
+    const std::vector<long> ids = sg.private_get_index_array();
+    std::vector<float> u;
+    if (q == 0) u = sg.private_get_data_array<0>();
+    else u = sg.private_get_data_array<1>();
+    bool match = true;
+    for (size_t i = 0; i < u.size(); ++i) {
+        auto c = sg.getCoord(ids[i / 64] * 64 + (long)(i % 64));
+        match &= std::fabs(u[i] - vmax * c.get(0) / 15.0f) < 1e-2f;
+    }
+    REQUIRE(match);
+}
+
+// testStencil_lap_simplified (:519-568): conv_cross Jacobi heat; only property 0 carries
+// the boundary values, the block store of conv_cross copies them to property 1
+static void testRefLapCross()
+{
+    const size_t sz[2] = {16, 16};
+    Grid2 sg(sz);
+    build_square(sg, {10.0f, 0.0f, 0.0f, 0.0f});
+    for (int it = 0; it < 1000; ++it) {
+        sg.conv_cross<0, 1, 1>(grid_key_dx<2, int>({0, 0}), grid_key_dx<2, int>({16, 16}),
+                               [] __device__(float &u, cross_stencil<2, float> &cs) {
+                                   return u + (cs.xm[0] + cs.xp[0] + cs.xm[1] + cs.xp[1] - 4.0 * u) * 0.1;
+                               });
+        sg.conv_cross<1, 0, 1>(grid_key_dx<2, int>({0, 0}), grid_key_dx<2, int>({16, 16}),
+                               [] __device__(float &u, cross_stencil<2, float> &cs) {
+                                   return u + (cs.xm[0] + cs.xp[0] + cs.xm[1] + cs.xp[1] - 4.0 * u) * 0.1;
+                               });
+    }
+    require_linear(sg, 0, 10.0f);
+}
+
+// testStencil_lap_no_cross_simplified (:570-645): conv with cp_block taps
+static void testRefLapConv()
+{
+    const size_t sz[2] = {16, 16};
+    Grid2 sg(sz);
+    build_square(sg, {10.0f, 10.0f, 0.0f, 0.0f});
+    typedef GetCpBlockType<Grid2, 0, 1>::type CpBlockType;
+    for (int it = 0; it < 1000; ++it) {
+        sg.conv<0, 1, 1>(grid_key_dx<2, int>({0, 0}), grid_key_dx<2, int>({16, 16}),
+                         [] __device__(CpBlockType & u, int i, int j) {
+                             float c = u(i, j);
+                             return c + (u(i - 1, j) + u(i + 1, j) + u(i, j - 1) + u(i, j + 1) - 4.0 * c) * 0.1;
+                         });
+        sg.conv<1, 0, 1>(grid_key_dx<2, int>({0, 0}), grid_key_dx<2, int>({16, 16}),
+                         [] __device__(CpBlockType & u, int i, int j) {
+                             float c = u(i, j);
+                             return c + (u(i - 1, j) + u(i + 1, j) + u(i, j - 1) + u(i, j + 1) - 4.0 * c) * 0.1;
+                         });
+    }
+    require_linear(sg, 0, 10.0f);
+}
+
+// testStencil_lap_no_cross_simplified2 (:647-736): conv2 on two fields at once
+static void testRefLapConv2()
+{
+    const size_t sz[2] = {16, 16};
+    Grid2 sg(sz);
+    build_square(sg, {10.0f, 5.0f, 10.0f, 5.0f});
+    typedef GetCpBlockType<Grid2, 0, 1>::type CpBlockType;
+    auto step = [] __device__(float &u_out, float &v_out, CpBlockType &u, CpBlockType &v, int i, int j) {
+        float cu = u(i, j);
+        float cv = v(i, j);
+        u_out = cu + (u(i - 1, j) + u(i + 1, j) + u(i, j - 1) + u(i, j + 1) - 4.0 * cu) * 0.1;
+        v_out = cv + (v(i - 1, j) + v(i + 1, j) + v(i, j - 1) + v(i, j + 1) - 4.0 * cv) * 0.1;
+    };
+    for (int it = 0; it < 1000; ++it) {
+        sg.conv2<0, 1, 2, 3, 1>(grid_key_dx<2, int>({0, 0}), grid_key_dx<2, int>({16, 16}), step);
+        sg.conv2<2, 3, 0, 1, 1>(grid_key_dx<2, int>({0, 0}), grid_key_dx<2, int>({16, 16}), step);
+    }
+    require_linear(sg, 0, 10.0f);
+    require_linear(sg, 1, 5.0f);
+}
+
+// testStencil_lap_no_cross_simplified_subset (:738-792): conv restricted to [3, 11]^2
+static void testRefConvSubset()
+{
+    const size_t sz[2] = {16, 16};
+    Grid2 sg(sz);
+    build_square(sg, {0.0f, 0.0f, 0.0f, 0.0f});
+    typedef GetCpBlockType<Grid2, 0, 1>::type CpBlockType;
+    sg.conv<0, 1, 1>(grid_key_dx<2, int>({3, 3}), grid_key_dx<2, int>({11, 11}),
+                     [] __device__(CpBlockType &, int, int) { return 5.0; });
+    bool match = true;
+    for (long i = 0; i < 64 * 4; ++i) {
+        auto coord = sg.getCoord(i);
+        const bool in = coord.get(0) >= 3 && coord.get(1) >= 3 && coord.get(0) <= 11 && coord.get(1) <= 11;
+        match &= sg.get<1>(coord) == (in ? 5.0f : 0.0f);
+    }
+    REQUIRE(match);
+}
+
 int main()
 {
+    testRefLapCross();
+    testRefLapConv();
+    testRefLapConv2();
+    testRefConvSubset();
     testConvLaplacian<SparseGridGpu<3, Props6>>();
     testConvLaplacian<SparseGridGpu_z<3, Props6>>();
     testConvCross();
