@@ -46,7 +46,10 @@
 
 #include "sparsegrid_gpu.h"
 #if defined(__HIPCC__)
-#include "SparseGridGpu_conv.hpp"
+#include <hip/hip_runtime.h>
+#define SG_HD __host__ __device__
+#else
+#define SG_HD
 #endif
 
 namespace sg {
@@ -86,16 +89,16 @@ template <unsigned int dim, typename T = long int> class grid_key_dx
     T k[dim];
 
 public:
-    grid_key_dx() : k{} {}
-    grid_key_dx(std::initializer_list<T> l)
+    SG_HD grid_key_dx() : k{} {}
+    SG_HD grid_key_dx(std::initializer_list<T> l)
     {
         unsigned int i = 0;
         for (T v : l)
             if (i < dim) k[i++] = v;
         for (; i < dim; ++i) k[i] = 0;
     }
-    T get(unsigned int i) const { return k[i]; }
-    void set_d(unsigned int i, T v) { k[i] = v; }
+    SG_HD T get(unsigned int i) const { return k[i]; }
+    SG_HD void set_d(unsigned int i, T v) { k[i] = v; }
 };
 
 template <unsigned int dim, typename T> class Box
@@ -154,6 +157,10 @@ template <unsigned int dim> struct NNFull {
     static constexpr int type = SG_NN_FULL;
     static constexpr int nNN = (dim == 2) ? 9 : 27;
 };
+
+#if defined(__HIPCC__)
+#include "SparseGridGpu_conv.hpp"
+#endif
 
 // ---- named stencils for applyStencils ----------------------------------------------------
 
@@ -536,7 +543,11 @@ public:
 
     // -- background / insertion / flush --
     template <unsigned int p> void setBackgroundValue(prop_type<p> v) { sg::check(sg_set_background(h_, (int)p, &v)); }
-    template <typename dim3T> void setGPUInsertBuffer(dim3T, dim3T) {}  // the insert buffer grows on demand
+#if !defined(__HIPCC__)
+    // host inserts only: the insert buffer grows on demand (the device insert buffer of
+    // hipcc translation units is below)
+    template <typename dim3T> void setGPUInsertBuffer(dim3T, dim3T) {}
+#endif
     void preFlush() {}
 
     // point inserts (insert<p>(coord) once per point): keys are getLinId values, one
@@ -561,6 +572,7 @@ public:
         int dummy[] = {0, (ops[v_reduce::prop] = v_reduce::op, 0)...};
         (void)dummy;
         sg::check(sg_flush(h_, ops));
+        ins_view_ok_ = false;  // the flush took the records of the kernels' insert buffer
     }
 
     // insertFlush<p>(coord) = value: insert one point and flush it immediately
@@ -599,6 +611,26 @@ public:
     }
 
 #if defined(__HIPCC__)
+    // -- device inserts (hipcc translation units) --
+    // setGPUInsertBuffer (SparseGridGpu.hpp:2270-2278): room for nBlock x nSlot chunk
+    // records (dim3 or counts) claimed by kernels that get toKernel(), until the next flush
+    template <typename B3, typename S3> void setGPUInsertBuffer(B3 nBlock, S3 nSlot)
+    {
+        sg::check(sg_insert_device_end(h_));  // records of an earlier buffer join the inserts
+        sg::check(sg_insert_device_begin(h_, count3(nBlock) * count3(nSlot), &ins_view_));
+        ins_view_ok_ = true;
+    }
+    // toKernel() (SparseGridGpu.hpp:1716-1751): the device view for insertBlock / insert<p>
+    sg::sparse_grid_gpu_ker<dim, blockEdgeSize, props_tuple, indexT> toKernel() const
+    {
+        if (!ins_view_ok_) throw sg::error("toKernel: call setGPUInsertBuffer first (the device view inserts)");
+        sg::sparse_grid_gpu_ker<dim, blockEdgeSize, props_tuple, indexT> k;
+        k.iv = ins_view_;
+        for (unsigned int q = 0; q < 3; ++q) k.bsz[q] = q < dim ? (int64_t)bsz_[q] : 1;
+        k.zorder = zorder ? 1 : 0;
+        return k;
+    }
+
     // -- conv family with user __device__ lambdas (SparseGridGpu.hpp:2568-2683), for
     //    translation units compiled by hipcc: include/SparseGridGpu_conv.hpp.  The box
     //    [start, stop] is inclusive; stencil_size is the ghost width the lambda may reach.
@@ -668,6 +700,9 @@ public:
     }
 
 private:
+    static int64_t count3(const dim3 &d) { return (int64_t)d.x * d.y * d.z; }
+    static int64_t count3(int64_t n) { return n; }
+    sg_insert_view ins_view_ = {};
     template <unsigned int R, int MODE, unsigned int s1, unsigned int s2, unsigned int s3, unsigned int d1,
               unsigned int d2, unsigned int d3, typename CoordT, typename F>
     void run_conv(const grid_key_dx<dim, CoordT> &start, const grid_key_dx<dim, CoordT> &stop, F f)
@@ -1018,6 +1053,7 @@ private:
     sg_grid *h_ = nullptr;
     size_t size_[dim];
     size_t bsz_[dim];
+    bool ins_view_ok_ = false;
 };
 
 // SparseGridGpu_z (src/SparseGridGpu/SparseGridGpu.hpp:3947-3972): Morton chunk ids

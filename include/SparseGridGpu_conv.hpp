@@ -289,6 +289,127 @@ inline int launch(sg_grid *h, const int64_t (&lo)[3], const int64_t (&hi)[3], F 
 }  // namespace conv_detail
 }  // namespace sg
 
+namespace sg {
+
+// a claimed chunk record of a device insert (insertBlock): get<p>()[offset] is property p
+// of the record's slot `offset`, get<nprops>() the mask bytes
+template <typename Tuple> struct insert_encap {
+    static constexpr unsigned int np = std::tuple_size<Tuple>::value;
+    void *prop[16];
+    uint8_t *mask;
+    template <unsigned int p>
+    __device__ typename std::conditional<(p < np), typename std::tuple_element<(p < np ? p : 0), Tuple>::type,
+                                         uint8_t>::type *
+    get() const
+    {
+        typedef typename std::conditional<(p < np), typename std::tuple_element<(p < np ? p : 0), Tuple>::type,
+                                          uint8_t>::type E;
+        if constexpr (p < np) return (E *)prop[p];
+        else return (E *)mask;
+    }
+};
+
+// toKernel() (SparseGridGpu.hpp:1716-1751) for device inserts (BlockMapGpu_ker.cuh:171-227,
+// SparseGridGpu_ker.cuh:445-490): init / insertBlock / insert<p> / flush_block_insert.
+// insertBlock claims one chunk record per distinct chunk id among the calling lanes of a
+// wave (the reference shares one per CUDA block); the records are merged by the next
+// flush in claim order.
+template <unsigned int dim, unsigned int B, typename Tuple, typename indexT> struct sparse_grid_gpu_ker {
+    static constexpr unsigned int dims = dim;
+    static constexpr unsigned int d = dim;
+    static constexpr unsigned int blockEdgeSize_ = B;
+    static constexpr unsigned int pMask = std::tuple_size<Tuple>::value;
+    static constexpr unsigned int chunk_size = (dim == 3) ? B * B * B : B * B;
+    sg_insert_view iv;
+    int64_t bsz[3];
+    int zorder;
+
+    __device__ void init() const {}
+    __device__ void flush_block_insert() const {}
+    __device__ static constexpr unsigned int getBlockEdgeSize() { return B; }
+
+    // chunk id of chunk coordinates (grid_smb row-major or grid_zmb Morton)
+    __device__ indexT chunk_id(const int64_t (&c)[3]) const
+    {
+        if (zorder) {
+            uint64_t r = 0;
+            for (int i = 0; i < 64 / (int)dim; ++i)
+                for (unsigned int q = 0; q < dim; ++q) r |= (uint64_t)((c[q] >> i) & 1) << (dim * i + q);
+            return (indexT)r;
+        }
+        int64_t v = c[dim - 1];
+        for (int q = (int)dim - 2; q >= 0; --q) v = v * bsz[q] + c[q];
+        return (indexT)v;
+    }
+    // point key = chunk id * chunk size + x-fastest offset inside the chunk
+    template <typename CoordT> __device__ indexT getLinId(const grid_key_dx<dim, CoordT> &k) const
+    {
+        int64_t c[3] = {0, 0, 0};
+        int off = 0, mul = 1;
+        for (unsigned int q = 0; q < dim; ++q) {
+            c[q] = (int64_t)k.get(q) / B;
+            off += (int)((int64_t)k.get(q) % B) * mul;
+            mul *= (int)B;
+        }
+        return chunk_id(c) * (indexT)chunk_size + off;
+    }
+
+    __device__ insert_encap<Tuple> insertBlock(indexT blockId) const
+    {
+        const int lane = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z)) & 63;
+        uint64_t todo = __ballot(1);
+        int64_t rec = 0;
+        while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const int64_t lid = __shfl((long long)blockId, leader);
+            const uint64_t peers = __ballot((int64_t)blockId == lid) & todo;
+            unsigned int idx = 0;
+            if (lane == leader) {
+                idx = atomicAdd(iv.count, 1u);
+                if ((int64_t)idx < iv.cap) iv.ids[idx] = lid;
+            }
+            idx = (unsigned int)__shfl((int)idx, leader);
+            if ((peers >> lane) & 1) rec = ((int64_t)idx < iv.cap) ? (int64_t)idx : iv.cap;  // the sink
+            todo &= ~peers;
+        }
+        insert_encap<Tuple> e;
+        for (unsigned int q = 0; q < 16; ++q) e.prop[q] = nullptr;
+        for (unsigned int q = 0; q < pMask; ++q)
+            e.prop[q] = (unsigned char *)iv.prop[q] + (size_t)rec * chunk_size * sizeof_prop(q);
+        e.mask = iv.mask + (size_t)rec * chunk_size;
+        return e;
+    }
+    // insert<p>(coord): the point exists after the next flush; returns its value of p
+    template <unsigned int p, typename CoordT>
+    __device__ typename std::tuple_element<p, Tuple>::type &insert(const grid_key_dx<dim, CoordT> &k) const
+    {
+        const indexT key = getLinId(k);
+        insert_encap<Tuple> e = insertBlock(key / (indexT)chunk_size);
+        const int off = (int)(key % (indexT)chunk_size);
+        e.mask[off] |= (uint8_t)SG_EXIST;
+        return e.template get<p>()[off];
+    }
+
+  private:
+    template <size_t... I> __device__ static size_t sizeof_at(unsigned int q, std::index_sequence<I...>)
+    {
+        size_t s = 0;
+        ((s = (q == I) ? sizeof(typename std::tuple_element<I, Tuple>::type) : s), ...);
+        return s;
+    }
+    __device__ static size_t sizeof_prop(unsigned int q)
+    {
+        return sizeof_at(q, std::make_index_sequence<std::tuple_size<Tuple>::value>());
+    }
+};
+
+}  // namespace sg
+
+// BlockMapGpu_ker<>::setExist (BlockMapGpu_ker.cuh:75): sets the EXIST bit of a mask byte
+template <typename = void> struct BlockMapGpu_ker {
+    __device__ static void setExist(unsigned char &m) { m |= (unsigned char)SG_EXIST; }
+};
+
 // the reference's names in user code: cross_stencil<dim, T> and
 // GetCpBlockType<grid, prop, stencil_size>::type (the type of the lambdas' block argument)
 using sg::cross_stencil;

@@ -325,6 +325,26 @@ typedef struct sg_kernel_view {
     int32_t nnn, dim, edge, chunk_size, zorder, nprops;
 } sg_kernel_view;
 int sg_to_kernel(sg_grid *g, int nn_type, sg_kernel_view *out);
+
+/* setGPUInsertBuffer + device insertBlock / insert<p> / flush_block_insert
+ * (src/SparseGridGpu/SparseGridGpu.hpp:2270-2278, BlockMapGpu.hpp:249-287,
+ * BlockMapGpu_ker.cuh:171-227): a zeroed staging area of `nrec` chunk records for the
+ * caller's kernels.  A kernel claims record r with atomicAdd(count, 1) (r >= cap: write
+ * record cap, a sink, instead), writes ids[r], sets mask bits and values at
+ * [r * chunk_size + offset]; sg_insert_device_end (or the next sg_flush) appends the
+ * claimed records as chunk records (sg_insert_chunks) and fails with SG_E_RANGE when
+ * more than `cap` were claimed.  Values of a property not written at an inserted point
+ * are 0. */
+typedef struct sg_insert_view {
+    unsigned int *count;   /* records claimed so far */
+    int64_t *ids;          /* [cap + 1] chunk id of each record */
+    uint8_t *mask;         /* [(cap + 1) * chunk_size] */
+    void *prop[16];        /* [(cap + 1) * chunk_size] per property */
+    int64_t cap;
+    int32_t chunk_size;
+} sg_insert_view;
+int sg_insert_device_begin(sg_grid *g, int64_t nrec, sg_insert_view *out);
+int sg_insert_device_end(sg_grid *g);
 /* countExistingElements / countBoundaryElements (src/SparseGridGpu/SparseGridGpu.hpp:2422-2490) */
 int sg_count_existing(sg_grid *g, int64_t *out);
 int sg_count_boundary(sg_grid *g, int64_t *out);

@@ -167,6 +167,8 @@ _SIGS = {
     "sg_get_neighbours": (_INT, [_P, _INT, _P, _INT]),
     "sg_device_ptr": (_INT, [_P, _INT, ctypes.POINTER(_P)]),
     "sg_to_kernel": (_INT, [_P, _INT, _P]),
+    "sg_insert_device_begin": (_INT, [_P, _I64, _P]),
+    "sg_insert_device_end": (_INT, [_P]),
     "sg_count_existing": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_count_boundary": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_pack_size": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),

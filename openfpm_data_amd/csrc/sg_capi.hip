@@ -341,6 +341,11 @@ int sg_flush(sg_grid *g, const int32_t *ops)
         if (op < SG_OP_NONE || op > SG_OP_OR) return fail(SG_E_ARG, "unknown reduction");
         if (op == SG_OP_OR && dtype_is_float(g->dt[p])) return fail(SG_E_ARG, "bitwise or on a float property");
     }
+    if (g->dins_open) {
+        // records the caller's kernels wrote join the pending inserts first
+        const int rc = insert_device_end(g);
+        if (rc) return rc;
+    }
     return flush_impl(g, ops);
 }
 
@@ -502,6 +507,18 @@ int sg_device_ptr(sg_grid *g, int prop, void **out)
     else if (valid_prop(g, prop)) *out = g->d_prop[prop];
     else return fail(SG_E_ARG, "bad property");
     return SG_OK;
+}
+
+int sg_insert_device_begin(sg_grid *g, int64_t nrec, sg_insert_view *out)
+{
+    if (!g || !out) return fail(SG_E_ARG, "null argument");
+    return insert_device_begin(g, nrec, out);
+}
+
+int sg_insert_device_end(sg_grid *g)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    return insert_device_end(g);
 }
 
 int sg_to_kernel(sg_grid *g, int nn_type, sg_kernel_view *out)

@@ -1021,6 +1021,67 @@ int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t
     return SG_OK;
 }
 
+// Chunk records written by the caller's kernels: a zeroed staging area of nrec records
+// (plus one sink record for claims beyond nrec) and a claim counter; insert_device_end
+// reads the counter back and appends the claimed records as chunk records.
+namespace {
+inline size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+int insert_device_begin(sg_grid *g, int64_t nrec, sg_insert_view *out)
+{
+    if (nrec <= 0) return fail(SG_E_ARG, "record count must be positive");
+    if (g->dins_open) return fail(SG_E_ARG, "a device insert buffer is open: end it (or flush) first");
+    const size_t csz = (size_t)g->csz, nr = (size_t)nrec + 1;
+    size_t total = 256 + round256(nr * 8) + round256(nr * csz);
+    for (int p = 0; p < g->np; ++p) total += round256(nr * csz * g->ts[p]);
+    int rc = ensure(g->pool, g->dins, total);
+    if (rc) return rc;
+    SG_HIP(hipMemsetAsync(g->dins.p, 0, total, g->stream));
+    unsigned char *b = (unsigned char *)g->dins.p;
+    memset(out, 0, sizeof(*out));
+    out->count = (unsigned int *)b;
+    b += 256;
+    out->ids = (int64_t *)b;
+    b += round256(nr * 8);
+    out->mask = (uint8_t *)b;
+    b += round256(nr * csz);
+    for (int p = 0; p < g->np; ++p) {
+        out->prop[p] = b;
+        b += round256(nr * csz * g->ts[p]);
+    }
+    out->cap = nrec;
+    out->chunk_size = g->csz;
+    g->dins_cap = nrec;
+    g->dins_open = true;
+    return SG_OK;
+}
+
+int insert_device_end(sg_grid *g)
+{
+    if (!g->dins_open) return SG_OK;
+    g->dins_open = false;
+    unsigned int cnt = 0;
+    SG_HIP(hipMemcpyAsync(&cnt, g->dins.p, sizeof(cnt), hipMemcpyDeviceToHost, g->stream));
+    SG_HIP(hipStreamSynchronize(g->stream));
+    if ((int64_t)cnt > g->dins_cap)
+        return fail(SG_E_RANGE, "device insert buffer overflow: " + std::to_string(cnt) + " chunk records claimed, " +
+                                    std::to_string(g->dins_cap) + " reserved");
+    if (cnt == 0) return SG_OK;
+    const size_t csz = (size_t)g->csz, nr = (size_t)g->dins_cap + 1;
+    unsigned char *b = (unsigned char *)g->dins.p + 256;
+    const int64_t *ids = (const int64_t *)b;
+    b += round256(nr * 8);
+    const uint8_t *mask = b;
+    b += round256(nr * csz);
+    const void *vals[kMaxProps] = {};
+    for (int p = 0; p < g->np; ++p) {
+        vals[p] = b;
+        b += round256(nr * csz * g->ts[p]);
+    }
+    return append_chunks(g, cnt, ids, mask, vals, SG_MEM_DEVICE);
+}
+
 namespace {
 
 // Deferred merge: the caller writes every touched chunk whole (the fused fold) from the

@@ -130,6 +130,12 @@ struct sg_grid {
     int fast_wmode = 3;
     // 8^3 tagBoundaries: chunks per wave (1, 2 or 4), all loaded before any is tagged
     int tag_cpw = 2;
+    // chunk records written by the caller's kernels (sg_insert_device_begin / _end):
+    // [count (256 B)] [ids, cap + 1] [masks] [values per property], the last record a
+    // sink for claims beyond the capacity
+    sg::DevBuf dins;
+    int64_t dins_cap = 0;
+    bool dins_open = false;
     // read the mask first and skip loading 8-point rows without points
     int fast_lskip = 0;
 
@@ -298,6 +304,8 @@ int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *m
 int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t *masks,
                   const void *const *values, int mem);
 int find_neighbours_impl(sg_grid *g, int type);
+int insert_device_begin(sg_grid *g, int64_t nrec, sg_insert_view *out);
+int insert_device_end(sg_grid *g);
 int tag_boundaries_impl(sg_grid *g, int type);
 // existing-point list of the point-list stencil mode (d_ep, ep_n)
 int existing_points_impl(sg_grid *g);

@@ -1,5 +1,5 @@
 // The conv family with user __device__ lambdas (SparseGridGpu.hpp:2568-2683 of the
-// reference), through the facade compiled by hipcc.  Each lambda form is checked against a
+// reference) and device inserts through toKernel(), through the facade compiled by hipcc.  Each lambda form is checked against a
 // path pinned elsewhere: the library's named stencils (themselves checked against the
 // oracle), or values computed on the host from the grid's host copy.
 // Build: hipcc -std=c++17 --offload-arch=gfx950 -ffp-contract=off -I include
@@ -447,8 +447,112 @@ static void testRefConvSubset()
     REQUIRE(match);
 }
 
+// ---- device inserts through toKernel() (SparseGridGpu_tests.cu:15-75, :209-282) ----
+
+template <unsigned int p, typename KerT> __global__ void insertValues(KerT sg)
+{
+    sg.init();
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    grid_key_dx<KerT::d, int> coord({x, y, z});
+    sg.template insert<p>(coord) = x;
+    sg.flush_block_insert();
+}
+
+template <unsigned int p1, unsigned int p2, typename KerT> __global__ void insertConstantValue2(KerT sg, float value)
+{
+    constexpr unsigned int pMask = KerT::pMask;
+    sg.init();
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    grid_key_dx<KerT::d, int> coord({x, y, z});
+    const long pos = sg.getLinId(coord);
+    const long chunk = pos / KerT::chunk_size;
+    const int offset = (int)(pos % KerT::chunk_size);
+    auto encap = sg.insertBlock(chunk);
+    encap.template get<p1>()[offset] = value;
+    encap.template get<p2>()[offset] = value;
+    BlockMapGpu_ker<>::setExist(encap.template get<pMask>()[offset]);
+    __syncthreads();
+    sg.flush_block_insert();
+}
+
+// testInsert (:209-244): 2D, 2 x 2 blocks of 8 x 8 threads, value = x, background 666
+static void testDeviceInsert2D()
+{
+    const size_t sz[2] = {16, 16};
+    SparseGridGpu<2, aggregate<float>, 8> sg(sz);
+    sg.setBackgroundValue<0>(666);
+    const dim3 gridSize(2, 2), blockSize(8, 8);
+    sg.setGPUInsertBuffer(gridSize, blockSize);
+    hipLaunchKernelGGL((insertValues<0>), gridSize, blockSize, 0, 0, sg.toKernel());
+    gpu::ofp_context_t ctx;
+    sg.flush<smax_<0>>(ctx, FLUSH_ON_DEVICE);
+    bool match = true;
+    for (long i = 0; i < 4 * 64; ++i) {
+        auto coord = sg.getCoord(i);
+        match &= sg.get<0>(coord) == (float)coord.get(0);
+    }
+    REQUIRE(match);
+    REQUIRE(sg.countExistingElements() == 256);
+}
+
+// testInsert3D (:246-282): 3D chunks of 4^3, 2 x 2 x 2 blocks of 4^3 threads
+static void testDeviceInsert3D()
+{
+    const size_t sz[3] = {8, 8, 8};
+    SparseGridGpu<3, aggregate<float>, 4> sg(sz);
+    sg.setBackgroundValue<0>(666);
+    const dim3 gridSize(2, 2, 2), blockSize(4, 4, 4);
+    sg.setGPUInsertBuffer(gridSize, blockSize);
+    hipLaunchKernelGGL((insertValues<0>), gridSize, blockSize, 0, 0, sg.toKernel());
+    gpu::ofp_context_t ctx;
+    sg.flush<smax_<0>>(ctx, FLUSH_ON_DEVICE);
+    bool match = true;
+    for (long i = 0; i < 64 * 4; ++i) {
+        auto coord = sg.getCoord(i);
+        match &= sg.get<0>(coord) == (float)coord.get(0);
+    }
+    REQUIRE(match);
+    REQUIRE(sg.countExistingElements() == 512);
+}
+
+// insertBlock + setExist of two properties (insertConstantValue2, :15-45) with 8 waves per
+// block writing one chunk, merged with smax; then an overflowing buffer is reported
+static void testDeviceInsertBlock()
+{
+    const size_t sz[3] = {16, 16, 16};
+    SparseGridGpu<3, aggregate<float, float, float>> sg(sz);
+    const dim3 gridSize(2, 2, 2), blockSize(8, 8, 8);
+    sg.setGPUInsertBuffer(gridSize, blockSize);
+    hipLaunchKernelGGL((insertConstantValue2<0, 2>), gridSize, blockSize, 0, 0, sg.toKernel(), 3.0f);
+    gpu::ofp_context_t ctx;
+    sg.flush<smax_<0>, smax_<1>, smax_<2>>(ctx, FLUSH_ON_DEVICE);
+    REQUIRE(sg.countExistingElements() == 4096);
+    const std::vector<float> a = sg.private_get_data_array<0>(), b = sg.private_get_data_array<1>(),
+                             c = sg.private_get_data_array<2>();
+    for (size_t i = 0; i < 4096; ++i) REQUIRE(a[i] == 3.0f && b[i] == 0.0f && c[i] == 3.0f);
+    // a buffer of one record for eight chunks: claims beyond it go to the sink record, and
+    // the flush reports the overflow
+    SparseGridGpu<3, aggregate<float, float, float>> s2(sz);
+    s2.setGPUInsertBuffer(1, 1);
+    hipLaunchKernelGGL((insertConstantValue2<0, 1>), gridSize, blockSize, 0, 0, s2.toKernel(), 1.0f);
+    bool thrown = false;
+    try {
+        s2.flush<smax_<0>, smax_<1>, smax_<2>>(ctx, FLUSH_ON_DEVICE);
+    } catch (const sg::error &) {
+        thrown = true;
+    }
+    REQUIRE(thrown);
+}
+
 int main()
 {
+    testDeviceInsert2D();
+    testDeviceInsert3D();
+    testDeviceInsertBlock();
     testRefLapCross();
     testRefLapConv();
     testRefLapConv2();
