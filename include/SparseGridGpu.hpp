@@ -19,8 +19,9 @@
  *   (src/Vector/cuda/map_vector_sparse_cuda_kernels.cuh), NNStar / NNFull, the test
  *   stencils HeatStencil and Conv3x3x3 (src/SparseGridGpu/tests/utils/SparseGridGpu_util_test.cuh).
  * Differences (a C ABI carries no device code):
- *   - insertion is host-batched (insertPoints / insertBlocks) instead of user __global__
- *     kernels calling insert<p>() on toKernel();
+ *   - insertion from user __global__ kernels (insert<p>() / insertBlock() on toKernel())
+ *     needs a hipcc translation unit; host code inserts in batches (insertPoints /
+ *     insertBlocks);
  *   - applyStencils takes the named stencils of this header (descriptors evaluated by
  *     library kernels), not arbitrary __device__ functors; user __device__ lambdas go
  *     through conv / conv2 / conv_cross / conv_cross_b / conv2_b / conv3_b, available in
