@@ -48,9 +48,9 @@
 #include "sparsegrid_gpu.h"
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
-#define SG_HD __host__ __device__
+#define SPARSEGRIDGPU_HD __host__ __device__
 #else
-#define SG_HD
+#define SPARSEGRIDGPU_HD
 #endif
 
 namespace sg {
@@ -90,16 +90,16 @@ template <unsigned int dim, typename T = long int> class grid_key_dx
     T k[dim];
 
 public:
-    SG_HD grid_key_dx() : k{} {}
-    SG_HD grid_key_dx(std::initializer_list<T> l)
+    SPARSEGRIDGPU_HD grid_key_dx() : k{} {}
+    SPARSEGRIDGPU_HD grid_key_dx(std::initializer_list<T> l)
     {
         unsigned int i = 0;
         for (T v : l)
             if (i < dim) k[i++] = v;
         for (; i < dim; ++i) k[i] = 0;
     }
-    SG_HD T get(unsigned int i) const { return k[i]; }
-    SG_HD void set_d(unsigned int i, T v) { k[i] = v; }
+    SPARSEGRIDGPU_HD T get(unsigned int i) const { return k[i]; }
+    SPARSEGRIDGPU_HD void set_d(unsigned int i, T v) { k[i] = v; }
 };
 
 template <unsigned int dim, typename T> class Box
