@@ -25,7 +25,7 @@
  *   - applyStencils takes the named stencils of this header (descriptors evaluated by
  *     library kernels), not arbitrary __device__ functors; user __device__ lambdas go
  *     through conv / conv2 / conv_cross / conv_cross_b / conv2_b / conv3_b, available in
- *     translation units compiled by hipcc (include/SparseGridGpu_conv.hpp, over the
+ *     translation units compiled by hipcc (include/SparseGridGpu_device.hpp, over the
  *     toKernel view sg_to_kernel);
  *   - failures throw sg::error (the reference prints and continues).
  */
@@ -160,7 +160,7 @@ template <unsigned int dim> struct NNFull {
 };
 
 #if defined(__HIPCC__)
-#include "SparseGridGpu_conv.hpp"
+#include "SparseGridGpu_device.hpp"
 #endif
 
 // ---- named stencils for applyStencils ----------------------------------------------------
@@ -633,7 +633,7 @@ public:
     }
 
     // -- conv family with user __device__ lambdas (SparseGridGpu.hpp:2568-2683), for
-    //    translation units compiled by hipcc: include/SparseGridGpu_conv.hpp.  The box
+    //    translation units compiled by hipcc: include/SparseGridGpu_device.hpp.  The box
     //    [start, stop] is inclusive; stencil_size is the ghost width the lambda may reach.
     // res = func(cpb, i, j[, k])
     template <unsigned int prop_src, unsigned int prop_dst, unsigned int stencil_size, typename lambda_f,

@@ -1,5 +1,6 @@
-/* SparseGridGpu_conv.hpp -- device path of the facade's conv family for user __device__
- * lambdas (hipcc only; included by SparseGridGpu.hpp when __HIPCC__ is defined).
+/* SparseGridGpu_device.hpp -- device side of the facade (hipcc only; included by
+ * SparseGridGpu.hpp when __HIPCC__ is defined): the conv family for user __device__ lambdas
+ * and the toKernel() view for device inserts (sparse_grid_gpu_ker, further below).
  *
  * Reference: SparseGridGpu::conv / conv2 / conv_cross / conv_cross_b / conv2_b / conv3_b
  * (src/SparseGridGpu/SparseGridGpu.hpp:2568-2683) with the functors stencil_cross_func_conv,
@@ -22,8 +23,8 @@
  * reference's block store does for those slots; empty slots keep the destination
  * background.
  */
-#ifndef SPARSEGRIDGPU_CONV_HPP
-#define SPARSEGRIDGPU_CONV_HPP
+#ifndef SPARSEGRIDGPU_DEVICE_HPP
+#define SPARSEGRIDGPU_DEVICE_HPP
 
 #include <hip/hip_runtime.h>
 
@@ -418,4 +419,4 @@ template <typename SG, unsigned int prop, unsigned int stencil_size> struct GetC
         type;
 };
 
-#endif  // SPARSEGRIDGPU_CONV_HPP
+#endif  // SPARSEGRIDGPU_DEVICE_HPP

@@ -309,7 +309,7 @@ int sg_device_ptr(sg_grid *g, int prop, void **out);
 
 /* SparseGridGpu::toKernel() (src/SparseGridGpu/SparseGridGpu.hpp:1716-1751): the grid's
  * device arrays, for kernels compiled by the caller (the facade's conv family,
- * include/SparseGridGpu_conv.hpp).  Builds the nn_type neighbour table when it is stale.
+ * include/SparseGridGpu_device.hpp).  Builds the nn_type neighbour table when it is stale.
  * The pointers stay valid until the next call that changes the chunk set or a property
  * array (flush, remove, unpack, a stencil writing into one of its sources, ...); work on
  * them belongs on `stream`, the handle's stream. */

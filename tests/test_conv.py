@@ -1,4 +1,4 @@
-"""The facade's conv family with user __device__ lambdas (include/SparseGridGpu_conv.hpp):
+"""The facade's conv family with user __device__ lambdas (include/SparseGridGpu_device.hpp):
 tests/cpp/test_conv.hip compiles with hipcc against the C ABI, and its checks pass on the
 GPU (each lambda form against the library's named stencils or host-computed values)."""
 import os
