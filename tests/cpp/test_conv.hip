@@ -548,8 +548,39 @@ static void testDeviceInsertBlock()
     REQUIRE(thrown);
 }
 
+// host point records first, then device chunk records into the same chunks: the flush
+// folds them in that order (sadd of integer-valued floats: host value + device value)
+static void testDeviceInsertAfterHostPoints()
+{
+    const size_t sz[3] = {8, 8, 8};
+    SparseGridGpu<3, aggregate<float>, 4> sg(sz);
+    std::vector<long> keys;
+    std::vector<float> vals;
+    for (int z = 0; z < 8; ++z)
+        for (int y = 0; y < 8; ++y)
+            for (int x = 0; x < 8; x += 2) {
+                keys.push_back(sg.getLinId(grid_key_dx<3, int>({x, y, z})));
+                vals.push_back(100.0f);
+            }
+    const void *v[] = {vals.data()};
+    sg.insertPoints(keys.data(), keys.size(), v);
+    const dim3 gridSize(2, 2, 2), blockSize(4, 4, 4);
+    sg.setGPUInsertBuffer(gridSize, blockSize);
+    hipLaunchKernelGGL((insertValues<0>), gridSize, blockSize, 0, 0, sg.toKernel());
+    gpu::ofp_context_t ctx;
+    sg.flush<sadd_<0>>(ctx, FLUSH_ON_DEVICE);
+    bool match = true;
+    for (int z = 0; z < 8; ++z)
+        for (int y = 0; y < 8; ++y)
+            for (int x = 0; x < 8; ++x)
+                match &= sg.get<0>(grid_key_dx<3, int>({x, y, z})) == (float)x + ((x % 2 == 0) ? 100.0f : 0.0f);
+    REQUIRE(match);
+    REQUIRE(sg.countExistingElements() == 512);
+}
+
 int main()
 {
+    testDeviceInsertAfterHostPoints();
     testDeviceInsert2D();
     testDeviceInsert3D();
     testDeviceInsertBlock();
