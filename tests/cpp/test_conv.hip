@@ -279,6 +279,30 @@ static void testBlockForms()
     REQUIRE(n > 10000);
 }
 
+// a ghost layer of 2 points (stencil_size 2): taps two points away, across chunk faces,
+// edges and the domain boundary (background 0 outside)
+static void testConvRadius2()
+{
+    const size_t sz[3] = {64, 64, 64};
+    SparseGridGpu<3, Props6> sg(sz);
+    build_shell(sg);
+    sg.template conv<0, 2, 2>(grid_key_dx<3, int>({0, 0, 0}), grid_key_dx<3, int>({63, 63, 63}),
+                              [] __device__(auto &u, int i, int j, int k) -> float {
+                                  return u(i + 2, j, k) - 2.0f * u(i, j - 2, k) + 3.0f * u(i - 1, j + 1, k + 2);
+                              });
+    const HostView h(sg);
+    size_t n = 0;
+    for (int z = 0; z < 64; ++z)
+        for (int y = 0; y < 64; ++y)
+            for (int x = 0; x < 64; ++x) {
+                if (h.flag(x, y, z) != 1) continue;
+                REQUIRE(h.get(2, x, y, z) ==
+                        h.get(0, x + 2, y, z) - 2.0f * h.get(0, x, y - 2, z) + 3.0f * h.get(0, x - 1, y + 1, z + 2));
+                ++n;
+            }
+    REQUIRE(n > 10000);
+}
+
 // 2D: conv of a 5-point Laplacian against the named Laplacian (FP64, integer values)
 static void testConv2D()
 {
@@ -594,6 +618,7 @@ int main()
     testConvBox();
     testConv2GrayScott();
     testBlockForms();
+    testConvRadius2();
     testConv2D();
     std::printf("conv tests passed\n");
     return 0;
