@@ -334,7 +334,8 @@ int sg_to_kernel(sg_grid *g, int nn_type, sg_kernel_view *out);
  * [r * chunk_size + offset]; sg_insert_device_end (or the next sg_flush) appends the
  * claimed records as chunk records (sg_insert_chunks) and fails with SG_E_RANGE when
  * more than `cap` were claimed.  Values of a property not written at an inserted point
- * are 0. */
+ * are 0.  The kernels run on the handle's stream (or finish before sg_insert_device_end:
+ * it reads the claim counter in the handle's stream order). */
 typedef struct sg_insert_view {
     unsigned int *count;   /* records claimed so far */
     int64_t *ids;          /* [cap + 1] chunk id of each record */
