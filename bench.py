@@ -373,9 +373,25 @@ def main():
 
     secondary = None
     if not args.no_secondary:
+        if dtype == np.float32:
+            # the same workload computed in FP64 (its own kernel shape; 33 B per slot)
+            log("FP64 stencil run")
+            a64 = argparse.Namespace(**vars(args))
+            a64.steps, a64.warmup = 200, 10
+            r64 = run_gpu(a64, ids, masks, data, np.float64, edge=edge)
+            b64 = slots * (2 * 8 * 2 + 1) + nch * 6 * 4
+            ach64 = b64 / (r64["kernel_ms"] * 1e-3) / 1e9
+            secondary_f64 = {"workload": "shell512_gray_scott_fp64", "steps": a64.steps,
+                             "ms_per_step": round(r64["kernel_ms"], 5),
+                             "value": round(r64["points"] / (r64["kernel_ms"] * 1e-3) / 1e9, 4), "unit": "Gpt-updates/s",
+                             "tuned": r64["tuned"],
+                             "roofline": {"bound": "hbm", "achieved": round(ach64, 2), "peak": HBM_PEAK_GBS,
+                                          "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4)}}
         log("flush workload")
         secondary = {"flush": run_flush_secondary(args.cpu_threads or min(16, os.cpu_count() or 1), not args.no_cpu),
                      "poisson": None}
+        if dtype == np.float32:
+            secondary["stencil_fp64"] = secondary_f64
         log("batched Poisson workload")
         secondary["poisson"] = run_poisson_secondary()
         tb = secondary["poisson"]["tuned"]
