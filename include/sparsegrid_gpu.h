@@ -174,7 +174,9 @@ int sg_set_background(sg_grid *g, int prop, const void *value);
  * values[p] may be NULL (value unspecified; the record only sets mask bits). */
 int sg_insert_points(sg_grid *g, int64_t n, const int64_t *keys,
                      const uint8_t *masks, const void *const *values, int mem);
-/* same, with integer coordinates [n][dim] instead of linear keys */
+/* same, with integer coordinates [n][dim] instead of linear keys; host coordinates
+ * outside the grid fail here (SG_E_RANGE), device ones fail the next sg_flush (which
+ * then drops the pending records), so the call does not wait for the device */
 int sg_insert_coords(sg_grid *g, int64_t n, const int32_t *coords,
                      const uint8_t *masks, const void *const *values, int mem);
 /* Chunk insertion: insertBlock(blockId) per chunk slot

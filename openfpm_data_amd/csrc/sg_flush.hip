@@ -89,8 +89,10 @@ __global__ void coords_to_keys_kernel(const int32_t *coords, int64_t *keys, int6
         ok = ok && c[d] >= 0 && c[d] < sz[d];
     }
     if (!ok) {
+        // flagged, and a key the flush's key check rejects (device coordinates are not
+        // read back at insertion)
         *bad = 1;
-        keys[i] = 0;
+        keys[i] = -1;
         return;
     }
     int64_t bc[3] = {0, 0, 0}, l = 0;
@@ -833,19 +835,14 @@ int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *m
         rc = ensure(g->pool, cb, 64);
         if (rc) return rc;
     }
+    // the kernel's flag word (not read back: invalid coordinates are caught by the flush)
     int *bad = (int *)((char *)cb.p + ((mem == SG_MEM_HOST) ? n * g->dim * sizeof(int32_t) : 0));
     bad = (int *)(((uintptr_t)bad + 7) & ~(uintptr_t)7);
-    SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
     hipLaunchKernelGGL(coords_to_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, dc,
                        g->d_ins_key + at, n, g->dim, g->edge, g->sz[0], g->sz[1], g->sz[2], chunk_lin(g), g->csz,
                        bad);
     SG_HIP(hipGetLastError());
-    if (mem == SG_MEM_DEVICE) {
-        int hbad = 0;
-        SG_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
-        SG_HIP(hipStreamSynchronize(g->stream));
-        if (hbad) return fail(SG_E_RANGE, "coordinate outside the grid");
-    }
+    // device coordinates outside the grid became key -1: the next flush rejects them
     rc = append_mask_values(g, at, n, masks, values, mem);
     if (rc) return rc;
     g->ins_n += n;

@@ -584,6 +584,28 @@ def test_to_kernel_view(G):
     g.close()
 
 
+def test_flush_rejects_device_coords_outside_grid(G):
+    """Coordinates handed over in device memory are checked by the flush: one outside
+    the grid fails it (SgError) and leaves the grid unchanged; host coordinates fail at
+    insertion."""
+    import torch
+    from openfpm_data_amd._capi import SgError
+    g = G(3, (32, 32, 32), [np.float32], block_edge=8)
+    g.insert_coords([[1, 2, 3]], [np.array([1.0], np.float32)])
+    g.flush(["smax"])
+    before = (g.chunk_ids().copy(), g.masks().copy(), g.prop(0).copy())
+    coords = torch.tensor([[4, 5, 6], [3, 40, 2], [7, 7, 7]], dtype=torch.int32, device="cuda")
+    g.insert_coords(coords, [torch.ones(3, dtype=torch.float32, device="cuda")])
+    with pytest.raises(SgError):
+        g.flush(["smax"])
+    assert g.pending_inserts() == 0
+    for a, b in zip(before, (g.chunk_ids(), g.masks(), g.prop(0))):
+        np.testing.assert_array_equal(a, b)
+    with pytest.raises(SgError):
+        g.insert_coords([[3, 40, 2]], [np.array([1.0], np.float32)])
+    g.close()
+
+
 def test_flush_rejects_device_keys_outside_grid(G):
     """Point keys handed over in device memory are checked by the flush: a key beyond the
     grid fails it (SgError), drops the pending records and leaves the grid unchanged."""
