@@ -178,11 +178,16 @@ template <typename T> __global__ void gather_kernel(const T *src, const uint32_t
 }
 
 // ---- flush kernels -------------------------------------------------------------
-template <typename K> __global__ void keys_to_sort_kernel(const int64_t *keys, K *sk, uint32_t *sv, int64_t n)
+// sort keys and record ids; with `bad`, also the range check of the keys (flag set when
+// one lies outside [0, limit))
+template <typename K>
+__global__ void keys_to_sort_kernel(const int64_t *keys, K *sk, uint32_t *sv, int64_t n, int64_t limit, int *bad)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        sk[i] = (K)keys[i];
+        const int64_t k = keys[i];
+        if (bad && (k < 0 || k >= limit)) *bad = 1;
+        sk[i] = (K)k;
         sv[i] = (uint32_t)i;
     }
 }
@@ -1290,7 +1295,7 @@ int flush_chunks_impl(sg_grid *g, const int32_t *ops)
     uint32_t *perm = nullptr;
     if (k32) {
         hipLaunchKernelGGL(keys_to_sort_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           g->d_cins_id, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+                           g->d_cins_id, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, n, (int64_t)0, (int *)nullptr);
         SG_HIP(hipGetLastError());
         uint32_t *ko;
         rc = radix_sort_pairs_u32(g, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, (uint32_t *)b_tk.p,
@@ -1298,7 +1303,7 @@ int flush_chunks_impl(sg_grid *g, const int32_t *ops)
         sk = ko;
     } else {
         hipLaunchKernelGGL(keys_to_sort_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           g->d_cins_id, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+                           g->d_cins_id, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, n, (int64_t)0, (int *)nullptr);
         SG_HIP(hipGetLastError());
         uint64_t *ko;
         rc = radix_sort_pairs_u64(g, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, (uint64_t *)b_tk.p,
@@ -1370,16 +1375,10 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     // grid is not touched before it is read; out-of-range keys only reorder scratch data)
     uint32_t *hw = host_words(g);
     if (!hw) return fail(SG_E_OOM, "pinned host allocation failed");
-    {
-        rc = ensure(g->pool, g->scratch[6], 64);
-        if (rc) return rc;
-        int *bad = (int *)g->scratch[6].p;
-        SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
-        hipLaunchKernelGGL(check_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key, n, limit,
-                           bad);
-        SG_HIP(hipGetLastError());
-        SG_HIP(hipMemcpyAsync(hw + 8, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
-    }
+    rc = ensure(g->pool, g->scratch[6], 64);
+    if (rc) return rc;
+    int *bad = (int *)g->scratch[6].p;
+    SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
     int bits = 0;
     while (bits < 63 && (((int64_t)1 << bits) < limit)) ++bits;
     const bool k32 = bits <= 32;
@@ -1396,7 +1395,7 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     uint32_t *perm = nullptr;
     if (k32) {
         hipLaunchKernelGGL(keys_to_sort_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           g->d_ins_key, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+                           g->d_ins_key, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, n, limit, bad);
         SG_HIP(hipGetLastError());
         uint32_t *ko;
         rc = radix_sort_pairs_u32(g, (uint32_t *)b_keys.p, (uint32_t *)b_vals.p, (uint32_t *)b_tk.p,
@@ -1404,7 +1403,7 @@ int flush_impl(sg_grid *g, const int32_t *ops)
         sk = ko;
     } else {
         hipLaunchKernelGGL(keys_to_sort_kernel<uint64_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                           g->d_ins_key, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, n);
+                           g->d_ins_key, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, n, limit, bad);
         SG_HIP(hipGetLastError());
         uint64_t *ko;
         rc = radix_sort_pairs_u64(g, (uint64_t *)b_keys.p, (uint32_t *)b_vals.p, (uint64_t *)b_tk.p,
@@ -1412,6 +1411,8 @@ int flush_impl(sg_grid *g, const int32_t *ops)
         sk = ko;
     }
     if (rc) return rc;
+    // the range flag of the sort-key pass, read with the first scan total below
+    SG_HIP(hipMemcpyAsync(hw + 8, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
     // the temporaries not holding the sorted result are free for reuse
     uint32_t *spare_a = (perm == (uint32_t *)b_vals.p) ? (uint32_t *)b_tv.p : (uint32_t *)b_vals.p;
 
