@@ -784,6 +784,27 @@ __device__ inline void box_finish(const BoxIn<T> &in, int64_t c, T *tw, T *__res
         T cf[27];
 #pragma unroll
         for (int q = 0; q < 27; ++q) cf[q] = std::is_integral<T>::value ? (T)sa.icoef[q] : (T)sa.coef[q];
+        T isum[8];
+        if constexpr (std::is_integral<T>::value) {
+            // wrapping integer sums do not depend on the order of the terms, so each of the
+            // 9 rows of the 3 x 3 (y, z) neighbourhood is read from LDS once (10 values)
+            // instead of once per tap
+#pragma unroll
+            for (int x = 0; x < 8; ++x) isum[x] = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const T *row = tw + ((z + k) * 10 + (y + j)) * 10;
+                    T r[10];
+#pragma unroll
+                    for (int q = 0; q < 10; ++q) r[q] = row[q];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int x = 0; x < 8; ++x) isum[x] = dev_add<T>(isum[x], dev_mul<T>(r[x + i], cf[i * 9 + j * 3 + k]));
+                }
+        }
         bool act[8];
         bool whole = true;
 #pragma unroll
@@ -794,13 +815,19 @@ __device__ inline void box_finish(const BoxIn<T> &in, int64_t c, T *tw, T *__res
             const uint8_t raw = (uint8_t)(mrow >> (8 * x));
             whole = whole && (act[x] || raw == 0);
             T tot = 0;
+            if constexpr (std::is_integral<T>::value) {
+                tot = isum[x];
+            } else {
+                // floating point: the reference order (x offset outermost, z innermost)
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+                for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j)
+                    for (int j = 0; j < 3; ++j)
 #pragma unroll
-                    for (int k = 0; k < 3; ++k)
-                        tot = dev_add<T>(tot, dev_mul<T>(tw[((z + k) * 10 + (y + j)) * 10 + (x + i)], cf[i * 9 + j * 3 + k]));
+                        for (int k = 0; k < 3; ++k)
+                            tot = dev_add<T>(tot,
+                                             dev_mul<T>(tw[((z + k) * 10 + (y + j)) * 10 + (x + i)], cf[i * 9 + j * 3 + k]));
+            }
             out[x] = act[x] ? tot : bg_dst;
         }
         if (whole) {
