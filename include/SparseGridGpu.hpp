@@ -84,6 +84,49 @@ template <typename... T> struct aggregate {
     typedef std::tuple<T...> type;
     static constexpr unsigned int max_prop = sizeof...(T);
 };
+namespace sg {
+template <typename Tuple> struct aggregate_of;
+template <typename... T> struct aggregate_of<std::tuple<T...>> { typedef ::aggregate<T...> type; };
+// scalar type of property p of an aggregate; p == number of properties is the mask byte
+template <typename Agg, unsigned int p, bool prop = (p < Agg::max_prop)> struct scalar_of {
+    typedef typename std::tuple_element<p, typename Agg::type>::type type;
+};
+template <typename Agg, unsigned int p> struct scalar_of<Agg, p, false> { typedef unsigned char type; };
+template <int v> struct int_ { static constexpr int value = v; };
+// the aggregate of a grid's chunks (AggregateBlockType / AggregateType of the device view):
+// the property types and the chunk size N, so that BlockTypeOf<...>::size is N
+template <unsigned int N, typename... T> struct block_aggregate {
+    typedef std::tuple<T...> type;
+    static constexpr unsigned int max_prop = sizeof...(T);
+    static constexpr unsigned int chunk_size = N;
+};
+template <unsigned int N, typename Tuple> struct block_aggregate_of;
+template <unsigned int N, typename... T> struct block_aggregate_of<N, std::tuple<T...>> {
+    typedef block_aggregate<N, T...> type;
+};
+template <typename T, unsigned int N> struct data_block_type {
+    typedef T scalarType;
+    static constexpr unsigned int size = N;
+};
+}  // namespace sg
+// ScalarTypeOf<AggregateT, p> of the stencil functors (the scalar of property p)
+template <typename Agg, unsigned int p> using ScalarTypeOf = typename sg::scalar_of<Agg, p>::type;
+// BlockTypeOf<AggregateType, p>: the chunk of property p (::size points of ScalarTypeOf)
+template <typename Agg, unsigned int p> using BlockTypeOf = sg::data_block_type<ScalarTypeOf<Agg, p>, Agg::chunk_size>;
+// IntPow<base, exp>::value (SparseGridGpu/TemplateUtils/mathUtils.hpp)
+template <unsigned int base, unsigned int exp> struct IntPow {
+    static constexpr unsigned int value = base * IntPow<base, exp - 1>::value;
+};
+template <unsigned int base> struct IntPow<base, 0> { static constexpr unsigned int value = 1; };
+namespace openfpm {
+// position of a chunk in the chunk list (openfpm::sparse_index, Vector/map_vector_sparse.hpp)
+template <typename T> struct sparse_index { T id; };
+}  // namespace openfpm
+// a point as chunk position + offset in the chunk (block_offset, SparseGridGpu_ker.cuh)
+template <typename T> struct block_offset {
+    T pos;
+    T off;
+};
 
 template <unsigned int dim, typename T = long int> class grid_key_dx
 {
@@ -174,6 +217,13 @@ template <unsigned int dim> inline void box_to_desc(sg_stencil_desc &d, const Bo
         d.box_hi[i] = box.getHigh(i);
     }
 }
+// a named stencil of this header: Stencil::desc(box, args...) exists
+template <typename Stencil, typename BoxT, typename... Args> struct has_desc {
+    template <typename S>
+    static auto test(int) -> decltype(S::desc(std::declval<const BoxT &>(), std::declval<Args>()...), std::true_type());
+    template <typename> static std::false_type test(...);
+    static constexpr bool value = decltype(test<Stencil>(0))::value;
+};
 inline sg_stencil_desc blank_desc(int kind, int apply)
 {
     sg_stencil_desc d;
@@ -439,13 +489,142 @@ private:
 };
 typedef Pack_stat Unpack_stat;
 
-template <unsigned int dim> struct default_edge { static constexpr unsigned int value = dim == 3 ? 8 : (dim == 2 ? 16 : 256); };
+// chunk edge and threadBlockSize defaults per dimension (SparseGridGpu.hpp:38-64)
+template <unsigned int dim> struct default_edge { typedef sg::int_<2> type; };
+template <> struct default_edge<1> { typedef sg::int_<256> type; typedef sg::int_<256> tb; };
+template <> struct default_edge<2> { typedef sg::int_<16> type; typedef sg::int_<256> tb; };
+template <> struct default_edge<3> { typedef sg::int_<8> type; typedef sg::int_<512> tb; };
 
-// zorder = true: chunk ids are Morton codes (grid_zmb), the reference's SparseGridGpu_z
-template <unsigned int dim, typename AggregateT, unsigned int blockEdgeSize = default_edge<dim>::value,
-          unsigned int threadBlockSize = 512, typename indexT = long int, bool zorder = false>
+// memory layouts of the reference's containers (memory_traits_inte: one array per property,
+// the chunk-major layout this library implements; memory_traits_lin is declared so that
+// code naming it compiles, and the grid rejects it)
+template <typename T> struct memory_traits_inte { typedef T type; };
+template <typename T> struct memory_traits_lin { typedef T type; };
+
+// Chunk linearizers (src/Grid/Geometry/grid_smb.hpp, grid_zmb.hpp): the domain size in points
+// and the chunk ids of grid_smb (row-major, x fastest) or grid_zmb (Morton: bit i of chunk
+// coordinate d at bit dim * i + d).  Inside a chunk the offset is x fastest for both.
+namespace sg {
+template <unsigned int dim, unsigned int blockEdgeSize, typename indexT, bool morton> class chunk_linearizer
+{
+public:
+    static constexpr bool is_morton = morton;
+    static constexpr indexT blockSize = IntPow<blockEdgeSize, dim>::value;
+    SPARSEGRIDGPU_HD chunk_linearizer() : blockSz{}, sz{} {}
+    // from the domain size in points (grid_smb.hpp:58-83)
+    template <typename I, typename = typename std::enable_if<std::is_integral<I>::value>::type>
+    SPARSEGRIDGPU_HD chunk_linearizer(const I (&size)[dim])
+    {
+        for (unsigned int d = 0; d < dim; ++d) {
+            sz[d] = (indexT)size[d];
+            blockSz[d] = (indexT)(size[d] / blockEdgeSize + ((size[d] % blockEdgeSize) != 0));
+        }
+    }
+    // a cube of domainBlockEdgeSize chunks per side (grid_smb.hpp:85-95)
+    SPARSEGRIDGPU_HD explicit chunk_linearizer(size_t domainBlockEdgeSize)
+    {
+        for (unsigned int d = 0; d < dim; ++d) {
+            blockSz[d] = (indexT)domainBlockEdgeSize;
+            sz[d] = (indexT)domainBlockEdgeSize * blockEdgeSize;
+        }
+    }
+#if defined(__HIPCC__)
+    // chunk counts from a dim3 (grid_smb.hpp:112-148)
+    SPARSEGRIDGPU_HD chunk_linearizer(const dim3 blockDimensions)
+    {
+        const unsigned int b[3] = {blockDimensions.x, blockDimensions.y, blockDimensions.z};
+        for (unsigned int d = 0; d < dim; ++d) {
+            blockSz[d] = (indexT)(d < 3 ? b[d] : 1u);
+            sz[d] = blockSz[d] * (indexT)blockEdgeSize;
+        }
+    }
+#endif
+    SPARSEGRIDGPU_HD const indexT (&getSize() const)[dim] { return sz; }
+    SPARSEGRIDGPU_HD const indexT &size(int i) const { return sz[i]; }
+    SPARSEGRIDGPU_HD const indexT (&getBlockSize() const)[dim] { return blockSz; }
+    // chunk id of chunk coordinates
+    template <typename CoordT> SPARSEGRIDGPU_HD indexT BlockLinId(const grid_key_dx<dim, CoordT> &bc) const
+    {
+        if (morton) {
+            uint64_t z = 0;
+            for (unsigned int d = 0; d < dim; ++d)
+                for (unsigned int i = 0; i < 64 / dim; ++i) z |= ((((uint64_t)bc.get(d)) >> i) & 1ull) << (dim * i + d);
+            return (indexT)z;
+        }
+        indexT b = (indexT)bc.get(dim - 1);
+        for (int d = (int)dim - 2; d >= 0; --d) b = b * blockSz[d] + (indexT)bc.get(d);
+        return b;
+    }
+    // chunk coordinates of chunk id b
+    SPARSEGRIDGPU_HD grid_key_dx<dim, int> BlockInvLinId(indexT b) const
+    {
+        grid_key_dx<dim, int> c;
+        for (unsigned int d = 0; d < dim; ++d) {
+            if (morton) {
+                uint64_t v = 0;
+                for (unsigned int i = 0; i < 64 / dim; ++i) v |= ((((uint64_t)b) >> (dim * i + d)) & 1ull) << i;
+                c.set_d(d, (int)v);
+            } else {
+                c.set_d(d, (int)(b % blockSz[d]));
+                b /= blockSz[d];
+            }
+        }
+        return c;
+    }
+    // point id = chunk id * blockSize + x-fastest offset (grid_smb.hpp:175-189)
+    template <typename CoordT> SPARSEGRIDGPU_HD indexT LinId(const grid_key_dx<dim, CoordT> &c) const
+    {
+        grid_key_dx<dim, CoordT> bc;
+        indexT l = 0;
+        for (int d = (int)dim - 1; d >= 0; --d) {
+            bc.set_d(d, (CoordT)(c.get(d) / (CoordT)blockEdgeSize));
+            l = l * blockEdgeSize + (indexT)(c.get(d) % (CoordT)blockEdgeSize);
+        }
+        return BlockLinId(bc) * blockSize + l;
+    }
+    SPARSEGRIDGPU_HD grid_key_dx<dim, int> InvLinId(indexT lin) const
+    {
+        grid_key_dx<dim, int> bc = BlockInvLinId(lin / blockSize), c;
+        indexT l = lin % blockSize;
+        for (unsigned int d = 0; d < dim; ++d) {
+            c.set_d(d, (int)(bc.get(d) * (int)blockEdgeSize + (int)(l % blockEdgeSize)));
+            l /= blockEdgeSize;
+        }
+        return c;
+    }
+
+protected:
+    indexT blockSz[dim];
+    indexT sz[dim];
+};
+}  // namespace sg
+
+template <unsigned int dim, unsigned int blockEdgeSize, typename indexT = long int>
+class grid_smb : public sg::chunk_linearizer<dim, blockEdgeSize, indexT, false>
+{
+public:
+    using sg::chunk_linearizer<dim, blockEdgeSize, indexT, false>::chunk_linearizer;
+    SPARSEGRIDGPU_HD grid_smb() {}
+};
+template <unsigned int dim, unsigned int blockEdgeSize, typename indexT = long int>
+class grid_zmb : public sg::chunk_linearizer<dim, blockEdgeSize, indexT, true>
+{
+public:
+    using sg::chunk_linearizer<dim, blockEdgeSize, indexT, true>::chunk_linearizer;
+    SPARSEGRIDGPU_HD grid_zmb() {}
+};
+
+template <unsigned int dim, typename AggregateT, unsigned int blockEdgeSize = default_edge<dim>::type::value,
+          unsigned int threadBlockSize = default_edge<dim>::tb::value, typename indexT = long int,
+          template <typename> class layout_base = memory_traits_inte,
+          typename linearizer = grid_smb<dim, blockEdgeSize, indexT>>
 class SparseGridGpu
 {
+    static_assert(std::is_same<layout_base<int>, memory_traits_inte<int>>::value,
+                  "the grid's arrays are one array per property (memory_traits_inte)");
+    // chunk ids are Morton codes (grid_zmb, SparseGridGpu_z) or row-major (grid_smb)
+    static constexpr bool zorder = linearizer::is_morton;
+
 public:
     static constexpr unsigned int dims = dim;
     static constexpr unsigned int blockEdgeSize_ = blockEdgeSize;
@@ -460,18 +639,17 @@ public:
         return s;
     }
 
+    // SparseGridGpu(const size_t (&res)[dim], r) (SparseGridGpu.hpp:1694-1698)
     explicit SparseGridGpu(const size_t (&res)[dim], unsigned int stencilSupportRadius = 1)
     {
-        int64_t sz[3] = {1, 1, 1};
-        for (unsigned int i = 0; i < dim; ++i) {
-            sz[i] = (int64_t)res[i];
-            size_[i] = res[i];
-            bsz_[i] = res[i] / blockEdgeSize + ((res[i] % blockEdgeSize) != 0);
-        }
-        int32_t dts[nprops > 0 ? nprops : 1];
-        fill_dtypes(dts, std::make_index_sequence<nprops>());
-        sg::check(sg_create((int)dim, sz, (int)blockEdgeSize, (int)nprops, dts, (int)stencilSupportRadius, &h_));
-        if (zorder) sg::check(sg_set_linearizer(h_, SG_LIN_ZMB));
+        init_(res, stencilSupportRadius);
+    }
+    // SparseGridGpu(linearizer &, r) (SparseGridGpu.hpp:1704-1714): the domain of a linearizer
+    explicit SparseGridGpu(const linearizer &gridGeometry, unsigned int stencilSupportRadius = 1)
+    {
+        size_t res[dim];
+        for (unsigned int i = 0; i < dim; ++i) res[i] = (size_t)gridGeometry.getSize()[i];
+        init_(res, stencilSupportRadius);
     }
     ~SparseGridGpu() { sg_destroy(h_); }
     SparseGridGpu(const SparseGridGpu &) = delete;
@@ -489,7 +667,10 @@ public:
         for (unsigned int q = 0; q < (nprops > 0 ? nprops : 1); ++q) {
             host_prop_[q] = std::move(o.host_prop_[q]);
             host_has_[q] = o.host_has_[q];
+            std::memcpy(bg_[q], o.bg_[q], sizeof(bg_[q]));
         }
+        host_dirty_ = o.host_dirty_;
+        nn_type_ = o.nn_type_;
     }
 
     sg_grid *handle() { return h_; }
@@ -543,7 +724,17 @@ public:
     }
 
     // -- background / insertion / flush --
-    template <unsigned int p> void setBackgroundValue(prop_type<p> v) { sg::check(sg_set_background(h_, (int)p, &v)); }
+    template <unsigned int p> void setBackgroundValue(prop_type<p> v)
+    {
+        sg::check(sg_set_background(h_, (int)p, &v));
+        std::memcpy(bg_[p], &v, sizeof(v));
+    }
+    template <unsigned int p> prop_type<p> getBackgroundValue() const
+    {
+        prop_type<p> v;
+        std::memcpy(&v, bg_[p], sizeof(v));
+        return v;
+    }
 #if !defined(__HIPCC__)
     // host inserts only: the insert buffer grows on demand (the device insert buffer of
     // hipcc translation units is below)
@@ -576,39 +767,90 @@ public:
         ins_view_ok_ = false;  // the flush took the records of the kernels' insert buffer
     }
 
-    // insertFlush<p>(coord) = value: insert one point and flush it immediately
-    template <unsigned int p> struct insert_ref {
-        SparseGridGpu *g;
-        indexT key;
-        insert_ref &operator=(prop_type<p> v)
-        {
-            const void *vals[nprops > 0 ? nprops : 1] = {};
-            vals[p] = &v;
-            g->insertPoints(&key, 1, vals);
-            int32_t ops[nprops > 0 ? nprops : 1];
-            for (unsigned int i = 0; i < nprops; ++i) ops[i] = SG_OP_NONE;
-            sg::check(sg_flush(g->h_, ops));
-            return *this;
-        }
-    };
-    template <unsigned int p, typename CoordT> insert_ref<p> insertFlush(const grid_key_dx<dim, CoordT> &coord)
+    // insertFlush<p>(coord) (SparseGridGpu.hpp:2780-2845, BlockMapGpu.hpp:186-222): insert the
+    // point into the host copy (taken from the device, every property, when there is none): a
+    // new chunk goes to its sorted place with mask 0 and the background, the point's mask
+    // becomes EXIST (1, as the reference writes, clearing PADDING), its other properties keep
+    // their values; returns the point's value of p in the host copy.  hostToDevice() sends the
+    // edited copy (chunks, masks, properties) to the device; until then get<p>(coord) reads it.
+    template <unsigned int p, typename CoordT> prop_type<p> &insertFlush(const grid_key_dx<dim, CoordT> &coord)
     {
-        return insert_ref<p>{this, getLinId(coord)};
+        if (!host_copy_) deviceToHost();
+        if (!host_has_[p]) {
+            if (host_dirty_) throw sg::error("insertFlush: property not in the edited host copy");
+            host_prop_[p].resize(host_mask_.size() * sizeof(prop_type<p>) + 1);
+            if (!host_mask_.empty()) sg::check(sg_get_prop(h_, (int)p, host_prop_[p].data(), SG_MEM_HOST));
+            host_has_[p] = true;
+        }
+        const indexT lin = getLinId(coord);
+        const size_t s = host_chunk_(lin / (indexT)chunk_size(), true) * chunk_size() + (size_t)(lin % (indexT)chunk_size());
+        host_mask_[s] = (uint8_t)SG_EXIST;
+        host_dirty_ = true;
+        return reinterpret_cast<prop_type<p> *>(host_prop_[p].data())[s];
+    }
+    // insertBlockFlush(coord, local_id) (SparseGridGpu.hpp:2799-2811): as insertFlush, returning
+    // the chunk's host arrays (get<p>()[local_id]) and the point's offset in local_id
+    struct host_block {
+        SparseGridGpu *g;
+        size_t base;
+        template <unsigned int q> prop_type<q> *get() const
+        {
+            if (!g->host_has_[q]) throw sg::error("insertBlockFlush: property not in the host copy");
+            return reinterpret_cast<prop_type<q> *>(g->host_prop_[q].data()) + base;
+        }
+        uint8_t *getMask() const { return g->host_mask_.data() + base; }
+    };
+    template <typename CoordT> host_block insertBlockFlush(const grid_key_dx<dim, CoordT> &coord, indexT &local_id)
+    {
+        if (!host_copy_) deviceToHost();
+        const indexT lin = getLinId(coord);
+        const size_t base = host_chunk_(lin / (indexT)chunk_size(), true) * chunk_size();
+        local_id = lin % (indexT)chunk_size();
+        host_mask_[base + (size_t)local_id] = (uint8_t)SG_EXIST;
+        host_dirty_ = true;
+        return host_block{this, base};
     }
 
     // -- neighbourhood, boundaries, stencils --
     template <typename NN = NNStar<dim>> void findNeighbours() { sg::check(sg_find_neighbours(h_, NN::type)); }
-    template <typename NN> void setNNType() { sg::check(sg_set_nn_type(h_, NN::type)); }
+    template <typename NN> void setNNType()
+    {
+        sg::check(sg_set_nn_type(h_, NN::type));
+        nn_type_ = NN::type;
+    }
     template <typename NN = NNStar<dim>, typename checker_type = No_check>
     void tagBoundaries(gpu::ofp_context_t &, checker_type = checker_type(),
                        tag_boundaries opt = NO_CALCULATE_EXISTING_POINTS)
     {
         sg::check(sg_tag_boundaries(h_, NN::type, (int)opt));
     }
-    template <typename Stencil, typename... Args> void applyStencils(const Box<dim, int> &box, StencilMode, Args... args)
+    // applyStencils<Stencil>(box, mode, args...) (SparseGridGpu.hpp:2704-2734).  A named
+    // stencil of this header (one with a static desc(box, args...)) runs as a library kernel;
+    // any other stencil is a user functor with the reference's protocol (typedef stencil_type;
+    // supportRadius; __device__ static stencil(sparseGrid, chunkId, chunkPos, offset, coord,
+    // dataBlockLoad, dataBlockStore, curMask, args...)), run by a kernel of
+    // SparseGridGpu_device.hpp in translation units compiled by hipcc.  After a functor has
+    // run, slots without a point are given the background again.
+    template <typename Stencil, typename... Args> void applyStencils(const Box<dim, int> &box, StencilMode mode, Args... args)
     {
-        sg_stencil_desc d = Stencil::desc(box, args...);
-        sg::check(sg_apply_stencil(h_, &d));
+        if constexpr (sg::has_desc<Stencil, Box<dim, int>, Args...>::value) {
+            sg_stencil_desc d = Stencil::desc(box, args...);
+            d.mode = (int)mode;
+            sg::check(sg_apply_stencil(h_, &d));
+        } else {
+#if defined(__HIPCC__)
+            apply_functor_<Stencil>(box, mode, args...);
+#else
+            static_assert(sg::has_desc<Stencil, Box<dim, int>, Args...>::value,
+                          "user stencil functors run in translation units compiled by hipcc");
+#endif
+        }
+    }
+    template <typename Stencil1, typename Stencil2, typename... Others, typename... Args>
+    void applyStencils(const Box<dim, int> &box, StencilMode mode, Args... args)
+    {
+        applyStencils<Stencil1>(box, mode, args...);
+        applyStencils<Stencil2, Others...>(box, mode, args...);
     }
 
 #if defined(__HIPCC__)
@@ -621,14 +863,20 @@ public:
         sg::check(sg_insert_device_begin(h_, count3(nBlock) * count3(nSlot), &ins_view_));
         ins_view_ok_ = true;
     }
-    // toKernel() (SparseGridGpu.hpp:1716-1751): the device view for insertBlock / insert<p>
-    sg::sparse_grid_gpu_ker<dim, blockEdgeSize, props_tuple, indexT> toKernel() const
+    // toKernel() (SparseGridGpu.hpp:1716-1751): the device view: reads (get<p>, exist,
+    // isPadding, get_sparse, getNNPoint), the stencil-functor helpers, and insertBlock /
+    // insert<p> when setGPUInsertBuffer opened an insert buffer.  It stays valid until the
+    // next call that changes the chunk set or a property array (flush, remove, ...).
+    typedef sg::sparse_grid_gpu_ker<dim, blockEdgeSize, props_tuple, indexT> kernel_view;
+    kernel_view toKernel() const
     {
-        if (!ins_view_ok_) throw sg::error("toKernel: call setGPUInsertBuffer first (the device view inserts)");
-        sg::sparse_grid_gpu_ker<dim, blockEdgeSize, props_tuple, indexT> k;
-        k.iv = ins_view_;
+        kernel_view k;
+        std::memset((void *)&k, 0, sizeof(k));
+        if (ins_view_ok_) k.iv = ins_view_;
+        sg::check(sg_to_kernel(h_, SG_NN_FULL, &k.gv));
         for (unsigned int q = 0; q < 3; ++q) k.bsz[q] = q < dim ? (int64_t)bsz_[q] : 1;
         k.zorder = zorder ? 1 : 0;
+        k.nn_star = nn_type_ == SG_NN_STAR;
         return k;
     }
 
@@ -700,7 +948,53 @@ public:
         run_conv<stencil_size, 5, prop_src1, prop_src2, prop_src3, prop_dst1, prop_dst2, prop_dst3>(start, stop, func);
     }
 
+    // the reference's forms with grid_key_dx<3> corners (SparseGridGpu.hpp:2568-2683), so that
+    // braced corners such as {0, 0} compile in any dimension
+    template <unsigned int prop_src, unsigned int prop_dst, unsigned int stencil_size, typename lambda_f, typename... ArgsT>
+    void conv(grid_key_dx<3> start, grid_key_dx<3> stop, lambda_f func, ArgsT... args)
+    {
+        conv<prop_src, prop_dst, stencil_size>(key_of_(start), key_of_(stop), func, args...);
+    }
+    template <unsigned int prop_src, unsigned int prop_dst, unsigned int stencil_size, typename lambda_f, typename... ArgsT>
+    void conv_cross(grid_key_dx<3> start, grid_key_dx<3> stop, lambda_f func, ArgsT... args)
+    {
+        conv_cross<prop_src, prop_dst, stencil_size>(key_of_(start), key_of_(stop), func, args...);
+    }
+    template <unsigned int prop_src, unsigned int prop_dst, unsigned int stencil_size, typename lambda_f, typename... ArgsT>
+    void conv_cross_b(grid_key_dx<3> start, grid_key_dx<3> stop, lambda_f func, ArgsT... args)
+    {
+        conv_cross_b<prop_src, prop_dst, stencil_size>(key_of_(start), key_of_(stop), func, args...);
+    }
+    template <unsigned int prop_src1, unsigned int prop_src2, unsigned int prop_dst1, unsigned int prop_dst2,
+              unsigned int stencil_size, typename lambda_f, typename... ArgsT>
+    void conv2(grid_key_dx<3> start, grid_key_dx<3> stop, lambda_f func, ArgsT... args)
+    {
+        conv2<prop_src1, prop_src2, prop_dst1, prop_dst2, stencil_size>(key_of_(start), key_of_(stop), func, args...);
+    }
+    template <unsigned int prop_src1, unsigned int prop_src2, unsigned int prop_dst1, unsigned int prop_dst2,
+              unsigned int stencil_size, typename lambda_f, typename... ArgsT>
+    void conv2_b(grid_key_dx<3> start, grid_key_dx<3> stop, lambda_f func, ArgsT... args)
+    {
+        conv2_b<prop_src1, prop_src2, prop_dst1, prop_dst2, stencil_size>(key_of_(start), key_of_(stop), func, args...);
+    }
+    template <unsigned int prop_src1, unsigned int prop_src2, unsigned int prop_src3, unsigned int prop_dst1,
+              unsigned int prop_dst2, unsigned int prop_dst3, unsigned int stencil_size, typename lambda_f,
+              typename... ArgsT>
+    void conv3_b(grid_key_dx<3> start, grid_key_dx<3> stop, lambda_f func, ArgsT... args)
+    {
+        conv3_b<prop_src1, prop_src2, prop_src3, prop_dst1, prop_dst2, prop_dst3, stencil_size>(key_of_(start),
+                                                                                                key_of_(stop), func,
+                                                                                                args...);
+    }
+
 private:
+    // (int corners: the forms above then call the deduced-corner forms, never themselves)
+    static grid_key_dx<dim, int> key_of_(const grid_key_dx<3> &k)
+    {
+        grid_key_dx<dim, int> r;
+        for (unsigned int d = 0; d < dim; ++d) r.set_d(d, (int)k.get(d));
+        return r;
+    }
     static int64_t count3(const dim3 &d) { return (int64_t)d.x * d.y * d.z; }
     static int64_t count3(int64_t n) { return n; }
     sg_insert_view ins_view_ = {};
@@ -723,10 +1017,22 @@ public:
 #endif
 
     // -- access --
+    // get<p>(coord) (SparseGridGpu.hpp:1897-1901): the point's value, the background where it
+    // does not exist; read from the host copy while it holds edits not yet sent by
+    // hostToDevice (insertFlush), else from the device
     template <unsigned int p, typename CoordT> prop_type<p> get(const grid_key_dx<dim, CoordT> &coord) const
     {
         int64_t k = (int64_t)getLinId(coord);
         prop_type<p> v;
+        if (host_dirty_ && host_has_[p]) {
+            const size_t c = const_cast<SparseGridGpu *>(this)->host_chunk_(k / (int64_t)chunk_size(), false);
+            const size_t s = c * chunk_size() + (size_t)(k % (int64_t)chunk_size());
+            if (c < host_ids_.size() && (host_mask_[s] & SG_EXIST))
+                std::memcpy(&v, host_prop_[p].data() + s * sizeof(v), sizeof(v));
+            else
+                std::memcpy(&v, bg_[p], sizeof(v));
+            return v;
+        }
         sg::check(sg_get_points(h_, (int)p, 1, &k, &v, SG_MEM_HOST));
         return v;
     }
@@ -762,6 +1068,8 @@ public:
     template <unsigned int... p> void deviceToHost()
     {
         sg::check(sg_synchronize(h_));
+        host_dirty_ = false;
+        for (unsigned int q = 0; q < (nprops > 0 ? nprops : 1); ++q) host_has_[q] = false;
         host_ids_ = private_get_index_array();
         host_mask_ = private_get_mask_array();
         for (unsigned int q = 0; q < nprops; ++q)
@@ -772,16 +1080,21 @@ public:
             }
         host_copy_ = true;
     }
+    // hostToDevice<p...>() (BlockMapGpu.hpp:230-241): the chunk list and the masks of the host
+    // copy (with the chunks insertFlush added), then properties p... (every property of the
+    // copy for an empty list); device properties not sent keep their values per chunk
     template <unsigned int... p> void hostToDevice()
     {
         require_host_copy();
-        if ((size_t)sg_num_chunks(h_) != host_ids_.size())
-            throw sg::error("hostToDevice: the grid's chunks changed since deviceToHost");
         for (unsigned int q = 0; q < nprops; ++q)
-            if ((sizeof...(p) == 0 && host_has_[q]) || (sizeof...(p) != 0 && listed<p...>(q))) {
-                if (!host_has_[q]) throw sg::error("hostToDevice: property not in the host copy");
+            if (sizeof...(p) != 0 && listed<p...>(q) && !host_has_[q])
+                throw sg::error("hostToDevice: property not in the host copy");
+        std::vector<int64_t> ids(host_ids_.begin(), host_ids_.end());
+        sg::check(sg_set_chunks(h_, (int64_t)ids.size(), ids.data(), host_mask_.data(), SG_MEM_HOST));
+        for (unsigned int q = 0; q < nprops; ++q)
+            if ((sizeof...(p) == 0 && host_has_[q]) || (sizeof...(p) != 0 && listed<p...>(q)))
                 if (!host_mask_.empty()) sg::check(sg_set_prop(h_, (int)q, host_prop_[q].data(), SG_MEM_HOST));
-            }
+        host_dirty_ = false;
     }
     template <unsigned int p> prop_type<p> &get(const sparse_grid_gpu_index<SparseGridGpu> &k)
     {
@@ -1038,7 +1351,86 @@ private:
         const size_t i = (size_t)k.get_cnk_pos_id() * chunk_size() + k.get_data_id();
         return reinterpret_cast<prop_type<p> *>(host_prop_[p].data())[i];
     }
+    // position of chunk id b in the host copy; when absent, the position it would take, where
+    // insert == true puts it (mask 0, the background in every copied property)
+    size_t host_chunk_(indexT b, bool insert)
+    {
+        const auto it = std::lower_bound(host_ids_.begin(), host_ids_.end(), b);
+        const size_t c = (size_t)(it - host_ids_.begin());
+        if (it != host_ids_.end() && *it == b) return c;
+        if (!insert) return host_ids_.size();
+        const size_t cs = chunk_size();
+        host_ids_.insert(it, b);
+        host_mask_.insert(host_mask_.begin() + c * cs, cs, (uint8_t)0);
+        for (unsigned int q = 0; q < nprops; ++q) {
+            if (!host_has_[q]) continue;
+            const size_t ts = prop_bytes(q);
+            std::vector<unsigned char> bgc(cs * ts);
+            for (size_t i = 0; i < cs; ++i) std::memcpy(bgc.data() + i * ts, bg_[q], ts);
+            host_prop_[q].insert(host_prop_[q].begin() + c * cs * ts, bgc.begin(), bgc.end());
+        }
+        return c;
+    }
+    void init_(const size_t (&res)[dim], unsigned int stencilSupportRadius)
+    {
+        int64_t sz[3] = {1, 1, 1};
+        for (unsigned int i = 0; i < dim; ++i) {
+            sz[i] = (int64_t)res[i];
+            size_[i] = res[i];
+            bsz_[i] = res[i] / blockEdgeSize + ((res[i] % blockEdgeSize) != 0);
+        }
+        int32_t dts[nprops > 0 ? nprops : 1];
+        fill_dtypes(dts, std::make_index_sequence<nprops>());
+        sg::check(sg_create((int)dim, sz, (int)blockEdgeSize, (int)nprops, dts, (int)stencilSupportRadius, &h_));
+        if (zorder) sg::check(sg_set_linearizer(h_, SG_LIN_ZMB));
+    }
+#if defined(__HIPCC__)
+    template <typename Stencil, typename... Args> void apply_functor_(const Box<dim, int> &box, StencilMode mode, Args... args)
+    {
+        static_assert(Stencil::supportRadius <= 1, "stencil functors of support radius 1");
+        static_assert(chunk_size() <= 1024, "one thread per chunk slot: at most 1024 points per chunk");
+        kernel_view k = toKernel();
+        k.nn_star = Stencil::stencil_type::type == SG_NN_STAR;
+        if (k.gv.nchunks == 0) return;
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (unsigned int d = 0; d < dim; ++d) {
+            lo[d] = box.getLow(d);
+            hi[d] = box.getHigh(d);
+        }
+        hipStream_t st = (hipStream_t)k.gv.stream;
+        if (mode == STENCIL_MODE_INPLACE_NO_SHARED) {
+            int64_t n = 0;
+            sg::check(sg_existing_points(h_, &n));
+            if (n > 0) {
+                int64_t *ep = nullptr;
+                if (hipMalloc((void **)&ep, (size_t)n * sizeof(int64_t)) != hipSuccess)
+                    throw sg::error("applyStencils: point list allocation failed");
+                const int rc = sg_get_existing_points(h_, ep, SG_MEM_DEVICE);
+                if (rc == SG_OK) {
+                    hipLaunchKernelGGL((sg::apply_stencil_functor_noshared_kernel<Stencil, kernel_view, Args...>),
+                                       dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, k, (const int64_t *)ep, n,
+                                       lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], args...);
+                }
+                const hipError_t le = hipGetLastError();
+                (void)hipStreamSynchronize(st);
+                (void)hipFree(ep);
+                sg::check(rc);
+                if (le != hipSuccess) throw sg::error(std::string("applyStencils: ") + hipGetErrorString(le));
+            }
+        } else {
+            hipLaunchKernelGGL((sg::apply_stencil_functor_kernel<Stencil, kernel_view, Args...>),
+                               dim3((unsigned)k.gv.nchunks), dim3(chunk_size()), 0, st, k, lo[0], lo[1], lo[2], hi[0],
+                               hi[1], hi[2], args...);
+            const hipError_t le = hipGetLastError();
+            if (le != hipSuccess) throw sg::error(std::string("applyStencils: ") + hipGetErrorString(le));
+        }
+        sg::check(sg_restore_background(h_, -1));
+    }
+#endif
     bool host_copy_ = false;
+    bool host_dirty_ = false;  // the host copy holds insertFlush edits not yet on the device
+    int nn_type_ = SG_NN_STAR;
+    unsigned char bg_[nprops > 0 ? nprops : 1][8] = {};
     std::vector<indexT> host_ids_;
     std::vector<uint8_t> host_mask_;
     std::vector<unsigned char> host_prop_[nprops > 0 ? nprops : 1];
@@ -1057,9 +1449,22 @@ private:
     bool ins_view_ok_ = false;
 };
 
-// SparseGridGpu_z (src/SparseGridGpu/SparseGridGpu.hpp:3947-3972): Morton chunk ids
-template <unsigned int dim, typename AggregateT, unsigned int blockEdgeSize = default_edge<dim>::value,
-          unsigned int threadBlockSize = 512, typename indexT = long int>
-using SparseGridGpu_z = SparseGridGpu<dim, AggregateT, blockEdgeSize, threadBlockSize, indexT, true>;
+// SparseGridGpu_z / SparseGridGpu_zi (Morton chunk ids) and SparseGridGpu_i (int indices)
+// (src/SparseGridGpu/SparseGridGpu.hpp:3947-3972)
+template <unsigned int dim, typename AggregateT, unsigned int blockEdgeSize = default_edge<dim>::type::value,
+          unsigned int threadBlockSize = default_edge<dim>::tb::value, typename indexT = long int,
+          template <typename> class layout_base = memory_traits_inte,
+          typename linearizer = grid_zmb<dim, blockEdgeSize, indexT>>
+using SparseGridGpu_z = SparseGridGpu<dim, AggregateT, blockEdgeSize, threadBlockSize, indexT, layout_base, linearizer>;
+template <unsigned int dim, typename AggregateT, unsigned int blockEdgeSize = default_edge<dim>::type::value,
+          unsigned int threadBlockSize = default_edge<dim>::tb::value, typename indexT = int,
+          template <typename> class layout_base = memory_traits_inte,
+          typename linearizer = grid_zmb<dim, blockEdgeSize, indexT>>
+using SparseGridGpu_zi = SparseGridGpu<dim, AggregateT, blockEdgeSize, threadBlockSize, indexT, layout_base, linearizer>;
+template <unsigned int dim, typename AggregateT, unsigned int blockEdgeSize = default_edge<dim>::type::value,
+          unsigned int threadBlockSize = default_edge<dim>::tb::value, typename indexT = int,
+          template <typename> class layout_base = memory_traits_inte,
+          typename linearizer = grid_smb<dim, blockEdgeSize, indexT>>
+using SparseGridGpu_i = SparseGridGpu<dim, AggregateT, blockEdgeSize, threadBlockSize, indexT, layout_base, linearizer>;
 
 #endif  // SPARSEGRIDGPU_FACADE_HPP

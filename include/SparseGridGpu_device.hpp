@@ -310,25 +310,52 @@ template <typename Tuple> struct insert_encap {
     }
 };
 
-// toKernel() (SparseGridGpu.hpp:1716-1751) for device inserts (BlockMapGpu_ker.cuh:171-227,
-// SparseGridGpu_ker.cuh:445-490): init / insertBlock / insert<p> / flush_block_insert.
-// insertBlock claims one chunk record per distinct chunk id among the calling lanes of a
-// wave (the reference shares one per CUDA block); the records are merged by the next
-// flush in claim order.
+// toKernel() (SparseGridGpu.hpp:1716-1751, SparseGridGpu_ker.cuh): the device view of a grid.
+//  - reads (SparseGridGpu_ker.cuh:296-443, :670-693; BlockMapGpu_ker.cuh:105-135): get<p>(coord)
+//    (the background where the point does not exist), get_sparse, exist, isPadding,
+//    getNNPoint, get<p>(block_offset);
+//  - the stencil-functor protocol of applyStencils (SparseGridGpu.hpp:954-1048, kernel
+//    SparseGridGpu_kernels.cuh:1037-1094): loadGhostBlock / storeBlock and the enlarged-block
+//    index helpers, for one workgroup of chunk-size threads per chunk;
+//  - device inserts (BlockMapGpu_ker.cuh:171-227, SparseGridGpu_ker.cuh:445-490): init /
+//    insertBlock / insert<p> / flush_block_insert, when setGPUInsertBuffer opened a buffer.
+//    insertBlock claims one chunk record per distinct chunk id among the calling lanes of a
+//    wave (the reference shares one per CUDA block); the records are merged by the next
+//    flush in claim order.
+// Ghost loads follow the neighbourhood type of the stencil (nn_star): NNStar reads the 2 dim
+// face neighbours (background outside the domain; edge and corner cells hold the background,
+// where the reference reads an unrelated neighbour), NNFull reads all 3^dim - 1 through the
+// full neighbour table, exactly as the reference's table (so at the low x / y faces an
+// out-of-domain neighbour aliases a chunk of the previous row, grid_smb.hpp:258-276).
 template <unsigned int dim, unsigned int B, typename Tuple, typename indexT> struct sparse_grid_gpu_ker {
     static constexpr unsigned int dims = dim;
     static constexpr unsigned int d = dim;
     static constexpr unsigned int blockEdgeSize_ = B;
     static constexpr unsigned int pMask = std::tuple_size<Tuple>::value;
     static constexpr unsigned int chunk_size = (dim == 3) ? B * B * B : B * B;
-    sg_insert_view iv;
+    static constexpr unsigned int blockSize = chunk_size;
+    static constexpr unsigned int stencilSupportRadius = 1;
+    static constexpr int EB = (int)B + 2;  // edge of the enlarged block (radius 1)
+    typedef typename block_aggregate_of<chunk_size, Tuple>::type AggregateBlockType;
+    typedef AggregateBlockType AggregateType;
+    typedef indexT indexT_;
+    template <unsigned int p>
+    using elem_t = typename std::conditional<(p < pMask), typename std::tuple_element<(p < pMask ? p : 0), Tuple>::type,
+                                             unsigned char>::type;
+
+    sg_insert_view iv;  // device insert buffer (count == nullptr: none open)
+    sg_kernel_view gv;  // the grid's arrays and full neighbour table
     int64_t bsz[3];
     int zorder;
+    int nn_star;        // ghost loads with NNStar semantics (else NNFull)
 
     __device__ void init() const {}
     __device__ void flush_block_insert() const {}
     __device__ static constexpr unsigned int getBlockEdgeSize() { return B; }
+    __device__ static constexpr unsigned int getBlockSize() { return chunk_size; }
+    __device__ static constexpr unsigned int getEnlargedBlockSize() { return dim == 3 ? EB * EB * EB : EB * EB; }
 
+    // ---- geometry (grid_smb / grid_zmb) ----
     // chunk id of chunk coordinates (grid_smb row-major or grid_zmb Morton)
     __device__ indexT chunk_id(const int64_t (&c)[3]) const
     {
@@ -342,6 +369,19 @@ template <unsigned int dim, unsigned int B, typename Tuple, typename indexT> str
         for (int q = (int)dim - 2; q >= 0; --q) v = v * bsz[q] + c[q];
         return (indexT)v;
     }
+    __device__ void chunk_coords(int64_t id, int64_t (&c)[3]) const
+    {
+        c[0] = c[1] = c[2] = 0;
+        if (zorder) {
+            for (int i = 0; i < 64 / (int)dim; ++i)
+                for (unsigned int q = 0; q < dim; ++q) c[q] |= ((id >> (dim * i + q)) & 1) << i;
+            return;
+        }
+        for (unsigned int q = 0; q < dim; ++q) {
+            c[q] = id % bsz[q];
+            id /= bsz[q];
+        }
+    }
     // point key = chunk id * chunk size + x-fastest offset inside the chunk
     template <typename CoordT> __device__ indexT getLinId(const grid_key_dx<dim, CoordT> &k) const
     {
@@ -354,9 +394,250 @@ template <unsigned int dim, unsigned int B, typename Tuple, typename indexT> str
         }
         return chunk_id(c) * (indexT)chunk_size + off;
     }
-
-    __device__ insert_encap<Tuple> insertBlock(indexT blockId) const
+    __device__ grid_key_dx<dim, int> getCoord(indexT lin) const
     {
+        int64_t c[3];
+        chunk_coords((int64_t)(lin / (indexT)chunk_size), c);
+        int off = (int)(lin % (indexT)chunk_size);
+        grid_key_dx<dim, int> k;
+        for (unsigned int q = 0; q < dim; ++q) {
+            k.set_d(q, (int)(c[q] * B + off % (int)B));
+            off /= (int)B;
+        }
+        return k;
+    }
+    template <typename CoordT> __device__ indexT getBlockLinId(const grid_key_dx<dim, CoordT> &blockCoord) const
+    {
+        int64_t c[3] = {0, 0, 0};
+        for (unsigned int q = 0; q < dim; ++q) c[q] = (int64_t)blockCoord.get(q);
+        return chunk_id(c);
+    }
+    __device__ grid_key_dx<dim, int> getBlockCoord(indexT blockId) const
+    {
+        int64_t c[3];
+        chunk_coords((int64_t)blockId, c);
+        grid_key_dx<dim, int> k;
+        for (unsigned int q = 0; q < dim; ++q) k.set_d(q, (int)c[q]);
+        return k;
+    }
+    // global coordinates of slot `offset` of the chunk at chunk coordinates blockCoord
+    template <typename CoordT>
+    __device__ grid_key_dx<dim, CoordT> getGlobalCoord(const grid_key_dx<dim, CoordT> &blockCoord, unsigned int offset) const
+    {
+        grid_key_dx<dim, CoordT> k;
+        for (unsigned int q = 0; q < dim; ++q) {
+            k.set_d(q, (CoordT)(blockCoord.get(q) * (CoordT)B + (CoordT)(offset % B)));
+            offset /= B;
+        }
+        return k;
+    }
+
+    // ---- reads ----
+    // position of chunk `blockId` in the chunk list, or nchunks (the background chunk)
+    __device__ int64_t chunk_pos(int64_t blockId) const
+    {
+        int64_t lo = 0, hi = gv.nchunks;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (gv.keys[mid] < blockId) lo = mid + 1;
+            else hi = mid;
+        }
+        return (lo < gv.nchunks && gv.keys[lo] == blockId) ? lo : gv.nchunks;
+    }
+    // get_sparse (SparseGridGpu_ker.cuh:409-414): chunk position (nchunks where the chunk does
+    // not exist, or the point is outside the domain) and offset of a point
+    template <typename CoordT>
+    __device__ void get_sparse(const grid_key_dx<dim, CoordT> &coord, unsigned int &dataBlockPos, unsigned int &offset) const
+    {
+        bool in = true;
+        for (unsigned int q = 0; q < dim; ++q) in = in && coord.get(q) >= 0 && (int64_t)coord.get(q) < gv.size[q];
+        int off = 0, mul = 1;
+        for (unsigned int q = 0; q < dim; ++q) {
+            off += (int)(((int64_t)coord.get(q) % B + B) % B) * mul;
+            mul *= (int)B;
+        }
+        offset = (unsigned int)off;
+        dataBlockPos = (unsigned int)(in ? chunk_pos((int64_t)(getLinId(coord) / (indexT)chunk_size)) : gv.nchunks);
+    }
+    // get<p>(coord) (SparseGridGpu_ker.cuh:401-406, BlockMapGpu_ker.cuh:119-135)
+    template <unsigned int p, typename CoordT> __device__ elem_t<p> get(const grid_key_dx<dim, CoordT> &coord) const
+    {
+        unsigned int pos, off;
+        get_sparse(coord, pos, off);
+        const int64_t s = (int64_t)pos * chunk_size + off;
+        const elem_t<p> *a = (const elem_t<p> *)gv.prop[p];
+        return (gv.mask[s] & SG_EXIST) ? a[s] : a[gv.nchunks * chunk_size + off];
+    }
+    // get<p>(block_offset) (SparseGridGpu_ker.cuh:424-443): the value at a chunk position
+    template <unsigned int p, typename CoordT> __device__ elem_t<p> &get(const block_offset<CoordT> &bo) const
+    {
+        elem_t<p> *a = (p < pMask) ? (elem_t<p> *)gv.prop[p < pMask ? p : 0] : (elem_t<p> *)gv.mask;
+        return a[(int64_t)bo.pos * chunk_size + bo.off];
+    }
+    template <typename CoordT> __device__ unsigned char getMask(const grid_key_dx<dim, CoordT> &coord) const
+    {
+        unsigned int pos, off;
+        get_sparse(coord, pos, off);
+        return gv.mask[(int64_t)pos * chunk_size + off];
+    }
+    template <typename CoordT> __device__ bool exist(const grid_key_dx<dim, CoordT> &coord) const
+    {
+        return getMask(coord) & SG_EXIST;
+    }
+    template <typename CoordT> __device__ bool isPadding(const grid_key_dx<dim, CoordT> &coord) const
+    {
+        return getMask(coord) & SG_PADDING;
+    }
+    // mask-byte helpers of BlockMapGpu_ker / SparseGridGpu_ker (bit 0 EXIST, bit 1 PADDING)
+    template <typename BitMaskT> __device__ static bool getBit(const BitMaskT &m, unsigned char pos) { return (m >> pos) & 1u; }
+    template <typename BitMaskT> __device__ static void setBit(BitMaskT &m, unsigned char pos) { m = m | (1u << pos); }
+    template <typename BitMaskT> __device__ static void unsetBit(BitMaskT &m, unsigned char pos) { m = m & ~(1u << pos); }
+    template <typename BitMaskT> __device__ static bool exist(const BitMaskT &m) { return getBit(m, 0); }
+    template <typename BitMaskT> __device__ static void setExist(BitMaskT &m) { setBit(m, 0); }
+    template <typename BitMaskT> __device__ static void unsetExist(BitMaskT &m) { unsetBit(m, 0); }
+    template <typename BitMaskT> __device__ static bool isPadding(const BitMaskT &m) { return getBit(m, 1); }
+    template <typename BitMaskT> __device__ static void setPadding(BitMaskT &m) { setBit(m, 1); }
+    template <typename BitMaskT> __device__ static void unsetPadding(BitMaskT &m) { unsetBit(m, 1); }
+
+    // position of the neighbour chunk of the chunk at `pos` in direction dd (entries -1/0/1)
+    // under the stencil's neighbourhood (see above); nchunks = the background chunk
+    __device__ int64_t neighbour_chunk(int64_t pos, const int (&dd)[3], bool star) const
+    {
+        int k = 0, pw = 1, nz = 0;
+        for (unsigned int q = 0; q < dim; ++q) {
+            k += (dd[q] + 1) * pw;
+            pw *= 3;
+            nz += dd[q] != 0;
+        }
+        if (nz == 0 || pos >= gv.nchunks) return pos;
+        if (!star) return gv.nn[pos * gv.nnn + k];
+        if (nz > 1) return gv.nchunks;
+        int64_t c[3];
+        chunk_coords(gv.keys[pos], c);
+        for (unsigned int q = 0; q < dim; ++q) {
+            const int64_t v = c[q] + dd[q];
+            if (v < 0 || v >= bsz[q]) return gv.nchunks;
+        }
+        return gv.nn[pos * gv.nnn + k];
+    }
+    // getNNPoint<NN>(pos, offset, mov) (SparseGridGpu_ker.cuh:296-327): the chunk position and
+    // offset of the point at displacement mov (at most one chunk edge) from slot `offset` of the
+    // chunk at pos
+    template <typename NN_type, typename indexT2>
+    __device__ block_offset<indexT2> getNNPoint(openfpm::sparse_index<unsigned int> pos, unsigned int offset,
+                                                const grid_key_dx<dim, indexT2> &mov) const
+    {
+        int dd[3] = {0, 0, 0};
+        int off = 0, mul = 1;
+        for (unsigned int q = 0; q < dim; ++q) {
+            int l = (int)(offset % B) + (int)mov.get(q);
+            offset /= B;
+            dd[q] = (l < 0) ? -1 : (l >= (int)B ? 1 : 0);
+            l -= dd[q] * (int)B;
+            off += l * mul;
+            mul *= (int)B;
+        }
+        block_offset<indexT2> bo;
+        bo.pos = (indexT2)neighbour_chunk((int64_t)pos.id, dd, NN_type::type == SG_NN_STAR);
+        bo.off = (indexT2)off;
+        return bo;
+    }
+
+    // ---- enlarged block (chunk + ghost layer of radius 1, x fastest, in LDS) ----
+    __device__ grid_key_dx<dim, int> getCoordInEnlargedBlock(unsigned int offset) const
+    {
+        grid_key_dx<dim, int> k;
+        for (unsigned int q = 0; q < dim; ++q) {
+            k.set_d(q, (int)(offset % B) + 1);
+            offset /= B;
+        }
+        return k;
+    }
+    template <typename CoordT> __device__ static unsigned int enlarged_lin(const grid_key_dx<dim, CoordT> &c)
+    {
+        unsigned int lin = 0;
+        for (int q = (int)dim - 1; q >= 0; --q) lin = lin * EB + (unsigned int)c.get(q);
+        return lin;
+    }
+    __device__ unsigned int getLinIdInEnlargedBlock(unsigned int offset) const
+    {
+        return enlarged_lin(getCoordInEnlargedBlock(offset));
+    }
+    __device__ unsigned int posToEnlargedBlockPos(unsigned int pos) const { return getLinIdInEnlargedBlock(pos); }
+    template <typename CoordT>
+    __device__ unsigned int getNeighbourLinIdInEnlargedBlock(const grid_key_dx<dim, CoordT> &base,
+                                                             const grid_key_dx<dim, CoordT> &offsets) const
+    {
+        grid_key_dx<dim, int> r;
+        for (unsigned int q = 0; q < dim; ++q) r.set_d(q, (int)(base.get(q) + offsets.get(q)));
+        return enlarged_lin(r);
+    }
+    template <typename CoordT>
+    __device__ unsigned int getNeighbourLinIdInEnlargedBlock(const grid_key_dx<dim, CoordT> &base, unsigned int dimension,
+                                                             char offset) const
+    {
+        grid_key_dx<dim, int> r;
+        for (unsigned int q = 0; q < dim; ++q) r.set_d(q, (int)base.get(q) + (q == dimension ? (int)offset : 0));
+        return enlarged_lin(r);
+    }
+    // loadGhostBlock<p> (SparseGridGpu_ker.cuh:555-570, loadGhostBlock_impl
+    // SparseGridGpu_ker_util.hpp:487-945): the chunk at pos and its ghost layer into
+    // sharedRegion (and the mask bytes into maskRegion), with the background wherever a
+    // point does not exist.  Every thread of the workgroup takes part; the caller
+    // synchronises before reading.
+    template <unsigned int p, typename AggrWrapperT>
+    __device__ void loadGhostBlock(const AggrWrapperT &, const openfpm::sparse_index<unsigned int> pos,
+                                   elem_t<p> *sharedRegion, unsigned char *maskRegion = nullptr) const
+    {
+        const elem_t<p> *a = (p < pMask) ? (const elem_t<p> *)gv.prop[p < pMask ? p : 0] : (const elem_t<p> *)gv.mask;
+        const int64_t bgbase = gv.nchunks * chunk_size;
+        const unsigned int nthr = blockDim.x * blockDim.y * blockDim.z;
+        const unsigned int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+        for (unsigned int e = tid; e < getEnlargedBlockSize(); e += nthr) {
+            unsigned int r = e;
+            int dd[3] = {0, 0, 0};
+            int off = 0, mul = 1;
+            for (unsigned int q = 0; q < dim; ++q) {
+                int l = (int)(r % EB) - 1;
+                r /= EB;
+                dd[q] = (l < 0) ? -1 : (l >= (int)B ? 1 : 0);
+                l -= dd[q] * (int)B;
+                off += l * mul;
+                mul *= (int)B;
+            }
+            const int64_t np = neighbour_chunk((int64_t)pos.id, dd, nn_star != 0);
+            const int64_t s = np * chunk_size + off;
+            const unsigned char m = gv.mask[s];
+            sharedRegion[e] = (p == pMask || (m & SG_EXIST)) ? a[s] : a[bgbase + off];
+            if (maskRegion) maskRegion[e] = m;
+        }
+    }
+    // storeBlock<p> (SparseGridGpu_ker.cuh:872-891): the inner part of sharedRegion to the chunk
+    template <unsigned int p, typename AggrWrapperT> __device__ void storeBlock(AggrWrapperT &block, elem_t<p> *sharedRegion) const
+    {
+        const unsigned int nthr = blockDim.x * blockDim.y * blockDim.z;
+        const unsigned int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+        for (unsigned int o = tid; o < chunk_size; o += nthr) block.template get<p>()[o] = sharedRegion[getLinIdInEnlargedBlock(o)];
+    }
+    // the arrays of the chunk at position pos (the dataBlockLoad / dataBlockStore of the functors)
+    __device__ insert_encap<Tuple> block_at(int64_t pos) const
+    {
+        insert_encap<Tuple> e;
+        for (unsigned int q = 0; q < 16; ++q) e.prop[q] = nullptr;
+        for (unsigned int q = 0; q < pMask; ++q)
+            e.prop[q] = (unsigned char *)gv.prop[q] + (size_t)pos * chunk_size * sizeof_prop(q);
+        e.mask = gv.mask + (size_t)pos * chunk_size;
+        return e;
+    }
+
+    // ---- device inserts ----
+    // insertBlock<nChunksPerBlocks>(blockId, stride) (BlockMapGpu_ker.cuh:199-227): the chunk
+    // record of blockId for the calling lanes; the template parameter and the stride of the
+    // reference's per-block sharing are accepted (records are shared per wave here)
+    template <unsigned int nChunksPerBlocks = 1>
+    __device__ insert_encap<Tuple> insertBlock(indexT blockId, unsigned int stride = 8192) const
+    {
+        (void)stride;
         const int lane = (int)(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z)) & 63;
         uint64_t todo = __ballot(1);
         int64_t rec = 0;
@@ -404,11 +685,65 @@ template <unsigned int dim, unsigned int B, typename Tuple, typename indexT> str
     }
 };
 
+// applyStencilInPlace (SparseGridGpu.hpp:954-1048, SparseGridGpu_kernels.cuh:1037-1094): one
+// workgroup of chunk-size threads per chunk; the functor is called by every thread with the
+// chunk's arrays as both dataBlockLoad and dataBlockStore (in place, as the reference) and
+// curMask = the slot's mask, 0 outside the inclusive box [lo, hi].
+template <typename Stencil, typename SGK, typename... Args>
+__global__ void apply_stencil_functor_kernel(SGK sg, int lo0, int lo1, int lo2, int hi0, int hi1, int hi2, Args... args)
+{
+    constexpr unsigned int dim = SGK::dims;
+    const int64_t pos = blockIdx.x;
+    if (pos >= sg.gv.nchunks) return;  // uniform over the workgroup
+    const unsigned int offset = threadIdx.x;
+    auto blk = sg.block_at(pos);
+    const int64_t id = sg.gv.keys[pos];
+    grid_key_dx<dim, int> pointCoord = sg.getCoord((typename SGK::indexT_)(id * SGK::chunk_size + offset));
+    unsigned char curMask = 0;
+    if (offset < SGK::chunk_size) {
+        curMask = sg.gv.mask[pos * SGK::chunk_size + offset];
+        const int lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+        for (unsigned int q = 0; q < dim; ++q)
+            if (pointCoord.get(q) < lo[q] || pointCoord.get(q) > hi[q]) curMask = 0;
+    }
+    openfpm::sparse_index<unsigned int> sp;
+    sp.id = (unsigned int)pos;
+    Stencil::stencil(sg, (unsigned int)id, sp, offset, pointCoord, blk, blk, curMask, args...);
+}
+
+// applyStencilInPlaceNoShared (SparseGridGpu.hpp:1051-1075, SparseGridGpu_kernels.cuh:1096-1192):
+// one thread per existing point of the point list ep (data positions chunk * chunk size + offset)
+template <typename Stencil, typename SGK, typename... Args>
+__global__ void apply_stencil_functor_noshared_kernel(SGK sg, const int64_t *ep, int64_t nep, int lo0, int lo1, int lo2,
+                                                      int hi0, int hi1, int hi2, Args... args)
+{
+    constexpr unsigned int dim = SGK::dims;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nep) return;
+    const int64_t pos = ep[t] / SGK::chunk_size;
+    const unsigned int offset = (unsigned int)(ep[t] % SGK::chunk_size);
+    auto blk = sg.block_at(pos);
+    const int64_t id = sg.gv.keys[pos];
+    grid_key_dx<dim, int> pointCoord = sg.getCoord((typename SGK::indexT_)(id * SGK::chunk_size + offset));
+    unsigned char curMask = sg.gv.mask[pos * SGK::chunk_size + offset];
+    const int lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+    for (unsigned int q = 0; q < dim; ++q)
+        if (pointCoord.get(q) < lo[q] || pointCoord.get(q) > hi[q]) curMask = 0;
+    openfpm::sparse_index<unsigned int> sp;
+    sp.id = (unsigned int)pos;
+    Stencil::stencil(sg, (unsigned int)id, sp, offset, pointCoord, blk, blk, curMask, args...);
+}
+
 }  // namespace sg
 
 // BlockMapGpu_ker<>::setExist (BlockMapGpu_ker.cuh:75): sets the EXIST bit of a mask byte
 template <typename = void> struct BlockMapGpu_ker {
     __device__ static void setExist(unsigned char &m) { m |= (unsigned char)SG_EXIST; }
+    template <typename M> __device__ static void unsetExist(M &m) { m &= (M)~SG_EXIST; }
+    template <typename M> __device__ static bool exist(const M &m) { return (m & SG_EXIST) != 0; }
+    template <typename M> __device__ static bool getBit(const M &m, unsigned char pos) { return (m >> pos) & 1u; }
+    template <typename M> __device__ static void setBit(M &m, unsigned char pos) { m = m | (1u << pos); }
+    template <typename M> __device__ static void unsetBit(M &m, unsigned char pos) { m = m & ~(1u << pos); }
 };
 
 // the reference's names in user code: cross_stencil<dim, T> and

@@ -299,6 +299,20 @@ int sg_get_masks(sg_grid *g, uint8_t *out, int mem);
 int sg_get_prop(sg_grid *g, int prop, void *out, int mem);
 /* overwrite property p of the existing chunks (hostToDevice<p>) */
 int sg_set_prop(sg_grid *g, int prop, const void *in, int mem);
+/* hostToDevice of the chunk list and the masks (BlockMapGpu.hpp:230-241, the index buffer
+ * and pMask of map_vector_sparse::hostToDevice): the grid's chunks become ids[n] (strictly
+ * increasing chunk ids of the domain) with masks [n][edge^dim].  A chunk that was already
+ * there keeps its property values, a new chunk takes the background; every slot whose new
+ * mask is 0 takes the background.  Pending inserts stay pending; neighbour tables and the
+ * existing-point list are rebuilt when next needed.  The host insertFlush of the facade
+ * (SparseGridGpu.hpp:2780-2845) edits a host copy that goes back through this call. */
+int sg_set_chunks(sg_grid *g, int64_t n, const int64_t *ids, const uint8_t *masks, int mem);
+/* Restore the layout invariant after the caller's kernels wrote the arrays of sg_to_kernel
+ * (a user applyStencils functor, a store through the device view): every slot whose mask is
+ * 0 takes the background of property prop (every property for prop == -1).  The reference
+ * substitutes the background for such slots on every read (loadGhostBlock_impl,
+ * SparseGridGpu_ker_util.hpp:551-552), so reads see the same values. */
+int sg_restore_background(sg_grid *g, int prop);
 /* get<p>(coord) for n point keys; background where the point does not exist
  * (src/SparseGridGpu/SparseGridGpu.hpp:1897-1901, BlockMapGpu.hpp:108-126) */
 int sg_get_points(sg_grid *g, int prop, int64_t n, const int64_t *keys,
@@ -435,6 +449,11 @@ int sg_get_links(sg_grid *g, int which, int32_t *pad_chunk, int16_t *pad_off, ui
  * the device blocks the library keeps from destroyed handles for reuse by new ones; the
  * chunk map, properties, masks and neighbour tables stay */
 int sg_trim(sg_grid *g);
+/* No counterpart in the reference: a streaming device copy of `bytes` (a multiple of 16,
+ * 16-byte aligned buffers) on `stream` (hipStream_t as void*), plain or with non-temporal
+ * loads and stores; the HBM ceiling that the stencil kernels are measured against. */
+#define SG_COPY_NONTEMPORAL 1
+int sg_stream_copy(void *dst, const void *src, int64_t bytes, int flags, void *stream);
 /* bytes of device memory the handle holds (resident arrays + cached temporaries) */
 int sg_device_bytes(sg_grid *g, int64_t *out);
 

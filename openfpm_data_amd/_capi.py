@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsg_hip.so")
 
 SG_OK = 0
+SG_E_ARG, SG_E_HIP, SG_E_OOM, SG_E_STATE, SG_E_RANGE = -1, -2, -3, -4, -5
 SG_MEM_HOST = 0
 SG_MEM_DEVICE = 1
 
@@ -43,6 +44,7 @@ TUNE_STENCIL_SKIP_EMPTY_ROWS = 4
 TUNE_TAG_CHUNKS_PER_WAVE = 7
 TUNE_BATCH_WAVES = 5
 TUNE_BATCH_MAX_WGS = 6
+COPY_NONTEMPORAL = 1
 LINK_DOWN = 0
 LINK_UP = 1
 
@@ -164,6 +166,8 @@ _SIGS = {
     "sg_get_prop": (_INT, [_P, _INT, _P, _INT]),
     "sg_set_prop": (_INT, [_P, _INT, _P, _INT]),
     "sg_get_points": (_INT, [_P, _INT, _I64, _P, _P, _INT]),
+    "sg_set_chunks": (_INT, [_P, _I64, _P, _P, _INT]),
+    "sg_restore_background": (_INT, [_P, _INT]),
     "sg_get_neighbours": (_INT, [_P, _INT, _P, _INT]),
     "sg_device_ptr": (_INT, [_P, _INT, ctypes.POINTER(_P)]),
     "sg_to_kernel": (_INT, [_P, _INT, _P]),
@@ -184,6 +188,7 @@ _SIGS = {
     "sg_get_links": (_INT, [_P, _INT, _P, _P, _P, _P, _P, _INT]),
     "sg_remove_copy_to_finalize": (_INT, [_P, _INT, _INT, _P]),
     "sg_trim": (_INT, [_P]),
+    "sg_stream_copy": (_INT, [_P, _P, _I64, _INT, _P]),
     "sg_device_bytes": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_batch_create": (_INT, [ctypes.POINTER(PoissonParams), _P, ctypes.POINTER(_P)]),
     "sg_batch_destroy": (_INT, [_P]),

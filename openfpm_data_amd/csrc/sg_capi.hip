@@ -481,6 +481,18 @@ int sg_set_prop(sg_grid *g, int prop, const void *in, int mem)
     return SG_OK;
 }
 
+int sg_set_chunks(sg_grid *g, int64_t n, const int64_t *ids, const uint8_t *masks, int mem)
+{
+    if (!g) return fail(SG_E_ARG, "null grid");
+    return set_chunks_impl(g, n, ids, masks, mem);
+}
+
+int sg_restore_background(sg_grid *g, int prop)
+{
+    if (!g || (prop != -1 && !valid_prop(g, prop))) return fail(SG_E_ARG, "bad property");
+    return restore_background_impl(g, prop);
+}
+
 int sg_get_points(sg_grid *g, int prop, int64_t n, const int64_t *keys, void *out, int mem)
 {
     if (!valid_prop(g, prop) || (n > 0 && (!keys || !out))) return fail(SG_E_ARG, "bad argument");

@@ -1550,4 +1550,110 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     return SG_OK;
 }
 
+// chunk c of a replaced chunk list: its values from the old chunk oldpos[c] (or the
+// background when the chunk is new), the background wherever the new mask is 0; chunk n
+// (the background chunk) holds the background
+template <typename T>
+__global__ __launch_bounds__(kWG) void remap_chunks_kernel(const T *__restrict__ src, T *__restrict__ dst,
+                                                          const int64_t *__restrict__ oldpos,
+                                                          const uint8_t *__restrict__ nmask, int64_t n, int csz, T bg)
+{
+    const int64_t c = blockIdx.x;
+    if (c > n) return;
+    const int64_t o = (c < n) ? oldpos[c] : -1;
+    for (int i = threadIdx.x; i < csz; i += kWG) {
+        const int64_t s = c * csz + i;
+        dst[s] = (o >= 0 && nmask[s] != 0) ? src[o * csz + i] : bg;
+    }
+}
+
+int set_chunks_impl(sg_grid *g, int64_t n, const int64_t *ids, const uint8_t *masks, int mem)
+{
+    if (n < 0 || (n > 0 && (!ids || !masks))) return fail(SG_E_ARG, "bad chunk list");
+    if (g->dins_open) return fail(SG_E_STATE, "device insert buffer open: call sg_insert_device_end first");
+    const int csz = g->csz;
+    // the ids on the host, checked: strictly increasing chunk ids of the domain
+    std::vector<int64_t> hid((size_t)n);
+    if (n > 0) {
+        if (mem == SG_MEM_DEVICE) {
+            SG_HIP(hipMemcpyAsync(hid.data(), ids, (size_t)n * 8, hipMemcpyDeviceToHost, g->stream));
+            SG_HIP(hipStreamSynchronize(g->stream));
+        } else {
+            memcpy(hid.data(), ids, (size_t)n * 8);
+        }
+    }
+    const ChunkLin L = chunk_lin(g);
+    for (int64_t c = 0; c < n; ++c) {
+        if (c > 0 && hid[c] <= hid[c - 1]) return fail(SG_E_ARG, "chunk ids not strictly increasing");
+        if (hid[c] < 0) return fail(SG_E_RANGE, "negative chunk id");
+        int64_t bc[3] = {0, 0, 0};
+        chunk_coords(L, hid[c], bc);
+        for (int d = 0; d < g->dim; ++d)
+            if (bc[d] < 0 || bc[d] >= g->bsz[d]) return fail(SG_E_RANGE, "chunk id outside the grid");
+        if (chunk_id(L, bc) != hid[c]) return fail(SG_E_RANGE, "chunk id outside the grid");
+    }
+    const int64_t ncap = std::max<int64_t>(n, 1);
+    int64_t *nkeys = (int64_t *)g->pool.alloc((size_t)ncap * 8);
+    uint8_t *nmask = (uint8_t *)g->pool.alloc((size_t)(ncap + 1) * csz);
+    void *nprop[kMaxProps] = {};
+    bool ok = nkeys && nmask;
+    for (int p = 0; p < g->np && ok; ++p) ok = (nprop[p] = g->pool.alloc((size_t)(ncap + 1) * csz * g->ts[p])) != nullptr;
+    int rc = SG_OK;
+    DevBuf &bp = g->scratch[5];
+    if (ok && (rc = ensure(g->pool, bp, (size_t)ncap * 8 + (size_t)ncap * 4 + 64))) ok = false;
+    if (!ok) {
+        if (nkeys) g->pool.free(nkeys);
+        if (nmask) g->pool.free(nmask);
+        for (int p = 0; p < g->np; ++p)
+            if (nprop[p]) g->pool.free(nprop[p]);
+        return rc ? rc : fail(SG_E_OOM, "chunk array allocation failed");
+    }
+    int64_t *oldpos = (int64_t *)bp.p;
+    uint32_t *isnew = (uint32_t *)(oldpos + ncap);
+    const hipMemcpyKind kind = (mem == SG_MEM_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (n > 0) {
+        SG_HIP(hipMemcpyAsync(nkeys, hid.data(), (size_t)n * 8, hipMemcpyHostToDevice, g->stream));
+        SG_HIP(hipMemcpyAsync(nmask, masks, (size_t)n * csz, kind, g->stream));
+        hipLaunchKernelGGL(old_lookup_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, nkeys, n, g->d_keys,
+                           g->nchunks, oldpos, isnew);
+        SG_HIP(hipGetLastError());
+    }
+    SG_HIP(hipMemsetAsync(nmask + (size_t)n * csz, 0, csz, g->stream));
+    for (int p = 0; p < g->np; ++p) {
+        dispatch_type(g->dt[p], [&](auto tag) {
+            typedef decltype(tag) T;
+            T v;
+            memcpy(&v, g->bg[p], sizeof(T));
+            hipLaunchKernelGGL(remap_chunks_kernel<T>, dim3((unsigned)(n + 1)), dim3(kWG), 0, g->stream,
+                               (const T *)g->d_prop[p], (T *)nprop[p], oldpos, nmask, n, csz, v);
+        });
+        SG_HIP(hipGetLastError());
+    }
+    // the host ids and masks are read by the queued copies: wait before they may go away
+    SG_HIP(hipStreamSynchronize(g->stream));
+    g->pool.free(g->d_keys);
+    g->pool.free(g->d_mask);
+    for (int p = 0; p < g->np; ++p) g->pool.free(g->d_prop[p]);
+    g->d_keys = nkeys;
+    g->d_mask = nmask;
+    for (int p = 0; p < g->np; ++p) g->d_prop[p] = nprop[p];
+    g->nchunks = n;
+    g->cap = ncap;
+    g->nn_valid[0] = g->nn_valid[1] = false;
+    g->ep_valid = false;
+    return SG_OK;
+}
+
+int restore_background_impl(sg_grid *g, int prop)
+{
+    for (int p = 0; p < g->np; ++p) {
+        if (prop >= 0 && p != prop) continue;
+        int rc = normalize_prop(g, p);
+        if (rc) return rc;
+    }
+    // the background chunk's mask stays 0
+    SG_HIP(hipMemsetAsync(g->d_mask + (size_t)g->nchunks * g->csz, 0, g->csz, g->stream));
+    return SG_OK;
+}
+
 }  // namespace sg

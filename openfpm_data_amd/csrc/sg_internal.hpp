@@ -314,10 +314,19 @@ int get_points_impl(sg_grid *g, int prop, int64_t n, const int64_t *keys, void *
 int fill_background_chunk(sg_grid *g, int prop);
 int normalize_prop(sg_grid *g, int prop);
 int count_mask_impl(sg_grid *g, int which, int64_t *out);
+// replace the chunk list and masks (hostToDevice of a host copy whose chunks changed)
+int set_chunks_impl(sg_grid *g, int64_t n, const int64_t *ids, const uint8_t *masks, int mem);
+// background into every mask-0 slot of property prop (every property for -1)
+int restore_background_impl(sg_grid *g, int prop);
 
 // device-wide primitives (sg_primitives.hip)
 // exclusive scan of n uint32 values; *total (host) receives the sum when non-null
 int exclusive_scan_u32(sg_grid *g, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total);
+// the same on `stream` with a caller-provided workspace of ws_words >= scan_ws_words(n) words;
+// d_total (device, may be null) receives the sum; allocates nothing and does not wait
+size_t scan_ws_words(int64_t n);
+int exclusive_scan_u32_ws(hipStream_t stream, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total,
+                          uint32_t *ws, size_t ws_words);
 // pinned host words of g (16 of them), allocated on first use; nullptr on failure
 uint32_t *host_words(sg_grid *g);
 // stable LSD radix sort of (key, value) pairs on the low `bits` bits of the key

@@ -228,9 +228,10 @@ __global__ __launch_bounds__(kWG) void unpack_records_kernel(UnpackIn U, int64_t
     int64_t bc[3] = {0, 0, 0};
     chunk_coords(PL, id, bc);
     const int64_t sz[3] = {sz0, sz1, sz2};
-    // a malformed section (scan not increasing or past n_pnt) is refused, not followed
+    // a malformed section (scan not starting at 0, not increasing or past n_pnt) is refused,
+    // not followed: the records [0, n_pnt) are then exactly the chunks' ranges
     const int64_t j0 = U.scan[i], j1 = U.scan[i + 1];
-    if (j0 > j1 || j1 > n_pnt) {
+    if (j0 > j1 || j1 > n_pnt || (i == 0 && j0 != 0)) {
         if (threadIdx.x == 0) *bad = 1;
         return;
     }
@@ -367,6 +368,7 @@ int64_t section_bytes(const sg_grid *g, int64_t n_cnk, int64_t n_pnt, int64_t ps
 struct PackPlan {
     int64_t stride = 0;
     uint32_t *cnt = nullptr, *flag = nullptr, *pscan = nullptr, *cscan = nullptr;
+    const int32_t *d_lo = nullptr, *d_hi = nullptr;  // device copy of the boxes, [nbox][dim]
     std::vector<int64_t> n_cnk, n_pnt;
 };
 
@@ -386,6 +388,8 @@ int plan_pack(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, PackPl
     int32_t *d_lo = (int32_t *)b_boxes.p, *d_hi = d_lo + (size_t)nbox * g->dim;
     SG_HIP(hipMemcpyAsync(d_lo, lo, bb, hipMemcpyHostToDevice, g->stream));
     SG_HIP(hipMemcpyAsync(d_hi, hi, bb, hipMemcpyHostToDevice, g->stream));
+    pl.d_lo = d_lo;
+    pl.d_hi = d_hi;
     pl.cnt = (uint32_t *)b_cnt.p;
     pl.flag = pl.cnt + per;
     pl.pscan = pl.flag + per;
@@ -408,36 +412,51 @@ int plan_pack(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, PackPl
     return SG_OK;
 }
 
-int pack_into_device(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, int nprop, const int32_t *props,
-                     int A, unsigned char *buf, int64_t cap, const PackPlan &pl)
+// header of one packed section (chunk count, box start, grid size) and the scan's last
+// entry (n_pnt), staged on the host for their copies to the device
+constexpr int kHdrMax = 8 + 2 * 3 * 4;
+struct SectionHead {
+    unsigned char b[kHdrMax];
+    uint32_t np;
+};
+
+// write the packed sections of the boxes to the device buffer buf (cap bytes) from a
+// plan: per box k, rows k * stride of cnt / pscan / cscan and corners d_lo / d_hi + k * dim;
+// head[] (nbox entries, host) stages the headers and must live until the call returns
+// (it waits for the stream before returning)
+int pack_sections(sg_grid *g, int nbox, const int32_t *lo, int nprop, const int32_t *props, int A,
+                  unsigned char *buf, int64_t cap, int64_t stride, const uint32_t *cnt, const uint32_t *pscan,
+                  const uint32_t *cscan, const int32_t *d_lo, const int32_t *d_hi, const int64_t *n_cnk,
+                  const int64_t *n_pnt, SectionHead *head)
 {
     const int64_t psz = point_bytes(g, nprop, props);
     int64_t at = 0;
     const int64_t n = g->nchunks;
+    const int hsz = 8 + 2 * g->dim * 4;
     for (int k = 0; k < nbox; ++k) {
-        const int64_t nc = pl.n_cnk[k], np = pl.n_pnt[k];
+        const int64_t nc = n_cnk[k], np = n_pnt[k];
         const int64_t sb = section_bytes(g, nc, np, psz, A);
         if (at + sb > cap) return fail(SG_E_RANGE, "pack buffer too small");
         unsigned char *s0 = buf + at;
         SG_HIP(hipMemsetAsync(s0, 0, (size_t)sb, g->stream));
         // header: chunk count, box start, grid size
-        std::vector<unsigned char> hdr(8 + 2 * g->dim * 4);
+        SectionHead &h = head[k];
         const uint64_t ncu = (uint64_t)nc;
-        memcpy(hdr.data(), &ncu, 8);
+        memcpy(h.b, &ncu, 8);
         for (int d = 0; d < g->dim; ++d) {
             const int32_t s = (int32_t)g->sz[d];
-            memcpy(hdr.data() + 8 + 4 * d, lo + k * g->dim + d, 4);
-            memcpy(hdr.data() + 8 + 4 * (g->dim + d), &s, 4);
+            memcpy(h.b + 8 + 4 * d, lo + k * g->dim + d, 4);
+            memcpy(h.b + 8 + 4 * (g->dim + d), &s, 4);
         }
-        SG_HIP(hipMemcpyAsync(s0, hdr.data(), hdr.size(), hipMemcpyHostToDevice, g->stream));
-        int64_t q = (int64_t)hdr.size();
+        h.np = (uint32_t)np;
+        SG_HIP(hipMemcpyAsync(s0, h.b, (size_t)hsz, hipMemcpyHostToDevice, g->stream));
+        int64_t q = hsz;
         PackOut O;
         O.A = A;
         O.ids = s0 + q;
         q += A * nc;
         O.scan = (uint32_t *)(s0 + q);
-        const uint32_t np32 = (uint32_t)np;
-        SG_HIP(hipMemcpyAsync(O.scan + nc, &np32, 4, hipMemcpyHostToDevice, g->stream));
+        SG_HIP(hipMemcpyAsync(O.scan + nc, &h.np, 4, hipMemcpyHostToDevice, g->stream));
         q += align_up((nc + 1) * 4, A);
         O.nprop = nprop;
         int64_t dq = q;
@@ -452,19 +471,24 @@ int pack_into_device(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi,
         q += align_up(np * 2, A);
         O.masks = s0 + q;
         if (n > 0 && np > 0) {
-            // box corners for the kernel: the plan's device copy
-            const int32_t *d_lo = (const int32_t *)g->scratch[0].p + (size_t)k * g->dim;
-            const int32_t *d_hi = (const int32_t *)g->scratch[0].p + (size_t)nbox * g->dim + (size_t)k * g->dim;
             hipLaunchKernelGGL(pack_write_kernel, dim3((unsigned)n), dim3(kWG), 0, g->stream, g->d_keys, n, g->d_mask,
-                               geom_of(g), d_lo, d_hi, pl.pscan + k * pl.stride, pl.cscan + k * pl.stride,
-                               pl.cnt + k * pl.stride, O);
+                               geom_of(g), d_lo + (size_t)k * g->dim, d_hi + (size_t)k * g->dim, pscan + k * stride,
+                               cscan + k * stride, cnt + k * stride, O);
             SG_HIP(hipGetLastError());
         }
         at += sb;
-        // host header/scan-tail sources are stack values: wait before they go away
-        SG_HIP(hipStreamSynchronize(g->stream));
     }
+    // the host header sources must outlive their copies
+    SG_HIP(hipStreamSynchronize(g->stream));
     return SG_OK;
+}
+
+int pack_into_device(sg_grid *g, int nbox, const int32_t *lo, int nprop, const int32_t *props, int A,
+                     unsigned char *buf, int64_t cap, const PackPlan &pl)
+{
+    std::vector<SectionHead> head((size_t)std::max(nbox, 1));
+    return pack_sections(g, nbox, lo, nprop, props, A, buf, cap, pl.stride, pl.cnt, pl.pscan, pl.cscan, pl.d_lo,
+                         pl.d_hi, pl.n_cnk.data(), pl.n_pnt.data(), head.data());
 }
 
 }  // namespace
@@ -502,12 +526,12 @@ int sg_pack(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, int npro
     for (int k = 0; k < nbox; ++k) tot += section_bytes(g, pl.n_cnk[k], pl.n_pnt[k], psz, index_bytes);
     if (tot > cap) return fail(SG_E_RANGE, "pack buffer too small");
     if (mem == SG_MEM_DEVICE) {
-        rc = pack_into_device(g, nbox, lo, hi, nprop, props, index_bytes, (unsigned char *)buf, cap, pl);
+        rc = pack_into_device(g, nbox, lo, nprop, props, index_bytes, (unsigned char *)buf, cap, pl);
         return rc;
     }
     DevBuf &st = g->scratch[2];
     if ((rc = ensure(g->pool, st, (size_t)tot + 64))) return rc;
-    if ((rc = pack_into_device(g, nbox, lo, hi, nprop, props, index_bytes, (unsigned char *)st.p, tot, pl)))
+    if ((rc = pack_into_device(g, nbox, lo, nprop, props, index_bytes, (unsigned char *)st.p, tot, pl)))
         return rc;
     SG_HIP(hipMemcpyAsync(buf, st.p, (size_t)tot, hipMemcpyDeviceToHost, g->stream));
     SG_HIP(hipStreamSynchronize(g->stream));
@@ -544,6 +568,7 @@ int sg_unpack(sg_grid *g, const void *buf, int64_t avail, int mem, const int32_t
     const int64_t psz = point_bytes(g, nprop, props);
     const int64_t sb = section_bytes(g, nc, np, psz, A);
     if (avail < sb) return fail(SG_E_RANGE, "packed section truncated");
+    if (nc == 0 && np != 0) return fail(SG_E_RANGE, "malformed packed section: points without chunks");
     *consumed = sb;
     if (np == 0) return SG_OK;
     // the section on the device

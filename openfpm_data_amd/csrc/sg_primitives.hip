@@ -16,12 +16,16 @@
 // CUB SortPairs does), which the flush relies on for its merge order.
 #include "sg_internal.hpp"
 
+#include <algorithm>
+
 namespace sg {
 namespace {
 
 constexpr int kScanWG = 256;
 constexpr int kScanIPT = 16;
 constexpr int kScanTile = kScanWG * kScanIPT;
+// a scan of up to kScanTile^(kScanMaxLevels-1) items (2^48 with 4096-item tiles)
+constexpr int kScanMaxLevels = 5;
 
 constexpr int kSortWG = 256;
 constexpr int kSortIPT = 16;
@@ -267,40 +271,53 @@ int radix_sort_pairs(sg_grid *g, K *keys, uint32_t *vals, K *keys_tmp, uint32_t 
 
 }  // namespace
 
-int exclusive_scan_u32(sg_grid *g, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total)
+// level sizes of a scan of n items: n, ceil(n/T), ceil(ceil(n/T)/T), ... until one tile
+static int scan_levels(int64_t n, int64_t (&sizes)[kScanMaxLevels])
 {
-    if (n <= 0) {
-        if (total) *total = 0;
-        return SG_OK;
+    int nl = 0;
+    sizes[nl++] = n;
+    while (sizes[nl - 1] > kScanTile && nl < kScanMaxLevels) {
+        sizes[nl] = (sizes[nl - 1] + kScanTile - 1) / kScanTile;
+        ++nl;
     }
-    // level sizes: n, ceil(n/T), ceil(ceil(n/T)/T), ... until one tile
-    std::vector<int64_t> sizes{n};
-    while (sizes.back() > kScanTile) sizes.push_back((sizes.back() + kScanTile - 1) / kScanTile);
+    return nl;
+}
+
+size_t scan_ws_words(int64_t n)
+{
+    if (n <= 0) return 0;
+    int64_t sizes[kScanMaxLevels];
+    const int nl = scan_levels(n, sizes);
     size_t need = 0;
-    for (size_t l = 1; l < sizes.size(); ++l) need += 2 * (size_t)sizes[l];
-    need += 2;
-    DevBuf &sb = g->scratch[11];
-    int rc = ensure(g->pool, sb, need * sizeof(uint32_t));
-    if (rc) return rc;
-    uint32_t *pool = (uint32_t *)sb.p;
+    for (int l = 1; l < nl; ++l) need += 2 * (size_t)sizes[l];
+    return need;
+}
+
+int exclusive_scan_u32_ws(hipStream_t stream, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *d_total,
+                          uint32_t *ws, size_t ws_words)
+{
+    if (n <= 0) return SG_OK;
+    int64_t sizes[kScanMaxLevels];
+    const int nl = scan_levels(n, sizes);
+    if (sizes[nl - 1] > kScanTile) return fail(SG_E_RANGE, "scan too long");
+    if (ws_words < scan_ws_words(n)) return fail(SG_E_RANGE, "scan workspace too small");
     // per level l >= 1: sums[l] (block sums of level l-1) and offs[l] (their scan)
-    std::vector<uint32_t *> sums(sizes.size(), nullptr), offs(sizes.size(), nullptr);
+    uint32_t *sums[kScanMaxLevels] = {}, *offs[kScanMaxLevels] = {};
     size_t at = 0;
-    for (size_t l = 1; l < sizes.size(); ++l) {
-        sums[l] = pool + at;
+    for (int l = 1; l < nl; ++l) {
+        sums[l] = ws + at;
         at += sizes[l];
-        offs[l] = pool + at;
+        offs[l] = ws + at;
         at += sizes[l];
     }
-    uint32_t *d_total = pool + at;
     auto aligned = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
     const uint32_t *lin = in;
-    for (size_t l = 0; l + 1 < sizes.size(); ++l) {
+    for (int l = 0; l + 1 < nl; ++l) {
         if (aligned(lin))
-            hipLaunchKernelGGL(scan_reduce_kernel<true>, dim3((unsigned)sizes[l + 1]), dim3(kScanWG), 0, g->stream,
+            hipLaunchKernelGGL(scan_reduce_kernel<true>, dim3((unsigned)sizes[l + 1]), dim3(kScanWG), 0, stream,
                                lin, sums[l + 1], sizes[l]);
         else
-            hipLaunchKernelGGL(scan_reduce_kernel<false>, dim3((unsigned)sizes[l + 1]), dim3(kScanWG), 0, g->stream,
+            hipLaunchKernelGGL(scan_reduce_kernel<false>, dim3((unsigned)sizes[l + 1]), dim3(kScanWG), 0, stream,
                                lin, sums[l + 1], sizes[l]);
         SG_HIP(hipGetLastError());
         lin = sums[l + 1];
@@ -308,21 +325,36 @@ int exclusive_scan_u32(sg_grid *g, const uint32_t *in, uint32_t *out, int64_t n,
     auto down = [&](const uint32_t *src, uint32_t *dst, const uint32_t *bo, int64_t len, uint32_t *tot) {
         const dim3 grid((unsigned)((len + kScanTile - 1) / kScanTile));
         if (aligned(src) && aligned(dst))
-            hipLaunchKernelGGL(scan_down_kernel<true>, grid, dim3(kScanWG), 0, g->stream, src, dst, bo, len, tot);
+            hipLaunchKernelGGL(scan_down_kernel<true>, grid, dim3(kScanWG), 0, stream, src, dst, bo, len, tot);
         else
-            hipLaunchKernelGGL(scan_down_kernel<false>, grid, dim3(kScanWG), 0, g->stream, src, dst, bo, len, tot);
+            hipLaunchKernelGGL(scan_down_kernel<false>, grid, dim3(kScanWG), 0, stream, src, dst, bo, len, tot);
     };
-    for (size_t l = sizes.size() - 1; l >= 1; --l) {
+    for (int l = nl - 1; l >= 1; --l) {
         // scan level l (its single top tile or using offsets from level l+1)
-        const uint32_t *bo = (l + 1 < sizes.size()) ? offs[l + 1] : nullptr;
+        const uint32_t *bo = (l + 1 < nl) ? offs[l + 1] : nullptr;
         down(sums[l], offs[l], bo, sizes[l], (uint32_t *)nullptr);
         SG_HIP(hipGetLastError());
     }
-    const uint32_t *bo0 = (sizes.size() > 1) ? offs[1] : nullptr;
-    down(in, out, bo0, n, total ? d_total : (uint32_t *)nullptr);
+    // the inclusive total is written by the thread owning item n-1 (works in place)
+    down(in, out, nl > 1 ? offs[1] : nullptr, n, d_total);
     SG_HIP(hipGetLastError());
+    return SG_OK;
+}
+
+int exclusive_scan_u32(sg_grid *g, const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total)
+{
+    if (n <= 0) {
+        if (total) *total = 0;
+        return SG_OK;
+    }
+    const size_t words = scan_ws_words(n) + 2;
+    DevBuf &sb = g->scratch[11];
+    int rc = ensure(g->pool, sb, words * sizeof(uint32_t));
+    if (rc) return rc;
+    uint32_t *d_total = (uint32_t *)sb.p + (words - 2);
+    if ((rc = exclusive_scan_u32_ws(g->stream, in, out, n, total ? d_total : nullptr, (uint32_t *)sb.p, words - 2)))
+        return rc;
     if (total) {
-        // the inclusive total is written by the thread owning item n-1 (works in place);
         // through a pinned word, so that copies queued before it need no wait of their own
         uint32_t *hw = host_words(g);
         if (!hw) return fail(SG_E_OOM, "pinned host allocation failed");
@@ -352,4 +384,53 @@ int radix_sort_pairs_u32(sg_grid *g, uint32_t *keys, uint32_t *vals, uint32_t *k
     return radix_sort_pairs<uint32_t>(g, keys, vals, keys_tmp, vals_tmp, n, bits, keys_out, vals_out);
 }
 
+// streaming copy of 16-byte vectors (the HBM ceiling measured beside the stencil): every
+// thread moves 4 vectors per loop trip, grid-stride, plain or non-temporal loads and stores
+typedef unsigned int v4u_t __attribute__((ext_vector_type(4)));
+template <bool NT> __global__ __launch_bounds__(256) void stream_copy_kernel(const v4u_t *__restrict__ src,
+                                                                            v4u_t *__restrict__ dst, int64_t nvec)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += 4 * stride) {
+        v4u_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t j = i + k * stride;
+            if (j < nvec) v[k] = NT ? __builtin_nontemporal_load(src + j) : src[j];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t j = i + k * stride;
+            if (j < nvec) {
+                if (NT) __builtin_nontemporal_store(v[k], dst + j);
+                else dst[j] = v[k];
+            }
+        }
+    }
+}
+
 }  // namespace sg
+
+extern "C" int sg_stream_copy(void *dst, const void *src, int64_t bytes, int flags, void *stream)
+{
+    using namespace sg;
+    if (bytes < 0 || (bytes > 0 && (!dst || !src))) return fail(SG_E_ARG, "bad copy arguments");
+    if (bytes % 16 || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return fail(SG_E_ARG, "the copy needs 16-byte aligned buffers and a multiple of 16 bytes");
+    const int64_t nvec = bytes / 16;
+    if (nvec == 0) return SG_OK;
+    int dev = 0, cus = 256;
+    SG_HIP(hipGetDevice(&dev));
+    SG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // 8 workgroups of 256 threads per CU, each thread moving 4 vectors per trip
+    const int64_t want = (nvec + 4 * 256 - 1) / (4 * 256);
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)cus * 8));
+    if (flags & SG_COPY_NONTEMPORAL)
+        hipLaunchKernelGGL(stream_copy_kernel<true>, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                           (const v4u_t *)src, (v4u_t *)dst, nvec);
+    else
+        hipLaunchKernelGGL(stream_copy_kernel<false>, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                           (const v4u_t *)src, (v4u_t *)dst, nvec);
+    SG_HIP(hipGetLastError());
+    return SG_OK;
+}
