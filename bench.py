@@ -223,7 +223,29 @@ def run_poisson_secondary():
     timed(C.POISSON_MIXED, True, True, tuning)  # warm-up (allocator, code objects)
     t_mix, r_mix = timed(C.POISSON_MIXED, True, True, tuning)
     t_plain, r_plain = timed(C.POISSON_PLAIN_FP64, False, False)
+    # the plain FP64 method batched the same way (shared launches, graph-replayed sweeps):
+    # separates the precision gain from the batching gain
+    t_pb, r_pb = timed(C.POISSON_PLAIN_FP64, True, True, tuning)
     sw = np.array([r["sweeps"] for r in r_mix])
+    # sweep kernel rate: one outer iteration of K FP32 red-black sweeps over all 256 grids
+    # (no graph: 2 K launches of rbsor_batch_kernel and two defect passes).  Algorithmic bytes
+    # per slot: 13 per half sweep (x read and written, f read, mask), 33 per defect pass.
+    K = 40
+    slots = int(sum(s["chunk_ids"].size for s in specs)) * 512
+    gs = grids()
+    P.solve(gs, P.make_params(tol=1e-30, omega=1.5, first_sweeps=K, max_outer=1, use_graph=False), tuning=tuning)
+    for g in gs:
+        g.close()
+    gs = grids()
+    t0 = time.perf_counter()
+    rk = P.solve(gs, P.make_params(tol=1e-30, omega=1.5, first_sweeps=K, max_outer=1, use_graph=False), tuning=tuning)
+    torch.cuda.synchronize()
+    t_k = time.perf_counter() - t0
+    for g in gs:
+        g.close()
+    assert all(r["sweeps"] == K for r in rk)
+    sweep_bytes = slots * (2 * K * 13 + 2 * 33)
+    sweep_gbs = sweep_bytes / t_k / 1e9
     return {"workload": "poisson_batch256", "grids": len(specs),
             "chunks": int(sum(s["chunk_ids"].size for s in specs)), "tol": 1e-10,
             "solves_per_s": round(len(specs) / t_mix, 2), "unit": "solves/s",
@@ -232,7 +254,14 @@ def run_poisson_secondary():
             "sweeps_mean": round(float(sw.mean()), 1), "sweeps_max": int(sw.max()),
             "per_grid_plain_fp64_solves_per_s": round(len(specs) / t_plain, 2),
             "per_grid_plain_fp64_converged": int(sum(r["converged"] for r in r_plain)),
-            "speedup_vs_per_grid_plain_fp64": round(t_plain / t_mix, 3), "tuned": tuned}
+            "speedup_vs_per_grid_plain_fp64": round(t_plain / t_mix, 3),
+            "batched_plain_fp64_solves_per_s": round(len(specs) / t_pb, 2),
+            "batched_plain_fp64_converged": int(sum(r["converged"] for r in r_pb)),
+            "speedup_vs_batched_plain_fp64": round(t_pb / t_mix, 3),
+            "roofline": {"rbsor_batch_sweeps": {"bound": "hbm", "achieved": round(sweep_gbs, 1), "peak": HBM_PEAK_GBS,
+                                                "unit": "GB/s", "frac": round(sweep_gbs / HBM_PEAK_GBS, 4),
+                                                "sweeps": K, "seconds": round(t_k, 5)}},
+            "tuned": tuned}
 
 
 def run_serving_secondary(n_open=384, n_cap=64, budget_gib=32.0, batch_tuning=None):
@@ -258,6 +287,46 @@ def run_serving_secondary(n_open=384, n_cap=64, budget_gib=32.0, batch_tuning=No
             "point_updates_per_s": round(s["point_updates_per_s"] / 1e9, 3), "point_updates_unit": "G/s",
             "served": s["served"], "converged": s["converged"], "rejected": s["rejected"],
             "max_live": s["max_live"], "peak_reserved_gib": round(s["peak_reserved_bytes"] / (1 << 30), 3)}
+
+
+def run_copy_bench(nbytes=1 << 30, reps=20):
+    """Streaming device copies of `nbytes` (sg_stream_copy, 16-byte vectors, plain and
+    non-temporal), HIP events on the copy's stream: GB/s counting the bytes read and written."""
+    import ctypes
+    import torch
+    from openfpm_data_amd import _capi as C
+
+    lib = C.load()
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.zeros_like(a)
+    stream = torch.cuda.Stream()
+    out = {"workload": "stream_copy", "bytes": nbytes, "reps": reps}
+    for name, flags in (("plain", 0), ("nontemporal", C.COPY_NONTEMPORAL)):
+        for _ in range(3):
+            C.check(lib.sg_stream_copy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()), nbytes, flags,
+                                       ctypes.c_void_p(stream.cuda_stream)))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            C.check(lib.sg_stream_copy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()), nbytes, flags,
+                                       ctypes.c_void_p(stream.cuda_stream)))
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms": round(ms, 4), "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac_of_peak": round(2 * nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    torch.cuda.synchronize()
+    assert bool((b == 1).all()), "copy result"
+    return out
+
+
+def host_cores():
+    """CPU cores this process may run on (its affinity set) and the host's count."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return avail, os.cpu_count() or avail
 
 
 def run_cpu_baseline(ids, masks, data, seconds, threads, edge=8):
@@ -294,14 +363,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core of the process's affinity set")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--no-secondary", action="store_true", help="skip the insertion/flush workload")
     ap.add_argument("--only", choices=["flush", "poisson", "serving"],
                     help="run one secondary workload alone and print its JSON (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 20 device copies of 1 GiB (reference for the FETCH/WRITE_SIZE counters)")
+    ap.add_argument("--copy-bench", action="store_true",
+                    help="only time the library's streaming copy of 1 GiB (plain and non-temporal): the "
+                         "HBM ceiling the stencil is measured against")
     args = ap.parse_args()
+    if args.copy_bench:
+        print(json.dumps(run_copy_bench()))
+        return
     if args.copy_calibration:
         import torch
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
@@ -318,7 +393,7 @@ def main():
               "figure is the kernel at its per-GPU shape); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.only:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cores()[0]
         if args.only == "flush":
             print(json.dumps(run_flush_secondary(threads, not args.no_cpu)))
         elif args.only == "poisson":
@@ -363,13 +438,18 @@ def main():
     traffic = traffic_from_profile("traffic_gray_scott_%s_shell512.txt" % args.dtype)
 
     cpu = None
+    avail, nproc = host_cores()
+    cpu_threads = args.cpu_threads or avail
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log("cpu baseline (%d threads)" % threads)
-        c = run_cpu_baseline(ids, masks, data, args.cpu_seconds, threads, edge)
-        cpu = {"value": round(c["value"], 6), "unit": "Gpt-updates/s", "cores": threads, "kind": "port",
+        log("cpu baseline (%d threads, then 1)" % cpu_threads)
+        c = run_cpu_baseline(ids, masks, data, args.cpu_seconds, cpu_threads, edge)
+        c1 = run_cpu_baseline(ids, masks, data, args.cpu_seconds, 1, edge)
+        cpu = {"value": round(c["value"], 6), "unit": "Gpt-updates/s", "cores": cpu_threads, "kind": "port",
+               "nproc": nproc, "cores_available": avail,
                "sample": "oracle (oracle/sgcpu.cpp, OpenMP) on the same 512^3 shell FP32 Gray-Scott grid, "
-                         "%d steps in %.1f s" % (c["steps"], c["seconds"])}
+                         "%d steps in %.1f s on %d threads" % (c["steps"], c["seconds"], cpu_threads),
+               "one_core": {"value": round(c1["value"], 6), "cores": 1,
+                            "sample": "%d steps in %.1f s on 1 thread" % (c1["steps"], c1["seconds"])}}
 
     secondary = None
     if not args.no_secondary:
@@ -388,7 +468,7 @@ def main():
                              "roofline": {"bound": "hbm", "achieved": round(ach64, 2), "peak": HBM_PEAK_GBS,
                                           "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4)}}
         log("flush workload")
-        secondary = {"flush": run_flush_secondary(args.cpu_threads or min(16, os.cpu_count() or 1), not args.no_cpu),
+        secondary = {"flush": run_flush_secondary(cpu_threads, not args.no_cpu),
                      "poisson": None}
         if dtype == np.float32:
             secondary["stencil_fp64"] = secondary_f64
