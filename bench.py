@@ -93,7 +93,9 @@ def run_gpu(args, ids, masks, data, dtype, edge=8):
     tuned = None
     if edge == 8:
         # launch shape by measurement (cached per device and size class)
-        tuned, _ = autotune.tune_stencil(g, step, "gray_scott", dtype, stream)
+        tuned, times = autotune.tune_stencil(g, step, "gray_scott", dtype, stream, force=args.retune)
+        for cfg, ms in sorted(times.items(), key=lambda kv: kv[1]):
+            log("  candidate %s: %.4f ms" % (cfg, ms))
         g.synchronize()
         # restart from the initial state: tuning stepped the grid
         g.set_prop(0, data[0].astype(dtype))
@@ -317,6 +319,17 @@ def run_copy_bench(nbytes=1 << 30, reps=20):
                      "frac_of_peak": round(2 * nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     torch.cuda.synchronize()
     assert bool((b == 1).all()), "copy result"
+    # a non-temporal copy reading what the previous kernel's non-temporal stores wrote
+    # (distinct values, b and c cleared first): does the second copy see the new data?
+    a.copy_(torch.arange(a.numel(), device="cuda", dtype=torch.float32))
+    b.zero_()
+    c = torch.zeros_like(a)
+    torch.cuda.synchronize()
+    for dst, src in ((b, a), (c, b)):
+        C.check(lib.sg_stream_copy(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()), nbytes,
+                                   C.COPY_NONTEMPORAL, ctypes.c_void_p(stream.cuda_stream)))
+    stream.synchronize()
+    out["nontemporal_roundtrip_equal"] = bool(torch.equal(c, a))
     return out
 
 
@@ -365,6 +378,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core of the process's affinity set")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--retune", action="store_true", help="time every stencil launch shape instead of the cache")
     ap.add_argument("--no-secondary", action="store_true", help="skip the insertion/flush workload")
     ap.add_argument("--only", choices=["flush", "poisson", "serving"],
                     help="run one secondary workload alone and print its JSON (profiling runs)")

@@ -476,6 +476,47 @@ private:
     std::vector<unsigned char> buf_;
 };
 
+// host memory of the whole-grid host pack (the reference's HeapMemory, openfpm_devices), and
+// a view of it (ExtPreAlloc<HeapMemory>): pack / unpack read and write at a Pack_stat offset
+class HeapMemory
+{
+public:
+    bool allocate(size_t n)
+    {
+        buf_.assign(n, 0);
+        return true;
+    }
+    bool resize(size_t n)
+    {
+        buf_.resize(n, 0);
+        return true;
+    }
+    size_t size() const { return buf_.size(); }
+    void *getPointer() { return buf_.data(); }
+    const void *getPointer() const { return buf_.data(); }
+    void incRef() {}
+    void decRef() {}
+
+private:
+    std::vector<unsigned char> buf_;
+};
+template <typename Mem> class ExtPreAlloc
+{
+public:
+    ExtPreAlloc(size_t n, Mem &m) : m_(&m)
+    {
+        if (m.size() < n) m.resize(n);
+    }
+    size_t size() const { return m_->size(); }
+    unsigned char *getPointer() { return (unsigned char *)m_->getPointer(); }
+    const unsigned char *getPointer() const { return (const unsigned char *)m_->getPointer(); }
+    void incRef() {}
+    void decRef() {}
+
+private:
+    Mem *m_;
+};
+
 // read position in a PackMemory (the reference's Pack_stat / Unpack_stat)
 class Pack_stat
 {
@@ -743,7 +784,8 @@ public:
     void preFlush() {}
 
     // point inserts (insert<p>(coord) once per point): keys are getLinId values, one
-    // host value array per property (nullptr: leave unset), masks default to EXIST
+    // host value array per property (nullptr: the records carry the property's background,
+    // which the flush merges like any value), masks default to EXIST
     void insertPoints(const indexT *keys, size_t n, const void *const *values, const uint8_t *masks = nullptr)
     {
         std::vector<int64_t> k(keys, keys + n);
@@ -1155,6 +1197,100 @@ public:
         deviation = nb > 1 ? std::sqrt(v / (nb - 1)) : 0.0;
     }
 
+    // -- whole-grid pack into host memory (packRequest(req) / pack(mem, sts) / unpack(mem, ps),
+    //    SparseGridGpu.hpp:2895-2959): the reference serialises its index buffer, its data
+    //    buffer and the linearizer.  Here, in host byte order, each part 8-byte aligned:
+    //      uint64 n | indexT ids[n]                                 (the chunk list)
+    //      uint64 n | uint64 k | int32 props[k] | T_q[n * chunk size] per listed property q
+    //               | uint8 mask[n * chunk size]                    (the data)
+    //      uint64 dim | uint64 size[dim]                            (the domain)
+    //    with every property when prp... is empty.  The mask travels always (the reference's
+    //    data buffer carries it only when every property is packed).  unpack replaces this
+    //    grid's chunk list and masks with the packed ones (the domains must be equal), sets
+    //    the packed properties and keeps the others per chunk (the background in new chunks).
+    template <int... prp> void packRequest(size_t &req) const
+    {
+        const size_t n = (size_t)sg_num_chunks(h_), cs = chunk_size();
+        const std::vector<int> pl = packed_props_<prp...>();
+        size_t b = 8 + align8_(n * sizeof(indexT)) + 16 + align8_(4 * pl.size());
+        for (int q : pl) b += align8_(n * cs * prop_bytes((unsigned)q));
+        b += align8_(n * cs) + 8 + 8 * dim;
+        req += b;
+    }
+    template <int... prp, typename Mem> void pack(ExtPreAlloc<Mem> &mem, Pack_stat &sts) const
+    {
+        size_t req = 0;
+        packRequest<prp...>(req);
+        if (sts.getOffset() + req > mem.size()) throw sg::error("pack: host memory too small");
+        unsigned char *out = mem.getPointer() + sts.getOffset();
+        std::memset(out, 0, req);
+        const std::vector<indexT> ids = private_get_index_array();
+        const uint64_t n = ids.size();
+        const size_t cs = chunk_size();
+        size_t at = 0;
+        put_(out, at, &n, 8);
+        put_(out, at, ids.data(), n * sizeof(indexT));
+        const std::vector<int> pl = packed_props_<prp...>();
+        const uint64_t k = pl.size();
+        put_(out, at, &n, 8);
+        put_(out, at, &k, 8);
+        put_(out, at, pl.data(), 4 * pl.size());
+        for (int q : pl) {
+            if (n) sg::check(sg_get_prop(h_, q, out + at, SG_MEM_HOST));
+            at += align8_(n * cs * prop_bytes((unsigned)q));
+        }
+        if (n) sg::check(sg_get_masks(h_, out + at, SG_MEM_HOST));
+        at += align8_(n * cs);
+        const uint64_t d = dim;
+        put_(out, at, &d, 8);
+        for (unsigned int i = 0; i < dim; ++i) {
+            const uint64_t v = size_[i];
+            put_(out, at, &v, 8);
+        }
+        sts.addOffset(at);
+    }
+    template <int... prp, typename Mem> void unpack(ExtPreAlloc<Mem> &mem, Unpack_stat &ps)
+    {
+        const unsigned char *in = mem.getPointer() + ps.getOffset();
+        const size_t avail = mem.size() - ps.getOffset();
+        size_t at = 0;
+        uint64_t n = 0, n2 = 0, k = 0;
+        get_(in, avail, at, &n, 8);
+        std::vector<indexT> ids((size_t)n);
+        get_(in, avail, at, ids.data(), n * sizeof(indexT));
+        get_(in, avail, at, &n2, 8);
+        get_(in, avail, at, &k, 8);
+        if (n2 != n || k > nprops) throw sg::error("unpack: malformed packed grid");
+        std::vector<int> pl((size_t)k);
+        get_(in, avail, at, pl.data(), 4 * k);
+        const size_t cs = chunk_size();
+        std::vector<const unsigned char *> src((size_t)k);
+        for (size_t i = 0; i < k; ++i) {
+            if (pl[i] < 0 || pl[i] >= (int)nprops) throw sg::error("unpack: malformed packed grid");
+            src[i] = in + at;
+            skip_(avail, at, n * cs * prop_bytes((unsigned)pl[i]));
+        }
+        const unsigned char *masks = in + at;
+        skip_(avail, at, n * cs);
+        uint64_t d = 0;
+        get_(in, avail, at, &d, 8);
+        if (d != dim) throw sg::error("unpack: packed grid of another dimension");
+        for (unsigned int i = 0; i < dim; ++i) {
+            uint64_t v = 0;
+            get_(in, avail, at, &v, 8);
+            if (v != size_[i]) throw sg::error("unpack: packed grid of another domain size");
+        }
+        std::vector<int64_t> ids64(ids.begin(), ids.end());
+        sg::check(sg_set_chunks(h_, (int64_t)n, ids64.data(), masks, SG_MEM_HOST));
+        const std::vector<int> want = packed_props_<prp...>();
+        for (size_t i = 0; i < k; ++i)
+            if (n && std::find(want.begin(), want.end(), pl[i]) != want.end())
+                sg::check(sg_set_prop(h_, pl[i], src[i], SG_MEM_HOST));
+        host_copy_ = false;
+        host_dirty_ = false;
+        ps.addOffset(at);
+    }
+
     // -- pack / unpack / remove (SparseGridGpu.hpp:3040-3733) --
     typedef SparseGridGpu_iterator_sub<dim, SparseGridGpu> sub_iterator;
     sub_iterator getIterator(const grid_key_dx<dim, int> &start, const grid_key_dx<dim, int> &stop) const
@@ -1298,6 +1434,30 @@ public:
     }
 
 private:
+    static size_t align8_(size_t b) { return (b + 7) & ~(size_t)7; }
+    static void put_(unsigned char *out, size_t &at, const void *src, size_t bytes)
+    {
+        if (bytes) std::memcpy(out + at, src, bytes);
+        at += align8_(bytes);
+    }
+    static void get_(const unsigned char *in, size_t avail, size_t &at, void *dst, size_t bytes)
+    {
+        if (at + bytes > avail) throw sg::error("unpack: packed grid truncated");
+        if (bytes) std::memcpy(dst, in + at, bytes);
+        at += align8_(bytes);
+    }
+    static void skip_(size_t avail, size_t &at, size_t bytes)
+    {
+        if (at + bytes > avail) throw sg::error("unpack: packed grid truncated");
+        at += align8_(bytes);
+    }
+    template <int... prp> static std::vector<int> packed_props_()
+    {
+        std::vector<int> pl = {prp...};
+        if (pl.empty())
+            for (unsigned int q = 0; q < nprops; ++q) pl.push_back((int)q);
+        return pl;
+    }
     int npack() const { return (int)(pack_lo_.size() / dim); }
     std::vector<int32_t> pack_lo_, pack_hi_;
 

@@ -171,12 +171,14 @@ int sg_set_background(sg_grid *g, int prop, const void *value);
  *  src/Vector/cuda/map_vector_sparse_cuda_ker.cuh:309-324).  Records are
  * appended to the handle's insert buffer in call order and merged by
  * sg_flush in that order.  masks may be NULL (every record EXIST);
- * values[p] may be NULL (value unspecified; the record only sets mask bits). */
+ * values[p] may be NULL: the records then carry the property's background value, which
+ * the flush merges with the property's reduction like any inserted value. */
 int sg_insert_points(sg_grid *g, int64_t n, const int64_t *keys,
                      const uint8_t *masks, const void *const *values, int mem);
 /* same, with integer coordinates [n][dim] instead of linear keys; host coordinates
- * outside the grid fail here (SG_E_RANGE), device ones fail the next sg_flush (which
- * then drops the pending records), so the call does not wait for the device */
+ * outside the grid fail here (SG_E_RANGE), device ones fail the next sg_flush (which then
+ * drops those records only: the other records stay pending, unmerged, for the next
+ * sg_flush), so the call does not wait for the device */
 int sg_insert_coords(sg_grid *g, int64_t n, const int32_t *coords,
                      const uint8_t *masks, const void *const *values, int mem);
 /* Chunk insertion: insertBlock(blockId) per chunk slot

@@ -5,8 +5,9 @@ What is tuned
 * 8^3 star stencils (Laplacian, heat, Gray-Scott): the kernel variant (one chunk per
   workgroup; persistent waves with a workgroups-per-XCD count; groups of 2 / 4 / 8 chunks
   per workgroup sharing x faces through LDS, whose LDS footprint grows with the group,
-  with one LDS tile per property or one tile shared by the two Gray-Scott properties) and
-  the store mode (whole rows, plain or non-temporal) -- ``tune_stencil``;
+  with one LDS tile per property or one tile shared by the two Gray-Scott properties), the
+  store mode (whole rows, plain or non-temporal) and the load mode (every row, or rows
+  without points skipped by the occupancy table) -- ``tune_stencil``;
 * batched red-black sweeps: waves (chunk tiles) per workgroup, i.e. the LDS tile footprint
   per workgroup, and a workgroup cap that makes waves loop over tiles -- ``tune_batch``;
 * the chunk edge of a point set for a stencil workload: grids are built at each candidate
@@ -42,6 +43,11 @@ STENCIL_CANDIDATES = [
     {"variant": 6, "stores": 3},
     {"variant": 7, "stores": 3},
     {"variant": 8, "stores": 3},
+    {"variant": 3, "stores": 3, "lskip": 2},
+    {"variant": 3, "stores": 3, "lskip": 3},
+    {"variant": 4, "stores": 3, "lskip": 2},
+    {"variant": 6, "stores": 3, "lskip": 2},
+    {"variant": 7, "stores": 3, "lskip": 2},
 ]
 
 BATCH_CANDIDATES = [
@@ -121,6 +127,7 @@ def apply_stencil_tuning(grid, cfg):
     if "wg_per_xcd" in cfg:
         grid.set_tuning(C.TUNE_STENCIL_WG_PER_XCD, cfg["wg_per_xcd"])
     grid.set_tuning(C.TUNE_STENCIL_STORES, cfg["stores"])
+    grid.set_tuning(C.TUNE_STENCIL_SKIP_EMPTY_ROWS, cfg.get("lskip", 0))
 
 
 def tune_stencil(grid, step, kind, dtype, stream, reps=30, force=False, candidates=None):

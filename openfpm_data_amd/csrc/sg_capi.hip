@@ -427,7 +427,7 @@ int sg_set_tuning(sg_grid *g, int key, int64_t value)
         return SG_OK;
     }
     if (key == SG_TUNE_STENCIL_SKIP_EMPTY_ROWS) {
-        if (value < 0 || value > 1) return fail(SG_E_RANGE, "skip-empty-rows must be 0 or 1");
+        if (value < 0 || value > 3) return fail(SG_E_RANGE, "skip-empty-rows must be 0, 1, 2 or 3");
         g->fast_lskip = (int)value;
         return SG_OK;
     }
@@ -515,7 +515,10 @@ int sg_get_neighbours(sg_grid *g, int nn_type, int32_t *out, int mem)
 int sg_device_ptr(sg_grid *g, int prop, void **out)
 {
     if (!g || !out) return fail(SG_E_ARG, "null argument");
-    if (prop == -1) *out = g->d_mask;
+    if (prop == -1) {
+        *out = g->d_mask;
+        g->occ_tab_valid = false;  // the caller may write masks
+    }
     else if (valid_prop(g, prop)) *out = g->d_prop[prop];
     else return fail(SG_E_ARG, "bad property");
     return SG_OK;
@@ -542,6 +545,7 @@ int sg_to_kernel(sg_grid *g, int nn_type, sg_kernel_view *out)
         if (rc) return rc;
     }
     memset(out, 0, sizeof(*out));
+    g->occ_tab_valid = false;  // the caller's kernels may write masks
     out->keys = g->d_keys;
     out->mask = g->d_mask;
     for (int p = 0; p < g->np; ++p) out->prop[p] = g->d_prop[p];

@@ -38,7 +38,9 @@ constexpr int kWG = 256;
 template <typename T> __device__ inline T dev_add(T a, T b)
 {
     if constexpr (std::is_integral<T>::value) {
-        typedef typename std::make_unsigned<T>::type U;
+        // wrapping arithmetic in an unsigned type of at least 32 bits (narrower types are
+        // promoted to signed int, where a product can overflow), truncated to T
+        typedef typename std::conditional<(sizeof(T) < 4), uint32_t, typename std::make_unsigned<T>::type>::type U;
         return (T)(U)((U)a + (U)b);
     } else {
         return a + b;
@@ -75,9 +77,10 @@ template <typename T> __global__ void fill_kernel(T *p, T v, int64_t n)
     if (i < n) p[i] = v;
 }
 
-// coordinates -> grid_smb linear ids (grid_smb.hpp:175-189); out-of-range -> flag
+// coordinates -> grid_smb linear ids (grid_smb.hpp:175-189); a coordinate outside the grid
+// gives key -1, which the next flush drops
 __global__ void coords_to_keys_kernel(const int32_t *coords, int64_t *keys, int64_t n, int dim, int edge, int64_t sz0,
-                                      int64_t sz1, int64_t sz2, ChunkLin L, int csz, int *bad)
+                                      int64_t sz1, int64_t sz2, ChunkLin L, int csz)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -89,9 +92,7 @@ __global__ void coords_to_keys_kernel(const int32_t *coords, int64_t *keys, int6
         ok = ok && c[d] >= 0 && c[d] < sz[d];
     }
     if (!ok) {
-        // flagged, and a key the flush's key check rejects (device coordinates are not
-        // read back at insertion)
-        *bad = 1;
+        // device coordinates are not read back at insertion: the flush's key check finds it
         keys[i] = -1;
         return;
     }
@@ -724,6 +725,22 @@ template <typename F> void dispatch_type(int dt, F &&f)
     }
 }
 
+// records whose key lies in [0, limit)
+__global__ void valid_key_kernel(const int64_t *keys, int64_t n, int64_t limit, uint32_t *flag)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = (keys[i] >= 0 && keys[i] < limit) ? 1u : 0u;
+}
+
+// stable compaction of ts-byte elements: src[i] -> dst[pos[i]] where flag[i]
+__global__ void compact_bytes_kernel(const uint8_t *src, uint8_t *dst, const uint32_t *flag, const uint32_t *pos,
+                                     int64_t n, int ts)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    for (int b = 0; b < ts; ++b) dst[(int64_t)pos[i] * ts + b] = src[i * ts + b];
+}
+
 int grow_insert_buffer(sg_grid *g, int64_t need)
 {
     if (need <= g->ins_cap) return SG_OK;
@@ -749,6 +766,45 @@ int grow_insert_buffer(sg_grid *g, int64_t need)
     g->d_ins_mask = nm;
     for (int p = 0; p < g->np; ++p) g->d_ins_val[p] = nv[p];
     g->ins_cap = ncap;
+    return SG_OK;
+}
+
+// drop the point records whose key is outside [0, limit) from the insert buffer, keeping
+// the others in order; *kept receives their count
+int drop_invalid_records(sg_grid *g, int64_t limit, int64_t *kept)
+{
+    const int64_t n = g->ins_n;
+    int rc;
+    DevBuf &b = g->scratch[8];
+    if ((rc = ensure(g->pool, b, (size_t)n * 8 + 64))) return rc;
+    uint32_t *flag = (uint32_t *)b.p, *pos = flag + n;
+    hipLaunchKernelGGL(valid_key_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key, n, limit, flag);
+    SG_HIP(hipGetLastError());
+    uint32_t nv = 0;
+    if ((rc = exclusive_scan_u32(g, flag, pos, n, &nv))) return rc;
+    int64_t *nk = (int64_t *)g->pool.alloc(g->ins_cap * sizeof(int64_t));
+    uint8_t *nm = (uint8_t *)g->pool.alloc(g->ins_cap);
+    if (!nk || !nm) return fail(SG_E_OOM, "insert buffer allocation failed");
+    void *nvl[kMaxProps] = {};
+    for (int p = 0; p < g->np; ++p)
+        if (!(nvl[p] = g->pool.alloc(g->ins_cap * g->ts[p]))) return fail(SG_E_OOM, "insert buffer allocation failed");
+    const dim3 grid(grid_1d(n, kWG)), blk(kWG);
+    hipLaunchKernelGGL(compact_bytes_kernel, grid, blk, 0, g->stream, (const uint8_t *)g->d_ins_key, (uint8_t *)nk,
+                       flag, pos, n, 8);
+    hipLaunchKernelGGL(compact_bytes_kernel, grid, blk, 0, g->stream, (const uint8_t *)g->d_ins_mask, nm, flag, pos, n,
+                       1);
+    for (int p = 0; p < g->np; ++p)
+        hipLaunchKernelGGL(compact_bytes_kernel, grid, blk, 0, g->stream, (const uint8_t *)g->d_ins_val[p],
+                           (uint8_t *)nvl[p], flag, pos, n, g->ts[p]);
+    SG_HIP(hipGetLastError());
+    g->pool.free(g->d_ins_key);
+    g->pool.free(g->d_ins_mask);
+    for (int p = 0; p < g->np; ++p) g->pool.free(g->d_ins_val[p]);
+    g->d_ins_key = nk;
+    g->d_ins_mask = nm;
+    for (int p = 0; p < g->np; ++p) g->d_ins_val[p] = nvl[p];
+    g->ins_n = nv;
+    *kept = nv;
     return SG_OK;
 }
 
@@ -836,18 +892,11 @@ int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *m
         if (rc) return rc;
         SG_HIP(hipMemcpyAsync(cb.p, coords, n * g->dim * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
         dc = (const int32_t *)cb.p;
-    } else {
-        rc = ensure(g->pool, cb, 64);
-        if (rc) return rc;
     }
-    // the kernel's flag word (not read back: invalid coordinates are caught by the flush)
-    int *bad = (int *)((char *)cb.p + ((mem == SG_MEM_HOST) ? n * g->dim * sizeof(int32_t) : 0));
-    bad = (int *)(((uintptr_t)bad + 7) & ~(uintptr_t)7);
     hipLaunchKernelGGL(coords_to_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, dc,
-                       g->d_ins_key + at, n, g->dim, g->edge, g->sz[0], g->sz[1], g->sz[2], chunk_lin(g), g->csz,
-                       bad);
+                       g->d_ins_key + at, n, g->dim, g->edge, g->sz[0], g->sz[1], g->sz[2], chunk_lin(g), g->csz);
     SG_HIP(hipGetLastError());
-    // device coordinates outside the grid became key -1: the next flush rejects them
+    // device coordinates outside the grid became key -1: the next flush drops those records
     rc = append_mask_values(g, at, n, masks, values, mem);
     if (rc) return rc;
     g->ins_n += n;
@@ -1354,6 +1403,7 @@ int flush_chunks_impl(sg_grid *g, const int32_t *ops)
     SG_HIP(hipGetLastError());
     g->cins_n = 0;
     g->nn_valid[0] = g->nn_valid[1] = false;
+    g->occ_tab_valid = false;
     g->ep_valid = false;
     return SG_OK;
 }
@@ -1430,8 +1480,11 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     uint32_t nu32 = 0;
     if ((rc = exclusive_scan_u32(g, head, hscan, n, &nu32))) return rc;
     if (hw[8]) {  // the key check above, complete now
-        g->ins_n = 0;
-        return fail(SG_E_RANGE, "point key outside the grid in the insert buffer");
+        // the records outside the grid are dropped, the others stay pending (nothing is merged)
+        int64_t kept = 0;
+        if ((rc = drop_invalid_records(g, limit, &kept))) return rc;
+        return fail(SG_E_RANGE, std::to_string(n - kept) + " inserted points outside the grid were dropped; the other " +
+                                    std::to_string(kept) + " stay pending (flush again to merge them)");
     }
     const int64_t nu = nu32;
     int64_t *ukey = (int64_t *)((char *)b_u.p + (((size_t)n * 4 + 15) & ~(size_t)15));
@@ -1546,6 +1599,7 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     g->ins_n = 0;
     // flush marks the neighbour table stale (SparseGridGpu.hpp:778), and the point list
     g->nn_valid[0] = g->nn_valid[1] = false;
+    g->occ_tab_valid = false;
     g->ep_valid = false;
     return SG_OK;
 }
@@ -1640,6 +1694,7 @@ int set_chunks_impl(sg_grid *g, int64_t n, const int64_t *ids, const uint8_t *ma
     g->nchunks = n;
     g->cap = ncap;
     g->nn_valid[0] = g->nn_valid[1] = false;
+    g->occ_tab_valid = false;
     g->ep_valid = false;
     return SG_OK;
 }

@@ -136,8 +136,18 @@ struct sg_grid {
     sg::DevBuf dins;
     int64_t dins_cap = 0;
     bool dins_open = false;
-    // read the mask first and skip loading 8-point rows without points
+    // 8^3 star stencils: 0 load every row; 1 read the mask first and skip loading 8-point
+    // rows without points; 2 / 3 skip the own rows (2 only) and ghost rows without points
+    // as the occupancy table says
     int fast_lskip = 0;
+    // occupancy table of the 8^3 star stencils, [n][16] int32 per chunk: the 6 star
+    // neighbour positions, the chunk's row occupancy (64 bits: row z*8+y has a slot with a
+    // non-zero mask), the occupancy of the 32 face ghost rows it reads (32 bits), and the
+    // row occupancy of its -x and +x neighbours; rebuilt when stale (chunk set or masks
+    // changed)
+    int32_t *d_occ_tab = nullptr;
+    int64_t occ_tab_cap = 0;
+    bool occ_tab_valid = false;
 
     // existing points (data positions chunk * csz + offset) for the point-list stencil
     // mode; rebuilt when stale (tagBoundaries with CALCULATE_EXISTING_POINTS builds it)
@@ -313,6 +323,8 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d);
 int get_points_impl(sg_grid *g, int prop, int64_t n, const int64_t *keys, void *out, int mem);
 int fill_background_chunk(sg_grid *g, int prop);
 int normalize_prop(sg_grid *g, int prop);
+// occupancy table of the 8^3 star stencils (sg_grid::d_occ_tab), rebuilt when stale
+int occ_table_impl(sg_grid *g);
 int count_mask_impl(sg_grid *g, int which, int64_t *out);
 // replace the chunk list and masks (hostToDevice of a host copy whose chunks changed)
 int set_chunks_impl(sg_grid *g, int64_t n, const int64_t *ids, const uint8_t *masks, int mem);

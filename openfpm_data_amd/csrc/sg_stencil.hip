@@ -28,7 +28,9 @@ constexpr int kWG = 256;
 template <typename T> __device__ inline T dev_add(T a, T b)
 {
     if constexpr (std::is_integral<T>::value) {
-        typedef typename std::make_unsigned<T>::type U;
+        // wrapping arithmetic in an unsigned type of at least 32 bits (narrower types are
+        // promoted to signed int, where a product can overflow), truncated to T
+        typedef typename std::conditional<(sizeof(T) < 4), uint32_t, typename std::make_unsigned<T>::type>::type U;
         return (T)(U)((U)a + (U)b);
     } else {
         return a + b;
@@ -37,7 +39,7 @@ template <typename T> __device__ inline T dev_add(T a, T b)
 template <typename T> __device__ inline T dev_mul(T a, T b)
 {
     if constexpr (std::is_integral<T>::value) {
-        typedef typename std::make_unsigned<T>::type U;
+        typedef typename std::conditional<(sizeof(T) < 4), uint32_t, typename std::make_unsigned<T>::type>::type U;
         return (T)(U)((U)a * (U)b);
     } else {
         return a * b;
@@ -1005,6 +1007,43 @@ __device__ inline void issue_ghosts(PropIn<T> &r, const T *__restrict__ arr, con
     if (load_xp) r.xp = arr[nbp[1] * 512 + lane * 8 + 0];
 }
 
+// the occupancy table row of a chunk (see sg_grid::d_occ_tab): star neighbour positions,
+// own row occupancy, face ghost-row occupancy, -x / +x neighbour row occupancy
+struct OccRow {
+    int64_t nbp[6];
+    uint64_t own, xm, xp;
+    uint32_t ghost;
+};
+
+__device__ inline void read_occ(OccRow &o, const int32_t *__restrict__ tab, int64_t c)
+{
+    const int32_t *row = tab + c * 16;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o.nbp[k] = row[k];
+    o.own = (uint64_t)(uint32_t)row[6] | ((uint64_t)(uint32_t)row[7] << 32);
+    o.ghost = (uint32_t)row[8];
+    o.xm = (uint64_t)(uint32_t)row[9] | ((uint64_t)(uint32_t)row[10] << 32);
+    o.xp = (uint64_t)(uint32_t)row[11] | ((uint64_t)(uint32_t)row[12] << 32);
+}
+
+// as issue_ghosts, reading only source rows that hold a point (the others hold the
+// background: library invariant)
+template <typename T>
+__device__ inline void issue_ghosts_occ(PropIn<T> &r, const T *__restrict__ arr, const OccRow &o, int lane, bool load_xm,
+                                        bool load_xp, T bg)
+{
+    const int face = lane >> 4;
+    const int64_t nbc = (face == 0) ? o.nbp[2] : (face == 1) ? o.nbp[3] : (face == 2) ? o.nbp[4] : o.nbp[5];
+    if ((o.ghost >> (lane >> 1)) & 1u) {
+        load_half(r.gh, arr + nbc * 512 + ghost_src_row(lane) * 8 + (lane & 1) * 4);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r.gh.v[i] = bg;
+    }
+    if (load_xm) r.xm = ((o.xm >> lane) & 1u) ? arr[o.nbp[0] * 512 + lane * 8 + 7] : bg;
+    if (load_xp) r.xp = ((o.xp >> lane) & 1u) ? arr[o.nbp[1] * 512 + lane * 8 + 0] : bg;
+}
+
 // LDS tile per property: rows (z, y) with z, y in [-1, 8] -> row index (z+1)*10 + (y+1),
 // 8 values per row.  Own rows and the 32 face rows are written; corner rows are unused.
 template <typename T> __device__ inline void stage_lds(const PropIn<T> &r, T *tile, int lane)
@@ -1073,6 +1112,27 @@ __device__ inline void issue_own(ChunkIn<T, KIND> &in, const uint8_t *__restrict
     in.mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
 }
 
+// own rows and mask of the chunk where the occupancy table says the row holds a point
+// (the others hold the background: library invariant)
+template <typename T, int KIND>
+__device__ inline void issue_own_occ(ChunkIn<T, KIND> &in, const uint8_t *__restrict__ mask, const T *__restrict__ src0,
+                                     const T *__restrict__ src1, int64_t c, int lane, const StencilArgs &sa,
+                                     uint64_t own)
+{
+    if ((own >> lane) & 1ull) {
+        load_row(in.u.own, src0 + c * 512 + lane * 8);
+        if constexpr (ChunkIn<T, KIND>::kTwo) load_row(in.v.own, src1 + c * 512 + lane * 8);
+        in.mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
+    } else {
+        in.mrow = 0;
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            in.u.own.v[x] = (T)sa.bgs[0];
+            if constexpr (ChunkIn<T, KIND>::kTwo) in.v.own.v[x] = (T)sa.bgs[1];
+        }
+    }
+}
+
 template <typename T, int KIND>
 __device__ inline void read_nn(int64_t (&nbp)[6], const int32_t *__restrict__ nn, int64_t c)
 {
@@ -1086,6 +1146,18 @@ __device__ inline void issue_chunk(ChunkIn<T, KIND> &in, const int32_t *__restri
                                    const T *__restrict__ src0, const T *__restrict__ src1, int64_t c, int lane,
                                    const StencilArgs &sa)
 {
+    if (sa.lskip >= 2) {
+        // nn is the occupancy table: rows without points are not loaded (lskip 3: ghost
+        // rows only, the own rows are loaded before the table row arrives)
+        if (sa.lskip == 3) issue_own<T, KIND>(in, mask, src0, src1, c, lane, sa);
+        __builtin_amdgcn_sched_barrier(0);
+        OccRow o;
+        read_occ(o, nn, c);
+        if (sa.lskip == 2) issue_own_occ<T, KIND>(in, mask, src0, src1, c, lane, sa, o.own);
+        issue_ghosts_occ<T>(in.u, src0, o, lane, true, true, (T)sa.bgs[0]);
+        if constexpr (ChunkIn<T, KIND>::kTwo) issue_ghosts_occ<T>(in.v, src1, o, lane, true, true, (T)sa.bgs[1]);
+        return;
+    }
     // own rows and mask first: they do not depend on the neighbour table
     issue_own<T, KIND>(in, mask, src0, src1, c, lane, sa);
     __builtin_amdgcn_sched_barrier(0);
@@ -1387,7 +1459,25 @@ __device__ inline void group_step(GroupLds<T, KIND, W, ONE> &L, int64_t grp, int
     const bool valid = c < n;  // every wave reaches the barrier
     ChunkIn<T, KIND> in;
     bool xm_grp = false, xp_grp = false;
-    if (valid) {
+    if (valid && sa.lskip >= 2) {
+        // nn is the occupancy table: rows without points are not loaded (lskip 3: ghost
+        // rows only, the own rows are loaded before the table row arrives)
+        if (sa.lskip == 3) issue_own<T, KIND>(in, mask, src0, src1, c, lane, sa);
+        __builtin_amdgcn_sched_barrier(0);
+        OccRow o;
+        read_occ(o, nn, c);
+        if (sa.lskip == 2) issue_own_occ<T, KIND>(in, mask, src0, src1, c, lane, sa, o.own);
+        xm_grp = (w > 0) && o.nbp[0] == c - 1;
+        xp_grp = (w < W - 1) && (c + 1 < n) && o.nbp[1] == c + 1;
+        issue_ghosts_occ<T>(in.u, src0, o, lane, !xm_grp, !xp_grp, (T)sa.bgs[0]);
+        if constexpr (kTwo) issue_ghosts_occ<T>(in.v, src1, o, lane, !xm_grp, !xp_grp, (T)sa.bgs[1]);
+        L.cols[w][0][0][lane] = in.u.own.v[0];
+        L.cols[w][0][1][lane] = in.u.own.v[7];
+        if constexpr (kTwo) {
+            L.cols[w][NP - 1][0][lane] = in.v.own.v[0];
+            L.cols[w][NP - 1][1][lane] = in.v.own.v[7];
+        }
+    } else if (valid) {
         issue_own<T, KIND>(in, mask, src0, src1, c, lane, sa);
         __builtin_amdgcn_sched_barrier(0);
         int64_t nbp[6];
@@ -1433,6 +1523,48 @@ __global__ __launch_bounds__(64 * W) void stencil_group_kernel(const int32_t *__
     const int64_t b = blockIdx.x;
     const int64_t grp = (b & 7) * groups_per_xcd + (b >> 3);
     group_step<T, KIND, W, ONE>(L, grp, w, lane, nn, n, mask, src0, src1, dst0, dst1, alias, sa, keys, bsz0, bsz1);
+}
+
+// row occupancy of 8^3 chunks: bit r of occ[c] set when row r (z*8 + y) of chunk c has a
+// slot with a non-zero mask; one wave per chunk, lane r reads its row's 8 mask bytes;
+// occ[n] (the background chunk) is 0
+__global__ __launch_bounds__(256) void row_occ_kernel(const uint8_t *__restrict__ mask, int64_t n,
+                                                      uint64_t *__restrict__ occ)
+{
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c > n) return;
+    const uint64_t m = (c < n) ? *(const uint64_t *)(mask + c * 512 + lane * 8) : 0ull;
+    const uint64_t b = __ballot(m != 0);
+    if (lane == 0) occ[c] = b;
+}
+
+// the occupancy table row of every chunk from the star neighbour table and the row
+// occupancy (one thread per chunk)
+__global__ void occ_table_kernel(const int32_t *__restrict__ nn, const uint64_t *__restrict__ occ, int64_t n,
+                                 int32_t *__restrict__ tab)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int64_t nbp[6];
+    for (int k = 0; k < 6; ++k) nbp[k] = nn[c * 8 + k];
+    uint32_t ghost = 0;
+    for (int j = 0; j < 32; ++j) {
+        const int face = j >> 3;
+        const uint64_t o = occ[nbp[2 + face]];
+        ghost |= (uint32_t)((o >> ghost_src_row(2 * j)) & 1ull) << j;
+    }
+    const uint64_t own = occ[c], xm = occ[nbp[0]], xp = occ[nbp[1]];
+    int32_t *row = tab + c * 16;
+    for (int k = 0; k < 6; ++k) row[k] = (int32_t)nbp[k];
+    row[6] = (int32_t)(uint32_t)own;
+    row[7] = (int32_t)(uint32_t)(own >> 32);
+    row[8] = (int32_t)ghost;
+    row[9] = (int32_t)(uint32_t)xm;
+    row[10] = (int32_t)(uint32_t)(xm >> 32);
+    row[11] = (int32_t)(uint32_t)xp;
+    row[12] = (int32_t)(uint32_t)(xp >> 32);
+    row[13] = row[14] = row[15] = 0;
 }
 
 template <typename T>
@@ -1740,6 +1872,12 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
     }
     const bool fast = g->dim == 3 && g->edge == 8 && type == SG_NN_STAR &&
                       (kind == SG_ST_LAP || kind == SG_ST_HEAT || kind == SG_ST_GRAY_SCOTT) && alias_ok;
+    // the fast kernels take the star table, or with lskip >= 2 the occupancy table
+    const int32_t *fnn = g->d_nn[type];
+    if (fast && g->fast_lskip >= 2) {
+        if ((rc = occ_table_impl(g))) return rc;
+        fnn = g->d_occ_tab;
+    }
     dispatch_type(dt, [&](auto tag) {
         typedef decltype(tag) T;
         const T *s0 = (const T *)g->d_prop[d->src[0]];
@@ -1755,7 +1893,7 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
                     if (g->fast_variant == 0) {
                         // one chunk per 64-lane workgroup
                         hipLaunchKernelGGL((stencil_wave_kernel<T, K>), dim3((unsigned)(per_xcd * 8)), dim3(64), 0,
-                                           g->stream, g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys,
+                                           g->stream, fnn, n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys,
                                            g->bsz[0], g->bsz[1], per_xcd);
                         return;
                     }
@@ -1769,7 +1907,7 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
                             const int64_t ngroups = (n + GW - 1) / GW;
                             const int64_t gpx = (ngroups + 7) / 8;
                             hipLaunchKernelGGL((stencil_group_kernel<T, K, GW, ONE>), dim3((unsigned)(gpx * 8)),
-                                               dim3(64 * GW), 0, g->stream, g->d_nn[type], n, g->d_mask, s0, s1, o0,
+                                               dim3(64 * GW), 0, g->stream, fnn, n, g->d_mask, s0, s1, o0,
                                                o1, al, sa, g->d_keys, g->bsz[0], g->bsz[1], gpx);
                         };
                         const std::false_type two_tiles;
@@ -1792,11 +1930,11 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
                     const unsigned nblk = (unsigned)(wg_per_xcd * 8);
                     if (g->fast_variant == 1)
                         hipLaunchKernelGGL((stencil_fast_kernel<T, K, 1>), dim3(nblk), dim3(256), 0, g->stream,
-                                           g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
+                                           fnn, n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
                                            g->bsz[1], per_xcd, per_wave);
                     else
                         hipLaunchKernelGGL((stencil_fast_kernel<T, K, 4>), dim3(nblk), dim3(256), 0, g->stream,
-                                           g->d_nn[type], n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
+                                           fnn, n, g->d_mask, s0, s1, o0, o1, al, sa, g->d_keys, g->bsz[0],
                                            g->bsz[1], per_xcd, per_wave);
                 };
                 if (kind == SG_ST_LAP) launch(std::integral_constant<int, SG_ST_LAP>());
@@ -1825,6 +1963,37 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
         rc = fill_background_chunk(g, p);
         if (rc) return rc;
     }
+    return SG_OK;
+}
+
+// occupancy table of the 8^3 star stencils (rebuilt when stale)
+int occ_table_impl(sg_grid *g)
+{
+    if (g->occ_tab_valid) return SG_OK;
+    int rc;
+    if (!g->nn_valid[SG_NN_STAR] && (rc = find_neighbours_impl(g, SG_NN_STAR))) return rc;
+    const int64_t n = g->nchunks;
+    if (n > g->occ_tab_cap || !g->d_occ_tab) {
+        g->pool.free(g->d_occ_tab);
+        const int64_t cap = std::max<int64_t>(n, 1);
+        g->d_occ_tab = (int32_t *)g->pool.alloc((size_t)cap * 16 * sizeof(int32_t));
+        if (!g->d_occ_tab) {
+            g->occ_tab_cap = 0;
+            return fail(SG_E_OOM, "occupancy table allocation failed");
+        }
+        g->occ_tab_cap = cap;
+    }
+    DevBuf &b = g->scratch[12];
+    if ((rc = ensure(g->pool, b, (size_t)(n + 1) * sizeof(uint64_t)))) return rc;
+    uint64_t *occ = (uint64_t *)b.p;
+    hipLaunchKernelGGL(row_occ_kernel, dim3((unsigned)(n / 4 + 1)), dim3(256), 0, g->stream, g->d_mask, n, occ);
+    SG_HIP(hipGetLastError());
+    if (n > 0) {
+        hipLaunchKernelGGL(occ_table_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_nn[SG_NN_STAR], occ,
+                           n, g->d_occ_tab);
+        SG_HIP(hipGetLastError());
+    }
+    g->occ_tab_valid = true;
     return SG_OK;
 }
 

@@ -51,6 +51,19 @@ def _ptr(x):
     return ctypes.c_void_p(x.ctypes.data)
 
 
+def _check_device(t, dtype, numel, what):
+    """A device tensor handed to the C ABI: contiguous, of the element size and kind (float or
+    integer) of `dtype` (integers of either signedness share the bytes), `numel` elements."""
+    dtype = np.dtype(dtype)
+    if not t.is_contiguous():
+        raise ValueError("%s: the device tensor must be contiguous" % what)
+    if t.element_size() != dtype.itemsize or t.is_floating_point() != (dtype.kind == "f"):
+        raise ValueError("%s: device tensor of %s where %s is expected" % (what, t.dtype, dtype.name))
+    if numel is not None and t.numel() != numel:
+        raise ValueError("%s: %d elements where %d are expected" % (what, t.numel(), numel))
+    return t
+
+
 def _mem(*xs):
     dev = [_is_device(x) for x in xs if x is not None]
     if dev and all(dev):
@@ -149,7 +162,7 @@ class SparseGridGpu:
         if not isinstance(values, (list, tuple)):
             values = [values]
         if len(values) != len(self.dtypes):
-            raise ValueError("one value array per property (None to leave a property unset)")
+            raise ValueError("one value array per property (None: the records carry the background)")
         keep = []
         arr = (ctypes.c_void_p * len(values))()
         for p, v in enumerate(values):
@@ -160,17 +173,30 @@ class SparseGridGpu:
                 v = np.ascontiguousarray(np.asarray(v, dtype=self.dtypes[p]).reshape(-1))
                 if v.size != n * per:
                     raise ValueError("property %d: expected %d values" % (p, n * per))
+            else:
+                _check_device(v, self.dtypes[p], n * per, "property %d" % p)
             keep.append(v)
             arr[p] = _ptr(v).value
         return arr, keep
+
+    def _masks(self, masks, count):
+        if masks is None:
+            return None
+        if _is_device(masks):
+            return _check_device(masks, np.uint8, count, "masks")
+        masks = np.ascontiguousarray(np.asarray(masks, dtype=np.uint8).reshape(-1))
+        if masks.size != count:
+            raise ValueError("masks: expected %d bytes" % count)
+        return masks
 
     def insert(self, keys, values=None, masks=None):
         """Point insertion (``insert<p>(coord)`` once per point) by linear key."""
         if not _is_device(keys):
             keys = np.ascontiguousarray(np.asarray(keys, dtype=np.int64).reshape(-1))
+        else:
+            _check_device(keys, np.int64, None, "keys")
         n = keys.numel() if _is_device(keys) else keys.size
-        if masks is not None and not _is_device(masks):
-            masks = np.ascontiguousarray(np.asarray(masks, dtype=np.uint8).reshape(-1))
+        masks = self._masks(masks, n)
         vals, keep = self._values(values, n)
         mem = _mem(keys, masks, *keep)
         C.check(self._lib.sg_insert_points(self._h, n, _ptr(keys), _ptr(masks), vals, mem))
@@ -181,9 +207,11 @@ class SparseGridGpu:
             coords = np.ascontiguousarray(np.asarray(coords, dtype=np.int32).reshape(-1, self.dim))
             n = coords.shape[0]
         else:
+            if coords.dim() != 2 or coords.shape[1] != self.dim:
+                raise ValueError("coords: device tensor of shape (n, %d) expected" % self.dim)
+            _check_device(coords, np.int32, None, "coords")
             n = coords.shape[0]
-        if masks is not None and not _is_device(masks):
-            masks = np.ascontiguousarray(np.asarray(masks, dtype=np.uint8).reshape(-1))
+        masks = self._masks(masks, n)
         vals, keep = self._values(values, n)
         mem = _mem(coords, masks, *keep)
         C.check(self._lib.sg_insert_coords(self._h, n, _ptr(coords), _ptr(masks), vals, mem))
@@ -194,9 +222,11 @@ class SparseGridGpu:
             chunk_ids = np.ascontiguousarray(np.asarray(chunk_ids, dtype=np.int64).reshape(-1))
             n = chunk_ids.size
         else:
+            _check_device(chunk_ids, np.int64, None, "chunk_ids")
             n = chunk_ids.numel()
-        if not _is_device(masks):
-            masks = np.ascontiguousarray(np.asarray(masks, dtype=np.uint8).reshape(-1))
+        masks = self._masks(masks, n * self.chunk_size)
+        if masks is None:
+            raise ValueError("insert_block needs the masks of the chunks")
         vals, keep = self._values(values, n, self.chunk_size)
         mem = _mem(chunk_ids, masks, *keep)
         C.check(self._lib.sg_insert_chunks(self._h, n, _ptr(chunk_ids), _ptr(masks), vals, mem))

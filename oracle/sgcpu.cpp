@@ -72,11 +72,13 @@ void dispatch(int dt, F &&f)
     }
 }
 
-// Wrapping add for integers (signed overflow done in the unsigned type), plain + for FP.
+// Wrapping add / mul for integers, done in an unsigned type of at least 32 bits (narrower
+// types would be promoted to signed int, where a product can overflow) and truncated to T;
+// plain + / * for FP.
 template <typename T> T op_add(T a, T b)
 {
     if constexpr (std::is_integral<T>::value) {
-        typedef typename std::make_unsigned<T>::type U;
+        typedef typename std::conditional<(sizeof(T) < 4), uint32_t, typename std::make_unsigned<T>::type>::type U;
         return (T)(U)((U)a + (U)b);
     } else {
         return a + b;
@@ -86,7 +88,7 @@ template <typename T> T op_add(T a, T b)
 template <typename T> T op_mul(T a, T b)
 {
     if constexpr (std::is_integral<T>::value) {
-        typedef typename std::make_unsigned<T>::type U;
+        typedef typename std::conditional<(sizeof(T) < 4), uint32_t, typename std::make_unsigned<T>::type>::type U;
         return (T)(U)((U)a * (U)b);
     } else {
         return a * b;

@@ -14,6 +14,7 @@
 //   the ball of the sub-iterator test into an empty grid
 //   testLinks: construct_link_dw / construct_link_up (SparseGridGpu.hpp:2090-2262) on a coarse
 //   ball and a fine ball: every link is a child / the parent of its padding point
+//   testHostPackUnpack: whole-grid pack into host memory (SparseGridGpu.hpp:2895-2959)
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -566,6 +567,88 @@ static void testInsertZ()
     REQUIRE(match);
 }
 
+// whole-grid pack into host memory (packRequest(req) / pack(mem, sts) / unpack(mem, ps),
+// SparseGridGpu.hpp:2895-2959): a grid packed and unpacked into an empty one gives the same
+// chunks, masks and properties; a pack of one property keeps the receiver's others
+static void testHostPackUnpack()
+{
+    typedef SparseGridGpu<3, aggregate<float, int>, 8> SG;
+    const size_t sz[3] = {64, 64, 64};
+    gpu::ofp_context_t ctx;
+    SG a(sz);
+    a.setBackgroundValue<1>(-3);
+    std::vector<long> keys;
+    std::vector<float> f;
+    std::vector<int> iv;
+    for (int z = 0; z < 64; ++z)
+        for (int y = 0; y < 64; ++y)
+            for (int x = 0; x < 64; ++x) {
+                const double r = std::sqrt((x - 32.0) * (x - 32.0) + (y - 32.0) * (y - 32.0) + (z - 32.0) * (z - 32.0));
+                if (r >= 12.0 && r < 17.0) {
+                    keys.push_back(a.getLinId(grid_key_dx<3, int>({x, y, z})));
+                    f.push_back((float)(0.5 * x - 0.25 * y + z));
+                    iv.push_back(x * 100 + y);
+                }
+            }
+    const void *v[] = {f.data(), iv.data()};
+    a.insertPoints(keys.data(), keys.size(), v);
+    a.flush<smax_<0>, smax_<1>>(ctx, FLUSH_ON_DEVICE);
+    a.findNeighbours();
+    a.tagBoundaries(ctx);
+    size_t req = 0;
+    a.packRequest(req);
+    HeapMemory pmem;
+    ExtPreAlloc<HeapMemory> &mem = *(new ExtPreAlloc<HeapMemory>(req, pmem));
+    mem.incRef();
+    Pack_stat sts;
+    a.pack(mem, sts);
+    REQUIRE(sts.getOffset() == req);
+    SG b(sz);
+    b.setBackgroundValue<1>(-3);
+    Unpack_stat ps;
+    b.unpack(mem, ps);
+    REQUIRE(ps.getOffset() == req);
+    REQUIRE(b.private_get_index_array() == a.private_get_index_array());
+    REQUIRE(b.private_get_mask_array() == a.private_get_mask_array());
+    REQUIRE(b.private_get_data_array<0>() == a.private_get_data_array<0>());
+    REQUIRE(b.private_get_data_array<1>() == a.private_get_data_array<1>());
+    REQUIRE(b.countBoundaryElements() == a.countBoundaryElements());
+    // property 1 alone into a grid holding property 0 in a few of the chunks
+    size_t req1 = 0;
+    a.packRequest<1>(req1);
+    REQUIRE(req1 < req);
+    HeapMemory pmem1;
+    ExtPreAlloc<HeapMemory> mem1(req1, pmem1);
+    Pack_stat sts1;
+    a.pack<1>(mem1, sts1);
+    SG c(sz);
+    c.setBackgroundValue<1>(-3);
+    std::vector<long> k2(keys.begin(), keys.begin() + 100);
+    std::vector<float> f2(100, 7.0f);
+    std::vector<int> i2(100, 5);
+    const void *v2[] = {f2.data(), i2.data()};
+    c.insertPoints(k2.data(), k2.size(), v2);
+    c.flush<smax_<0>, smax_<1>>(ctx, FLUSH_ON_DEVICE);
+    Unpack_stat ps1;
+    c.unpack<1>(mem1, ps1);
+    REQUIRE(c.private_get_index_array() == a.private_get_index_array());
+    REQUIRE(c.private_get_data_array<1>() == a.private_get_data_array<1>());
+    for (long k : k2) REQUIRE(c.get<0>(c.getCoord(k)) == 7.0f);
+    REQUIRE(c.get<0>(c.getCoord(keys[200])) == 0.0f);
+    // another domain: refused
+    const size_t sz2[3] = {32, 64, 64};
+    SparseGridGpu<3, aggregate<float, int>, 8> d(sz2);
+    Unpack_stat ps2;
+    bool threw = false;
+    try {
+        d.unpack(mem, ps2);
+    } catch (const sg::error &) {
+        threw = true;
+    }
+    REQUIRE(threw);
+    delete &mem;
+}
+
 int main()
 {
     testInsert();
@@ -581,6 +664,7 @@ int main()
     testIteratorHost();
     testCopyTo();
     testLinks();
+    testHostPackUnpack();
     if (failures) {
         std::printf("%d failure(s)\n", failures);
         return 1;

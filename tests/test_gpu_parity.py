@@ -263,7 +263,9 @@ def test_blobs_flush_subsample(G, oracle):
 @pytest.mark.parametrize("variant,stores,lskip", [(0, 0, 0), (0, 1, 1), (1, 2, 0), (2, 1, 0), (3, 2, 1),
                                                   (4, 2, 0), (4, 0, 1), (5, 1, 0), (4, 3, 0), (3, 3, 1),
                                                   (0, 3, 0), (1, 3, 1), (6, 3, 0), (7, 3, 1), (8, 3, 0),
-                                                  (6, 2, 1), (7, 0, 0), (8, 1, 1)])
+                                                  (6, 2, 1), (7, 0, 0), (8, 1, 1), (3, 3, 2), (3, 3, 3),
+                                                  (4, 3, 2), (6, 3, 2), (7, 3, 3), (0, 3, 2), (1, 3, 2),
+                                                  (2, 0, 3), (5, 1, 2), (8, 2, 3)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_fast_stencil_variants(G, oracle, variant, stores, lskip, dtype):
     """Every selectable launch shape of the 8^3 star-stencil kernel gives the oracle's
@@ -304,11 +306,12 @@ def test_grouped_stencil_variants_agree_on_a_large_shell(G, oracle):
     u0 = rng.uniform(0.3, 1.0, len(pts)).astype(np.float32)
     v0 = rng.uniform(0.0, 0.4, len(pts)).astype(np.float32)
 
-    def run(cls, variant=None, stores=None):
+    def run(cls, variant=None, stores=None, lskip=0):
         g = cls(3, (176, 176, 176), [np.float32] * 4, block_edge=8)
         if variant is not None:
             g.set_tuning(C.TUNE_STENCIL_VARIANT, variant)
             g.set_tuning(C.TUNE_STENCIL_STORES, stores)
+            g.set_tuning(C.TUNE_STENCIL_SKIP_EMPTY_ROWS, lskip)
         g.insert_coords(pts, [u0, v0, np.zeros_like(u0), np.zeros_like(u0)])
         g.flush(["smax"] * 4)
         for _ in range(3):
@@ -322,7 +325,50 @@ def test_grouped_stencil_variants_agree_on_a_large_shell(G, oracle):
             g = run(G, variant, stores)
             S.assert_same_state(g, ref)
             g.close()
+    for variant in (3, 4, 6):
+        for lskip in (2, 3):
+            g = run(G, variant, 3, lskip)
+            S.assert_same_state(g, ref)
+            g.close()
     ref.close()
+
+
+@pytest.mark.parametrize("lskip", [2, 3])
+def test_occupancy_table_follows_mask_changes(G, oracle, lskip):
+    """The row-occupancy table of the skipping loads is rebuilt after every change of the
+    masks: points removed (whole rows emptied) and points added by a flush; each following
+    Gray-Scott step equals the oracle's, with non-zero backgrounds in the skipped rows."""
+    rng = np.random.default_rng(77)
+    pts = W.shell_points(64, 14.0, 26.0)
+    u0 = rng.uniform(0.3, 1.0, len(pts)).astype(np.float32)
+    v0 = rng.uniform(0.0, 0.4, len(pts)).astype(np.float32)
+    gs = []
+    for cls in (G, oracle.SparseGridCpu):
+        g = cls(3, (64, 64, 64), [np.float32] * 4, block_edge=8)
+        g.set_background_value(0, 0.25)
+        g.set_background_value(1, -0.5)
+        if cls is G:
+            g.set_tuning(C.TUNE_STENCIL_VARIANT, 3)
+            g.set_tuning(C.TUNE_STENCIL_STORES, 3)
+            g.set_tuning(C.TUNE_STENCIL_SKIP_EMPTY_ROWS, lskip)
+        g.insert_coords(pts, [u0, v0, np.zeros_like(u0), np.zeros_like(u0)])
+        g.flush(["smax"] * 4)
+        g.conv2_gray_scott(0, 1, 2, 3, **W.GS_PARAMS)
+        g.conv2_gray_scott(2, 3, 0, 1, **W.GS_PARAMS)
+        # empty whole x-rows: every point with y % 3 == 0 and z % 2 == 0
+        sel = pts[(pts[:, 1] % 3 == 0) & (pts[:, 2] % 2 == 0)]
+        g.remove_points(g.lin_id(sel))
+        g.conv2_gray_scott(0, 1, 2, 3, **W.GS_PARAMS)
+        g.conv2_gray_scott(2, 3, 0, 1, **W.GS_PARAMS)
+        # refill some of them and add a new chunk
+        extra = np.concatenate([sel[::2], np.array([[2, 2, 2], [3, 2, 2]], np.int32)])
+        g.insert_coords(extra, [np.full(len(extra), 0.75, np.float32), np.full(len(extra), 0.125, np.float32),
+                                np.zeros(len(extra), np.float32), np.zeros(len(extra), np.float32)])
+        g.flush(["sright"] * 4)
+        g.conv2_gray_scott(0, 1, 2, 3, **W.GS_PARAMS)
+        g.conv2_gray_scott(2, 3, 0, 1, **W.GS_PARAMS)
+        gs.append(g)
+    S.assert_same_state(gs[0], gs[1])
 
 
 def test_blobs1024_full_flush_tag_box(G, oracle):
@@ -586,8 +632,8 @@ def test_to_kernel_view(G):
 
 def test_flush_rejects_device_coords_outside_grid(G):
     """Coordinates handed over in device memory are checked by the flush: one outside
-    the grid fails it (SgError) and leaves the grid unchanged; host coordinates fail at
-    insertion."""
+    the grid fails it (SgError), the grid is unchanged and the valid records stay pending;
+    host coordinates fail at insertion."""
     import torch
     from openfpm_data_amd._capi import SgError
     g = G(3, (32, 32, 32), [np.float32], block_edge=8)
@@ -598,17 +644,20 @@ def test_flush_rejects_device_coords_outside_grid(G):
     g.insert_coords(coords, [torch.ones(3, dtype=torch.float32, device="cuda")])
     with pytest.raises(SgError):
         g.flush(["smax"])
-    assert g.pending_inserts() == 0
+    assert g.pending_inserts() == 2
     for a, b in zip(before, (g.chunk_ids(), g.masks(), g.prop(0))):
         np.testing.assert_array_equal(a, b)
     with pytest.raises(SgError):
         g.insert_coords([[3, 40, 2]], [np.array([1.0], np.float32)])
+    g.flush(["smax"])
+    assert g.pending_inserts() == 0 and g.count_existing_elements() == 3
     g.close()
 
 
 def test_flush_rejects_device_keys_outside_grid(G):
     """Point keys handed over in device memory are checked by the flush: a key beyond the
-    grid fails it (SgError), drops the pending records and leaves the grid unchanged."""
+    grid fails it (SgError) and is dropped, the grid is unchanged, the other records stay
+    pending and the next flush merges them."""
     import torch
     from openfpm_data_amd._capi import SgError
     g = G(3, (32, 32, 32), [np.float32], block_edge=8)
@@ -620,9 +669,11 @@ def test_flush_rejects_device_keys_outside_grid(G):
     g.insert(keys, [vals])
     with pytest.raises(SgError):
         g.flush(["smax"])
-    assert g.pending_inserts() == 0
+    assert g.pending_inserts() == 2
     for a, b in zip(before, (g.chunk_ids(), g.masks(), g.prop(0))):
         np.testing.assert_array_equal(a, b)
+    g.flush(["smax"])
+    assert g.count_existing_elements() == 3
     g.close()
 
 
@@ -651,3 +702,61 @@ def test_operations_on_an_empty_grid(G, oracle):
         assert h.num_chunks() == 0
         g.close()
         h.close()
+
+
+def test_device_inputs_are_checked(G):
+    """Device tensors handed to the inserts are checked for element type, contiguity and
+    element count before they reach the C ABI (a wrong one raises, nothing is inserted)."""
+    import torch
+    g = G(3, (32, 32, 32), [np.float32, np.int32], block_edge=8)
+    c = torch.tensor([[1, 2, 3], [4, 5, 6]], dtype=torch.int32, device="cuda")
+    v = [torch.ones(2, device="cuda"), torch.ones(2, dtype=torch.int32, device="cuda")]
+    bad = [
+        (c.to(torch.int64), v, None),                      # int64 coordinates
+        (c.t().contiguous().t(), v, None),                 # strided coordinates
+        (c, [v[0].double(), v[1]], None),                  # float64 values for a float32 property
+        (c, [v[0], v[1].float()], None),                   # float values for an int32 property
+        (c, [torch.ones(3, device="cuda"), v[1]], None),   # wrong count
+        (c, v, torch.ones(2, dtype=torch.int32, device="cuda")),  # int32 masks
+    ]
+    for coords, vals, masks in bad:
+        with pytest.raises(ValueError):
+            g.insert_coords(coords, vals, masks=masks)
+    assert g.pending_inserts() == 0
+    g.insert_coords(c, v)
+    g.flush(["smax", "smax"])
+    assert g.count_existing_elements() == 2
+    g.close()
+
+
+def test_device_coordinates_outside_the_grid(G, oracle):
+    """Device coordinates outside the grid are found by the next flush: it fails, drops only
+    those records and keeps the others pending; the next flush merges them as the oracle
+    merges the valid records alone."""
+    import torch
+    from openfpm_data_amd._capi import SgError
+    rng = np.random.default_rng(81)
+    good = rng.integers(0, 40, size=(500, 3)).astype(np.int32)
+    bad = np.array([[40, 0, 0], [-1, 3, 3], [5, 99, 5]], np.int32)
+    coords = np.concatenate([good[:200], bad[:2], good[200:], bad[2:]])
+    vals = rng.uniform(-1, 1, len(coords)).astype(np.float32)
+    g = G(3, (40, 40, 40), [np.float32], block_edge=8)
+    g.insert_coords(torch.from_numpy(coords).cuda(), [torch.from_numpy(vals).cuda()])
+    with pytest.raises(SgError, match="3 inserted points outside the grid"):
+        g.flush(["sright"])
+    assert g.pending_inserts() == 500 and g.num_chunks() == 0
+    g.flush(["sright"])
+    keep = np.all((coords >= 0) & (coords < 40), axis=1)
+    o = oracle.SparseGridCpu(3, (40, 40, 40), [np.float32], block_edge=8)
+    o.insert_coords(coords[keep], [vals[keep]])
+    o.flush(["sright"])
+    S.assert_same_state(g, o)
+    g.close()
+    o.close()
+
+
+def test_flush_known_answers_of_the_reference(G):
+    """BlockMapGpu_tests.cu:126-264 and BlockMapGpu_kernels_tests.cu:16-492 (segmented fold
+    with old data, merge maps, conflict resolution), end to end through insert + flush on
+    the GPU, against the transcribed answers."""
+    S.check_flush_known_answers(G)

@@ -436,3 +436,9 @@ def test_zmb_grid_equals_smb_grid(oracle, dim, edge):
         np.testing.assert_array_equal(g.inv_lin_id(keys), coords)
     for a, b in zip(dense[0], dense[1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_flush_known_answers_of_the_reference(oracle):
+    """BlockMapGpu_tests.cu:126-264 and BlockMapGpu_kernels_tests.cu:16-492 (segmented fold
+    with old data, merge maps, conflict resolution), end to end through insert + flush."""
+    S.check_flush_known_answers(oracle.SparseGridCpu)
