@@ -385,6 +385,21 @@ int sg_pack_size(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, int
 /* pack + packFinalize: write the packed boxes (in box order) to buf (cap bytes) */
 int sg_pack(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, int nprop,
             const int32_t *props, int index_bytes, void *buf, int64_t cap, int mem);
+/* sg_pack into device memory without allocating.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (loops replayed under hipGraph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The count and scan temporaries live in
+ * one static device array of SG_PACK_NOALLOC_SCRATCH_BYTES per GPU (held until the process
+ * ends) instead of memory the grid allocates; no hipMalloc / hipFree is called.  Limits:
+ * nbox <= SG_PACK_NOALLOC_MAX_BOXES and 16 * nbox * (chunks + 1) bytes of rows plus the
+ * corners and the scan workspace within the scratch (SG_E_RANGE otherwise, nothing done).
+ * dev_buf is device memory of cap bytes; the bytes written equal sg_pack's with
+ * SG_MEM_DEVICE, *written = their count; returns when the pack is complete.  Every call on
+ * one GPU shares that GPU's scratch: calls that overlap on one GPU (several host threads,
+ * or grids on several streams) race. */
+#define SG_PACK_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define SG_PACK_NOALLOC_MAX_BOXES 64
+int sg_pack_noalloc(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, int nprop,
+                    const int32_t *props, int index_bytes, void *dev_buf, int64_t cap, int64_t *written);
 /* unpack (SparseGridGpu.hpp:3665-3733): read the packed box at buf; its points land at
  * (packed coordinate - box_start + dst_lo) and are queued for
  * sg_remove_add_unpack_finalize; *consumed = bytes of the section (the next box starts

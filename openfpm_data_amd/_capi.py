@@ -45,6 +45,9 @@ TUNE_TAG_CHUNKS_PER_WAVE = 7
 TUNE_BATCH_WAVES = 5
 TUNE_BATCH_MAX_WGS = 6
 COPY_NONTEMPORAL = 1
+# sg_pack_noalloc limits (static per-GPU scratch)
+PACK_NOALLOC_SCRATCH_BYTES = 32 << 20
+PACK_NOALLOC_MAX_BOXES = 64
 LINK_DOWN = 0
 LINK_UP = 1
 
@@ -177,6 +180,7 @@ _SIGS = {
     "sg_count_boundary": (_INT, [_P, ctypes.POINTER(_I64)]),
     "sg_pack_size": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),
     "sg_pack": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, _P, _I64, _INT]),
+    "sg_pack_noalloc": (_INT, [_P, _INT, _P, _P, _INT, _P, _INT, _P, _I64, ctypes.POINTER(_I64)]),
     "sg_unpack": (_INT, [_P, _P, _I64, _INT, _P, _INT, _P, _INT, ctypes.POINTER(_I64)]),
     "sg_remove_box": (_INT, [_P, _P, _P]),
     "sg_remove_points": (_INT, [_P, _I64, _P, _INT]),
@@ -237,8 +241,13 @@ class SgError(RuntimeError):
     pass
 
 
+def last_error():
+    """The calling thread's last error message of the library."""
+    msg = load().sg_last_error()
+    return msg.decode() if msg else ""
+
+
 def check(rc):
     if rc != SG_OK:
-        msg = load().sg_last_error()
-        raise SgError("libsg_hip error %d: %s" % (rc, msg.decode() if msg else ""))
+        raise SgError("libsg_hip error %d: %s" % (rc, last_error()))
     return rc

@@ -24,6 +24,7 @@
 #include "sg_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace sg {
 namespace {
@@ -412,22 +413,35 @@ int plan_pack(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, PackPl
     return SG_OK;
 }
 
-// header of one packed section (chunk count, box start, grid size) and the scan's last
-// entry (n_pnt), staged on the host for their copies to the device
-constexpr int kHdrMax = 8 + 2 * 3 * 4;
+// header of one packed section: chunk count, box start and grid size, and the scan's last
+// entry (n_pnt); passed by value to the kernel that writes them, so nothing is staged on
+// the host
 struct SectionHead {
-    unsigned char b[kHdrMax];
+    uint64_t nc;
     uint32_t np;
+    int dim;
+    int32_t lo[3], sz[3];
 };
 
+__global__ void pack_header_kernel(unsigned char *s0, uint32_t *scan_last, SectionHead h)
+{
+    if (threadIdx.x != 0) return;
+    // sections start at multiples of 4 bytes only: byte copies
+    copy_scalar(s0, (const unsigned char *)&h.nc, 8);
+    for (int d = 0; d < h.dim; ++d) {
+        copy_scalar(s0 + 8 + 4 * d, (const unsigned char *)&h.lo[d], 4);
+        copy_scalar(s0 + 8 + 4 * (h.dim + d), (const unsigned char *)&h.sz[d], 4);
+    }
+    copy_scalar((unsigned char *)scan_last, (const unsigned char *)&h.np, 4);
+}
+
 // write the packed sections of the boxes to the device buffer buf (cap bytes) from a
-// plan: per box k, rows k * stride of cnt / pscan / cscan and corners d_lo / d_hi + k * dim;
-// head[] (nbox entries, host) stages the headers and must live until the call returns
-// (it waits for the stream before returning)
+// plan: per box k, rows k * stride of cnt / pscan / cscan and corners d_lo / d_hi + k * dim
+// (device), lo + k * dim (host); queued on the grid's stream, allocates nothing
 int pack_sections(sg_grid *g, int nbox, const int32_t *lo, int nprop, const int32_t *props, int A,
                   unsigned char *buf, int64_t cap, int64_t stride, const uint32_t *cnt, const uint32_t *pscan,
                   const uint32_t *cscan, const int32_t *d_lo, const int32_t *d_hi, const int64_t *n_cnk,
-                  const int64_t *n_pnt, SectionHead *head)
+                  const int64_t *n_pnt)
 {
     const int64_t psz = point_bytes(g, nprop, props);
     int64_t at = 0;
@@ -439,25 +453,23 @@ int pack_sections(sg_grid *g, int nbox, const int32_t *lo, int nprop, const int3
         if (at + sb > cap) return fail(SG_E_RANGE, "pack buffer too small");
         unsigned char *s0 = buf + at;
         SG_HIP(hipMemsetAsync(s0, 0, (size_t)sb, g->stream));
-        // header: chunk count, box start, grid size
-        SectionHead &h = head[k];
-        const uint64_t ncu = (uint64_t)nc;
-        memcpy(h.b, &ncu, 8);
-        for (int d = 0; d < g->dim; ++d) {
-            const int32_t s = (int32_t)g->sz[d];
-            memcpy(h.b + 8 + 4 * d, lo + k * g->dim + d, 4);
-            memcpy(h.b + 8 + 4 * (g->dim + d), &s, 4);
-        }
-        h.np = (uint32_t)np;
-        SG_HIP(hipMemcpyAsync(s0, h.b, (size_t)hsz, hipMemcpyHostToDevice, g->stream));
         int64_t q = hsz;
         PackOut O;
         O.A = A;
         O.ids = s0 + q;
         q += A * nc;
         O.scan = (uint32_t *)(s0 + q);
-        SG_HIP(hipMemcpyAsync(O.scan + nc, &h.np, 4, hipMemcpyHostToDevice, g->stream));
         q += align_up((nc + 1) * 4, A);
+        SectionHead h{};
+        h.nc = (uint64_t)nc;
+        h.np = (uint32_t)np;
+        h.dim = g->dim;
+        for (int d = 0; d < g->dim; ++d) {
+            h.lo[d] = lo[k * g->dim + d];
+            h.sz[d] = (int32_t)g->sz[d];
+        }
+        hipLaunchKernelGGL(pack_header_kernel, dim3(1), dim3(64), 0, g->stream, s0, O.scan + nc, h);
+        SG_HIP(hipGetLastError());
         O.nprop = nprop;
         int64_t dq = q;
         for (int pi = 0; pi < nprop; ++pi) {
@@ -478,17 +490,57 @@ int pack_sections(sg_grid *g, int nbox, const int32_t *lo, int nprop, const int3
         }
         at += sb;
     }
-    // the host header sources must outlive their copies
+    return SG_OK;
+}
+
+// the packed sections from a plan, complete when the call returns
+int pack_into_device(sg_grid *g, int nbox, const int32_t *lo, int nprop, const int32_t *props, int A,
+                     unsigned char *buf, int64_t cap, const PackPlan &pl)
+{
+    int rc = pack_sections(g, nbox, lo, nprop, props, A, buf, cap, pl.stride, pl.cnt, pl.pscan, pl.cscan, pl.d_lo,
+                           pl.d_hi, pl.n_cnk.data(), pl.n_pnt.data());
+    if (rc) return rc;
     SG_HIP(hipStreamSynchronize(g->stream));
     return SG_OK;
 }
 
-int pack_into_device(sg_grid *g, int nbox, const int32_t *lo, int nprop, const int32_t *props, int A,
-                     unsigned char *buf, int64_t cap, const PackPlan &pl)
+// ---- pack without device allocation (sg_pack_noalloc) -----------------------------
+// The count / scan temporaries live in one static device array per GPU.  Host code never
+// uses the array's name as an address: its device address comes from hipGetSymbolAddress,
+// taken once per device with that device current, and only that address is passed on.
+static __device__ __align__(256) unsigned char scratch[SG_PACK_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device
+int noalloc_scratch(void **out)
 {
-    std::vector<SectionHead> head((size_t)std::max(nbox, 1));
-    return pack_sections(g, nbox, lo, nprop, props, A, buf, cap, pl.stride, pl.cnt, pl.pscan, pl.cscan, pl.d_lo,
-                         pl.d_hi, pl.n_cnk.data(), pl.n_pnt.data(), head.data());
+    int dev = -1;
+    SG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(SG_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SG_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SG_OK;
+}
+
+constexpr size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// scratch bytes of a pack of nbox boxes over n chunks: the box corners, the four count /
+// scan rows of every box, the per-box totals and the scan workspace
+size_t noalloc_pack_bytes(int nbox, int dim, int64_t n)
+{
+    const size_t stride = (size_t)n + 1;
+    return round256((size_t)2 * nbox * dim * 4) + round256((size_t)4 * nbox * stride * 4) +
+           round256((size_t)2 * nbox * 4) + round256(scan_ws_words((int64_t)stride) * 4);
 }
 
 }  // namespace
@@ -535,6 +587,76 @@ int sg_pack(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, int npro
         return rc;
     SG_HIP(hipMemcpyAsync(buf, st.p, (size_t)tot, hipMemcpyDeviceToHost, g->stream));
     SG_HIP(hipStreamSynchronize(g->stream));
+    return SG_OK;
+}
+
+int sg_pack_noalloc(sg_grid *g, int nbox, const int32_t *lo, const int32_t *hi, int nprop, const int32_t *props,
+                    int index_bytes, void *dev_buf, int64_t cap, int64_t *written)
+{
+    if (!g || !written || (!dev_buf && cap > 0)) return fail(SG_E_ARG, "null argument");
+    int rc = check_pack_args(g, nbox, lo, hi, nprop, props, index_bytes);
+    if (rc) return rc;
+    if (nbox > SG_PACK_NOALLOC_MAX_BOXES)
+        return fail(SG_E_RANGE, "sg_pack_noalloc: " + std::to_string(nbox) + " boxes, at most " +
+                                    std::to_string(SG_PACK_NOALLOC_MAX_BOXES));
+    const int64_t n = g->nchunks, stride = n + 1;
+    const size_t need = noalloc_pack_bytes(nbox, g->dim, n);
+    if (need > (size_t)SG_PACK_NOALLOC_SCRATCH_BYTES)
+        return fail(SG_E_RANGE, "sg_pack_noalloc: " + std::to_string(nbox) + " boxes over " + std::to_string(n) +
+                                    " chunks need " + std::to_string(need) + " scratch bytes, more than the " +
+                                    std::to_string((size_t)SG_PACK_NOALLOC_SCRATCH_BYTES) + " reserved");
+    *written = 0;
+    if (nbox == 0) return SG_OK;
+    void *base = nullptr;
+    if ((rc = noalloc_scratch(&base))) return rc;
+    // carve the scratch: corners | cnt, flag, pscan, cscan rows | totals | scan workspace
+    unsigned char *p = (unsigned char *)base;
+    const size_t bb = (size_t)nbox * g->dim * 4, per = (size_t)nbox * stride;
+    int32_t *d_lo = (int32_t *)p, *d_hi = d_lo + (size_t)nbox * g->dim;
+    p += round256(2 * bb);
+    uint32_t *cnt = (uint32_t *)p, *flag = cnt + per, *pscan = flag + per, *cscan = pscan + per;
+    p += round256(4 * per * 4);
+    uint32_t *d_tot = (uint32_t *)p;
+    p += round256((size_t)2 * nbox * 4);
+    uint32_t *ws = (uint32_t *)p;
+    const size_t ws_words = scan_ws_words(stride);
+
+    SG_HIP(hipMemcpyAsync(d_lo, lo, bb, hipMemcpyHostToDevice, g->stream));
+    SG_HIP(hipMemcpyAsync(d_hi, hi, bb, hipMemcpyHostToDevice, g->stream));
+    // the scratch keeps the previous call's contents: the rows' trailing entries (which the
+    // count pass does not write) must read 0
+    SG_HIP(hipMemsetAsync(cnt, 0, per * 2 * sizeof(uint32_t), g->stream));
+    if (n > 0) {
+        Boxes B{nbox, d_lo, d_hi};
+        hipLaunchKernelGGL(pack_count_kernel, dim3((unsigned)n), dim3(kWG), 0, g->stream, g->d_keys, n, g->d_mask,
+                           geom_of(g), B, cnt, flag, stride);
+        SG_HIP(hipGetLastError());
+    }
+    for (int k = 0; k < nbox; ++k) {
+        if ((rc = exclusive_scan_u32_ws(g->stream, cnt + k * stride, pscan + k * stride, stride, d_tot + 2 * k, ws,
+                                        ws_words)))
+            return rc;
+        if ((rc = exclusive_scan_u32_ws(g->stream, flag + k * stride, cscan + k * stride, stride, d_tot + 2 * k + 1,
+                                        ws, ws_words)))
+            return rc;
+    }
+    uint32_t h_tot[2 * SG_PACK_NOALLOC_MAX_BOXES];
+    SG_HIP(hipMemcpyAsync(h_tot, d_tot, (size_t)2 * nbox * 4, hipMemcpyDeviceToHost, g->stream));
+    SG_HIP(hipStreamSynchronize(g->stream));
+    int64_t n_pnt[SG_PACK_NOALLOC_MAX_BOXES], n_cnk[SG_PACK_NOALLOC_MAX_BOXES];
+    const int64_t psz = point_bytes(g, nprop, props);
+    int64_t tot = 0;
+    for (int k = 0; k < nbox; ++k) {
+        n_pnt[k] = h_tot[2 * k];
+        n_cnk[k] = h_tot[2 * k + 1];
+        tot += section_bytes(g, n_cnk[k], n_pnt[k], psz, index_bytes);
+    }
+    if (tot > cap) return fail(SG_E_RANGE, "pack buffer too small: " + std::to_string(tot) + " bytes needed");
+    if ((rc = pack_sections(g, nbox, lo, nprop, props, index_bytes, (unsigned char *)dev_buf, cap, stride, cnt, pscan,
+                            cscan, d_lo, d_hi, n_cnk, n_pnt)))
+        return rc;
+    SG_HIP(hipStreamSynchronize(g->stream));
+    *written = tot;
     return SG_OK;
 }
 
