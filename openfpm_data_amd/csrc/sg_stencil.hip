@@ -1007,23 +1007,40 @@ __device__ inline void issue_ghosts(PropIn<T> &r, const T *__restrict__ arr, con
     if (load_xp) r.xp = arr[nbp[1] * 512 + lane * 8 + 0];
 }
 
-// the occupancy table row of a chunk (see sg_grid::d_occ_tab): star neighbour positions,
-// own row occupancy, face ghost-row occupancy, -x / +x neighbour row occupancy
+// the occupancy table row of a chunk (see sg_grid::d_occ_tab): -x / +x neighbour
+// positions, the +-y / +-z neighbour this lane's ghost half row comes from, own row
+// occupancy, face ghost-row occupancy, -x / +x neighbour row occupancy
 struct OccRow {
-    int64_t nbp[6];
+    int64_t nxm, nxp, face_nb;
     uint64_t own, xm, xp;
     uint32_t ghost;
 };
 
-__device__ inline void read_occ(OccRow &o, const int32_t *__restrict__ tab, int64_t c)
+// c is uniform across the wave: every entry goes through readfirstlane, and the lane's
+// face neighbour is selected among those values (a select among the fields or array
+// elements of a row held in memory is folded into an indexed load from a stack copy of
+// the row, whose wait stalls every load issued before it)
+__device__ inline void read_occ(OccRow &o, const int32_t *__restrict__ tab, int64_t c, int lane)
 {
     const int32_t *row = tab + c * 16;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) o.nbp[k] = row[k];
-    o.own = (uint64_t)(uint32_t)row[6] | ((uint64_t)(uint32_t)row[7] << 32);
-    o.ghost = (uint32_t)row[8];
-    o.xm = (uint64_t)(uint32_t)row[9] | ((uint64_t)(uint32_t)row[10] << 32);
-    o.xp = (uint64_t)(uint32_t)row[11] | ((uint64_t)(uint32_t)row[12] << 32);
+    const int32_t nxm = __builtin_amdgcn_readfirstlane(row[0]), nxp = __builtin_amdgcn_readfirstlane(row[1]);
+    const int32_t nym = __builtin_amdgcn_readfirstlane(row[2]), nyp = __builtin_amdgcn_readfirstlane(row[3]);
+    const int32_t nzm = __builtin_amdgcn_readfirstlane(row[4]), nzp = __builtin_amdgcn_readfirstlane(row[5]);
+    const uint32_t own0 = (uint32_t)__builtin_amdgcn_readfirstlane(row[6]);
+    const uint32_t own1 = (uint32_t)__builtin_amdgcn_readfirstlane(row[7]);
+    const uint32_t gh = (uint32_t)__builtin_amdgcn_readfirstlane(row[8]);
+    const uint32_t xm0 = (uint32_t)__builtin_amdgcn_readfirstlane(row[9]);
+    const uint32_t xm1 = (uint32_t)__builtin_amdgcn_readfirstlane(row[10]);
+    const uint32_t xp0 = (uint32_t)__builtin_amdgcn_readfirstlane(row[11]);
+    const uint32_t xp1 = (uint32_t)__builtin_amdgcn_readfirstlane(row[12]);
+    const int face = lane >> 4;  // 0: -y, 1: +y, 2: -z, 3: +z
+    o.nxm = nxm;
+    o.nxp = nxp;
+    o.face_nb = (face & 2) ? ((face & 1) ? nzp : nzm) : ((face & 1) ? nyp : nym);
+    o.own = (uint64_t)own0 | ((uint64_t)own1 << 32);
+    o.ghost = gh;
+    o.xm = (uint64_t)xm0 | ((uint64_t)xm1 << 32);
+    o.xp = (uint64_t)xp0 | ((uint64_t)xp1 << 32);
 }
 
 // as issue_ghosts, reading only source rows that hold a point (the others hold the
@@ -1032,16 +1049,14 @@ template <typename T>
 __device__ inline void issue_ghosts_occ(PropIn<T> &r, const T *__restrict__ arr, const OccRow &o, int lane, bool load_xm,
                                         bool load_xp, T bg)
 {
-    const int face = lane >> 4;
-    const int64_t nbc = (face == 0) ? o.nbp[2] : (face == 1) ? o.nbp[3] : (face == 2) ? o.nbp[4] : o.nbp[5];
     if ((o.ghost >> (lane >> 1)) & 1u) {
-        load_half(r.gh, arr + nbc * 512 + ghost_src_row(lane) * 8 + (lane & 1) * 4);
+        load_half(r.gh, arr + o.face_nb * 512 + ghost_src_row(lane) * 8 + (lane & 1) * 4);
     } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i) r.gh.v[i] = bg;
     }
-    if (load_xm) r.xm = ((o.xm >> lane) & 1u) ? arr[o.nbp[0] * 512 + lane * 8 + 7] : bg;
-    if (load_xp) r.xp = ((o.xp >> lane) & 1u) ? arr[o.nbp[1] * 512 + lane * 8 + 0] : bg;
+    if (load_xm) r.xm = ((o.xm >> lane) & 1u) ? arr[o.nxm * 512 + lane * 8 + 7] : bg;
+    if (load_xp) r.xp = ((o.xp >> lane) & 1u) ? arr[o.nxp * 512 + lane * 8 + 0] : bg;
 }
 
 // LDS tile per property: rows (z, y) with z, y in [-1, 8] -> row index (z+1)*10 + (y+1),
@@ -1152,7 +1167,7 @@ __device__ inline void issue_chunk(ChunkIn<T, KIND> &in, const int32_t *__restri
         if (sa.lskip == 3) issue_own<T, KIND>(in, mask, src0, src1, c, lane, sa);
         __builtin_amdgcn_sched_barrier(0);
         OccRow o;
-        read_occ(o, nn, c);
+        read_occ(o, nn, c, lane);
         if (sa.lskip == 2) issue_own_occ<T, KIND>(in, mask, src0, src1, c, lane, sa, o.own);
         issue_ghosts_occ<T>(in.u, src0, o, lane, true, true, (T)sa.bgs[0]);
         if constexpr (ChunkIn<T, KIND>::kTwo) issue_ghosts_occ<T>(in.v, src1, o, lane, true, true, (T)sa.bgs[1]);
@@ -1465,10 +1480,10 @@ __device__ inline void group_step(GroupLds<T, KIND, W, ONE> &L, int64_t grp, int
         if (sa.lskip == 3) issue_own<T, KIND>(in, mask, src0, src1, c, lane, sa);
         __builtin_amdgcn_sched_barrier(0);
         OccRow o;
-        read_occ(o, nn, c);
+        read_occ(o, nn, c, lane);
         if (sa.lskip == 2) issue_own_occ<T, KIND>(in, mask, src0, src1, c, lane, sa, o.own);
-        xm_grp = (w > 0) && o.nbp[0] == c - 1;
-        xp_grp = (w < W - 1) && (c + 1 < n) && o.nbp[1] == c + 1;
+        xm_grp = (w > 0) && o.nxm == c - 1;
+        xp_grp = (w < W - 1) && (c + 1 < n) && o.nxp == c + 1;
         issue_ghosts_occ<T>(in.u, src0, o, lane, !xm_grp, !xp_grp, (T)sa.bgs[0]);
         if constexpr (kTwo) issue_ghosts_occ<T>(in.v, src1, o, lane, !xm_grp, !xp_grp, (T)sa.bgs[1]);
         L.cols[w][0][0][lane] = in.u.own.v[0];
