@@ -282,11 +282,17 @@ int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, s
  * with the rows passed through LDS so that each store instruction writes 1 KB of the
  * chunk contiguously */
 #define SG_TUNE_STENCIL_STORES 3
-/* 8^3 star-stencil loads: 1 = read the mask first and skip rows without points */
+/* 8^3 star-stencil loads: 0 every row; 1 read the mask first and skip rows without points;
+ * 2 skip the own and ghost rows without points as the per-chunk occupancy table says
+ * (rebuilt after any mask change); 3 as 2 for the ghost rows only */
 #define SG_TUNE_STENCIL_SKIP_EMPTY_ROWS 4
 /* 8^3 tagBoundaries: chunks per wave, 1, 2 (default) or 4; a wave issues the loads of all
  * its chunks before tagging any */
 #define SG_TUNE_TAG_CHUNKS_PER_WAVE 7
+/* flush of point records: 0 (default) sorts (chunk id, record) pairs and folds each
+ * chunk's records in LDS where chunk ids fit 32 bits and a record 32 bytes; 1 always
+ * sorts point keys.  Both give the same result. */
+#define SG_TUNE_FLUSH_PATH 8
 int sg_set_tuning(sg_grid *g, int key, int64_t value);
 
 /* ---- queries ---------------------------------------------------------------- */

@@ -426,6 +426,11 @@ int sg_set_tuning(sg_grid *g, int key, int64_t value)
         g->fast_wmode = (int)value;
         return SG_OK;
     }
+    if (key == SG_TUNE_FLUSH_PATH) {
+        if (value != 0 && value != 1) return fail(SG_E_RANGE, "flush path must be 0 or 1");
+        g->flush_path = (int)value;
+        return SG_OK;
+    }
     if (key == SG_TUNE_STENCIL_SKIP_EMPTY_ROWS) {
         if (value < 0 || value > 3) return fail(SG_E_RANGE, "skip-empty-rows must be 0, 1, 2 or 3");
         g->fast_lskip = (int)value;

@@ -635,6 +635,310 @@ __global__ __launch_bounds__(256) void write_chunks_kernel(const int64_t *__rest
     }
 }
 
+// ---- point records grouped by chunk ------------------------------------------------
+// A record of the chunk-grouped flush, 16 * nv bytes: uint16 slot in the chunk, uint8
+// mask, then the value of every property at its natural alignment from byte 3 on.
+struct RecLayout {
+    int nv, bytes;
+    int off[kMaxProps];
+};
+
+__device__ inline void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// insert buffer -> (chunk id, record) pairs in insertion order; a key outside [0, limit)
+// sets *bad (the flush then drops those records before anything is merged)
+__global__ __launch_bounds__(kWG) void chunk_records_kernel(const int64_t *__restrict__ keys,
+                                                           const uint8_t *__restrict__ mask, int64_t n, int csz,
+                                                           int64_t limit, FoldProps F, RecLayout R,
+                                                           uint32_t *__restrict__ ck, unsigned char *__restrict__ rec,
+                                                           int *__restrict__ bad)
+{
+    // each thread assembles its record in LDS (small stores at the layout's offsets), then
+    // the block writes its records out as whole 16-byte vectors
+    __shared__ __attribute__((aligned(16))) unsigned char s_rec[kWG * 32];
+    const int64_t i0 = (int64_t)blockIdx.x * kWG;
+    const int64_t i = i0 + threadIdx.x;
+    if (i < n) {
+        const int64_t k = keys[i];
+        const bool ok = k >= 0 && k < limit;
+        if (!ok) *bad = 1;
+        const int64_t kk = ok ? k : 0;
+        ck[i] = (uint32_t)(kk / csz);
+        unsigned char *d = s_rec + threadIdx.x * R.bytes;
+        *(uint16_t *)d = (uint16_t)(kk % csz);
+        d[2] = mask[i];
+        for (int q = 0; q < F.np; ++q) {
+            const unsigned char *s = F.rval[q] + i * F.ts[q];
+            unsigned char *t = d + R.off[q];
+            switch (F.ts[q]) {
+            case 8: *(uint64_t *)t = *(const uint64_t *)s; break;
+            case 4: *(uint32_t *)t = *(const uint32_t *)s; break;
+            case 2: *(uint16_t *)t = *(const uint16_t *)s; break;
+            default: *t = *s; break;
+            }
+        }
+    }
+    __syncthreads();
+    const int64_t nv = ((n - i0 < kWG) ? (n - i0) : kWG) * R.nv;
+    for (int64_t v = threadIdx.x; v < nv; v += kWG) ((uint4 *)rec)[i0 * R.nv + v] = ((const uint4 *)s_rec)[v];
+}
+
+// LDS fold state of one chunk, per wave: seen[csz] (0xff where a record reached the
+// slot, else 0), am[csz] (OR of their masks), then per property csz values at voff[q]
+struct FoldState {
+    int bytes;  // per wave, a multiple of 16
+    int roff;   // staging of a batch of 64 records
+    int voff[kMaxProps];
+};
+
+// the records of one batch for one property, rank by rank (a rank's records have distinct
+// slots); first / ao come from the batch's mask pass.  The operator is a template
+// parameter, so the rounds run without dispatch.
+template <typename T, int OP>
+__device__ inline void fold_batch_rounds(unsigned char *vals, const unsigned char *bp, int off, int rank, int maxr,
+                                         bool first, uint8_t ao, uint8_t bm)
+{
+    T *a = (T *)vals + off;
+    const T b = (rank >= 0) ? *(const T *)bp : T();
+    const bool aE = ao & 1, bE = bm & 1;
+    for (int r = 0; r <= maxr; ++r) {
+        if (rank == r) {
+            if (first) {
+                *a = b;
+            } else if (OP == SG_OP_NONE) {
+                if (bE) *a = b;
+            } else if (aE && bE) {
+                *a = dev_reduce<T>(OP, *a, b);
+            } else if (bE) {
+                *a = b;
+            }
+        }
+        if (maxr > 0) wave_sync_lds();
+    }
+}
+
+template <typename T>
+__device__ inline void fold_batch_prop(int op, unsigned char *vals, const unsigned char *bp, int off, int rank,
+                                       int maxr, bool first, uint8_t ao, uint8_t bm)
+{
+    switch (op) {
+    case SG_OP_MAX: fold_batch_rounds<T, SG_OP_MAX>(vals, bp, off, rank, maxr, first, ao, bm); break;
+    case SG_OP_MIN: fold_batch_rounds<T, SG_OP_MIN>(vals, bp, off, rank, maxr, first, ao, bm); break;
+    case SG_OP_ADD: fold_batch_rounds<T, SG_OP_ADD>(vals, bp, off, rank, maxr, first, ao, bm); break;
+    case SG_OP_RIGHT: fold_batch_rounds<T, SG_OP_RIGHT>(vals, bp, off, rank, maxr, first, ao, bm); break;
+    case SG_OP_LEFT: fold_batch_rounds<T, SG_OP_LEFT>(vals, bp, off, rank, maxr, first, ao, bm); break;
+    case SG_OP_OR: fold_batch_rounds<T, SG_OP_OR>(vals, bp, off, rank, maxr, first, ao, bm); break;
+    default: fold_batch_rounds<T, SG_OP_NONE>(vals, bp, off, rank, maxr, first, ao, bm); break;
+    }
+    wave_sync_lds();
+}
+
+// one property of a touched chunk: old contents (OLD; the background for a new chunk)
+// combined with the folded records (fold_old) and written once at the final position
+template <typename T, bool OLD>
+__device__ inline void write_fold_prop(int op, T bgv, const unsigned char *__restrict__ vals,
+                                       const uint8_t *__restrict__ seen, const uint8_t *__restrict__ am,
+                                       const T *__restrict__ src, const uint8_t *__restrict__ smask, T *__restrict__ dst,
+                                       int csz, int lane)
+{
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    constexpr int PPV = 16 / (int)sizeof(T);
+    if ((csz % 16) == 0) {
+        // gfx9 counts stores in vmcnt: a load between stores makes its wait cover the stores
+        // issued before it.  So a new chunk's loop has no load at all, and an old chunk's
+        // loop loads a group of vectors (values and mask bytes) before storing any of them.
+        typedef typename std::conditional<PPV == 16, v4u,
+                typename std::conditional<PPV == 8, uint64_t,
+                typename std::conditional<PPV == 4, uint32_t, uint16_t>::type>::type>::type MaskW;
+        const int nvec = csz / PPV;
+        constexpr int G = 4;
+        for (int v0 = lane; v0 < nvec; v0 += 64 * G) {
+            union {
+                v4u q;
+                T t[PPV];
+            } u[G];
+            union {
+                MaskW w;
+                uint8_t b[PPV];
+            } om[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int v = v0 + 64 * g;
+                if (OLD && v < nvec) {
+                    u[g].q = ((const v4u *)src)[v];
+                    om[g].w = ((const MaskW *)smask)[v];
+                } else {
+#pragma unroll
+                    for (int x = 0; x < PPV; ++x) {
+                        u[g].t[x] = bgv;
+                        om[g].b[x] = 0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int v = v0 + 64 * g;
+                if (v >= nvec) break;
+                // the vector's seen bytes in one LDS read (most vectors receive no record)
+                union {
+                    MaskW w;
+                    uint8_t b[PPV];
+                } sw;
+                sw.w = ((const MaskW *)seen)[v];
+                bool any = false;
+#pragma unroll
+                for (int x = 0; x < PPV; ++x) any = any || sw.b[x] != 0;
+                if (any) {
+#pragma unroll
+                    for (int x = 0; x < PPV; ++x) {
+                        if (sw.b[x]) {
+                            const int s = v * PPV + x;
+                            T o = u[g].t[x];
+                            fold_old<T>(op, ((const T *)vals)[s], am[s], &o, om[g].b[x], bgv);
+                            u[g].t[x] = o;
+                        }
+                    }
+                }
+                __builtin_nontemporal_store(u[g].q, (v4u *)dst + v);
+            }
+        }
+        return;
+    }
+    for (int s = lane; s < csz; s += 64) {
+        T o = OLD ? src[s] : bgv;
+        if (seen[s]) fold_old<T>(op, ((const T *)vals)[s], am[s], &o, OLD ? smask[s] : (uint8_t)0, bgv);
+        dst[s] = o;
+    }
+}
+
+// One wave per touched chunk j.  Its records are a contiguous run of the sorted records,
+// in insertion order (the sort is stable).  64 at a time they are loaded with one
+// coalesced load (the next 64 are loaded while the current ones are folded) and staged in
+// LDS; each lane's record is applied to the LDS state of its slot, records of one batch
+// that share a slot in lane order (one rank per round: ranks come from ballots over the
+// slot bits).  Then every array of the chunk is combined with the chunk's old contents
+// and written once, contiguously.
+template <int NV>
+__global__ __launch_bounds__(256) void fold_chunk_records_kernel(const uint4 *__restrict__ rec,
+                                                                 const uint32_t *__restrict__ ustart,
+                                                                 const int64_t *__restrict__ cpos,
+                                                                 const int64_t *__restrict__ oldpos,
+                                                                 const uint32_t *__restrict__ isnew, int64_t nnew,
+                                                                 int csz, int obits, int waves, FoldState S,
+                                                                 FoldProps F, RecLayout R, FusedImg I,
+                                                                 uint8_t *__restrict__ omask)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_fold[];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= waves) return;  // waves are independent: no workgroup barrier
+    const int64_t j = (int64_t)blockIdx.x * waves + w;
+    if (j >= nnew) return;
+    unsigned char *st = smem_fold + (size_t)w * S.bytes;
+    uint8_t *seen = st, *am = st + csz;
+    uint4 *stage = (uint4 *)(st + S.roff);  // the batch's records, lane l at stage[l * NV]
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const uint32_t r0 = ustart[j], r1 = ustart[j + 1];
+    uint4 nxt[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) nxt[v] = (r0 + lane < r1) ? rec[(size_t)(r0 + lane) * NV + v] : make_uint4(0, 0, 0, 0);
+    {
+        const bool fresh = isnew[j] != 0;
+        const int64_t src = fresh ? 0 : oldpos[j], dst = cpos[j];
+        if (csz % 16 == 0) {
+            for (int v = lane; v < csz / 16; v += 64) ((uint4 *)seen)[v] = make_uint4(0, 0, 0, 0);
+        } else {
+            for (int s = lane; s < csz; s += 64) seen[s] = 0;
+        }
+        for (uint32_t b0 = r0; b0 < r1; b0 += 64) {
+            const uint32_t k = b0 + (uint32_t)lane;
+            const bool valid = k < r1;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) stage[lane * NV + v] = nxt[v];
+            // the next 64 records of the chunk, in flight while these are folded
+            const uint32_t kn = k + 64;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) nxt[v] = (kn < r1) ? rec[(size_t)kn * NV + v] : make_uint4(0, 0, 0, 0);
+            wave_sync_lds();
+            const unsigned char *rp = (const unsigned char *)(stage + lane * NV);
+            const int off = valid ? (int)*(const uint16_t *)rp : 0;
+            const uint8_t bm = rp[2];
+            uint64_t peers = __ballot(valid);
+            for (int b = 0; b < obits; ++b) {
+                const bool bit = (off >> b) & 1;
+                const uint64_t bal = __ballot(bit);
+                peers &= bit ? bal : ~bal;
+            }
+            const int rank = valid ? (int)__popcll(peers & lt) : -1;
+            int maxr = 0;
+            while (__ballot(rank > maxr) != 0) ++maxr;
+            // the mask pass, rank by rank: whether the record is its slot's first and the OR
+            // of the masks of the slot's earlier records
+            bool first = false;
+            uint8_t ao = 0;
+            for (int r = 0; r <= maxr; ++r) {
+                if (rank == r) {
+                    first = seen[off] == 0;
+                    ao = first ? (uint8_t)0 : am[off];
+                    am[off] = (uint8_t)(ao | bm);
+                    seen[off] = 0xff;
+                }
+                wave_sync_lds();
+            }
+            // the values, property by property (type and operator dispatched once per batch)
+            for (int q = 0; q < F.np; ++q) {
+                unsigned char *vals = st + S.voff[q];
+                const unsigned char *bp = rp + R.off[q];
+                const int op = F.op[q];
+                switch (F.dt[q]) {
+#define SG_CASE(CODE, T)                                                                                        \
+    case CODE: fold_batch_prop<T>(op, vals, bp, off, rank, maxr, first, ao, bm); break;
+                    SG_FOLD_TYPES(SG_CASE)
+#undef SG_CASE
+                }
+            }
+        }
+        const uint8_t *smask = fresh ? nullptr : I.smask + src * csz;
+        for (int q = 0; q < F.np; ++q) {
+            const size_t cb = (size_t)csz * F.ts[q];
+            const unsigned char *sp = fresh ? nullptr : I.src[q] + src * cb;
+            unsigned char *dp = F.out[q] + dst * cb;
+            switch (F.dt[q]) {
+#define SG_CASE(CODE, T)                                                                                        \
+    case CODE: {                                                                                                \
+        T bgv;                                                                                                  \
+        __builtin_memcpy(&bgv, &F.bg[q], sizeof(T));                                                            \
+        if (fresh)                                                                                              \
+            write_fold_prop<T, false>(F.op[q], bgv, st + S.voff[q], seen, am, nullptr, nullptr, (T *)dp, csz, lane); \
+        else                                                                                                    \
+            write_fold_prop<T, true>(F.op[q], bgv, st + S.voff[q], seen, am, (const T *)sp, smask, (T *)dp, csz,  \
+                                     lane);                                                                     \
+        break;                                                                                                  \
+    }
+                SG_FOLD_TYPES(SG_CASE)
+#undef SG_CASE
+            }
+        }
+        // the mask: old | OR of the records
+        uint8_t *dm = omask + dst * csz;
+        if (csz % 16 == 0) {
+            typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+            for (int v = lane; v < csz / 16; v += 64) {
+                v4u q = smask ? ((const v4u *)smask)[v] : (v4u){0u, 0u, 0u, 0u};
+                // seen bytes are 0 or 0xff: the OR of the records' masks where a record came
+                q |= ((const v4u *)am)[v] & ((const v4u *)seen)[v];
+                __builtin_nontemporal_store(q, (v4u *)dm + v);
+            }
+        } else {
+            for (int s = lane; s < csz; s += 64) dm[s] = (uint8_t)((smask ? smask[s] : 0) | (seen[s] ? am[s] : 0));
+        }
+    }
+}
+
 __global__ void uchunk_index_kernel(const uint32_t *hscan, const uint32_t *head, int64_t nu, uint32_t *uchunk)
 {
     int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1408,6 +1712,138 @@ int flush_chunks_impl(sg_grid *g, const int32_t *ops)
     return SG_OK;
 }
 
+// Flush of point records grouped by chunk: the records become (chunk id, record of 16 or
+// 32 bytes) pairs, stably sorted by chunk id with the records moving along (no gather
+// through a permutation), and one wave per touched chunk folds the chunk's run of
+// records in LDS and writes the chunk once.  The fold each point receives is the
+// point-key flush's: its records in insertion order, then the old value.  Taken when the
+// chunk ids fit 32 bits, a record fits 32 bytes and a chunk's fold state fits 64 KB of
+// LDS (*taken = false otherwise, nothing done).
+int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
+{
+    *taken = false;
+    const int64_t n = g->ins_n;
+    const int csz = g->csz;
+    const int64_t limit = key_limit(g);
+    const int64_t nblocks = limit / csz;
+    int cbits = 0;
+    while (cbits < 63 && (((int64_t)1 << cbits) < nblocks)) ++cbits;
+    if (cbits > 32 || csz > 65536 || n >= ((int64_t)1 << 31)) return SG_OK;
+    RecLayout R;
+    {
+        int at = 3;
+        for (int p = 0; p < g->np; ++p) {
+            at = (at + g->ts[p] - 1) / g->ts[p] * g->ts[p];
+            R.off[p] = at;
+            at += g->ts[p];
+        }
+        if (at > 32) return SG_OK;
+        R.nv = at <= 16 ? 1 : 2;
+        R.bytes = 16 * R.nv;
+    }
+    FoldState S;
+    {
+        size_t at = (size_t)2 * csz;
+        for (int p = 0; p < g->np; ++p) {
+            at = (at + 15) / 16 * 16;
+            S.voff[p] = (int)at;
+            at += (size_t)csz * g->ts[p];
+        }
+        at = (at + 15) / 16 * 16;
+        S.roff = (int)at;
+        at += (size_t)64 * R.bytes;
+        if (at > 65536) return SG_OK;
+        S.bytes = (int)at;
+    }
+    *taken = true;
+    int rc;
+    uint32_t *hw = host_words(g);
+    if (!hw) return fail(SG_E_OOM, "pinned host allocation failed");
+    if ((rc = ensure(g->pool, g->scratch[6], 64))) return rc;
+    int *bad = (int *)g->scratch[6].p;
+    SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
+
+    FoldProps F;
+    F.np = g->np;
+    for (int p = 0; p < g->np; ++p) {
+        F.dt[p] = g->dt[p];
+        F.ts[p] = g->ts[p];
+        F.op[p] = ops ? ops[p] : SG_OP_NONE;
+        F.rval[p] = (const unsigned char *)g->d_ins_val[p];
+        F.bg[p] = 0;
+        memcpy(&F.bg[p], g->bg[p], g->ts[p]);
+    }
+    // (chunk id, record) pairs and the sort's second buffers
+    DevBuf &b_ck = g->scratch[0], &b_ckt = g->scratch[2], &b_rec = g->scratch[7], &b_rect = g->scratch[13];
+    if ((rc = ensure(g->pool, b_ck, (size_t)n * 4 + 64)) || (rc = ensure(g->pool, b_ckt, (size_t)n * 4 + 64)) ||
+        (rc = ensure(g->pool, b_rec, (size_t)n * R.bytes + 64)) ||
+        (rc = ensure(g->pool, b_rect, (size_t)n * R.bytes + 64)))
+        return rc;
+    hipLaunchKernelGGL(chunk_records_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key,
+                       g->d_ins_mask, n, csz, limit, F, R, (uint32_t *)b_ck.p, (unsigned char *)b_rec.p, bad);
+    SG_HIP(hipGetLastError());
+    uint32_t *ck = nullptr;
+    uint4 *rec = nullptr;
+    if ((rc = radix_sort_records_u32(g, (uint32_t *)b_ck.p, (uint4 *)b_rec.p, (uint32_t *)b_ckt.p,
+                                     (uint4 *)b_rect.p, R.nv, n, cbits, &ck, &rec)))
+        return rc;
+    // the range flag, read with the scan total below
+    SG_HIP(hipMemcpyAsync(hw + 8, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+    // touched chunks: run heads of the sorted chunk ids + scan
+    DevBuf &b_h = g->scratch[4];
+    if ((rc = ensure(g->pool, b_h, (size_t)n * 8 + 64))) return rc;
+    uint32_t *head = (uint32_t *)b_h.p, *hscan = head + n;
+    hipLaunchKernelGGL(run_head_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                       (const uint32_t *)ck, head, n);
+    SG_HIP(hipGetLastError());
+    uint32_t nn32 = 0;
+    if ((rc = exclusive_scan_u32(g, head, hscan, n, &nn32))) return rc;
+    if (hw[8]) {
+        // the records outside the grid are dropped, the others stay pending (nothing is merged)
+        int64_t kept = 0;
+        if ((rc = drop_invalid_records(g, limit, &kept))) return rc;
+        return fail(SG_E_RANGE, std::to_string(n - kept) + " inserted points outside the grid were dropped; the other " +
+                                    std::to_string(kept) + " stay pending (flush again to merge them)");
+    }
+    const int64_t nnew = nn32;
+    // unique chunk ids and the first record of each (ustart[nnew] = n)
+    if ((rc = ensure(g->pool, g->scratch[12], (size_t)nnew * 8 + (size_t)(nnew + 1) * 4 + 64))) return rc;
+    int64_t *nkey = (int64_t *)g->scratch[12].p;
+    uint32_t *ustart = (uint32_t *)(nkey + nnew);
+    hipLaunchKernelGGL(unique_points_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
+                       (const uint32_t *)ck, head, hscan, n, nkey, ustart, nn32);
+    SG_HIP(hipGetLastError());
+
+    MergeDefer df;
+    int64_t *cpos = nullptr;
+    if ((rc = merge_chunk_set(g, nkey, nnew, &cpos, &df))) return rc;
+    FusedImg I;
+    for (int p = 0; p < g->np; ++p) {
+        I.src[p] = (const unsigned char *)df.src_prop[p];
+        F.out[p] = (unsigned char *)g->d_prop[p];
+    }
+    I.smask = df.src_mask;
+    int obits = 0;
+    while ((1 << obits) < csz) ++obits;
+    const int waves = std::max(1, std::min(4, 65536 / S.bytes));
+    // one wave per touched chunk (a wave's loop then runs once)
+    const int64_t groups = (nnew + waves - 1) / waves;
+    const dim3 fgrid((unsigned)groups), fblk(64 * waves);
+    if (R.nv == 1)
+        hipLaunchKernelGGL(fold_chunk_records_kernel<1>, fgrid, fblk, (size_t)waves * S.bytes, g->stream, rec, ustart,
+                           cpos, df.oldpos, df.isnew, nnew, csz, obits, waves, S, F, R, I, g->d_mask);
+    else
+        hipLaunchKernelGGL(fold_chunk_records_kernel<2>, fgrid, fblk, (size_t)waves * S.bytes, g->stream, rec, ustart,
+                           cpos, df.oldpos, df.isnew, nnew, csz, obits, waves, S, F, R, I, g->d_mask);
+    SG_HIP(hipGetLastError());
+    finish_merge(g, df);
+    g->ins_n = 0;
+    g->nn_valid[0] = g->nn_valid[1] = false;
+    g->occ_tab_valid = false;
+    g->ep_valid = false;
+    return SG_OK;
+}
+
 }  // namespace
 
 int flush_impl(sg_grid *g, const int32_t *ops)
@@ -1417,6 +1853,11 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     const int64_t n = g->ins_n;
     if (n == 0) return SG_OK;  // nothing inserted (map_vector_sparse.hpp:1199-1213)
     if (n >= ((int64_t)1 << 31)) return fail(SG_E_RANGE, "more than 2^31 pending inserts");
+    if (g->flush_path == 0) {
+        bool taken = false;
+        const int rc0 = flush_grouped_impl(g, ops, &taken);
+        if (taken || rc0) return rc0;
+    }
     const int csz = g->csz;
     const int64_t limit = key_limit(g);
     int rc;

@@ -130,6 +130,9 @@ struct sg_grid {
     int fast_wmode = 3;
     // 8^3 tagBoundaries: chunks per wave (1, 2 or 4), all loaded before any is tagged
     int tag_cpw = 2;
+    // flush of point records: 0 grouped by chunk where the records and chunk ids allow it,
+    // 1 always by point key
+    int flush_path = 0;
     // chunk records written by the caller's kernels (sg_insert_device_begin / _end):
     // [count (256 B)] [ids, cap + 1] [masks] [values per property], the last record a
     // sink for claims beyond the capacity
@@ -348,6 +351,10 @@ int radix_sort_pairs_u64(sg_grid *g, uint64_t *keys, uint32_t *vals, uint64_t *k
 int radix_sort_pairs_u32(sg_grid *g, uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp,
                          uint32_t *vals_tmp, int64_t n, int bits, uint32_t **keys_out,
                          uint32_t **vals_out);
+// the same for (key, record of nv 16-byte vectors, nv = 1 or 2): the records move with
+// their keys
+int radix_sort_records_u32(sg_grid *g, uint32_t *keys, uint4 *recs, uint32_t *keys_tmp, uint4 *recs_tmp, int nv,
+                           int64_t n, int bits, uint32_t **keys_out, uint4 **recs_out);
 
 inline unsigned grid_1d(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 

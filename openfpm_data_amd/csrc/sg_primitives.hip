@@ -8,9 +8,10 @@
 //   pass = histogram (LDS atomics) -> device scan of the digit-major count table
 //          -> scatter: a stable in-tile rank computed by wave64 ballot matching (for
 //             each item, the lanes holding the same digit are found with 8 ballots;
-//             rank = popcount of lower peer lanes + counts of earlier waves and earlier
-//             items of the tile) places the tile in LDS in digit order, which is then
-//             written out so that each digit's run is contiguous in global memory.
+//             rank = popcount of lower peer lanes + the wave's earlier items of the
+//             digit + the digit's items in earlier waves) places the tile in LDS in digit
+//             order, which is then written out so that each digit's run is contiguous in
+//             global memory.
 // Stability: items are ranked in index order inside a tile and tiles are ranked in
 // tile order by the scanned table, so equal keys keep their input order (as
 // CUB SortPairs does), which the flush relies on for its merge order.
@@ -171,7 +172,11 @@ __global__ __launch_bounds__(kSortWG) void radix_hist_kernel(const K *keys, uint
 
 // Scatter of one pass: the tile's items are first placed in LDS in (digit, index) order
 // (stable in-tile ranks from wave64 ballot matching), then written out in that order, so
-// each digit's run of the tile goes to consecutive global addresses.
+// each digit's run of the tile goes to consecutive global addresses.  Wave w owns the
+// tile's items [w * 64 * IPT, (w + 1) * 64 * IPT), item j of a lane at w * 64 * IPT +
+// j * 64 + lane, so index order is (wave, j, lane): a wave ranks its items among equal
+// digits by ballots and a running count per digit of its own (LDS, in wave order, no
+// barrier); after one barrier the waves' counts are scanned per digit.
 template <typename K>
 __global__ __launch_bounds__(kSortWG) void radix_scatter_kernel(const K *keys, const uint32_t *vals, K *okeys,
                                                                 uint32_t *ovals, const uint32_t *offsets,
@@ -182,24 +187,30 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_kernel(const K *keys, c
     __shared__ uint32_t s_val[kSortTile];
     __shared__ uint32_t s_gbase[256];  // global start of digit d for this tile
     __shared__ uint32_t s_lstart[256]; // start of digit d inside the tile
-    __shared__ uint32_t s_run[256];    // items of digit d placed so far
     __shared__ uint32_t s_wcnt[kSortWG / 64][256];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     s_gbase[t] = offsets[(int64_t)t * ntiles + blockIdx.x];
     uint32_t tot;
     s_lstart[t] = block_exclusive_scan(counts[(int64_t)t * ntiles + blockIdx.x], &tot);
-    s_run[t] = 0;
 #pragma unroll
     for (int q = 0; q < kSortWG / 64; ++q) s_wcnt[q][t] = 0;
-    __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    const int64_t base = (int64_t)blockIdx.x * kSortTile + (int64_t)w * 64 * kSortIPT;
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    // every item of the thread is loaded before the first is ranked
+    K kk[kSortIPT];
+    uint32_t vv[kSortIPT];
+#pragma unroll
     for (int j = 0; j < kSortIPT; ++j) {
-        const int64_t i = base + (int64_t)j * kSortWG + t;
-        const bool valid = i < n;
-        const K key = valid ? keys[i] : (K)0;
-        const uint32_t val = valid ? vals[i] : 0u;
-        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const int64_t i = base + (int64_t)j * 64 + lane;
+        kk[j] = (i < n) ? keys[i] : (K)0;
+        vv[j] = (i < n) ? vals[i] : 0u;
+    }
+    __syncthreads();  // s_wcnt cleared
+    uint32_t wr[kSortIPT];
+#pragma unroll
+    for (int j = 0; j < kSortIPT; ++j) {
+        const bool valid = base + (int64_t)j * 64 + lane < n;
+        const uint32_t d = (uint32_t)(kk[j] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -207,26 +218,34 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_kernel(const K *keys, c
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (valid && rank == 0) s_wcnt[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = s_lstart[d] + s_run[d] + rank;
-            for (int q = 0; q < w; ++q) pos += s_wcnt[q][d];
-            s_key[pos] = key;
-            s_val[pos] = val;
-        }
-        __syncthreads();
-        uint32_t sum = 0;
+        const uint32_t lr = (uint32_t)__popcll(peers & lt);
+        const uint32_t before = valid ? s_wcnt[w][d] : 0u;
+        wr[j] = before + lr;
+        if (valid && lr == 0) s_wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        uint32_t run = 0;
 #pragma unroll
         for (int q = 0; q < kSortWG / 64; ++q) {
-            sum += s_wcnt[q][t];
-            s_wcnt[q][t] = 0;
+            const uint32_t c = s_wcnt[q][t];
+            s_wcnt[q][t] = run;
+            run += c;
         }
-        s_run[t] += sum;
-        __syncthreads();
     }
-    const int m = (int)((n - base < kSortTile) ? (n - base) : kSortTile);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortIPT; ++j) {
+        if (base + (int64_t)j * 64 + lane < n) {
+            const uint32_t d = (uint32_t)(kk[j] >> shift) & 255u;
+            const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + wr[j];
+            s_key[pos] = kk[j];
+            s_val[pos] = vv[j];
+        }
+    }
+    __syncthreads();
+    const int64_t tbase = (int64_t)blockIdx.x * kSortTile;
+    const int m = (int)((n - tbase < kSortTile) ? (n - tbase) : kSortTile);
     for (int k = t; k < m; k += kSortWG) {
         const K key = s_key[k];
         const uint32_t d = (uint32_t)(key >> shift) & 255u;
@@ -269,7 +288,176 @@ int radix_sort_pairs(sg_grid *g, K *keys, uint32_t *vals, K *keys_tmp, uint32_t 
     return SG_OK;
 }
 
+// ---- stable LSD radix sort of (uint32 key, record of NV 16-byte vectors) ------------
+// The same passes with the records travelling with their keys, so that a sorted run is
+// read contiguously afterwards (no gather through a permutation).  Tiles of 2048 items
+// keep the LDS staging (keys + records) small enough for several workgroups per CU.
+constexpr int kRecIPT = 8;
+constexpr int kRecTile = kSortWG * kRecIPT;
+
+// the digit counts of kHistTiles consecutive tiles per workgroup (every key of the
+// workgroup loaded before the first is counted)
+constexpr int kHistTiles = 4;
+__global__ __launch_bounds__(kSortWG) void radix_hist_rec_kernel(const uint32_t *keys, uint32_t *counts, int64_t n,
+                                                                 int shift, int ntiles)
+{
+    __shared__ uint32_t s_hist[kHistTiles][256];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < kHistTiles; ++h) s_hist[h][t] = 0;
+    uint32_t d[kHistTiles][kRecIPT];
+#pragma unroll
+    for (int h = 0; h < kHistTiles; ++h) {
+        const int64_t base = ((int64_t)blockIdx.x * kHistTiles + h) * kRecTile;
+#pragma unroll
+        for (int j = 0; j < kRecIPT; ++j) {
+            const int64_t i = base + (int64_t)j * kSortWG + t;
+            d[h][j] = (i < n) ? (keys[i] >> shift) & 255u : 256u;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < kHistTiles; ++h)
+#pragma unroll
+        for (int j = 0; j < kRecIPT; ++j)
+            if (d[h][j] < 256u) atomicAdd(&s_hist[h][d[h][j]], 1u);
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < kHistTiles; ++h) {
+        const int64_t tile = (int64_t)blockIdx.x * kHistTiles + h;
+        if (tile < ntiles) counts[(int64_t)t * ntiles + tile] = s_hist[h][t];
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(kSortWG) void radix_scatter_rec_kernel(const uint32_t *__restrict__ keys,
+                                                                    const uint4 *__restrict__ recs,
+                                                                    uint32_t *__restrict__ okeys,
+                                                                    uint4 *__restrict__ orecs,
+                                                                    const uint32_t *__restrict__ offsets,
+                                                                    const uint32_t *__restrict__ counts, int64_t n,
+                                                                    int shift, int ntiles)
+{
+    __shared__ uint32_t s_key[kRecTile];
+    __shared__ uint4 s_rec[kRecTile * NV];
+    __shared__ uint32_t s_gbase[256];
+    __shared__ uint32_t s_lstart[256];
+    __shared__ uint32_t s_wcnt[kSortWG / 64][256];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    s_gbase[t] = offsets[(int64_t)t * ntiles + blockIdx.x];
+    uint32_t tot;
+    s_lstart[t] = block_exclusive_scan(counts[(int64_t)t * ntiles + blockIdx.x], &tot);
+#pragma unroll
+    for (int q = 0; q < kSortWG / 64; ++q) s_wcnt[q][t] = 0;
+    // Wave w owns the tile's items [w * 64 * IPT, (w + 1) * 64 * IPT), item j of a lane at
+    // w * 64 * IPT + j * 64 + lane, so index order is (wave, j, lane): a wave ranks its
+    // items among equal digits by ballots and a running count per digit of its own (LDS,
+    // in wave order, no barrier); after one barrier the waves' counts are scanned per digit.
+    const int64_t base = (int64_t)blockIdx.x * kRecTile + (int64_t)w * 64 * kRecIPT;
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    // every item of the thread is loaded before the first is ranked
+    uint32_t kk[kRecIPT];
+    uint4 rr[kRecIPT][NV];
+#pragma unroll
+    for (int j = 0; j < kRecIPT; ++j) {
+        const int64_t i = base + (int64_t)j * 64 + lane;
+        const bool valid = i < n;
+        kk[j] = valid ? keys[i] : 0u;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) rr[j][v] = valid ? recs[i * NV + v] : make_uint4(0, 0, 0, 0);
+    }
+    __syncthreads();  // s_wcnt cleared
+    uint32_t wr[kRecIPT];
+#pragma unroll
+    for (int j = 0; j < kRecIPT; ++j) {
+        const bool valid = base + (int64_t)j * 64 + lane < n;
+        const uint32_t d = (kk[j] >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t lr = (uint32_t)__popcll(peers & lt);
+        const uint32_t before = valid ? s_wcnt[w][d] : 0u;
+        wr[j] = before + lr;
+        if (valid && lr == 0) s_wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        // per digit: the exclusive prefix of the waves' counts
+        uint32_t run = 0;
+#pragma unroll
+        for (int q = 0; q < kSortWG / 64; ++q) {
+            const uint32_t c = s_wcnt[q][t];
+            s_wcnt[q][t] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRecIPT; ++j) {
+        if (base + (int64_t)j * 64 + lane < n) {
+            const uint32_t d = (kk[j] >> shift) & 255u;
+            const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + wr[j];
+            s_key[pos] = kk[j];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) s_rec[pos * NV + v] = rr[j][v];
+        }
+    }
+    __syncthreads();
+    const int64_t tbase = (int64_t)blockIdx.x * kRecTile;
+    const int m = (int)((n - tbase < kRecTile) ? (n - tbase) : kRecTile);
+    for (int k = t; k < m; k += kSortWG) {
+        const uint32_t key = s_key[k];
+        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t pos = s_gbase[d] + (uint32_t)k - s_lstart[d];
+        okeys[pos] = key;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) orecs[(int64_t)pos * NV + v] = s_rec[k * NV + v];
+    }
+}
+
 }  // namespace
+
+int radix_sort_records_u32(sg_grid *g, uint32_t *keys, uint4 *recs, uint32_t *keys_tmp, uint4 *recs_tmp, int nv,
+                           int64_t n, int bits, uint32_t **keys_out, uint4 **recs_out)
+{
+    *keys_out = keys;
+    *recs_out = recs;
+    if (n <= 1 || bits <= 0) return SG_OK;
+    if (nv != 1 && nv != 2) return fail(SG_E_ARG, "records of 1 or 2 16-byte vectors");
+    if (n >= (int64_t)1 << 31) return fail(SG_E_RANGE, "radix sort supports fewer than 2^31 items");
+    const int ntiles = (int)((n + kRecTile - 1) / kRecTile);
+    DevBuf &cbuf = g->scratch[10];
+    int rc = ensure(g->pool, cbuf, (size_t)256 * ntiles * sizeof(uint32_t) * 2);
+    if (rc) return rc;
+    uint32_t *counts = (uint32_t *)cbuf.p;
+    uint32_t *offs = counts + (size_t)256 * ntiles;
+    uint32_t *src_k = keys, *dst_k = keys_tmp;
+    uint4 *src_r = recs, *dst_r = recs_tmp;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(radix_hist_rec_kernel, dim3((ntiles + kHistTiles - 1) / kHistTiles), dim3(kSortWG), 0,
+                           g->stream, src_k, counts, n, shift,
+                           ntiles);
+        SG_HIP(hipGetLastError());
+        rc = exclusive_scan_u32(g, counts, offs, (int64_t)256 * ntiles, nullptr);
+        if (rc) return rc;
+        if (nv == 1)
+            hipLaunchKernelGGL(radix_scatter_rec_kernel<1>, dim3(ntiles), dim3(kSortWG), 0, g->stream, src_k, src_r,
+                               dst_k, dst_r, offs, counts, n, shift, ntiles);
+        else
+            hipLaunchKernelGGL(radix_scatter_rec_kernel<2>, dim3(ntiles), dim3(kSortWG), 0, g->stream, src_k, src_r,
+                               dst_k, dst_r, offs, counts, n, shift, ntiles);
+        SG_HIP(hipGetLastError());
+        std::swap(src_k, dst_k);
+        std::swap(src_r, dst_r);
+    }
+    *keys_out = src_k;
+    *recs_out = src_r;
+    return SG_OK;
+}
 
 // level sizes of a scan of n items: n, ceil(n/T), ceil(ceil(n/T)/T), ... until one tile
 static int scan_levels(int64_t n, int64_t (&sizes)[kScanMaxLevels])

@@ -94,8 +94,13 @@ def _rand_vals(rng, dt, n):
     return rng.integers(lo, hi, n).astype(dt)
 
 
-def run_flush_program(G, dim, edge, size, dtypes, steps, background=None):
+def run_flush_program(G, dim, edge, size, dtypes, steps, background=None, flush_path=None):
+    """Run a flush program on a new grid of class G; flush_path (library grids only)
+    selects the flush of point records (0 grouped by chunk, 1 by point key)."""
     g = G(dim, size, dtypes, block_edge=edge)
+    if flush_path is not None:
+        from openfpm_data_amd import _capi as C
+        g.set_tuning(C.TUNE_FLUSH_PATH, flush_path)
     if background is not None:
         for p, b in enumerate(background):
             g.set_background_value(p, b)

@@ -43,20 +43,53 @@ def test_insert_known_answer(G, golden, name):
     np.testing.assert_array_equal(g.get(0, keys), g.inv_lin_id(keys)[:, 0].astype(np.float32))
 
 
-@pytest.mark.parametrize("seed", range(8))
-def test_flush_random_programs(G, oracle, seed):
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("seed", range(10))
+def test_flush_random_programs(G, oracle, seed, path):
+    """Random insert / flush programs against the oracle and the dictionary model, with
+    the point records grouped by chunk (path 0: records of 16 bytes, of 32 bytes, and
+    beyond 32 bytes, where the point-key flush runs) and by point key (path 1)."""
     rng = np.random.default_rng(200 + seed)
     dim = 3 if seed % 2 else 2
     edge = [2, 4, 8][seed % 3]
     size = [int(rng.integers(edge, 6 * edge)) for _ in range(dim)]
     dtypes = [[np.float64], [np.int32, np.uint32], [np.float32, np.int64, np.uint8],
-              [np.uint16, np.int8, np.float64, np.uint64]][seed % 4]
+              [np.uint16, np.int8, np.float64, np.uint64], [np.float64, np.int64, np.float64, np.uint64],
+              ][seed % 5]
     steps = S.random_flush_program(seed, dim, edge, size, dtypes, nflush=3, nrec=150)
     bgs = [3, 1, 2, 5][: len(dtypes)]
-    a = S.run_flush_program(G, dim, edge, size, dtypes, steps, background=bgs)
+    a = S.run_flush_program(G, dim, edge, size, dtypes, steps, background=bgs, flush_path=path)
     b = S.run_flush_program(oracle.SparseGridCpu, dim, edge, size, dtypes, steps, background=bgs)
     S.assert_same_state(a, b)
     S.compare_to_model(a, S.model_flush(dim, edge, size, dtypes, steps, background=bgs), dtypes)
+
+
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("seed", range(4))
+def test_flush_crowded_chunks(G, oracle, seed, path):
+    """Thousands of point records in a few chunks, most of them on a handful of slots
+    (many batches of 64 records per chunk and many records per slot within a batch), with
+    order-sensitive folds (FP sadd of non-integer values, sright, sleft): bit-exact
+    against the oracle on both flush paths."""
+    rng = np.random.default_rng(900 + seed)
+    dim, edge = [(3, 4), (3, 8), (2, 16), (2, 2)][seed]
+    size = [4 * edge] * dim
+    cs = edge ** dim
+    dtypes = [np.float32, np.float64, np.int32]
+    steps = []
+    for f in range(2):
+        n = 5000
+        ch = rng.integers(0, 3, n)
+        off = np.where(rng.random(n) < 0.8, rng.integers(0, 4, n), rng.integers(0, cs, n))
+        keys = ch.astype(np.int64) * cs + off
+        masks = rng.choice(np.array([1, 1, 1, 3, 0, 2], dtype=np.uint8), n)
+        vals = [rng.uniform(-50, 50, n).astype(np.float32), rng.uniform(-50, 50, n),
+                rng.integers(-1000, 1000, n).astype(np.int32)]
+        steps.append(([("points", keys, masks, vals)], [["sadd", "sright", "sleft"], ["sadd", "sadd", "smax"]][f]))
+    a = S.run_flush_program(G, dim, edge, size, dtypes, steps, background=[1, 2, 3], flush_path=path)
+    b = S.run_flush_program(oracle.SparseGridCpu, dim, edge, size, dtypes, steps, background=[1, 2, 3])
+    S.assert_same_state(a, b)
+    S.compare_to_model(a, S.model_flush(dim, edge, size, dtypes, steps, background=[1, 2, 3]), dtypes)
 
 
 @pytest.mark.parametrize("order", ["chunks_only", "chunks_first"])

@@ -81,7 +81,9 @@ def test_served_results_match_alone(build_thread):
                      "tol": 1e-10 if i % 2 == 0 else 1e-6})
     biggest = max(S.resident_bytes(len(r["chunk_ids"])) + S.build_bytes(len(r["chunk_ids"])) for r in reqs)
     srv = S.PoissonServer(budget_bytes=3 * biggest, keep_solutions=True, build_thread=build_thread)
-    arrivals = np.arange(8) * 0.002
+    # four requests arrive together (their lifetimes overlap whatever the solve speed: the
+    # budget admits at least two), the others one by one while those are on the GPU
+    arrivals = np.concatenate([np.zeros(4), 0.002 * np.arange(1, 5)])
     recs, summ = srv.run(reqs, arrivals)
     srv.close()
     assert summ["served"] == 8 and summ["converged"] == 8
