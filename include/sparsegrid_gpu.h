@@ -284,7 +284,8 @@ int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, s
 #define SG_TUNE_STENCIL_STORES 3
 /* 8^3 star-stencil loads: 0 every row; 1 read the mask first and skip rows without points;
  * 2 skip the own and ghost rows without points as the per-chunk occupancy table says
- * (rebuilt after any mask change); 3 as 2 for the ghost rows only */
+ * (rebuilt after any mask change); 3 as 2 for the ghost rows only; 4 as 2 with each row's
+ * mask read from two 64-bit planes per chunk built with the table */
 #define SG_TUNE_STENCIL_SKIP_EMPTY_ROWS 4
 /* 8^3 tagBoundaries: chunks per wave, 1, 2 (default) or 4; a wave issues the loads of all
  * its chunks before tagging any */
@@ -293,6 +294,9 @@ int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, s
  * chunk's records in LDS where chunk ids fit 32 bits and a record 32 bytes; 1 always
  * sorts point keys.  Both give the same result. */
 #define SG_TUNE_FLUSH_PATH 8
+/* 8^3 tagBoundaries: 0 reads the neighbours' EXIST bits from per-chunk bit-planes (64 bytes
+ * per chunk, rebuilt after mask changes); 1 (default) from their mask rows.  Same result. */
+#define SG_TUNE_TAG_VARIANT 9
 int sg_set_tuning(sg_grid *g, int key, int64_t value);
 
 /* ---- queries ---------------------------------------------------------------- */

@@ -48,6 +48,9 @@ STENCIL_CANDIDATES = [
     {"variant": 4, "stores": 3, "lskip": 2},
     {"variant": 6, "stores": 3, "lskip": 2},
     {"variant": 7, "stores": 3, "lskip": 2},
+    {"variant": 3, "stores": 3, "lskip": 4},
+    {"variant": 6, "stores": 3, "lskip": 4},
+    {"variant": 7, "stores": 3, "lskip": 4},
 ]
 
 BATCH_CANDIDATES = [

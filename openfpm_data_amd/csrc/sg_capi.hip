@@ -426,13 +426,18 @@ int sg_set_tuning(sg_grid *g, int key, int64_t value)
         g->fast_wmode = (int)value;
         return SG_OK;
     }
+    if (key == SG_TUNE_TAG_VARIANT) {
+        if (value != 0 && value != 1) return fail(SG_E_RANGE, "tagBoundaries variant must be 0 or 1");
+        g->tag_variant = (int)value;
+        return SG_OK;
+    }
     if (key == SG_TUNE_FLUSH_PATH) {
         if (value != 0 && value != 1) return fail(SG_E_RANGE, "flush path must be 0 or 1");
         g->flush_path = (int)value;
         return SG_OK;
     }
     if (key == SG_TUNE_STENCIL_SKIP_EMPTY_ROWS) {
-        if (value < 0 || value > 3) return fail(SG_E_RANGE, "skip-empty-rows must be 0, 1, 2 or 3");
+        if (value < 0 || value > 4) return fail(SG_E_RANGE, "skip-empty-rows must be in [0, 4]");
         g->fast_lskip = (int)value;
         return SG_OK;
     }
@@ -523,6 +528,7 @@ int sg_device_ptr(sg_grid *g, int prop, void **out)
     if (prop == -1) {
         *out = g->d_mask;
         g->occ_tab_valid = false;  // the caller may write masks
+        g->ebits_valid = false;
     }
     else if (valid_prop(g, prop)) *out = g->d_prop[prop];
     else return fail(SG_E_ARG, "bad property");
@@ -551,6 +557,7 @@ int sg_to_kernel(sg_grid *g, int nn_type, sg_kernel_view *out)
     }
     memset(out, 0, sizeof(*out));
     g->occ_tab_valid = false;  // the caller's kernels may write masks
+    g->ebits_valid = false;
     out->keys = g->d_keys;
     out->mask = g->d_mask;
     for (int p = 0; p < g->np; ++p) out->prop[p] = g->d_prop[p];

@@ -1708,6 +1708,7 @@ int flush_chunks_impl(sg_grid *g, const int32_t *ops)
     g->cins_n = 0;
     g->nn_valid[0] = g->nn_valid[1] = false;
     g->occ_tab_valid = false;
+    g->ebits_valid = false;
     g->ep_valid = false;
     return SG_OK;
 }
@@ -1840,6 +1841,7 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
     g->ins_n = 0;
     g->nn_valid[0] = g->nn_valid[1] = false;
     g->occ_tab_valid = false;
+    g->ebits_valid = false;
     g->ep_valid = false;
     return SG_OK;
 }
@@ -2041,6 +2043,7 @@ int flush_impl(sg_grid *g, const int32_t *ops)
     // flush marks the neighbour table stale (SparseGridGpu.hpp:778), and the point list
     g->nn_valid[0] = g->nn_valid[1] = false;
     g->occ_tab_valid = false;
+    g->ebits_valid = false;
     g->ep_valid = false;
     return SG_OK;
 }
@@ -2136,6 +2139,7 @@ int set_chunks_impl(sg_grid *g, int64_t n, const int64_t *ids, const uint8_t *ma
     g->cap = ncap;
     g->nn_valid[0] = g->nn_valid[1] = false;
     g->occ_tab_valid = false;
+    g->ebits_valid = false;
     g->ep_valid = false;
     return SG_OK;
 }

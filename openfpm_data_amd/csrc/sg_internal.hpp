@@ -141,7 +141,7 @@ struct sg_grid {
     bool dins_open = false;
     // 8^3 star stencils: 0 load every row; 1 read the mask first and skip loading 8-point
     // rows without points; 2 / 3 skip the own rows (2 only) and ghost rows without points
-    // as the occupancy table says
+    // as the occupancy table says; 4 as 2 with each row's mask taken from the mask planes
     int fast_lskip = 0;
     // occupancy table of the 8^3 star stencils, [n][16] int32 per chunk: the 6 star
     // neighbour positions, the chunk's row occupancy (64 bits: row z*8+y has a slot with a
@@ -151,6 +151,20 @@ struct sg_grid {
     int32_t *d_occ_tab = nullptr;
     int64_t occ_tab_cap = 0;
     bool occ_tab_valid = false;
+    // mask planes of the 8^3 star stencils' skip mode 4, [n + 1][16] uint64 (words 0-7: mask
+    // byte non-zero, 8-15: EXIST without PADDING; word z bit x + 8 y), built with the
+    // occupancy table and valid with it
+    uint64_t *d_mplanes = nullptr;
+    int64_t mplanes_cap = 0;
+    // EXIST bit-planes of 8^3 chunks, [n + 1][8] uint64 (word z: bit x + 8 y is the EXIST
+    // bit of point (x, y, z)), the background chunk's zero; rebuilt when stale (masks or
+    // the chunk set may have changed; tagBoundaries changes only PADDING bits)
+    uint64_t *d_ebits = nullptr;
+    int64_t ebits_cap = 0;
+    bool ebits_valid = false;
+    // 8^3 tagBoundaries: 0 neighbours' EXIST bits from the bit-planes, 1 from the mask rows
+    // (faster when the planes have to be built first, as after every flush)
+    int tag_variant = 1;
 
     // existing points (data positions chunk * csz + offset) for the point-list stencil
     // mode; rebuilt when stale (tagBoundaries with CALCULATE_EXISTING_POINTS builds it)
@@ -328,6 +342,8 @@ int fill_background_chunk(sg_grid *g, int prop);
 int normalize_prop(sg_grid *g, int prop);
 // occupancy table of the 8^3 star stencils (sg_grid::d_occ_tab), rebuilt when stale
 int occ_table_impl(sg_grid *g);
+// EXIST bit-planes of 8^3 chunks (sg_grid::d_ebits), rebuilt when stale
+int ebits_impl(sg_grid *g);
 int count_mask_impl(sg_grid *g, int which, int64_t *out);
 // replace the chunk list and masks (hostToDevice of a host copy whose chunks changed)
 int set_chunks_impl(sg_grid *g, int64_t n, const int64_t *ids, const uint8_t *masks, int mem);

@@ -768,6 +768,7 @@ int sg_remove_points(sg_grid *g, int64_t n, const int64_t *keys, int mem)
         dk = (const int64_t *)g->scratch[9].p;
     }
     g->occ_tab_valid = false;
+    g->ebits_valid = false;
     hipLaunchKernelGGL(remove_points_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_keys, g->nchunks,
                        g->d_mask, dk, n, g->csz, grid_props(g));
     SG_HIP(hipGetLastError());
@@ -793,6 +794,7 @@ int remove_queued_boxes(sg_grid *g)
         SG_HIP(hipMemcpyAsync(d_hi, g->rem_hi.data(), bb, hipMemcpyHostToDevice, g->stream));
         Boxes B{nb, d_lo, d_hi};
         g->occ_tab_valid = false;
+        g->ebits_valid = false;
         hipLaunchKernelGGL(remove_boxes_kernel, dim3((unsigned)g->nchunks), dim3(kWG), 0, g->stream, g->d_keys,
                            g->nchunks, g->d_mask, geom_of(g), B, grid_props(g));
         SG_HIP(hipGetLastError());

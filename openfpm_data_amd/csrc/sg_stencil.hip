@@ -215,6 +215,9 @@ struct StencilArgs {
     double bgs[2];  // source backgrounds
     int lskip;      // 8^3 fast path: read the mask first and skip loading rows without points
     int zorder;     // chunk ids are Morton codes (grid_zmb)
+    // skip mode 4: per chunk 16 words, word z (0-7) bit x + 8 y = mask byte non-zero, word
+    // 8 + z the same for EXIST without PADDING (the masks as the stencil needs them)
+    const uint64_t *mplanes;
     double coef[27];
     int64_t icoef[27];
 };
@@ -688,6 +691,105 @@ __global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_kernel(int64_t n, 
     }
 }
 
+// ---- EXIST bit-planes of 8^3 chunks -------------------------------------------------
+// One wave per chunk, lane = row (y = l & 7, z = l >> 3): the row's 8 EXIST bits go to bits
+// 8y .. 8y + 7 of the chunk's word z, the 8 lanes of one z combine theirs by shuffles.
+// Chunk n (the background chunk) gets zero words.
+__global__ __launch_bounds__(256) void ebits_kernel(const uint8_t *__restrict__ mask, int64_t n,
+                                                    uint64_t *__restrict__ eb)
+{
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c > n) return;
+    const uint64_t row = (c < n) ? *(const uint64_t *)(mask + c * 512 + lane * 8) : 0ull;
+    const uint64_t w = (uint64_t)exist_bits8(row) << (8 * (lane & 7));
+    uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+#pragma unroll
+    for (int s = 1; s < 8; s <<= 1) {
+        lo |= (uint32_t)__shfl_xor((int)lo, s, 64);
+        hi |= (uint32_t)__shfl_xor((int)hi, s, 64);
+    }
+    if ((lane & 7) == 0) eb[c * 8 + (lane >> 3)] = (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// 8^3 tagBoundaries reading the neighbours' EXIST bits from the bit-planes (tagBoundaries,
+// SparseGridGpu_kernels.cuh:667-737; same tile and decision as tag_boundaries8_kernel): a
+// lane's row word and, for lanes 0-35, one ghost row word of the 10 x 10 tile come from
+// 8-byte plane words of the chunk and its neighbours, which share 64-byte lines, instead
+// of mask rows and single mask bytes of the neighbour chunks.
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_bits_kernel(int64_t n, const int32_t *__restrict__ nn,
+                                                                         int nnn, int type,
+                                                                         const uint64_t *__restrict__ eb,
+                                                                         uint8_t *__restrict__ mask)
+{
+    __shared__ uint16_t tile[WAVES][100];
+    // uniform wave index: the neighbour-table row is read by scalar loads
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * WAVES + w;
+    if (c >= n) return;  // waves are independent: no workgroup barrier below
+    const int y = lane & 7, z = lane >> 3;
+    int gyy, gzz;
+    ghost_row_yz(lane < 36 ? lane : 0, gyy, gzz);
+    const int gdy = (gyy < 0) ? -1 : (gyy > 7 ? 1 : 0), gdz = (gzz < 0) ? -1 : (gzz > 7 ? 1 : 0);
+    const int gly = gyy - gdy * 8, glz = gzz - gdz * 8;
+    const int32_t *row_nn = nn + c * nnn;
+    auto chunk_of = [&](int dx, int dy, int dz) -> int64_t {
+        const int e = nn_entry(type, dx, dy, dz);
+        if (e == -2) return c;
+        return (e < 0) ? -1 : (int64_t)row_nn[e];
+    };
+    // plane word z of the chunk at displacement (dx, dy, dz); 0 where the table has no entry
+    auto word = [&](int dx, int dy, int dz, int pz) -> uint64_t {
+        const int64_t nb = chunk_of(dx, dy, dz);
+        return (nb < 0) ? 0ull : eb[nb * 8 + pz];
+    };
+    const uint64_t own = *(const uint64_t *)(mask + c * 512 + lane * 8);
+    const uint64_t pc = eb[c * 8 + z], pxm = word(-1, 0, 0, z), pxp = word(1, 0, 0, z);
+    uint64_t g0 = 0, gm = 0, gp = 0;
+    if (lane < 36) {
+        g0 = word(0, gdy, gdz, glz);
+        if (type == SG_NN_FULL) {
+            gm = word(-1, gdy, gdz, glz);
+            gp = word(1, gdy, gdz, glz);
+        }
+    }
+    tile[w][(z + 1) * 10 + (y + 1)] =
+        (uint16_t)(((pxm >> (8 * y + 7)) & 1u) | (((pc >> (8 * y)) & 0xFFu) << 1) | (((pxp >> (8 * y)) & 1u) << 9));
+    if (lane < 36)
+        tile[w][(gzz + 1) * 10 + (gyy + 1)] = (uint16_t)(((gm >> (8 * gly + 7)) & 1u) |
+                                                         (((g0 >> (8 * gly)) & 0xFFu) << 1) |
+                                                         (((gp >> (8 * gly)) & 1u) << 9));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint16_t *t = tile[w];
+    const uint32_t self = t[(z + 1) * 10 + (y + 1)];
+    uint32_t all;  // bit j: every neighbour of x = j - 1 exists
+    if (type == SG_NN_FULL) {
+        all = 0x3FFu;
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const uint32_t r = t[(z + dz) * 10 + (y + dy)];
+                all &= r & (r << 1) & (r >> 1);
+            }
+    } else {
+        all = (self << 1) & (self >> 1) & t[(z + 1) * 10 + y] & t[(z + 1) * 10 + y + 2] & t[z * 10 + y + 1] &
+              t[(z + 2) * 10 + y + 1];
+    }
+    const uint32_t pad = (self >> 1) & ~(all >> 1) & 0xFFu;  // existing and not surrounded
+    uint64_t out = own;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t m = (own >> (8 * i)) & 0xFFull;
+        const uint64_t nm = ((pad >> i) & 1u) ? (m | 2ull) : (m & ~2ull);
+        out = (m & 1) ? ((out & ~(0xFFull << (8 * i))) | (nm << (8 * i))) : out;
+    }
+    if (out != own) *(uint64_t *)(mask + c * 512 + lane * 8) = out;
+}
+
 // ---- 3D 8^3 27-point box ----------------------------------------------------------
 // One wave per chunk, lane = x-row.  The wave stages a 10 x 10 x 10 tile of the source
 // (rows (z, y) in [-1, 8]^2, 10 values with the x ghosts) from the chunk and its 26
@@ -1127,8 +1229,20 @@ __device__ inline void issue_own(ChunkIn<T, KIND> &in, const uint8_t *__restrict
     in.mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
 }
 
+// bit x of b -> bit 8 x
+__device__ inline uint64_t spread8(uint32_t b)
+{
+    uint64_t r = 0;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) r |= (uint64_t)((b >> x) & 1u) << (8 * x);
+    return r;
+}
+
 // own rows and mask of the chunk where the occupancy table says the row holds a point
-// (the others hold the background: library invariant)
+// (the others hold the background: library invariant).  Skip mode 4 takes the row's mask
+// from the two 64-bit planes (2 x 8 bytes per row, shared by the 8 rows of a z-plane,
+// instead of the 8 mask bytes): byte x = 1 for an interior point, 2 for any other
+// non-zero mask, 0 for none -- the three cases the fast kernels distinguish.
 template <typename T, int KIND>
 __device__ inline void issue_own_occ(ChunkIn<T, KIND> &in, const uint8_t *__restrict__ mask, const T *__restrict__ src0,
                                      const T *__restrict__ src1, int64_t c, int lane, const StencilArgs &sa,
@@ -1137,7 +1251,14 @@ __device__ inline void issue_own_occ(ChunkIn<T, KIND> &in, const uint8_t *__rest
     if ((own >> lane) & 1ull) {
         load_row(in.u.own, src0 + c * 512 + lane * 8);
         if constexpr (ChunkIn<T, KIND>::kTwo) load_row(in.v.own, src1 + c * 512 + lane * 8);
-        in.mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
+        if (sa.lskip == 4) {
+            const int z = lane >> 3, y8 = 8 * (lane & 7);
+            const uint32_t nz = (uint32_t)(sa.mplanes[c * 16 + z] >> y8) & 0xFFu;
+            const uint32_t it = (uint32_t)(sa.mplanes[c * 16 + 8 + z] >> y8) & 0xFFu;
+            in.mrow = spread8(it) | (spread8(nz & ~it) << 1);
+        } else {
+            in.mrow = *(const uint64_t *)(mask + c * 512 + lane * 8);
+        }
     } else {
         in.mrow = 0;
 #pragma unroll
@@ -1168,7 +1289,7 @@ __device__ inline void issue_chunk(ChunkIn<T, KIND> &in, const int32_t *__restri
         __builtin_amdgcn_sched_barrier(0);
         OccRow o;
         read_occ(o, nn, c, lane);
-        if (sa.lskip == 2) issue_own_occ<T, KIND>(in, mask, src0, src1, c, lane, sa, o.own);
+        if (sa.lskip == 2 || sa.lskip == 4) issue_own_occ<T, KIND>(in, mask, src0, src1, c, lane, sa, o.own);
         issue_ghosts_occ<T>(in.u, src0, o, lane, true, true, (T)sa.bgs[0]);
         if constexpr (ChunkIn<T, KIND>::kTwo) issue_ghosts_occ<T>(in.v, src1, o, lane, true, true, (T)sa.bgs[1]);
         return;
@@ -1481,7 +1602,7 @@ __device__ inline void group_step(GroupLds<T, KIND, W, ONE> &L, int64_t grp, int
         __builtin_amdgcn_sched_barrier(0);
         OccRow o;
         read_occ(o, nn, c, lane);
-        if (sa.lskip == 2) issue_own_occ<T, KIND>(in, mask, src0, src1, c, lane, sa, o.own);
+        if (sa.lskip == 2 || sa.lskip == 4) issue_own_occ<T, KIND>(in, mask, src0, src1, c, lane, sa, o.own);
         xm_grp = (w > 0) && o.nxm == c - 1;
         xp_grp = (w < W - 1) && (c + 1 < n) && o.nxp == c + 1;
         issue_ghosts_occ<T>(in.u, src0, o, lane, !xm_grp, !xp_grp, (T)sa.bgs[0]);
@@ -1542,9 +1663,11 @@ __global__ __launch_bounds__(64 * W) void stencil_group_kernel(const int32_t *__
 
 // row occupancy of 8^3 chunks: bit r of occ[c] set when row r (z*8 + y) of chunk c has a
 // slot with a non-zero mask; one wave per chunk, lane r reads its row's 8 mask bytes;
-// occ[n] (the background chunk) is 0
+// occ[n] (the background chunk) is 0.  Also the chunk's two mask planes (StencilArgs::
+// mplanes): the row's 8 bits go to bits 8y .. 8y + 7 of word z, the 8 lanes of one z
+// combine theirs by shuffles.
 __global__ __launch_bounds__(256) void row_occ_kernel(const uint8_t *__restrict__ mask, int64_t n,
-                                                      uint64_t *__restrict__ occ)
+                                                      uint64_t *__restrict__ occ, uint64_t *__restrict__ planes)
 {
     const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -1552,6 +1675,24 @@ __global__ __launch_bounds__(256) void row_occ_kernel(const uint8_t *__restrict_
     const uint64_t m = (c < n) ? *(const uint64_t *)(mask + c * 512 + lane * 8) : 0ull;
     const uint64_t b = __ballot(m != 0);
     if (lane == 0) occ[c] = b;
+    uint32_t nz = 0, it = 0;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        const uint32_t cm = (uint32_t)(m >> (8 * x)) & 0xFFu;
+        nz |= (uint32_t)(cm != 0) << x;
+        it |= (uint32_t)((cm & SG_EXIST) && !(cm & SG_PADDING)) << x;
+    }
+    const int sh = 8 * (lane & 7);
+    uint32_t w[4] = {sh < 32 ? nz << sh : 0u, sh >= 32 ? nz << (sh - 32) : 0u, sh < 32 ? it << sh : 0u,
+                     sh >= 32 ? it << (sh - 32) : 0u};
+#pragma unroll
+    for (int s = 1; s < 8; s <<= 1)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] |= (uint32_t)__shfl_xor((int)w[k], s, 64);
+    if ((lane & 7) == 0) {
+        planes[c * 16 + (lane >> 3)] = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+        planes[c * 16 + 8 + (lane >> 3)] = (uint64_t)w[2] | ((uint64_t)w[3] << 32);
+    }
 }
 
 // the occupancy table row of every chunk from the star neighbour table and the row
@@ -1698,8 +1839,19 @@ int tag_boundaries_impl(sg_grid *g, int type)
 {
     const int64_t n = g->nchunks;
     if (n == 0) return SG_OK;
+    // PADDING bits change: the stencils' mask planes (EXIST without PADDING) go stale with
+    // the occupancy table they are built with; the EXIST bit-planes stay valid
+    g->occ_tab_valid = false;
     int rc = ensure_nn(g, type);
     if (rc) return rc;
+    if (g->dim == 3 && g->edge == 8 && g->tag_variant == 0) {
+        // other waves read only the bit-planes, which tagging (PADDING bits) leaves valid
+        if ((rc = ebits_impl(g))) return rc;
+        hipLaunchKernelGGL((tag_boundaries8_bits_kernel<4>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, g->stream, n,
+                           g->d_nn[type], g->nnn(type), type, g->d_ebits, g->d_mask);
+        SG_HIP(hipGetLastError());
+        return SG_OK;
+    }
     if (g->dim == 3 && g->edge == 8) {
         // the tile kernel writes masks in place: rows are read by other waves only through
         // ghost words (EXIST bits), which this kernel never changes
@@ -1791,6 +1943,7 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
     sa.bgd[0] = sa.bgd[1] = 0.0;
     sa.bgs[0] = sa.bgs[1] = 0.0;
     sa.lskip = g->fast_lskip;
+    sa.mplanes = nullptr;
     sa.zorder = g->lin == SG_LIN_ZMB;
     for (int q = 0; q < nsrc && q < 2; ++q) {
         if (g->dt[d->src[q]] == SG_F32) {
@@ -1892,6 +2045,7 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
     if (fast && g->fast_lskip >= 2) {
         if ((rc = occ_table_impl(g))) return rc;
         fnn = g->d_occ_tab;
+        sa.mplanes = g->d_mplanes;
     }
     dispatch_type(dt, [&](auto tag) {
         typedef decltype(tag) T;
@@ -1981,6 +2135,27 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
     return SG_OK;
 }
 
+// EXIST bit-planes of 8^3 chunks (rebuilt when stale)
+int ebits_impl(sg_grid *g)
+{
+    if (g->ebits_valid) return SG_OK;
+    const int64_t n = g->nchunks;
+    if (n + 1 > g->ebits_cap || !g->d_ebits) {
+        g->pool.free(g->d_ebits);
+        const int64_t cap = n + 1;
+        g->d_ebits = (uint64_t *)g->pool.alloc((size_t)cap * 8 * sizeof(uint64_t));
+        if (!g->d_ebits) {
+            g->ebits_cap = 0;
+            return fail(SG_E_OOM, "EXIST bit-plane allocation failed");
+        }
+        g->ebits_cap = cap;
+    }
+    hipLaunchKernelGGL(ebits_kernel, dim3((unsigned)(n / 4 + 1)), dim3(256), 0, g->stream, g->d_mask, n, g->d_ebits);
+    SG_HIP(hipGetLastError());
+    g->ebits_valid = true;
+    return SG_OK;
+}
+
 // occupancy table of the 8^3 star stencils (rebuilt when stale)
 int occ_table_impl(sg_grid *g)
 {
@@ -1998,10 +2173,20 @@ int occ_table_impl(sg_grid *g)
         }
         g->occ_tab_cap = cap;
     }
+    if (n + 1 > g->mplanes_cap || !g->d_mplanes) {
+        g->pool.free(g->d_mplanes);
+        g->d_mplanes = (uint64_t *)g->pool.alloc((size_t)(n + 1) * 16 * sizeof(uint64_t));
+        if (!g->d_mplanes) {
+            g->mplanes_cap = 0;
+            return fail(SG_E_OOM, "mask plane allocation failed");
+        }
+        g->mplanes_cap = n + 1;
+    }
     DevBuf &b = g->scratch[12];
     if ((rc = ensure(g->pool, b, (size_t)(n + 1) * sizeof(uint64_t)))) return rc;
     uint64_t *occ = (uint64_t *)b.p;
-    hipLaunchKernelGGL(row_occ_kernel, dim3((unsigned)(n / 4 + 1)), dim3(256), 0, g->stream, g->d_mask, n, occ);
+    hipLaunchKernelGGL(row_occ_kernel, dim3((unsigned)(n / 4 + 1)), dim3(256), 0, g->stream, g->d_mask, n, occ,
+                       g->d_mplanes);
     SG_HIP(hipGetLastError());
     if (n > 0) {
         hipLaunchKernelGGL(occ_table_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_nn[SG_NN_STAR], occ,

@@ -298,7 +298,8 @@ def test_blobs_flush_subsample(G, oracle):
                                                   (0, 3, 0), (1, 3, 1), (6, 3, 0), (7, 3, 1), (8, 3, 0),
                                                   (6, 2, 1), (7, 0, 0), (8, 1, 1), (3, 3, 2), (3, 3, 3),
                                                   (4, 3, 2), (6, 3, 2), (7, 3, 3), (0, 3, 2), (1, 3, 2),
-                                                  (2, 0, 3), (5, 1, 2), (8, 2, 3)])
+                                                  (2, 0, 3), (5, 1, 2), (8, 2, 3), (3, 3, 4), (6, 3, 4),
+                                                  (4, 3, 4), (0, 3, 4), (1, 3, 4), (7, 2, 4), (5, 1, 4)])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_fast_stencil_variants(G, oracle, variant, stores, lskip, dtype):
     """Every selectable launch shape of the 8^3 star-stencil kernel gives the oracle's
@@ -359,14 +360,14 @@ def test_grouped_stencil_variants_agree_on_a_large_shell(G, oracle):
             S.assert_same_state(g, ref)
             g.close()
     for variant in (3, 4, 6):
-        for lskip in (2, 3):
+        for lskip in (2, 3, 4):
             g = run(G, variant, 3, lskip)
             S.assert_same_state(g, ref)
             g.close()
     ref.close()
 
 
-@pytest.mark.parametrize("lskip", [2, 3])
+@pytest.mark.parametrize("lskip", [2, 3, 4])
 def test_occupancy_table_follows_mask_changes(G, oracle, lskip):
     """The row-occupancy table of the skipping loads is rebuilt after every change of the
     masks: points removed (whole rows emptied) and points added by a flush; each following
@@ -584,11 +585,12 @@ def test_tag_and_box_8cubed_random(G, oracle, seed):
         S.assert_same_state(a, b, exist_only=False)
 
 
-@pytest.mark.parametrize("cpw", [1, 2, 4])
+@pytest.mark.parametrize("variant,cpw", [(0, 2), (1, 1), (1, 2), (1, 4)])
 @pytest.mark.parametrize("nn", [C.NN_STAR, C.NN_FULL])
-def test_tag_boundaries_chunks_per_wave(G, oracle, cpw, nn):
-    """The 8^3 tagBoundaries kernel with 1, 2 or 4 chunks per wave (masks with every bit
-    pattern) tags the same points as the oracle."""
+def test_tag_boundaries_chunks_per_wave(G, oracle, variant, cpw, nn):
+    """The 8^3 tagBoundaries kernels (neighbour EXIST bits from the bit-planes, or from the
+    mask rows with 1, 2 or 4 chunks per wave; masks with every bit pattern) tag the same
+    points as the oracle."""
     rng = np.random.default_rng(1100 + cpw)
     size = [int(rng.integers(60, 90)) for _ in range(3)]
     steps = S.random_flush_program(7 + cpw, 3, 8, size, [np.int32], nflush=2, nrec=4000, ops=["smax"])
@@ -596,12 +598,72 @@ def test_tag_boundaries_chunks_per_wave(G, oracle, cpw, nn):
     for cls in (G, oracle.SparseGridCpu):
         g = S.run_flush_program(cls, 3, 8, size, [np.int32], steps)
         if cls is G:
+            g.set_tuning(C.TUNE_TAG_VARIANT, variant)
             g.set_tuning(C.TUNE_TAG_CHUNKS_PER_WAVE, cpw)
         g.set_nn_type(nn)
         g.tag_boundaries(nn)
         grids.append(g)
     S.assert_same_state(grids[0], grids[1])
     assert grids[0].count_boundary_elements() == grids[1].count_boundary_elements() > 0
+
+
+@pytest.mark.parametrize("nn", [C.NN_STAR, C.NN_FULL])
+def test_tag_boundaries_after_mask_changes(G, oracle, nn):
+    """The EXIST bit-planes follow every change of the masks: tagBoundaries after a flush
+    that adds points to existing and new chunks, after removing points and a box, and
+    after the masks were written through the device pointer, matches the oracle each
+    time (tagging itself changes only PADDING bits and keeps the planes)."""
+    import torch
+    rng = np.random.default_rng(77)
+    size = (72, 64, 80)
+    steps = S.random_flush_program(31, 3, 8, size, [np.int32], nflush=1, nrec=3000, ops=["smax"])
+    grids = [S.run_flush_program(cls, 3, 8, size, [np.int32], steps) for cls in (G, oracle.SparseGridCpu)]
+
+    def tag_and_compare():
+        for g in grids:
+            g.set_nn_type(nn)
+            g.tag_boundaries(nn)
+        S.assert_same_state(grids[0], grids[1])
+
+    tag_and_compare()
+    tag_and_compare()  # again, on tagged masks
+    pts = np.stack([rng.integers(0, s, 2000) for s in size], axis=1).astype(np.int32)
+    vals = rng.integers(-100, 100, len(pts)).astype(np.int32)
+    for g in grids:
+        g.insert_coords(pts, [vals])
+        g.flush(["smax"])
+    tag_and_compare()
+    keys = grids[0].lin_id(S.existing_points(grids[0])[::3])
+    for g in grids:
+        g.remove_points(keys)
+        g.remove_box((8, 8, 8), (40, 30, 50))
+        g.remove_add_unpack_finalize([])
+    tag_and_compare()
+    # masks written by the caller through the device pointer (EXIST cleared on every fifth
+    # slot); the oracle gets the same masks as chunk records of a new grid
+    m2 = grids[0].masks().copy()
+    m2.reshape(-1)[::5] &= 0xFE
+    ids = grids[0].chunk_ids()
+
+    class _DeviceBytes:
+        def __init__(self, ptr, n):
+            self.__cuda_array_interface__ = {"shape": (n,), "typestr": "|u1", "data": (ptr, False),
+                                             "version": 2}
+
+    grids[0].synchronize()
+    dev = torch.as_tensor(_DeviceBytes(grids[0].device_ptr(-1), m2.size), device="cuda")
+    dev.copy_(torch.from_numpy(m2.reshape(-1)).cuda())
+    torch.cuda.synchronize()
+    o = oracle.SparseGridCpu(3, size, [np.int32], block_edge=8)
+    o.insert_block(ids, m2, [np.zeros(m2.shape, np.int32)])
+    o.flush(["smax"])
+    for g in (grids[0], o):
+        g.set_nn_type(nn)
+        g.tag_boundaries(nn)
+    np.testing.assert_array_equal(grids[0].chunk_ids(), o.chunk_ids())
+    np.testing.assert_array_equal(grids[0].masks(), o.masks())
+    for g in grids + [o]:
+        g.close()
 
 
 def test_new_grids_reuse_blocks_of_destroyed_ones(G, oracle):
