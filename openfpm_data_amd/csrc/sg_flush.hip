@@ -756,7 +756,10 @@ __device__ inline void write_fold_prop(int op, T bgv, const unsigned char *__res
                 typename std::conditional<PPV == 8, uint64_t,
                 typename std::conditional<PPV == 4, uint32_t, uint16_t>::type>::type>::type MaskW;
         const int nvec = csz / PPV;
-        constexpr int G = 4;
+        // groups of G vectors per lane: only an old chunk's loop loads, and for 1- and
+        // 2-byte values a group would hold 16 or 8 unpacked values per vector (the kernel's
+        // register peak, which sets its occupancy), so those load one vector at a time
+        constexpr int G = OLD ? (sizeof(T) >= 4 ? 4 : 1) : 1;
         for (int v0 = lane; v0 < nvec; v0 += 64 * G) {
             union {
                 v4u q;
@@ -831,13 +834,14 @@ __global__ __launch_bounds__(256) void fold_chunk_records_kernel(const uint4 *__
                                                                  const uint32_t *__restrict__ isnew, int64_t nnew,
                                                                  int csz, int obits, int waves, FoldState S,
                                                                  FoldProps F, RecLayout R, FusedImg I,
-                                                                 uint8_t *__restrict__ omask)
+                                                                 uint8_t *__restrict__ omask, int skip_fresh)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_fold[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (w >= waves) return;  // waves are independent: no workgroup barrier
     const int64_t j = (int64_t)blockIdx.x * waves + w;
     if (j >= nnew) return;
+    if (skip_fresh && isnew[j] != 0) return;  // folded by fold_fresh_chunks_kernel
     unsigned char *st = smem_fold + (size_t)w * S.bytes;
     uint8_t *seen = st, *am = st + csz;
     uint4 *stage = (uint4 *)(st + S.roff);  // the batch's records, lane l at stage[l * NV]
@@ -936,6 +940,222 @@ __global__ __launch_bounds__(256) void fold_chunk_records_kernel(const uint4 *__
         } else {
             for (int s = lane; s < csz; s += 64) dm[s] = (uint8_t)((smask ? smask[s] : 0) | (seen[s] ? am[s] : 0));
         }
+    }
+}
+
+// ---- fold of the records of brand-new chunks -------------------------------------
+// A new chunk has no old contents, so its LDS state can be the chunk's image itself:
+// every array starts as the background (masks 0), the records fold into it, and it is
+// copied out as 16-byte vectors.  Record fields come straight from the registers the
+// records were loaded into (no staging), the masks are ORed in by LDS atomics (order-
+// free), and only the values of a slot reached by several records of one batch are
+// folded rank by rank, in lane (= insertion) order.
+
+struct FreshProp {
+    int dt, op, ts, roff;  // type code, reduction, value bytes, byte offset in the record
+    int voff, pad;         // LDS image offset of the property's chunk
+    unsigned char *out;    // chunk array
+    uint64_t pat;          // background replicated over 8 bytes
+};
+
+struct FreshArgs {
+    int np;
+    int moff, soff;  // LDS offsets of the mask image and the seen bits
+    int bytes;       // LDS bytes per wave
+    FreshProp p[kMaxProps];
+};
+
+// word i (uniform) of a record held in NV 16-byte registers
+template <int NV> __device__ inline uint32_t rec_word(const uint4 *r, int i)
+{
+    const uint4 v = (NV == 1 || i < 4) ? r[0] : r[NV - 1];
+    const int k = i & 3;
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+// the value of a property at byte offset roff (uniform, naturally aligned)
+template <typename T, int NV> __device__ inline T rec_value(const uint4 *r, int roff)
+{
+    T t;
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = (uint64_t)rec_word<NV>(r, roff >> 2) | ((uint64_t)rec_word<NV>(r, (roff >> 2) + 1) << 32);
+        __builtin_memcpy(&t, &u, 8);
+    } else if constexpr (sizeof(T) == 4) {
+        const uint32_t u = rec_word<NV>(r, roff >> 2);
+        __builtin_memcpy(&t, &u, 4);
+    } else {
+        const uint32_t u = rec_word<NV>(r, roff >> 2) >> (8 * (roff & 3));
+        if constexpr (sizeof(T) == 2) {
+            const uint16_t h = (uint16_t)u;
+            __builtin_memcpy(&t, &h, 2);
+        } else {
+            const uint8_t h = (uint8_t)u;
+            __builtin_memcpy(&t, &h, 1);
+        }
+    }
+    return t;
+}
+
+// One property of one batch into the chunk image.  first: the record is its slot's first
+// (its value is taken whatever the masks); aE: an earlier record of the slot carries
+// EXIST; bE: this one does.  The same rule as the old-chunk fold's record rounds.
+template <typename T, int OP, int NV>
+__device__ inline void fresh_prop(unsigned char *img, const uint4 *r, int roff, int off, int rank, int maxr,
+                                  bool first, bool aE, bool bE, bool all_first)
+{
+    T *a = (T *)img + off;
+    const T b = rec_value<T, NV>(r, roff);
+    if (all_first) {
+        // one record per slot, none seen before: the values are the records'
+        if (rank == 0) *a = b;
+        return;
+    }
+    for (int k = 0; k <= maxr; ++k) {
+        if (rank == k) {
+            const T cur = *a;
+            T v;
+            if (OP == SG_OP_NONE) v = bE ? b : cur;
+            else v = (aE && bE) ? dev_reduce<T>(OP, cur, b) : (bE ? b : cur);
+            *a = first ? b : v;
+        }
+        if (k < maxr) wave_sync_lds();
+    }
+}
+
+template <typename T, int NV>
+__device__ inline void fresh_prop_op(int op, unsigned char *img, const uint4 *r, int roff, int off, int rank, int maxr,
+                                     bool first, bool aE, bool bE, bool all_first)
+{
+    switch (op) {
+    case SG_OP_MAX: fresh_prop<T, SG_OP_MAX, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
+    case SG_OP_MIN: fresh_prop<T, SG_OP_MIN, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
+    case SG_OP_ADD: fresh_prop<T, SG_OP_ADD, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
+    case SG_OP_RIGHT: fresh_prop<T, SG_OP_RIGHT, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
+    case SG_OP_LEFT: fresh_prop<T, SG_OP_LEFT, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
+    case SG_OP_OR: fresh_prop<T, SG_OP_OR, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
+    default: fresh_prop<T, SG_OP_NONE, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
+    }
+}
+
+// One wave per touched chunk; chunks that existed before are left to
+// fold_chunk_records_kernel (their waves return at once).  Needs csz % 16 == 0.
+template <int NV>
+__global__ __launch_bounds__(256) void fold_fresh_chunks_kernel(const uint4 *__restrict__ rec,
+                                                                const uint32_t *__restrict__ ustart,
+                                                                const int64_t *__restrict__ cpos,
+                                                                const uint32_t *__restrict__ isnew, int64_t nnew,
+                                                                int csz, int obits, int waves, FreshArgs A,
+                                                                uint8_t *__restrict__ omask)
+{
+    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_fresh[];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= waves) return;  // waves are independent: no workgroup barrier
+    const int64_t j = (int64_t)blockIdx.x * waves + w;
+    if (j >= nnew || isnew[j] == 0) return;
+    unsigned char *img = smem_fresh + (size_t)w * A.bytes;
+    uint8_t *mimg = img + A.moff;
+    uint32_t *seen = (uint32_t *)(img + A.soff);
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const uint32_t r0 = ustart[j], r1 = ustart[j + 1];
+    uint4 nxt[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) nxt[v] = (r0 + lane < r1) ? rec[(size_t)(r0 + lane) * NV + v] : make_uint4(0, 0, 0, 0);
+    // the image: background values, masks 0, no slot seen
+    for (int q = 0; q < A.np; ++q) {
+        const FreshProp &P = A.p[q];
+        const uint32_t lo = (uint32_t)P.pat, hi = (uint32_t)(P.pat >> 32);
+        const v4u pat = {lo, hi, lo, hi};
+        v4u *d = (v4u *)(img + P.voff);
+        for (int v = lane; v < csz * P.ts / 16; v += 64) d[v] = pat;
+    }
+    for (int v = lane; v < csz / 16; v += 64) ((v4u *)mimg)[v] = (v4u){0u, 0u, 0u, 0u};
+    for (int v = lane; v < (csz + 31) / 32; v += 64) seen[v] = 0u;
+    wave_sync_lds();
+    const int64_t dst = cpos[j];
+    for (uint32_t b0 = r0; b0 < r1; b0 += 64) {
+        uint4 cur[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) cur[v] = nxt[v];
+        const uint32_t k = b0 + (uint32_t)lane;
+        const bool valid = k < r1;
+        const uint32_t kn = k + 64;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) nxt[v] = (kn < r1) ? rec[(size_t)kn * NV + v] : make_uint4(0, 0, 0, 0);
+        const int off = valid ? (int)(cur[0].x & 0xffffu) : 0;
+        const uint8_t m = (uint8_t)(cur[0].x >> 16);
+        // records of this batch that share the slot: ballots over the slot bits
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < obits; ++b) {
+            const bool bit = (off >> b) & 1;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const int rank = valid ? (int)__popcll(peers & lt) : -1;
+        int maxr = 0;
+        while (__ballot(rank > maxr) != 0) ++maxr;
+        // the slot's state before this batch, read before any update of the batch
+        const bool pseen = valid && ((seen[off >> 5] >> (off & 31)) & 1u);
+        const bool pE = valid && (mimg[off] & 1);
+        wave_sync_lds();
+        if (valid) {
+            atomicOr((uint32_t *)mimg + (off >> 2), (uint32_t)m << (8 * (off & 3)));
+            if (rank == 0 && !pseen) atomicOr(seen + (off >> 5), 1u << (off & 31));
+        }
+        const bool bE = valid && (m & 1);
+        const uint64_t exist = __ballot(bE);
+        const bool aE = pE || ((exist & peers & lt) != 0);
+        const bool first = rank == 0 && !pseen;
+        const bool all_first = maxr == 0 && __ballot(valid && !first) == 0;
+        for (int q = 0; q < A.np; ++q) {
+            const FreshProp &P = A.p[q];
+            unsigned char *vimg = img + P.voff;
+            switch (P.dt) {
+#define SG_CASE(CODE, T)                                                                                        \
+    case CODE: fresh_prop_op<T, NV>(P.op, vimg, cur, P.roff, off, rank, maxr, first, aE, bE, all_first); break;
+                SG_FOLD_TYPES(SG_CASE)
+#undef SG_CASE
+            }
+        }
+        wave_sync_lds();
+    }
+    // slots reached only by records without any mask bit hold the background (the old-
+    // chunk fold gives the same with no old point); 8 slots per lane and step
+    for (int s0 = lane * 8; s0 < csz; s0 += 512) {
+        const uint32_t sb = (seen[s0 >> 5] >> (s0 & 31)) & 0xffu;
+        const uint64_t mb = *(const uint64_t *)(mimg + s0);
+        uint32_t fix = 0;
+#pragma unroll
+        for (int x = 0; x < 8; ++x) fix |= (((mb >> (8 * x)) & 0xffu) == 0 ? 1u : 0u) << x;
+        fix &= sb;
+        while (fix) {
+            const int s = s0 + __builtin_ctz(fix);
+            fix &= fix - 1;
+            for (int q = 0; q < A.np; ++q) {
+                const FreshProp &P = A.p[q];
+                unsigned char *vp = img + P.voff + (size_t)s * P.ts;
+                switch (P.ts) {
+                case 8: *(uint64_t *)vp = P.pat; break;
+                case 4: *(uint32_t *)vp = (uint32_t)P.pat; break;
+                case 2: *(uint16_t *)vp = (uint16_t)P.pat; break;
+                default: *vp = (unsigned char)P.pat; break;
+                }
+            }
+        }
+    }
+    wave_sync_lds();
+    // the image out, every array's chunk as whole 16-byte vectors
+    for (int q = 0; q < A.np; ++q) {
+        const FreshProp &P = A.p[q];
+        const int nvec = csz * P.ts / 16;
+        const v4u *s = (const v4u *)(img + P.voff);
+        v4u *d = (v4u *)(P.out + (size_t)dst * csz * P.ts);
+        for (int v = lane; v < nvec; v += 64) __builtin_nontemporal_store(s[v], d + v);
+    }
+    {
+        const v4u *s = (const v4u *)mimg;
+        v4u *d = (v4u *)(omask + dst * csz);
+        for (int v = lane; v < csz / 16; v += 64) __builtin_nontemporal_store(s[v], d + v);
     }
 }
 
@@ -1444,6 +1664,7 @@ namespace {
 struct MergeDefer {
     const int64_t *oldpos = nullptr;  // [nnew] old position of each touched chunk
     const uint32_t *isnew = nullptr;  // [nnew] 1: brand-new chunk
+    int64_t nfresh = 0;               // brand-new chunks among the nnew
     const uint8_t *src_mask = nullptr;
     const void *src_prop[kMaxProps] = {};
     void *free_keys = nullptr, *free_mask = nullptr, *free_prop[kMaxProps] = {};
@@ -1489,6 +1710,7 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
     if (df) {
         df->oldpos = oldpos;
         df->isnew = isnew;
+        df->nfresh = nb;
         df->src_mask = g->d_mask;
         for (int p = 0; p < g->np; ++p) df->src_prop[p] = g->d_prop[p];
     }
@@ -1826,17 +2048,62 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
     I.smask = df.src_mask;
     int obits = 0;
     while ((1 << obits) < csz) ++obits;
-    const int waves = std::max(1, std::min(4, 65536 / S.bytes));
-    // one wave per touched chunk (a wave's loop then runs once)
-    const int64_t groups = (nnew + waves - 1) / waves;
-    const dim3 fgrid((unsigned)groups), fblk(64 * waves);
-    if (R.nv == 1)
-        hipLaunchKernelGGL(fold_chunk_records_kernel<1>, fgrid, fblk, (size_t)waves * S.bytes, g->stream, rec, ustart,
-                           cpos, df.oldpos, df.isnew, nnew, csz, obits, waves, S, F, R, I, g->d_mask);
-    else
-        hipLaunchKernelGGL(fold_chunk_records_kernel<2>, fgrid, fblk, (size_t)waves * S.bytes, g->stream, rec, ustart,
-                           cpos, df.oldpos, df.isnew, nnew, csz, obits, waves, S, F, R, I, g->d_mask);
-    SG_HIP(hipGetLastError());
+    // brand-new chunks: folded in an image of the chunk (csz % 16 == 0, image <= 64 KB)
+    FreshArgs A;
+    bool fresh_ok = (csz % 16) == 0 && df.nfresh > 0;
+    {
+        size_t at = 0;
+        A.np = g->np;
+        for (int p = 0; p < g->np; ++p) {
+            FreshProp &P = A.p[p];
+            P.dt = g->dt[p];
+            P.op = F.op[p];
+            P.ts = g->ts[p];
+            P.roff = R.off[p];
+            P.voff = (int)at;
+            P.pad = 0;
+            P.out = F.out[p];
+            unsigned char pat[8];
+            for (int b = 0; b < 8; b += g->ts[p]) memcpy(pat + b, g->bg[p], g->ts[p]);
+            memcpy(&P.pat, pat, 8);
+            at += (size_t)csz * g->ts[p];
+        }
+        A.moff = (int)at;
+        at += csz;
+        A.soff = (int)at;
+        at += (size_t)(csz + 31) / 32 * 4;
+        at = (at + 15) / 16 * 16;
+        A.bytes = (int)std::min<size_t>(at, 1 << 30);
+        fresh_ok = fresh_ok && at <= 65536;
+    }
+    if (fresh_ok) {
+        const int fw = std::max(1, std::min(4, 65536 / A.bytes));
+        const dim3 fgrid((unsigned)((nnew + fw - 1) / fw)), fblk(64 * fw);
+        if (R.nv == 1)
+            hipLaunchKernelGGL(fold_fresh_chunks_kernel<1>, fgrid, fblk, (size_t)fw * A.bytes, g->stream, rec, ustart,
+                               cpos, df.isnew, nnew, csz, obits, fw, A, g->d_mask);
+        else
+            hipLaunchKernelGGL(fold_fresh_chunks_kernel<2>, fgrid, fblk, (size_t)fw * A.bytes, g->stream, rec, ustart,
+                               cpos, df.isnew, nnew, csz, obits, fw, A, g->d_mask);
+        SG_HIP(hipGetLastError());
+    }
+    if (!fresh_ok || df.nfresh < nnew) {
+        // chunks that existed before (and new ones when the image path is not taken)
+        const int waves = std::max(1, std::min(4, 65536 / S.bytes));
+        // one wave per touched chunk (a wave's loop then runs once)
+        const int64_t groups = (nnew + waves - 1) / waves;
+        const dim3 fgrid((unsigned)groups), fblk(64 * waves);
+        const int skip_fresh = fresh_ok ? 1 : 0;
+        if (R.nv == 1)
+            hipLaunchKernelGGL(fold_chunk_records_kernel<1>, fgrid, fblk, (size_t)waves * S.bytes, g->stream, rec,
+                               ustart, cpos, df.oldpos, df.isnew, nnew, csz, obits, waves, S, F, R, I, g->d_mask,
+                               skip_fresh);
+        else
+            hipLaunchKernelGGL(fold_chunk_records_kernel<2>, fgrid, fblk, (size_t)waves * S.bytes, g->stream, rec,
+                               ustart, cpos, df.oldpos, df.isnew, nnew, csz, obits, waves, S, F, R, I, g->d_mask,
+                               skip_fresh);
+        SG_HIP(hipGetLastError());
+    }
     finish_merge(g, df);
     g->ins_n = 0;
     g->nn_valid[0] = g->nn_valid[1] = false;
