@@ -104,6 +104,106 @@ __global__ void coords_to_keys_kernel(const int32_t *coords, int64_t *keys, int6
     keys[i] = chunk_id(L, bc) * csz + l;
 }
 
+// the values of the properties of one batch of point inserts: source arrays (nullptr:
+// the background), insert-buffer destinations
+struct StageVals {
+    int np;
+    int ts[kMaxProps];
+    const unsigned char *src[kMaxProps];
+    unsigned char *dst[kMaxProps];
+    uint64_t bg[kMaxProps];
+};
+
+constexpr int kStageItems = 4;  // records per thread of the streaming insert kernels
+
+// Point inserts given by device coordinates, in one pass: the key (as
+// coords_to_keys_kernel, in 32-bit arithmetic: coordinates are int32), the mask (masks
+// or EXIST) and every property's value, each to its insert-buffer array.  A block takes
+// kWG * kStageItems consecutive records, a thread every kWG-th of them, all loads of a
+// step issued before its stores.
+__global__ __launch_bounds__(kWG) void stage_coords_kernel(const int32_t *__restrict__ coords,
+                                                           const uint8_t *__restrict__ masks, int64_t n, int dim,
+                                                           int edge, int sz0, int sz1, int sz2, ChunkLin L, int csz,
+                                                           int64_t *__restrict__ keys, uint8_t *__restrict__ omask,
+                                                           StageVals V)
+{
+    const int64_t i0 = (int64_t)blockIdx.x * (kWG * kStageItems) + threadIdx.x;
+    const int sz[3] = {sz0, sz1, sz2};
+    int c[kStageItems][3];
+#pragma unroll
+    for (int t = 0; t < kStageItems; ++t) {
+        const int64_t i = i0 + (int64_t)t * kWG;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c[t][d] = (d < dim && i < n) ? coords[i * dim + d] : 0;
+    }
+    const uint32_t e = (uint32_t)edge;
+#pragma unroll
+    for (int t = 0; t < kStageItems; ++t) {
+        const int64_t i = i0 + (int64_t)t * kWG;
+        if (i >= n) break;
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ok = ok && (d >= dim || (c[t][d] >= 0 && c[t][d] < sz[d]));
+        int64_t key = -1;  // outside the grid: the next flush's key check finds it
+        if (ok) {
+            int64_t bc[3] = {0, 0, 0};
+            uint32_t l = 0;
+#pragma unroll
+            for (int d = 2; d >= 0; --d) {
+                if (d >= dim) continue;
+                const uint32_t q = (uint32_t)c[t][d] / e;
+                bc[d] = q;
+                l = l * e + ((uint32_t)c[t][d] - q * e);
+            }
+            key = chunk_id(L, bc) * csz + l;
+        }
+        keys[i] = key;
+    }
+    {
+        uint8_t m[kStageItems];
+#pragma unroll
+        for (int t = 0; t < kStageItems; ++t) {
+            const int64_t i = i0 + (int64_t)t * kWG;
+            m[t] = (masks && i < n) ? masks[i] : (uint8_t)SG_EXIST;
+        }
+#pragma unroll
+        for (int t = 0; t < kStageItems; ++t) {
+            const int64_t i = i0 + (int64_t)t * kWG;
+            if (i < n) omask[i] = m[t];
+        }
+    }
+    for (int p = 0; p < V.np; ++p) {
+        const unsigned char *s = V.src[p];
+        unsigned char *d = V.dst[p];
+        const uint64_t bg = V.bg[p];
+        uint64_t v[kStageItems];
+#pragma unroll
+        for (int t = 0; t < kStageItems; ++t) {
+            const int64_t i = i0 + (int64_t)t * kWG;
+            v[t] = bg;
+            if (s && i < n) {
+                switch (V.ts[p]) {
+                case 8: v[t] = ((const uint64_t *)s)[i]; break;
+                case 4: v[t] = ((const uint32_t *)s)[i]; break;
+                case 2: v[t] = ((const uint16_t *)s)[i]; break;
+                default: v[t] = s[i]; break;
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kStageItems; ++t) {
+            const int64_t i = i0 + (int64_t)t * kWG;
+            if (i >= n) break;
+            switch (V.ts[p]) {
+            case 8: ((uint64_t *)d)[i] = v[t]; break;
+            case 4: ((uint32_t *)d)[i] = (uint32_t)v[t]; break;
+            case 2: ((uint16_t *)d)[i] = (uint16_t)v[t]; break;
+            default: d[i] = (unsigned char)v[t]; break;
+            }
+        }
+    }
+}
+
 __global__ void check_keys_kernel(const int64_t *keys, int64_t n, int64_t limit, int *bad)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1001,15 +1101,10 @@ template <typename T, int NV> __device__ inline T rec_value(const uint4 *r, int 
 // EXIST; bE: this one does.  The same rule as the old-chunk fold's record rounds.
 template <typename T, int OP, int NV>
 __device__ inline void fresh_prop(unsigned char *img, const uint4 *r, int roff, int off, int rank, int maxr,
-                                  bool first, bool aE, bool bE, bool all_first)
+                                  bool first, bool aE, bool bE)
 {
     T *a = (T *)img + off;
     const T b = rec_value<T, NV>(r, roff);
-    if (all_first) {
-        // one record per slot, none seen before: the values are the records'
-        if (rank == 0) *a = b;
-        return;
-    }
     for (int k = 0; k <= maxr; ++k) {
         if (rank == k) {
             const T cur = *a;
@@ -1024,22 +1119,55 @@ __device__ inline void fresh_prop(unsigned char *img, const uint4 *r, int roff, 
 
 template <typename T, int NV>
 __device__ inline void fresh_prop_op(int op, unsigned char *img, const uint4 *r, int roff, int off, int rank, int maxr,
-                                     bool first, bool aE, bool bE, bool all_first)
+                                     bool first, bool aE, bool bE)
 {
     switch (op) {
-    case SG_OP_MAX: fresh_prop<T, SG_OP_MAX, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
-    case SG_OP_MIN: fresh_prop<T, SG_OP_MIN, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
-    case SG_OP_ADD: fresh_prop<T, SG_OP_ADD, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
-    case SG_OP_RIGHT: fresh_prop<T, SG_OP_RIGHT, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
-    case SG_OP_LEFT: fresh_prop<T, SG_OP_LEFT, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
-    case SG_OP_OR: fresh_prop<T, SG_OP_OR, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
-    default: fresh_prop<T, SG_OP_NONE, NV>(img, r, roff, off, rank, maxr, first, aE, bE, all_first); break;
+    case SG_OP_MAX: fresh_prop<T, SG_OP_MAX, NV>(img, r, roff, off, rank, maxr, first, aE, bE); break;
+    case SG_OP_MIN: fresh_prop<T, SG_OP_MIN, NV>(img, r, roff, off, rank, maxr, first, aE, bE); break;
+    case SG_OP_ADD: fresh_prop<T, SG_OP_ADD, NV>(img, r, roff, off, rank, maxr, first, aE, bE); break;
+    case SG_OP_RIGHT: fresh_prop<T, SG_OP_RIGHT, NV>(img, r, roff, off, rank, maxr, first, aE, bE); break;
+    case SG_OP_LEFT: fresh_prop<T, SG_OP_LEFT, NV>(img, r, roff, off, rank, maxr, first, aE, bE); break;
+    case SG_OP_OR: fresh_prop<T, SG_OP_OR, NV>(img, r, roff, off, rank, maxr, first, aE, bE); break;
+    default: fresh_prop<T, SG_OP_NONE, NV>(img, r, roff, off, rank, maxr, first, aE, bE); break;
     }
 }
 
-// One wave per touched chunk; chunks that existed before are left to
-// fold_chunk_records_kernel (their waves return at once).  Needs csz % 16 == 0.
-template <int NV>
+// the record's value bits of one property into the image, when every record of the batch
+// is its slot's first (no fold: only the value size matters)
+template <int NV> __device__ inline void fresh_store_bits(unsigned char *img, const uint4 *r, int roff, int ts, int off)
+{
+    switch (ts) {
+    case 8: ((uint64_t *)img)[off] = rec_value<uint64_t, NV>(r, roff); break;
+    case 4: ((uint32_t *)img)[off] = rec_value<uint32_t, NV>(r, roff); break;
+    case 2: ((uint16_t *)img)[off] = rec_value<uint16_t, NV>(r, roff); break;
+    default: img[off] = rec_value<uint8_t, NV>(r, roff); break;
+    }
+}
+
+// f(q) for every property: unrolled when the count NP is known at compile time
+template <int Q, int N, typename F> __device__ inline void props_unrolled(F &f)
+{
+    if constexpr (Q < N) {
+        f(Q);
+        props_unrolled<Q + 1, N>(f);
+    }
+}
+
+template <int NP, typename F> __device__ inline void for_props(int np, F &&f)
+{
+    if constexpr (NP > 0) {
+        props_unrolled<0, NP>(f);
+    } else {
+        for (int q = 0; q < np; ++q) f(q);
+    }
+}
+
+// One wave per touched chunk; chunks that existed before (isnew[j] == 0) are left to
+// fold_chunk_records_kernel, their waves return at once (isnew == nullptr: every
+// touched chunk is new).  NP > 0: the property count, known at compile time (the
+// per-property loops unroll and their parameters are loaded once); NP == 0: A.np.
+// Needs csz % 16 == 0.
+template <int NV, int NP>
 __global__ __launch_bounds__(256) void fold_fresh_chunks_kernel(const uint4 *__restrict__ rec,
                                                                 const uint32_t *__restrict__ ustart,
                                                                 const int64_t *__restrict__ cpos,
@@ -1049,10 +1177,12 @@ __global__ __launch_bounds__(256) void fold_fresh_chunks_kernel(const uint4 *__r
 {
     typedef unsigned int v4u __attribute__((ext_vector_type(4)));
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_fresh[];
+    const int np = NP > 0 ? NP : A.np;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (w >= waves) return;  // waves are independent: no workgroup barrier
     const int64_t j = (int64_t)blockIdx.x * waves + w;
-    if (j >= nnew || isnew[j] == 0) return;
+    if (j >= nnew) return;
+    if (isnew && isnew[j] == 0) return;
     unsigned char *img = smem_fresh + (size_t)w * A.bytes;
     uint8_t *mimg = img + A.moff;
     uint32_t *seen = (uint32_t *)(img + A.soff);
@@ -1062,17 +1192,16 @@ __global__ __launch_bounds__(256) void fold_fresh_chunks_kernel(const uint4 *__r
 #pragma unroll
     for (int v = 0; v < NV; ++v) nxt[v] = (r0 + lane < r1) ? rec[(size_t)(r0 + lane) * NV + v] : make_uint4(0, 0, 0, 0);
     // the image: background values, masks 0, no slot seen
-    for (int q = 0; q < A.np; ++q) {
+    for_props<NP>(np, [&](int q) {
         const FreshProp &P = A.p[q];
         const uint32_t lo = (uint32_t)P.pat, hi = (uint32_t)(P.pat >> 32);
         const v4u pat = {lo, hi, lo, hi};
         v4u *d = (v4u *)(img + P.voff);
         for (int v = lane; v < csz * P.ts / 16; v += 64) d[v] = pat;
-    }
+    });
     for (int v = lane; v < csz / 16; v += 64) ((v4u *)mimg)[v] = (v4u){0u, 0u, 0u, 0u};
     for (int v = lane; v < (csz + 31) / 32; v += 64) seen[v] = 0u;
     wave_sync_lds();
-    const int64_t dst = cpos[j];
     for (uint32_t b0 = r0; b0 < r1; b0 += 64) {
         uint4 cur[NV];
 #pragma unroll
@@ -1102,20 +1231,27 @@ __global__ __launch_bounds__(256) void fold_fresh_chunks_kernel(const uint4 *__r
             atomicOr((uint32_t *)mimg + (off >> 2), (uint32_t)m << (8 * (off & 3)));
             if (rank == 0 && !pseen) atomicOr(seen + (off >> 5), 1u << (off & 31));
         }
-        const bool bE = valid && (m & 1);
-        const uint64_t exist = __ballot(bE);
-        const bool aE = pE || ((exist & peers & lt) != 0);
         const bool first = rank == 0 && !pseen;
-        const bool all_first = maxr == 0 && __ballot(valid && !first) == 0;
-        for (int q = 0; q < A.np; ++q) {
-            const FreshProp &P = A.p[q];
-            unsigned char *vimg = img + P.voff;
-            switch (P.dt) {
+        if (maxr == 0 && __ballot(valid && !first) == 0) {
+            // one record per slot, none reached before: the values are the records'
+            for_props<NP>(np, [&](int q) {
+                const FreshProp &P = A.p[q];
+                if (valid) fresh_store_bits<NV>(img + P.voff, cur, P.roff, P.ts, off);
+            });
+        } else {
+            const bool bE = valid && (m & 1);
+            const uint64_t exist = __ballot(bE);
+            const bool aE = pE || ((exist & peers & lt) != 0);
+            for_props<NP>(np, [&](int q) {
+                const FreshProp &P = A.p[q];
+                unsigned char *vimg = img + P.voff;
+                switch (P.dt) {
 #define SG_CASE(CODE, T)                                                                                        \
-    case CODE: fresh_prop_op<T, NV>(P.op, vimg, cur, P.roff, off, rank, maxr, first, aE, bE, all_first); break;
-                SG_FOLD_TYPES(SG_CASE)
+    case CODE: fresh_prop_op<T, NV>(P.op, vimg, cur, P.roff, off, rank, maxr, first, aE, bE); break;
+                    SG_FOLD_TYPES(SG_CASE)
 #undef SG_CASE
-            }
+                }
+            });
         }
         wave_sync_lds();
     }
@@ -1131,7 +1267,7 @@ __global__ __launch_bounds__(256) void fold_fresh_chunks_kernel(const uint4 *__r
         while (fix) {
             const int s = s0 + __builtin_ctz(fix);
             fix &= fix - 1;
-            for (int q = 0; q < A.np; ++q) {
+            for (int q = 0; q < np; ++q) {
                 const FreshProp &P = A.p[q];
                 unsigned char *vp = img + P.voff + (size_t)s * P.ts;
                 switch (P.ts) {
@@ -1145,13 +1281,14 @@ __global__ __launch_bounds__(256) void fold_fresh_chunks_kernel(const uint4 *__r
     }
     wave_sync_lds();
     // the image out, every array's chunk as whole 16-byte vectors
-    for (int q = 0; q < A.np; ++q) {
+    const int64_t dst = cpos[j];
+    for_props<NP>(np, [&](int q) {
         const FreshProp &P = A.p[q];
         const int nvec = csz * P.ts / 16;
         const v4u *s = (const v4u *)(img + P.voff);
         v4u *d = (v4u *)(P.out + (size_t)dst * csz * P.ts);
         for (int v = lane; v < nvec; v += 64) __builtin_nontemporal_store(s[v], d + v);
-    }
+    });
     {
         const v4u *s = (const v4u *)mimg;
         v4u *d = (v4u *)(omask + dst * csz);
@@ -1416,6 +1553,26 @@ int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *m
         if (rc) return rc;
         SG_HIP(hipMemcpyAsync(cb.p, coords, n * g->dim * sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
         dc = (const int32_t *)cb.p;
+    } else {
+        // device arrays: keys, masks and values in one pass
+        StageVals V;
+        V.np = g->np;
+        for (int p = 0; p < g->np; ++p) {
+            V.ts[p] = g->ts[p];
+            V.src[p] = values ? (const unsigned char *)values[p] : nullptr;
+            V.dst[p] = (unsigned char *)g->d_ins_val[p] + at * g->ts[p];
+            V.bg[p] = 0;
+            memcpy(&V.bg[p], g->bg[p], g->ts[p]);
+        }
+        int szi[3];
+        for (int d = 0; d < 3; ++d) szi[d] = (int)std::min<int64_t>(g->sz[d], INT32_MAX);
+        hipLaunchKernelGGL(stage_coords_kernel, dim3(grid_1d(n, kWG * kStageItems)), dim3(kWG), 0, g->stream, dc,
+                           masks, n, g->dim,
+                           g->edge, szi[0], szi[1], szi[2], chunk_lin(g), g->csz, g->d_ins_key + at,
+                           g->d_ins_mask + at, V);
+        SG_HIP(hipGetLastError());
+        g->ins_n += n;
+        return SG_OK;
     }
     hipLaunchKernelGGL(coords_to_keys_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, dc,
                        g->d_ins_key + at, n, g->dim, g->edge, g->sz[0], g->sz[1], g->sz[2], chunk_lin(g), g->csz);
@@ -2079,12 +2236,29 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
     if (fresh_ok) {
         const int fw = std::max(1, std::min(4, 65536 / A.bytes));
         const dim3 fgrid((unsigned)((nnew + fw - 1) / fw)), fblk(64 * fw);
-        if (R.nv == 1)
-            hipLaunchKernelGGL(fold_fresh_chunks_kernel<1>, fgrid, fblk, (size_t)fw * A.bytes, g->stream, rec, ustart,
-                               cpos, df.isnew, nnew, csz, obits, fw, A, g->d_mask);
-        else
-            hipLaunchKernelGGL(fold_fresh_chunks_kernel<2>, fgrid, fblk, (size_t)fw * A.bytes, g->stream, rec, ustart,
-                               cpos, df.isnew, nnew, csz, obits, fw, A, g->d_mask);
+        // every touched chunk new: no per-chunk check
+        const uint32_t *chk = df.nfresh < nnew ? df.isnew : nullptr;
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, fgrid, fblk, (size_t)fw * A.bytes, g->stream, rec, ustart, cpos, chk, nnew, csz,
+                               obits, fw, A, g->d_mask);
+        };
+        if (R.nv == 1) {
+            switch (g->np) {
+            case 1: launch(fold_fresh_chunks_kernel<1, 1>); break;
+            case 2: launch(fold_fresh_chunks_kernel<1, 2>); break;
+            case 3: launch(fold_fresh_chunks_kernel<1, 3>); break;
+            case 4: launch(fold_fresh_chunks_kernel<1, 4>); break;
+            default: launch(fold_fresh_chunks_kernel<1, 0>); break;
+            }
+        } else {
+            switch (g->np) {
+            case 1: launch(fold_fresh_chunks_kernel<2, 1>); break;
+            case 2: launch(fold_fresh_chunks_kernel<2, 2>); break;
+            case 3: launch(fold_fresh_chunks_kernel<2, 3>); break;
+            case 4: launch(fold_fresh_chunks_kernel<2, 4>); break;
+            default: launch(fold_fresh_chunks_kernel<2, 0>); break;
+            }
+        }
         SG_HIP(hipGetLastError());
     }
     if (!fresh_ok || df.nfresh < nnew) {

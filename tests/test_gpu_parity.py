@@ -44,20 +44,22 @@ def test_insert_known_answer(G, golden, name):
 
 
 @pytest.mark.parametrize("path", [0, 1])
-@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("seed", range(12))
 def test_flush_random_programs(G, oracle, seed, path):
     """Random insert / flush programs against the oracle and the dictionary model, with
     the point records grouped by chunk (path 0: records of 16 bytes, of 32 bytes, and
-    beyond 32 bytes, where the point-key flush runs) and by point key (path 1)."""
+    beyond 32 bytes, where the point-key flush runs; one to four properties, whose loops
+    the new-chunk fold unrolls, and six, which it loops over) and by point key (path 1)."""
     rng = np.random.default_rng(200 + seed)
     dim = 3 if seed % 2 else 2
     edge = [2, 4, 8][seed % 3]
     size = [int(rng.integers(edge, 6 * edge)) for _ in range(dim)]
     dtypes = [[np.float64], [np.int32, np.uint32], [np.float32, np.int64, np.uint8],
               [np.uint16, np.int8, np.float64, np.uint64], [np.float64, np.int64, np.float64, np.uint64],
-              ][seed % 5]
+              [np.uint8, np.int16, np.float32, np.int8, np.uint16, np.int32],
+              ][seed % 6]
     steps = S.random_flush_program(seed, dim, edge, size, dtypes, nflush=3, nrec=150)
-    bgs = [3, 1, 2, 5][: len(dtypes)]
+    bgs = [3, 1, 2, 5, 7, 9][: len(dtypes)]
     a = S.run_flush_program(G, dim, edge, size, dtypes, steps, background=bgs, flush_path=path)
     b = S.run_flush_program(oracle.SparseGridCpu, dim, edge, size, dtypes, steps, background=bgs)
     S.assert_same_state(a, b)
