@@ -300,6 +300,104 @@ template <typename K> __global__ void run_head_kernel(const K *sk, uint32_t *hea
     if (i < n) head[i] = (i == 0 || sk[i - 1] != sk[i]) ? 1u : 0u;
 }
 
+// Runs of equal keys in a sorted uint32 array, in tiles of kRunTile items: thread t of a
+// tile holds items t * kRunIPT .. + kRunIPT - 1 (consecutive, read as 16-byte vectors);
+// item i starts a run when i == 0 or its key differs from item i - 1's.
+constexpr int kRunIPT = 16;
+constexpr int kRunTile = kWG * kRunIPT;
+
+__device__ inline void run_items(const uint32_t *__restrict__ sk, int64_t n, int64_t i0, uint32_t (&k)[kRunIPT],
+                                 uint32_t &prev)
+{
+    if (i0 + kRunIPT <= n) {
+#pragma unroll
+        for (int v = 0; v < kRunIPT / 4; ++v) {
+            const uint4 q = ((const uint4 *)(sk + i0))[v];
+            k[4 * v] = q.x;
+            k[4 * v + 1] = q.y;
+            k[4 * v + 2] = q.z;
+            k[4 * v + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kRunIPT; ++j) k[j] = (i0 + j < n) ? sk[i0 + j] : 0u;
+    }
+    // the item before the thread's first (none for item 0: its key is taken as different)
+    prev = (i0 > 0 && i0 <= n) ? sk[i0 - 1] : ~k[0];
+}
+
+// exclusive prefix of v over the kWG threads of the block (*total: the block's sum)
+__device__ inline uint32_t run_block_scan(uint32_t v, uint32_t *total)
+{
+    __shared__ uint32_t s_w[kWG / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < kWG / 64; ++q) {
+        const uint32_t s = s_w[q];
+        pre += (q < w) ? s : 0u;
+        tot += s;
+    }
+    *total = tot;
+    return pre + inc - v;
+}
+
+// the number of runs starting in each tile
+__global__ __launch_bounds__(kWG) void run_count_kernel(const uint32_t *__restrict__ sk, int64_t n,
+                                                       uint32_t *__restrict__ cnt)
+{
+    const int64_t i0 = (int64_t)blockIdx.x * kRunTile + (int64_t)threadIdx.x * kRunIPT;
+    uint32_t k[kRunIPT], prev;
+    run_items(sk, n, i0, k, prev);
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kRunIPT; ++j) {
+        c += (i0 + j < n && k[j] != prev) ? 1u : 0u;
+        prev = k[j];
+    }
+    uint32_t tot;
+    run_block_scan(c, &tot);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+// each run's key and first item at the run's rank (off: exclusive scan of the tiles'
+// counts; start[nu] = n)
+__global__ __launch_bounds__(kWG) void run_emit_kernel(const uint32_t *__restrict__ sk, int64_t n,
+                                                      const uint32_t *__restrict__ off, int64_t *__restrict__ key,
+                                                      uint32_t *__restrict__ start, uint32_t nu)
+{
+    const int64_t i0 = (int64_t)blockIdx.x * kRunTile + (int64_t)threadIdx.x * kRunIPT;
+    uint32_t k[kRunIPT], prev;
+    run_items(sk, n, i0, k, prev);
+    uint32_t heads = 0, c = 0;
+#pragma unroll
+    for (int j = 0; j < kRunIPT; ++j) {
+        const bool h = i0 + j < n && k[j] != prev;
+        heads |= (h ? 1u : 0u) << j;
+        c += h ? 1u : 0u;
+        prev = k[j];
+    }
+    uint32_t tot;
+    uint32_t u = off[blockIdx.x] + run_block_scan(c, &tot);
+#pragma unroll
+    for (int j = 0; j < kRunIPT; ++j) {
+        if ((heads >> j) & 1u) {
+            key[u] = (int64_t)k[j];
+            start[u] = (uint32_t)(i0 + j);
+            ++u;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[nu] = (uint32_t)n;
+}
+
 // unique points (construct_index_unique, src/util/cuda/kernels.cuh:34-54)
 template <typename K>
 __global__ void unique_points_kernel(const K *sk, const uint32_t *head, const uint32_t *hscan, int64_t n,
@@ -1920,7 +2018,15 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
             U.src[a] = (const uint4 *)(a == 0 ? (const void *)g->d_mask : g->d_prop[a - 1]);
             U.dst[a] = (uint4 *)(a == 0 ? (void *)nmask : nprop[a - 1]);
         }
-        if (flat) {
+        if (flat && df && m == 0) {
+            // no old chunk: the caller's fold writes every chunk, only the background
+            // chunk (position total) is initialised here
+            UnionArrays B = U;
+            for (int a = 0; a < B.na; ++a) B.dst[a] += (size_t)total << B.vshift[a];
+            hipLaunchKernelGGL(build_union_kernel, dim3(1), dim3(kWG), 0, g->stream, B, src_of, (int64_t)1,
+                               (const uint8_t *)nullptr);
+            SG_HIP(hipGetLastError());
+        } else if (flat) {
             uint8_t *skip = nullptr;
             if (df) {
                 // the touched chunks are written whole by the caller's fused fold
@@ -2169,15 +2275,17 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
         return rc;
     // the range flag, read with the scan total below
     SG_HIP(hipMemcpyAsync(hw + 8, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
-    // touched chunks: run heads of the sorted chunk ids + scan
+    // touched chunks: runs of equal sorted chunk ids, counted per tile, tile offsets
+    // scanned, then each run's id and first record written at its offset
+    const int64_t rtiles = (n + kRunTile - 1) / kRunTile;
     DevBuf &b_h = g->scratch[4];
-    if ((rc = ensure(g->pool, b_h, (size_t)n * 8 + 64))) return rc;
-    uint32_t *head = (uint32_t *)b_h.p, *hscan = head + n;
-    hipLaunchKernelGGL(run_head_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                       (const uint32_t *)ck, head, n);
+    if ((rc = ensure(g->pool, b_h, (size_t)rtiles * 8 + 64))) return rc;
+    uint32_t *rcnt = (uint32_t *)b_h.p, *roff = rcnt + rtiles;
+    hipLaunchKernelGGL(run_count_kernel, dim3((unsigned)rtiles), dim3(kWG), 0, g->stream, (const uint32_t *)ck, n,
+                       rcnt);
     SG_HIP(hipGetLastError());
     uint32_t nn32 = 0;
-    if ((rc = exclusive_scan_u32(g, head, hscan, n, &nn32))) return rc;
+    if ((rc = exclusive_scan_u32(g, rcnt, roff, rtiles, &nn32))) return rc;
     if (hw[8]) {
         // the records outside the grid are dropped, the others stay pending (nothing is merged)
         int64_t kept = 0;
@@ -2190,8 +2298,8 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
     if ((rc = ensure(g->pool, g->scratch[12], (size_t)nnew * 8 + (size_t)(nnew + 1) * 4 + 64))) return rc;
     int64_t *nkey = (int64_t *)g->scratch[12].p;
     uint32_t *ustart = (uint32_t *)(nkey + nnew);
-    hipLaunchKernelGGL(unique_points_kernel<uint32_t>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream,
-                       (const uint32_t *)ck, head, hscan, n, nkey, ustart, nn32);
+    hipLaunchKernelGGL(run_emit_kernel, dim3((unsigned)rtiles), dim3(kWG), 0, g->stream, (const uint32_t *)ck, n, roff,
+                       nkey, ustart, nn32);
     SG_HIP(hipGetLastError());
 
     MergeDefer df;

@@ -570,6 +570,18 @@ __device__ inline uint32_t exist_bits8(uint64_t row)
     return b;
 }
 
+// a mask row (8 bytes) with the PADDING bit of every existing point set to bit x of pad
+// and cleared otherwise; points without EXIST are left as they are
+__device__ inline uint64_t apply_padding(uint64_t row, uint32_t pad)
+{
+    // pad bit x -> bit 8x (four bits at a time: x * 0x00204081 places bit x at 8x with
+    // no two partial products overlapping)
+    const uint64_t p = (uint64_t)(((pad & 0xFu) * 0x00204081u) & 0x01010101u) |
+                       ((uint64_t)((((pad >> 4) & 0xFu) * 0x00204081u) & 0x01010101u) << 32);
+    const uint64_t e = row & 0x0101010101010101ull;
+    return (row & ~(e << 1)) | ((p & e) << 1);
+}
+
 // ghost row i (0..35) of the 10 x 10 tile: (yy, zz) in [-1, 8]^2 outside [0, 7]^2
 // (selects, not branches: branchy stores through the references end up in scratch)
 __device__ inline void ghost_row_yz(int i, int &yy, int &zz)
@@ -601,12 +613,16 @@ __device__ inline int nn_entry(int type, int dx, int dy, int dz)
 // staged in LDS and its points tagged.
 template <int WAVES, int CPW>
 __global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_kernel(int64_t n, const int32_t *__restrict__ nn,
-                                                                    int nnn, int type, uint8_t *__restrict__ mask)
+                                                                    int nnn, int type, uint8_t *__restrict__ mask,
+                                                                    int64_t per_xcd)
 {
     __shared__ uint16_t tile[WAVES][CPW][100];
     // uniform wave index: the neighbour-table rows are read by scalar loads
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int64_t cb = ((int64_t)blockIdx.x * WAVES + w) * CPW;
+    // workgroup b runs in XCD b & 7, which walks its own contiguous range of workgroups'
+    // chunks: neighbours' mask lines then mostly come from the same XCD's L2
+    const int64_t wg = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int64_t cb = (wg * WAVES + w) * CPW;
     if (cb >= n) return;  // waves are independent: no workgroup barrier below
     const int y = lane & 7, z = lane >> 3;
     int gyy, gzz;
@@ -680,13 +696,7 @@ __global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_kernel(int64_t n, 
         }
         const uint32_t pad = (self >> 1) & ~(all >> 1) & 0xFFu;  // existing and not surrounded
         const uint64_t o = own[k];
-        uint64_t out = o;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const uint64_t m = (o >> (8 * i)) & 0xFFull;
-            const uint64_t nm = ((pad >> i) & 1u) ? (m | 2ull) : (m & ~2ull);
-            out = (m & 1) ? ((out & ~(0xFFull << (8 * i))) | (nm << (8 * i))) : out;
-        }
+        const uint64_t out = apply_padding(o, pad);
         if (out != o) *(uint64_t *)(mask + c * 512 + lane * 8) = out;
     }
 }
@@ -716,78 +726,99 @@ __global__ __launch_bounds__(256) void ebits_kernel(const uint8_t *__restrict__ 
 // SparseGridGpu_kernels.cuh:667-737; same tile and decision as tag_boundaries8_kernel): a
 // lane's row word and, for lanes 0-35, one ghost row word of the 10 x 10 tile come from
 // 8-byte plane words of the chunk and its neighbours, which share 64-byte lines, instead
-// of mask rows and single mask bytes of the neighbour chunks.
-template <int WAVES>
+// of mask rows and single mask bytes of the neighbour chunks.  A wave takes CPW
+// consecutive chunks; their neighbour-table rows come by scalar loads and each lane picks
+// its ghost rows' chunks from them by selects, so every plane word of the wave's chunks
+// is loaded in one step, with no dependent load.
+template <int WAVES, int CPW>
 __global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_bits_kernel(int64_t n, const int32_t *__restrict__ nn,
                                                                          int nnn, int type,
                                                                          const uint64_t *__restrict__ eb,
-                                                                         uint8_t *__restrict__ mask)
+                                                                         uint8_t *__restrict__ mask,
+                                                                         int64_t per_xcd)
 {
-    __shared__ uint16_t tile[WAVES][100];
-    // uniform wave index: the neighbour-table row is read by scalar loads
+    __shared__ uint16_t tile[WAVES][CPW][100];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int64_t c = (int64_t)blockIdx.x * WAVES + w;
-    if (c >= n) return;  // waves are independent: no workgroup barrier below
+    // workgroup b runs in XCD b & 7, which walks its own contiguous range of workgroups'
+    // chunks: neighbours' plane words then mostly come from the same XCD's L2
+    const int64_t wg = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int64_t cb = (wg * WAVES + w) * CPW;
+    if (cb >= n) return;  // waves are independent: no workgroup barrier below
     const int y = lane & 7, z = lane >> 3;
     int gyy, gzz;
     ghost_row_yz(lane < 36 ? lane : 0, gyy, gzz);
     const int gdy = (gyy < 0) ? -1 : (gyy > 7 ? 1 : 0), gdz = (gzz < 0) ? -1 : (gzz > 7 ? 1 : 0);
     const int gly = gyy - gdy * 8, glz = gzz - gdz * 8;
-    const int32_t *row_nn = nn + c * nnn;
-    auto chunk_of = [&](int dx, int dy, int dz) -> int64_t {
-        const int e = nn_entry(type, dx, dy, dz);
-        if (e == -2) return c;
-        return (e < 0) ? -1 : (int64_t)row_nn[e];
-    };
-    // plane word z of the chunk at displacement (dx, dy, dz); 0 where the table has no entry
-    auto word = [&](int dx, int dy, int dz, int pz) -> uint64_t {
-        const int64_t nb = chunk_of(dx, dy, dz);
-        return (nb < 0) ? 0ull : eb[nb * 8 + pz];
-    };
-    const uint64_t own = *(const uint64_t *)(mask + c * 512 + lane * 8);
-    const uint64_t pc = eb[c * 8 + z], pxm = word(-1, 0, 0, z), pxp = word(1, 0, 0, z);
-    uint64_t g0 = 0, gm = 0, gp = 0;
-    if (lane < 36) {
-        g0 = word(0, gdy, gdz, glz);
-        if (type == SG_NN_FULL) {
-            gm = word(-1, gdy, gdz, glz);
-            gp = word(1, gdy, gdz, glz);
-        }
+    const bool full = type == SG_NN_FULL;
+    // table entries of this lane's ghost row chunks (dx = 0, -1, +1; -1: none)
+    const int e0 = lane < 36 ? nn_entry(type, 0, gdy, gdz) : -1;
+    const int em = (lane < 36 && full) ? nn_entry(type, -1, gdy, gdz) : -1;
+    const int ep = (lane < 36 && full) ? nn_entry(type, 1, gdy, gdz) : -1;
+    const int exm = nn_entry(type, -1, 0, 0), exp_ = nn_entry(type, 1, 0, 0);
+    // the mask rows and the chunks' table rows (entry l in lane l) first: the plane words
+    // depend on the table
+    uint64_t own[CPW], pc[CPW], pxm[CPW], pxp[CPW], g0[CPW], gm[CPW], gp[CPW];
+    int32_t rowv[CPW];
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) own[k] = (cb + k < n) ? *(const uint64_t *)(mask + (cb + k) * 512 + lane * 8) : 0ull;
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) rowv[k] = (cb + k < n && lane < nnn) ? nn[(cb + k) * nnn + lane] : -1;
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+        const int64_t c = cb + k;
+        pc[k] = pxm[k] = pxp[k] = g0[k] = gm[k] = gp[k] = 0;
+        if (c >= n) continue;
+        // uniform entries by readlane, this lane's ghost entries by a lane shuffle
+        const int32_t nxm = __builtin_amdgcn_readlane(rowv[k], exm), nxp = __builtin_amdgcn_readlane(rowv[k], exp_);
+        const int32_t n0 = __shfl(rowv[k], e0 < 0 ? 0 : e0, 64);
+        const int32_t nm = __shfl(rowv[k], em < 0 ? 0 : em, 64);
+        const int32_t np = __shfl(rowv[k], ep < 0 ? 0 : ep, 64);
+        pc[k] = eb[c * 8 + z];
+        if (nxm >= 0) pxm[k] = eb[(int64_t)nxm * 8 + z];
+        if (nxp >= 0) pxp[k] = eb[(int64_t)nxp * 8 + z];
+        if (e0 >= 0 && n0 >= 0) g0[k] = eb[(int64_t)n0 * 8 + glz];
+        if (em >= 0 && nm >= 0) gm[k] = eb[(int64_t)nm * 8 + glz];
+        if (ep >= 0 && np >= 0) gp[k] = eb[(int64_t)np * 8 + glz];
     }
-    tile[w][(z + 1) * 10 + (y + 1)] =
-        (uint16_t)(((pxm >> (8 * y + 7)) & 1u) | (((pc >> (8 * y)) & 0xFFu) << 1) | (((pxp >> (8 * y)) & 1u) << 9));
-    if (lane < 36)
-        tile[w][(gzz + 1) * 10 + (gyy + 1)] = (uint16_t)(((gm >> (8 * gly + 7)) & 1u) |
-                                                         (((g0 >> (8 * gly)) & 0xFFu) << 1) |
-                                                         (((gp >> (8 * gly)) & 1u) << 9));
+#pragma unroll
+    for (int k = 0; k < CPW; ++k) {
+        if (cb + k >= n) continue;
+        tile[w][k][(z + 1) * 10 + (y + 1)] = (uint16_t)(((pxm[k] >> (8 * y + 7)) & 1u) |
+                                                        (((pc[k] >> (8 * y)) & 0xFFu) << 1) |
+                                                        (((pxp[k] >> (8 * y)) & 1u) << 9));
+        if (lane < 36)
+            tile[w][k][(gzz + 1) * 10 + (gyy + 1)] = (uint16_t)(((gm[k] >> (8 * gly + 7)) & 1u) |
+                                                                (((g0[k] >> (8 * gly)) & 0xFFu) << 1) |
+                                                                (((gp[k] >> (8 * gly)) & 1u) << 9));
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint16_t *t = tile[w];
-    const uint32_t self = t[(z + 1) * 10 + (y + 1)];
-    uint32_t all;  // bit j: every neighbour of x = j - 1 exists
-    if (type == SG_NN_FULL) {
-        all = 0x3FFu;
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
+    for (int k = 0; k < CPW; ++k) {
+        const int64_t c = cb + k;
+        if (c >= n) continue;
+        const uint16_t *t = tile[w][k];
+        const uint32_t self = t[(z + 1) * 10 + (y + 1)];
+        uint32_t all;  // bit j: every neighbour of x = j - 1 exists
+        if (full) {
+            all = 0x3FFu;
 #pragma unroll
-            for (int dy = 0; dy < 3; ++dy) {
-                const uint32_t r = t[(z + dz) * 10 + (y + dy)];
-                all &= r & (r << 1) & (r >> 1);
-            }
-    } else {
-        all = (self << 1) & (self >> 1) & t[(z + 1) * 10 + y] & t[(z + 1) * 10 + y + 2] & t[z * 10 + y + 1] &
-              t[(z + 2) * 10 + y + 1];
+            for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) {
+                    const uint32_t r = t[(z + dz) * 10 + (y + dy)];
+                    all &= r & (r << 1) & (r >> 1);
+                }
+        } else {
+            all = (self << 1) & (self >> 1) & t[(z + 1) * 10 + y] & t[(z + 1) * 10 + y + 2] & t[z * 10 + y + 1] &
+                  t[(z + 2) * 10 + y + 1];
+        }
+        const uint32_t pad = (self >> 1) & ~(all >> 1) & 0xFFu;  // existing and not surrounded
+        const uint64_t o = own[k];
+        const uint64_t out = apply_padding(o, pad);
+        if (out != o) *(uint64_t *)(mask + c * 512 + lane * 8) = out;
     }
-    const uint32_t pad = (self >> 1) & ~(all >> 1) & 0xFFu;  // existing and not surrounded
-    uint64_t out = own;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint64_t m = (own >> (8 * i)) & 0xFFull;
-        const uint64_t nm = ((pad >> i) & 1u) ? (m | 2ull) : (m & ~2ull);
-        out = (m & 1) ? ((out & ~(0xFFull << (8 * i))) | (nm << (8 * i))) : out;
-    }
-    if (out != own) *(uint64_t *)(mask + c * 512 + lane * 8) = out;
 }
 
 // ---- 3D 8^3 27-point box ----------------------------------------------------------
@@ -1847,8 +1878,15 @@ int tag_boundaries_impl(sg_grid *g, int type)
     if (g->dim == 3 && g->edge == 8 && g->tag_variant == 0) {
         // other waves read only the bit-planes, which tagging (PADDING bits) leaves valid
         if ((rc = ebits_impl(g))) return rc;
-        hipLaunchKernelGGL((tag_boundaries8_bits_kernel<4>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, g->stream, n,
-                           g->d_nn[type], g->nnn(type), type, g->d_ebits, g->d_mask);
+        auto launch = [&](auto cpw) {
+            constexpr int CPW = decltype(cpw)::value;
+            const int64_t per_wg = 4 * CPW, per_xcd = ((n + per_wg - 1) / per_wg + 7) / 8;
+            hipLaunchKernelGGL((tag_boundaries8_bits_kernel<4, CPW>), dim3((unsigned)(per_xcd * 8)), dim3(256), 0,
+                               g->stream, n, g->d_nn[type], g->nnn(type), type, g->d_ebits, g->d_mask, per_xcd);
+        };
+        if (g->tag_cpw == 1) launch(std::integral_constant<int, 1>());
+        else if (g->tag_cpw == 2) launch(std::integral_constant<int, 2>());
+        else launch(std::integral_constant<int, 4>());
         SG_HIP(hipGetLastError());
         return SG_OK;
     }
@@ -1857,9 +1895,9 @@ int tag_boundaries_impl(sg_grid *g, int type)
         // ghost words (EXIST bits), which this kernel never changes
         auto launch = [&](auto cpw) {
             constexpr int CPW = decltype(cpw)::value;
-            const int64_t per_wg = 4 * CPW;
-            hipLaunchKernelGGL((tag_boundaries8_kernel<4, CPW>), dim3((unsigned)((n + per_wg - 1) / per_wg)), dim3(256),
-                               0, g->stream, n, g->d_nn[type], g->nnn(type), type, g->d_mask);
+            const int64_t per_wg = 4 * CPW, per_xcd = ((n + per_wg - 1) / per_wg + 7) / 8;
+            hipLaunchKernelGGL((tag_boundaries8_kernel<4, CPW>), dim3((unsigned)(per_xcd * 8)), dim3(256), 0,
+                               g->stream, n, g->d_nn[type], g->nnn(type), type, g->d_mask, per_xcd);
         };
         if (g->tag_cpw == 1) launch(std::integral_constant<int, 1>());
         else if (g->tag_cpw == 2) launch(std::integral_constant<int, 2>());

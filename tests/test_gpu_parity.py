@@ -587,11 +587,11 @@ def test_tag_and_box_8cubed_random(G, oracle, seed):
         S.assert_same_state(a, b, exist_only=False)
 
 
-@pytest.mark.parametrize("variant,cpw", [(0, 2), (1, 1), (1, 2), (1, 4)])
+@pytest.mark.parametrize("variant,cpw", [(0, 1), (0, 2), (0, 4), (1, 1), (1, 2), (1, 4)])
 @pytest.mark.parametrize("nn", [C.NN_STAR, C.NN_FULL])
 def test_tag_boundaries_chunks_per_wave(G, oracle, variant, cpw, nn):
-    """The 8^3 tagBoundaries kernels (neighbour EXIST bits from the bit-planes, or from the
-    mask rows with 1, 2 or 4 chunks per wave; masks with every bit pattern) tag the same
+    """The 8^3 tagBoundaries kernels (neighbour EXIST bits from the bit-planes or from the
+    mask rows, with 1, 2 or 4 chunks per wave; masks with every bit pattern) tag the same
     points as the oracle."""
     rng = np.random.default_rng(1100 + cpw)
     size = [int(rng.integers(60, 90)) for _ in range(3)]
