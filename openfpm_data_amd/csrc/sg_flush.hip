@@ -502,10 +502,11 @@ __global__ void place_new_keys_kernel(const int64_t *bn, int64_t nb, const int64
 // so every store stream is sequential.
 struct UnionArrays {
     int na;
-    const uint4 *src[kMaxProps + 1];
-    uint4 *dst[kMaxProps + 1];
-    uint64_t pat[kMaxProps + 1];  // fill value replicated over 8 bytes
-    int vshift[kMaxProps + 1];    // log2 of the 16-byte vectors per chunk
+    // mask, properties and the EXIST bit-planes (when kept)
+    const uint4 *src[kMaxProps + 2];
+    uint4 *dst[kMaxProps + 2];
+    uint64_t pat[kMaxProps + 2];  // fill value replicated over 8 bytes
+    int vshift[kMaxProps + 2];    // log2 of the 16-byte vectors per chunk
 };
 
 // skip (optional): output chunks written by someone else (the fused fold)
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(kWG) void build_union_kernel(UnionArrays U, const i
     const int64_t stride = (int64_t)gridDim.x * kWG;
     // constant indices into the kernel argument (a runtime index would copy it to scratch)
 #pragma unroll
-    for (int a = 0; a < kMaxProps + 1; ++a) {
+    for (int a = 0; a < kMaxProps + 2; ++a) {
         if (a >= U.na) break;
         const int sh = U.vshift[a];
         const int64_t nv = nout << sh, lane_mask = ((int64_t)1 << sh) - 1;
@@ -1017,6 +1018,14 @@ __device__ inline void write_fold_prop(int op, T bgv, const unsigned char *__res
     }
 }
 
+// EXIST bits (bit 0 of each byte) of an 8-byte mask row, byte i -> bit i: the bits sit
+// at 8i, and multiplying by 0x0102040810204080 moves bit 8i to 56 + i with every partial
+// product at its own bit position (no carries)
+__device__ inline uint32_t exist_byte(uint64_t row)
+{
+    return (uint32_t)(((row & 0x0101010101010101ull) * 0x0102040810204080ull) >> 56);
+}
+
 // One wave per touched chunk j.  Its records are a contiguous run of the sorted records,
 // in insertion order (the sort is stable).  64 at a time they are loaded with one
 // coalesced load (the next 64 are loaded while the current ones are folded) and staged in
@@ -1032,7 +1041,8 @@ __global__ __launch_bounds__(256) void fold_chunk_records_kernel(const uint4 *__
                                                                  const uint32_t *__restrict__ isnew, int64_t nnew,
                                                                  int csz, int obits, int waves, FoldState S,
                                                                  FoldProps F, RecLayout R, FusedImg I,
-                                                                 uint8_t *__restrict__ omask, int skip_fresh)
+                                                                 uint8_t *__restrict__ omask, int skip_fresh,
+                                                                 uint64_t *__restrict__ eb)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_fold[];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -1134,6 +1144,12 @@ __global__ __launch_bounds__(256) void fold_chunk_records_kernel(const uint4 *__
                 // seen bytes are 0 or 0xff: the OR of the records' masks where a record came
                 q |= ((const v4u *)am)[v] & ((const v4u *)seen)[v];
                 __builtin_nontemporal_store(q, (v4u *)dm + v);
+                if (eb) {
+                    // 8^3 chunks: the vector holds rows 2v and 2v + 1, whose EXIST bits are
+                    // bytes 2v and 2v + 1 of the chunk's 64 plane bytes
+                    const uint64_t lo = ((uint64_t)q.y << 32) | q.x, hi = ((uint64_t)q.w << 32) | q.z;
+                    ((uint16_t *)eb)[dst * 32 + v] = (uint16_t)(exist_byte(lo) | (exist_byte(hi) << 8));
+                }
             }
         } else {
             for (int s = lane; s < csz; s += 64) dm[s] = (uint8_t)((smask ? smask[s] : 0) | (seen[s] ? am[s] : 0));
@@ -1271,7 +1287,7 @@ __global__ __launch_bounds__(256) void fold_fresh_chunks_kernel(const uint4 *__r
                                                                 const int64_t *__restrict__ cpos,
                                                                 const uint32_t *__restrict__ isnew, int64_t nnew,
                                                                 int csz, int obits, int waves, FreshArgs A,
-                                                                uint8_t *__restrict__ omask)
+                                                                uint8_t *__restrict__ omask, uint64_t *__restrict__ eb)
 {
     typedef unsigned int v4u __attribute__((ext_vector_type(4)));
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_fresh[];
@@ -1380,6 +1396,12 @@ __global__ __launch_bounds__(256) void fold_fresh_chunks_kernel(const uint4 *__r
     wave_sync_lds();
     // the image out, every array's chunk as whole 16-byte vectors
     const int64_t dst = cpos[j];
+    if (eb) {
+        // 8^3 chunks: EXIST bit-plane word z has row (y, z)'s bits as byte y, i.e. byte
+        // l = 8z + y of the chunk's 64 plane bytes is lane l's row
+        const uint64_t row = *(const uint64_t *)(mimg + lane * 8);
+        ((uint8_t *)eb)[dst * 64 + lane] = exist_byte(row);
+    }
     for_props<NP>(np, [&](int q) {
         const FreshProp &P = A.p[q];
         const int nvec = csz * P.ts / 16;
@@ -1920,6 +1942,9 @@ struct MergeDefer {
     const int64_t *oldpos = nullptr;  // [nnew] old position of each touched chunk
     const uint32_t *isnew = nullptr;  // [nnew] 1: brand-new chunk
     int64_t nfresh = 0;               // brand-new chunks among the nnew
+    // in: keep the EXIST bit-planes (8^3 chunks) through the merge, the caller's fold
+    // writing those of the touched chunks; out: false when they could not be kept
+    bool planes = false;
     const uint8_t *src_mask = nullptr;
     const void *src_prop[kMaxProps] = {};
     void *free_keys = nullptr, *free_mask = nullptr, *free_prop[kMaxProps] = {};
@@ -2001,6 +2026,7 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
         UnionArrays U;
         U.na = 1 + g->np;
         bool flat = true;
+        uint64_t *neb = nullptr;
         for (int a = 0; a < U.na && flat; ++a) {
             const size_t cb = (size_t)csz * (a == 0 ? 1 : g->ts[a - 1]);
             flat = cb % 16 == 0 && ((cb / 16) & (cb / 16 - 1)) == 0;
@@ -2017,6 +2043,19 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
             memcpy(&U.pat[a], pat, 8);
             U.src[a] = (const uint4 *)(a == 0 ? (const void *)g->d_mask : g->d_prop[a - 1]);
             U.dst[a] = (uint4 *)(a == 0 ? (void *)nmask : nprop[a - 1]);
+        }
+        if (df && df->planes) {
+            // the bit-planes (64 bytes per chunk, 0 for a new one) move with the chunks
+            neb = flat ? (uint64_t *)g->pool.alloc((size_t)(ncap + 1) * 64) : nullptr;
+            if (neb) {
+                const int a = U.na++;
+                U.vshift[a] = 2;
+                U.pat[a] = 0;
+                U.src[a] = (const uint4 *)g->d_ebits;
+                U.dst[a] = (uint4 *)neb;
+            } else {
+                df->planes = false;
+            }
         }
         if (flat && df && m == 0) {
             // no old chunk: the caller's fold writes every chunk, only the background
@@ -2089,7 +2128,14 @@ int merge_chunk_set(sg_grid *g, const int64_t *nkey, int64_t nnew, int64_t **cpo
         for (int p = 0; p < g->np; ++p) g->d_prop[p] = nprop[p];
         g->nchunks = total;
         g->cap = ncap;
+        if (neb) {
+            // the old planes are read only by the union pass above (stream order)
+            g->pool.free(g->d_ebits);
+            g->d_ebits = neb;
+            g->ebits_cap = ncap + 1;
+        }
     }
+    if (df && df->planes && (!g->d_ebits || g->ebits_cap < g->nchunks + 1)) df->planes = false;
 
     *cpos_out = cpos;
     return SG_OK;
@@ -2303,8 +2349,13 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
     SG_HIP(hipGetLastError());
 
     MergeDefer df;
+    // 8^3 chunks: the EXIST bit-planes are kept up to date when they are valid (or the
+    // grid has no chunk yet): the fold writes those of the touched chunks
+    df.planes = g->dim == 3 && g->edge == 8 && (g->ebits_valid || g->nchunks == 0);
     int64_t *cpos = nullptr;
     if ((rc = merge_chunk_set(g, nkey, nnew, &cpos, &df))) return rc;
+    const bool planes = df.planes;
+    uint64_t *eb = planes ? g->d_ebits : nullptr;
     FusedImg I;
     for (int p = 0; p < g->np; ++p) {
         I.src[p] = (const unsigned char *)df.src_prop[p];
@@ -2348,7 +2399,7 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
         const uint32_t *chk = df.nfresh < nnew ? df.isnew : nullptr;
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, fgrid, fblk, (size_t)fw * A.bytes, g->stream, rec, ustart, cpos, chk, nnew, csz,
-                               obits, fw, A, g->d_mask);
+                               obits, fw, A, g->d_mask, eb);
         };
         if (R.nv == 1) {
             switch (g->np) {
@@ -2379,18 +2430,18 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
         if (R.nv == 1)
             hipLaunchKernelGGL(fold_chunk_records_kernel<1>, fgrid, fblk, (size_t)waves * S.bytes, g->stream, rec,
                                ustart, cpos, df.oldpos, df.isnew, nnew, csz, obits, waves, S, F, R, I, g->d_mask,
-                               skip_fresh);
+                               skip_fresh, eb);
         else
             hipLaunchKernelGGL(fold_chunk_records_kernel<2>, fgrid, fblk, (size_t)waves * S.bytes, g->stream, rec,
                                ustart, cpos, df.oldpos, df.isnew, nnew, csz, obits, waves, S, F, R, I, g->d_mask,
-                               skip_fresh);
+                               skip_fresh, eb);
         SG_HIP(hipGetLastError());
     }
     finish_merge(g, df);
     g->ins_n = 0;
     g->nn_valid[0] = g->nn_valid[1] = false;
     g->occ_tab_valid = false;
-    g->ebits_valid = false;
+    g->ebits_valid = planes;
     g->ep_valid = false;
     return SG_OK;
 }

@@ -163,8 +163,8 @@ struct sg_grid {
     int64_t ebits_cap = 0;
     bool ebits_valid = false;
     // 8^3 tagBoundaries: 0 neighbours' EXIST bits from the bit-planes, 1 from the mask rows
-    // (faster when the planes have to be built first, as after every flush)
-    int tag_variant = 1;
+    // (faster when the planes would have to be built first), 2 the planes when current
+    int tag_variant = 2;
 
     // existing points (data positions chunk * csz + offset) for the point-list stencil
     // mode; rebuilt when stale (tagBoundaries with CALCULATE_EXISTING_POINTS builds it)

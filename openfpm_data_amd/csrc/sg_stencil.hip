@@ -702,24 +702,29 @@ __global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_kernel(int64_t n, 
 }
 
 // ---- EXIST bit-planes of 8^3 chunks -------------------------------------------------
-// One wave per chunk, lane = row (y = l & 7, z = l >> 3): the row's 8 EXIST bits go to bits
-// 8y .. 8y + 7 of the chunk's word z, the 8 lanes of one z combine theirs by shuffles.
-// Chunk n (the background chunk) gets zero words.
+// 64 bytes per chunk: byte l = 8z + y holds the EXIST bits of row (y, z), bit x for point
+// x (so word z has bit 8y + x).  A thread per row, kEbitsRows consecutive rows each (all
+// loads first); the byte is gathered by one multiply: bit 0 of byte i sits at 8i, and
+// times 0x0102040810204080 it lands on 56 + i, every partial product at its own bit.
+// Chunk n (the background chunk) gets zero bytes.
+constexpr int kEbitsRows = 8;
 __global__ __launch_bounds__(256) void ebits_kernel(const uint8_t *__restrict__ mask, int64_t n,
                                                     uint64_t *__restrict__ eb)
 {
-    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (c > n) return;
-    const uint64_t row = (c < n) ? *(const uint64_t *)(mask + c * 512 + lane * 8) : 0ull;
-    const uint64_t w = (uint64_t)exist_bits8(row) << (8 * (lane & 7));
-    uint32_t lo = (uint32_t)w, hi = (uint32_t)(w >> 32);
+    const int64_t rows = (n + 1) * 64;
+    const int64_t r0 = (int64_t)blockIdx.x * (256 * kEbitsRows) + threadIdx.x;
+    uint64_t row[kEbitsRows];
 #pragma unroll
-    for (int s = 1; s < 8; s <<= 1) {
-        lo |= (uint32_t)__shfl_xor((int)lo, s, 64);
-        hi |= (uint32_t)__shfl_xor((int)hi, s, 64);
+    for (int k = 0; k < kEbitsRows; ++k) {
+        const int64_t r = r0 + (int64_t)k * 256;
+        row[k] = (r < n * 64) ? *(const uint64_t *)(mask + r * 8) : 0ull;
     }
-    if ((lane & 7) == 0) eb[c * 8 + (lane >> 3)] = (uint64_t)lo | ((uint64_t)hi << 32);
+#pragma unroll
+    for (int k = 0; k < kEbitsRows; ++k) {
+        const int64_t r = r0 + (int64_t)k * 256;
+        if (r < rows)
+            ((uint8_t *)eb)[r] = (uint8_t)(((row[k] & 0x0101010101010101ull) * 0x0102040810204080ull) >> 56);
+    }
 }
 
 // 8^3 tagBoundaries reading the neighbours' EXIST bits from the bit-planes (tagBoundaries,
@@ -1875,7 +1880,7 @@ int tag_boundaries_impl(sg_grid *g, int type)
     g->occ_tab_valid = false;
     int rc = ensure_nn(g, type);
     if (rc) return rc;
-    if (g->dim == 3 && g->edge == 8 && g->tag_variant == 0) {
+    if (g->dim == 3 && g->edge == 8 && (g->tag_variant == 0 || (g->tag_variant == 2 && g->ebits_valid))) {
         // other waves read only the bit-planes, which tagging (PADDING bits) leaves valid
         if ((rc = ebits_impl(g))) return rc;
         auto launch = [&](auto cpw) {
@@ -2188,7 +2193,8 @@ int ebits_impl(sg_grid *g)
         }
         g->ebits_cap = cap;
     }
-    hipLaunchKernelGGL(ebits_kernel, dim3((unsigned)(n / 4 + 1)), dim3(256), 0, g->stream, g->d_mask, n, g->d_ebits);
+    hipLaunchKernelGGL(ebits_kernel, dim3(grid_1d((n + 1) * 64, 256 * kEbitsRows)), dim3(256), 0, g->stream, g->d_mask,
+                       n, g->d_ebits);
     SG_HIP(hipGetLastError());
     g->ebits_valid = true;
     return SG_OK;

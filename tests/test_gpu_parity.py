@@ -635,6 +635,21 @@ def test_tag_boundaries_after_mask_changes(G, oracle, nn):
         g.insert_coords(pts, [vals])
         g.flush(["smax"])
     tag_and_compare()
+    # a flush into existing chunks only (no chunk added: planes rewritten in place), with
+    # new points beside the existing ones
+    ex = S.existing_points(grids[0])
+    near = np.clip(ex[::7] + rng.integers(-1, 2, (len(ex[::7]), 3)), 0, np.array(size) - 1)
+    same = (near // 8 == ex[::7] // 8).all(axis=1)
+    near = near[same].astype(np.int32)
+    nvals = rng.integers(-100, 100, len(near)).astype(np.int32)
+    for g in grids:
+        g.insert_coords(near, [nvals])
+        g.flush(["smax"])
+    tag_and_compare()
+    for variant in (0, 1):
+        grids[0].set_tuning(C.TUNE_TAG_VARIANT, variant)
+        tag_and_compare()
+    grids[0].set_tuning(C.TUNE_TAG_VARIANT, 2)
     keys = grids[0].lin_id(S.existing_points(grids[0])[::3])
     for g in grids:
         g.remove_points(keys)
