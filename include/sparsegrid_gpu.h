@@ -294,10 +294,10 @@ int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, s
  * chunk's records in LDS where chunk ids fit 32 bits and a record 32 bytes; 1 always
  * sorts point keys.  Both give the same result. */
 #define SG_TUNE_FLUSH_PATH 8
-/* 8^3 tagBoundaries: 0 reads the neighbours' EXIST bits from per-chunk bit-planes (64 bytes
- * per chunk: kept current by the flush of point records grouped by chunk, rebuilt after
- * other mask changes); 1 from their mask rows; 2 (default) the bit-planes when they are
- * current, else the mask rows.  Same result. */
+/* 8^3 tagBoundaries: 0 (default) works on per-chunk EXIST bit-planes (64 bytes per chunk,
+ * one 64-bit word per z slice: kept current by the flush of point records grouped by
+ * chunk, rebuilt after other mask changes); 1 reads the neighbours' mask rows.  Same
+ * result. */
 #define SG_TUNE_TAG_VARIANT 9
 int sg_set_tuning(sg_grid *g, int key, int64_t value);
 

@@ -427,7 +427,7 @@ int sg_set_tuning(sg_grid *g, int key, int64_t value)
         return SG_OK;
     }
     if (key == SG_TUNE_TAG_VARIANT) {
-        if (value < 0 || value > 2) return fail(SG_E_RANGE, "tagBoundaries variant must be 0, 1 or 2");
+        if (value != 0 && value != 1) return fail(SG_E_RANGE, "tagBoundaries variant must be 0 or 1");
         g->tag_variant = (int)value;
         return SG_OK;
     }

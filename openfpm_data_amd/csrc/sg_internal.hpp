@@ -162,9 +162,9 @@ struct sg_grid {
     uint64_t *d_ebits = nullptr;
     int64_t ebits_cap = 0;
     bool ebits_valid = false;
-    // 8^3 tagBoundaries: 0 neighbours' EXIST bits from the bit-planes, 1 from the mask rows
-    // (faster when the planes would have to be built first), 2 the planes when current
-    int tag_variant = 2;
+    // 8^3 tagBoundaries: 0 on the EXIST bit-planes (built first when stale), 1 on the
+    // neighbours' mask rows
+    int tag_variant = 0;
 
     // existing points (data positions chunk * csz + offset) for the point-list stencil
     // mode; rebuilt when stale (tagBoundaries with CALCULATE_EXISTING_POINTS builds it)

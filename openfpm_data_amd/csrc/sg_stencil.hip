@@ -727,102 +727,111 @@ __global__ __launch_bounds__(256) void ebits_kernel(const uint8_t *__restrict__ 
     }
 }
 
-// 8^3 tagBoundaries reading the neighbours' EXIST bits from the bit-planes (tagBoundaries,
-// SparseGridGpu_kernels.cuh:667-737; same tile and decision as tag_boundaries8_kernel): a
-// lane's row word and, for lanes 0-35, one ghost row word of the 10 x 10 tile come from
-// 8-byte plane words of the chunk and its neighbours, which share 64-byte lines, instead
-// of mask rows and single mask bytes of the neighbour chunks.  A wave takes CPW
-// consecutive chunks; their neighbour-table rows come by scalar loads and each lane picks
-// its ghost rows' chunks from them by selects, so every plane word of the wave's chunks
-// is loaded in one step, with no dependent load.
-template <int WAVES, int CPW>
-__global__ __launch_bounds__(64 * WAVES) void tag_boundaries8_bits_kernel(int64_t n, const int32_t *__restrict__ nn,
-                                                                         int nnn, int type,
-                                                                         const uint64_t *__restrict__ eb,
-                                                                         uint8_t *__restrict__ mask,
-                                                                         int64_t per_xcd)
+// ---- 8^3 tagBoundaries on whole plane words -----------------------------------------
+// Plane word z of a chunk holds its slice z: bit 8y + x = EXIST of point (x, y, z).  The
+// neighbours of every point of a slice in x and y are shifts of the word, filled in at
+// the slice's edges from the same word of the x / y neighbour chunks:
+//   xlo(C, L): E(x - 1, y) (x = 0 from L's x = 7), xhi(C, R): E(x + 1, y),
+//   ylo(C, M): E(x, y - 1) (y = 0 from M's y = 7), yhi(C, P): E(x, y + 1).
+__device__ inline uint64_t pw_xlo(uint64_t c, uint64_t l)
 {
-    __shared__ uint16_t tile[WAVES][CPW][100];
+    return ((c << 1) & 0xFEFEFEFEFEFEFEFEull) | ((l >> 7) & 0x0101010101010101ull);
+}
+__device__ inline uint64_t pw_xhi(uint64_t c, uint64_t r)
+{
+    return ((c >> 1) & 0x7F7F7F7F7F7F7F7Full) | ((r << 7) & 0x8080808080808080ull);
+}
+__device__ inline uint64_t pw_ylo(uint64_t c, uint64_t m) { return (c << 8) | (m >> 56); }
+__device__ inline uint64_t pw_yhi(uint64_t c, uint64_t p) { return (c >> 8) | (p << 56); }
+
+// A wave takes 6 consecutive chunks; lane l < 60 -> chunk k = l / 10 and slice zz =
+// l % 10 - 1 in [-1, 8] (slices -1 and 8 are slice 7 of the chunk below and slice 0 of the
+// chunk above).  Each lane loads the plane words of its slice and its x / y neighbour
+// chunks (table entries and words in two dependent steps), and reduces them to S = AND
+// over the 3 x 3 (x, y) neighbourhood (FULL) or to its star in the slice (STAR); slices
+// z - 1 and z + 1 come from lanes l -/+ 1.  Not surrounded = E & ~(S(z-1) & S(z) &
+// S(z+1)), or E & ~(star & E(z-1) & E(z+1)).  Then, lane = row, the chunks' mask rows
+// get their PADDING bits (apply_padding).  Same result as tag_boundaries8_kernel.
+template <int TYPE>
+__global__ __launch_bounds__(256) void tag_planes_kernel(int64_t n, const int32_t *__restrict__ nn, int nnn,
+                                                        const uint64_t *__restrict__ eb, uint8_t *__restrict__ mask,
+                                                        int64_t per_xcd)
+{
+    constexpr int CPW = 6;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    // workgroup b runs in XCD b & 7, which walks its own contiguous range of workgroups'
-    // chunks: neighbours' plane words then mostly come from the same XCD's L2
+    // workgroup b runs in XCD b & 7, which walks its own contiguous range of the chunks
     const int64_t wg = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    const int64_t cb = (wg * WAVES + w) * CPW;
-    if (cb >= n) return;  // waves are independent: no workgroup barrier below
-    const int y = lane & 7, z = lane >> 3;
-    int gyy, gzz;
-    ghost_row_yz(lane < 36 ? lane : 0, gyy, gzz);
-    const int gdy = (gyy < 0) ? -1 : (gyy > 7 ? 1 : 0), gdz = (gzz < 0) ? -1 : (gzz > 7 ? 1 : 0);
-    const int gly = gyy - gdy * 8, glz = gzz - gdz * 8;
-    const bool full = type == SG_NN_FULL;
-    // table entries of this lane's ghost row chunks (dx = 0, -1, +1; -1: none)
-    const int e0 = lane < 36 ? nn_entry(type, 0, gdy, gdz) : -1;
-    const int em = (lane < 36 && full) ? nn_entry(type, -1, gdy, gdz) : -1;
-    const int ep = (lane < 36 && full) ? nn_entry(type, 1, gdy, gdz) : -1;
-    const int exm = nn_entry(type, -1, 0, 0), exp_ = nn_entry(type, 1, 0, 0);
-    // the mask rows and the chunks' table rows (entry l in lane l) first: the plane words
-    // depend on the table
-    uint64_t own[CPW], pc[CPW], pxm[CPW], pxp[CPW], g0[CPW], gm[CPW], gp[CPW];
-    int32_t rowv[CPW];
+    const int64_t cb = (wg * 4 + w) * CPW;
+    if (cb >= n) return;  // waves are independent: no workgroup barrier
+    // the chunks' mask rows (lane = row), used last, loaded first
+    uint64_t own[CPW];
 #pragma unroll
-    for (int k = 0; k < CPW; ++k) own[k] = (cb + k < n) ? *(const uint64_t *)(mask + (cb + k) * 512 + lane * 8) : 0ull;
+    for (int q = 0; q < CPW; ++q)
+        own[q] = (cb + q < n) ? *(const uint64_t *)(mask + (cb + q) * 512 + lane * 8) : 0ull;
+    const int k = lane / 10, s = lane - 10 * k, zz = s - 1;
+    const int64_t c = cb + k;
+    const bool act = lane < 10 * CPW && c < n;
+    const int dz = zz < 0 ? -1 : (zz > 7 ? 1 : 0), wz = zz & 7;
+    const int32_t *row = nn + c * nnn;
+    uint64_t S, E;
+    if (TYPE == SG_NN_FULL) {
+        // chunks (dx, dy, dz): entry (dx + 1) + 3 (dy + 1) + 9 (dz + 1); the chunk itself at
+        // 13.  The three entries of one dy are consecutive: one 12-byte load each.
+        int64_t nb[3][3];
 #pragma unroll
-    for (int k = 0; k < CPW; ++k) rowv[k] = (cb + k < n && lane < nnn) ? nn[(cb + k) * nnn + lane] : -1;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int32_t t3[3] = {-1, -1, -1};
+            if (act) __builtin_memcpy(t3, row + 9 * (dz + 1) + 3 * (dy + 1), 12);
 #pragma unroll
-    for (int k = 0; k < CPW; ++k) {
-        const int64_t c = cb + k;
-        pc[k] = pxm[k] = pxp[k] = g0[k] = gm[k] = gp[k] = 0;
-        if (c >= n) continue;
-        // uniform entries by readlane, this lane's ghost entries by a lane shuffle
-        const int32_t nxm = __builtin_amdgcn_readlane(rowv[k], exm), nxp = __builtin_amdgcn_readlane(rowv[k], exp_);
-        const int32_t n0 = __shfl(rowv[k], e0 < 0 ? 0 : e0, 64);
-        const int32_t nm = __shfl(rowv[k], em < 0 ? 0 : em, 64);
-        const int32_t np = __shfl(rowv[k], ep < 0 ? 0 : ep, 64);
-        pc[k] = eb[c * 8 + z];
-        if (nxm >= 0) pxm[k] = eb[(int64_t)nxm * 8 + z];
-        if (nxp >= 0) pxp[k] = eb[(int64_t)nxp * 8 + z];
-        if (e0 >= 0 && n0 >= 0) g0[k] = eb[(int64_t)n0 * 8 + glz];
-        if (em >= 0 && nm >= 0) gm[k] = eb[(int64_t)nm * 8 + glz];
-        if (ep >= 0 && np >= 0) gp[k] = eb[(int64_t)np * 8 + glz];
-    }
-#pragma unroll
-    for (int k = 0; k < CPW; ++k) {
-        if (cb + k >= n) continue;
-        tile[w][k][(z + 1) * 10 + (y + 1)] = (uint16_t)(((pxm[k] >> (8 * y + 7)) & 1u) |
-                                                        (((pc[k] >> (8 * y)) & 0xFFu) << 1) |
-                                                        (((pxp[k] >> (8 * y)) & 1u) << 9));
-        if (lane < 36)
-            tile[w][k][(gzz + 1) * 10 + (gyy + 1)] = (uint16_t)(((gm[k] >> (8 * gly + 7)) & 1u) |
-                                                                (((g0[k] >> (8 * gly)) & 0xFFu) << 1) |
-                                                                (((gp[k] >> (8 * gly)) & 1u) << 9));
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-    for (int k = 0; k < CPW; ++k) {
-        const int64_t c = cb + k;
-        if (c >= n) continue;
-        const uint16_t *t = tile[w][k];
-        const uint32_t self = t[(z + 1) * 10 + (y + 1)];
-        uint32_t all;  // bit j: every neighbour of x = j - 1 exists
-        if (full) {
-            all = 0x3FFu;
-#pragma unroll
-            for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-                for (int dy = 0; dy < 3; ++dy) {
-                    const uint32_t r = t[(z + dz) * 10 + (y + dy)];
-                    all &= r & (r << 1) & (r >> 1);
-                }
-        } else {
-            all = (self << 1) & (self >> 1) & t[(z + 1) * 10 + y] & t[(z + 1) * 10 + y + 2] & t[z * 10 + y + 1] &
-                  t[(z + 2) * 10 + y + 1];
+            for (int dx = -1; dx <= 1; ++dx)
+                nb[dy + 1][dx + 1] = (dz == 0 && dy == 0 && dx == 0) ? (act ? c : -1) : (int64_t)t3[dx + 1];
         }
-        const uint32_t pad = (self >> 1) & ~(all >> 1) & 0xFFu;  // existing and not surrounded
-        const uint64_t o = own[k];
-        const uint64_t out = apply_padding(o, pad);
-        if (out != o) *(uint64_t *)(mask + c * 512 + lane * 8) = out;
+        uint64_t W[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) W[r][x] = nb[r][x] >= 0 ? eb[nb[r][x] * 8 + wz] : 0ull;
+        uint64_t X[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) X[r] = W[r][1] & pw_xlo(W[r][1], W[r][0]) & pw_xhi(W[r][1], W[r][2]);
+        S = X[1] & pw_ylo(X[1], X[0]) & pw_yhi(X[1], X[2]);
+        E = W[1][1];
+    } else {
+        // star table: -x +x -y +y -z +z; slices -1 / 8 only need the word of chunk (0, 0, dz)
+        const int64_t self = !act ? -1 : (dz == 0 ? c : (int64_t)row[dz < 0 ? 4 : 5]);
+        int64_t f[4] = {-1, -1, -1, -1};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) f[q] = (act && dz == 0) ? (int64_t)row[q] : -1;
+        E = self >= 0 ? eb[self * 8 + wz] : 0ull;
+        uint64_t F[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) F[q] = f[q] >= 0 ? eb[f[q] * 8 + wz] : 0ull;
+        S = E & pw_xlo(E, F[0]) & pw_xhi(E, F[1]) & pw_ylo(E, F[2]) & pw_yhi(E, F[3]);
+    }
+    // slices z - 1 and z + 1 from the neighbour lanes (valid for the slices 0..7 of a chunk)
+    const uint32_t slo = (uint32_t)S, shi = (uint32_t)(S >> 32), elo = (uint32_t)E, ehi = (uint32_t)(E >> 32);
+    uint64_t below, above;
+    if (TYPE == SG_NN_FULL) {
+        below = (uint64_t)(uint32_t)__shfl_up((int)slo, 1, 64) | ((uint64_t)(uint32_t)__shfl_up((int)shi, 1, 64) << 32);
+        above = (uint64_t)(uint32_t)__shfl_down((int)slo, 1, 64) |
+                ((uint64_t)(uint32_t)__shfl_down((int)shi, 1, 64) << 32);
+    } else {
+        below = (uint64_t)(uint32_t)__shfl_up((int)elo, 1, 64) | ((uint64_t)(uint32_t)__shfl_up((int)ehi, 1, 64) << 32);
+        above = (uint64_t)(uint32_t)__shfl_down((int)elo, 1, 64) |
+                ((uint64_t)(uint32_t)__shfl_down((int)ehi, 1, 64) << 32);
+    }
+    const uint64_t pad = E & ~(S & below & above);
+    // lane = row (y, z) = (l & 7, l >> 3) of chunk q: its slice's pad word is lane
+    // q * 10 + z + 1's
+    const uint32_t plo = (uint32_t)pad, phi = (uint32_t)(pad >> 32);
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+        const int src = q * 10 + (lane >> 3) + 1;
+        const uint64_t pw = (uint64_t)(uint32_t)__shfl((int)plo, src, 64) |
+                            ((uint64_t)(uint32_t)__shfl((int)phi, src, 64) << 32);
+        if (cb + q >= n) continue;
+        const uint64_t out = apply_padding(own[q], (uint32_t)(pw >> (8 * (lane & 7))) & 0xFFu);
+        if (out != own[q]) *(uint64_t *)(mask + (cb + q) * 512 + lane * 8) = out;
     }
 }
 
@@ -1880,18 +1889,17 @@ int tag_boundaries_impl(sg_grid *g, int type)
     g->occ_tab_valid = false;
     int rc = ensure_nn(g, type);
     if (rc) return rc;
-    if (g->dim == 3 && g->edge == 8 && (g->tag_variant == 0 || (g->tag_variant == 2 && g->ebits_valid))) {
+    if (g->dim == 3 && g->edge == 8 && g->tag_variant == 0) {
         // other waves read only the bit-planes, which tagging (PADDING bits) leaves valid
         if ((rc = ebits_impl(g))) return rc;
-        auto launch = [&](auto cpw) {
-            constexpr int CPW = decltype(cpw)::value;
-            const int64_t per_wg = 4 * CPW, per_xcd = ((n + per_wg - 1) / per_wg + 7) / 8;
-            hipLaunchKernelGGL((tag_boundaries8_bits_kernel<4, CPW>), dim3((unsigned)(per_xcd * 8)), dim3(256), 0,
-                               g->stream, n, g->d_nn[type], g->nnn(type), type, g->d_ebits, g->d_mask, per_xcd);
-        };
-        if (g->tag_cpw == 1) launch(std::integral_constant<int, 1>());
-        else if (g->tag_cpw == 2) launch(std::integral_constant<int, 2>());
-        else launch(std::integral_constant<int, 4>());
+        const int64_t per_xcd = ((n + 23) / 24 + 7) / 8;
+        const unsigned grid = (unsigned)(per_xcd * 8);
+        if (type == SG_NN_FULL)
+            hipLaunchKernelGGL((tag_planes_kernel<SG_NN_FULL>), dim3(grid), dim3(256), 0, g->stream, n, g->d_nn[type],
+                               g->nnn(type), g->d_ebits, g->d_mask, per_xcd);
+        else
+            hipLaunchKernelGGL((tag_planes_kernel<SG_NN_STAR>), dim3(grid), dim3(256), 0, g->stream, n, g->d_nn[type],
+                               g->nnn(type), g->d_ebits, g->d_mask, per_xcd);
         SG_HIP(hipGetLastError());
         return SG_OK;
     }
