@@ -287,8 +287,8 @@ int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, s
  * (rebuilt after any mask change); 3 as 2 for the ghost rows only; 4 as 2 with each row's
  * mask read from two 64-bit planes per chunk built with the table */
 #define SG_TUNE_STENCIL_SKIP_EMPTY_ROWS 4
-/* 8^3 tagBoundaries: chunks per wave, 1, 2 (default) or 4; a wave issues the loads of all
- * its chunks before tagging any */
+/* 8^3 tagBoundaries on the mask rows (SG_TUNE_TAG_VARIANT 1): chunks per wave, 1, 2
+ * (default) or 4; a wave issues the loads of all its chunks before tagging any */
 #define SG_TUNE_TAG_CHUNKS_PER_WAVE 7
 /* flush of point records: 0 (default) sorts (chunk id, record) pairs and folds each
  * chunk's records in LDS where chunk ids fit 32 bits and a record 32 bytes; 1 always
