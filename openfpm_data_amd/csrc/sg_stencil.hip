@@ -940,19 +940,42 @@ __device__ inline void box_finish(const BoxIn<T> &in, int64_t c, T *tw, T *__res
             // instead of once per tap
 #pragma unroll
             for (int x = 0; x < 8; ++x) isum[x] = 0;
+            bool uni = true;  // every tap has the same coefficient (uniform across the wave)
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
+            for (int q = 1; q < 27; ++q) uni = uni && cf[q] == cf[0];
+            if (uni) {
+                // c (a + b + ...) = c a + c b + ... in wrapping arithmetic: the 27 values are
+                // summed (row triples first) and multiplied once, no multiply per tap
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const T *row = tw + ((z + k) * 10 + (y + j)) * 10;
-                    T r[10];
+                for (int j = 0; j < 3; ++j)
 #pragma unroll
-                    for (int q = 0; q < 10; ++q) r[q] = row[q];
+                    for (int k = 0; k < 3; ++k) {
+                        const T *row = tw + ((z + k) * 10 + (y + j)) * 10;
+                        T r[10];
 #pragma unroll
-                    for (int i = 0; i < 3; ++i)
+                        for (int q = 0; q < 10; ++q) r[q] = row[q];
 #pragma unroll
-                        for (int x = 0; x < 8; ++x) isum[x] = dev_add<T>(isum[x], dev_mul<T>(r[x + i], cf[i * 9 + j * 3 + k]));
-                }
+                        for (int x = 0; x < 8; ++x)
+                            isum[x] = dev_add<T>(isum[x], dev_add<T>(dev_add<T>(r[x], r[x + 1]), r[x + 2]));
+                    }
+#pragma unroll
+                for (int x = 0; x < 8; ++x) isum[x] = dev_mul<T>(isum[x], cf[0]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const T *row = tw + ((z + k) * 10 + (y + j)) * 10;
+                        T r[10];
+#pragma unroll
+                        for (int q = 0; q < 10; ++q) r[q] = row[q];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+#pragma unroll
+                            for (int x = 0; x < 8; ++x)
+                                isum[x] = dev_add<T>(isum[x], dev_mul<T>(r[x + i], cf[i * 9 + j * 3 + k]));
+                    }
+            }
         }
         bool act[8];
         bool whole = true;

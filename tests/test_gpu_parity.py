@@ -578,12 +578,15 @@ def test_tag_and_box_8cubed_random(G, oracle, seed):
     S.assert_same_state(a, b)  # masks after tagging
     for p in range(2):
         b.set_prop(p, a.prop(p))  # PADDING-only slots have no specified value after a flush
-    for apply, box in [(C.APPLY_EXISTING, None), (C.APPLY_INTERIOR, (lo, hi)), (C.APPLY_EXISTING, (lo, hi))]:
+    # random taps, and one coefficient for all taps (the integer kernel then sums the 27
+    # values and multiplies once)
+    for cf, (apply, box) in [(coef, (C.APPLY_EXISTING, None)), (coef, (C.APPLY_INTERIOR, (lo, hi))),
+                             (coef, (C.APPLY_EXISTING, (lo, hi))), (np.full(27, -3), (C.APPLY_EXISTING, None))]:
         for g in grids:
             if integer:
-                g.apply_stencil(C.make_desc(C.ST_BOX, [0], [1], apply, box, (), [int(v) for v in coef]))
+                g.apply_stencil(C.make_desc(C.ST_BOX, [0], [1], apply, box, (), [int(v) for v in cf]))
             else:
-                g.apply_stencil(C.make_desc(C.ST_BOX, [0], [1], apply, box, [float(v) for v in coef]))
+                g.apply_stencil(C.make_desc(C.ST_BOX, [0], [1], apply, box, [float(v) for v in cf]))
         S.assert_same_state(a, b, exist_only=False)
 
 
