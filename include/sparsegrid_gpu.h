@@ -299,6 +299,9 @@ int sg_poisson_solve(sg_grid *const *grids, int n, const sg_poisson_params *p, s
  * chunk, rebuilt after other mask changes); 1 reads the neighbours' mask rows.  Same
  * result. */
 #define SG_TUNE_TAG_VARIANT 9
+/* 8^3 27-point box: 1 (default) two consecutive chunks per wave, sharing the x ghosts
+ * between them when they are x neighbours; 0 one chunk per wave.  Same result. */
+#define SG_TUNE_BOX_PAIRS 10
 int sg_set_tuning(sg_grid *g, int key, int64_t value);
 
 /* ---- queries ---------------------------------------------------------------- */

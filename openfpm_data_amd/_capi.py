@@ -44,6 +44,7 @@ TUNE_STENCIL_SKIP_EMPTY_ROWS = 4
 TUNE_TAG_CHUNKS_PER_WAVE = 7
 TUNE_FLUSH_PATH = 8
 TUNE_TAG_VARIANT = 9
+TUNE_BOX_PAIRS = 10
 TUNE_BATCH_WAVES = 5
 TUNE_BATCH_MAX_WGS = 6
 COPY_NONTEMPORAL = 1

@@ -426,6 +426,11 @@ int sg_set_tuning(sg_grid *g, int key, int64_t value)
         g->fast_wmode = (int)value;
         return SG_OK;
     }
+    if (key == SG_TUNE_BOX_PAIRS) {
+        if (value != 0 && value != 1) return fail(SG_E_RANGE, "box pairing must be 0 or 1");
+        g->box_pairs = (int)value;
+        return SG_OK;
+    }
     if (key == SG_TUNE_TAG_VARIANT) {
         if (value != 0 && value != 1) return fail(SG_E_RANGE, "tagBoundaries variant must be 0 or 1");
         g->tag_variant = (int)value;

@@ -165,6 +165,8 @@ struct sg_grid {
     // 8^3 tagBoundaries: 0 on the EXIST bit-planes (built first when stale), 1 on the
     // neighbours' mask rows
     int tag_variant = 0;
+    // 8^3 27-point box: two consecutive chunks per wave, x ghosts between them shared
+    int box_pairs = 1;
 
     // existing points (data positions chunk * csz + offset) for the point-list stencil
     // mode; rebuilt when stale (tagBoundaries with CALCULATE_EXISTING_POINTS builds it)

@@ -852,7 +852,8 @@ template <typename T> struct BoxIn {
 
 template <typename T>
 __device__ inline void box_load(BoxIn<T> &in, int64_t c, const int32_t *__restrict__ nn,
-                                const uint8_t *__restrict__ mask, const T *__restrict__ src, int lane)
+                                const uint8_t *__restrict__ mask, const T *__restrict__ src, int lane,
+                                bool skip_xm = false, bool skip_xp = false)
 {
     const int32_t *row_nn = nn + c * 27;
     auto chunk_of = [&](int dx, int dy, int dz) -> int64_t {
@@ -862,8 +863,9 @@ __device__ inline void box_load(BoxIn<T> &in, int64_t c, const int32_t *__restri
     const T *sr = src + c * 512 + lane * 8;
 #pragma unroll
     for (int i = 0; i < 8; ++i) in.own[i] = sr[i];
-    in.oxm = src[chunk_of(-1, 0, 0) * 512 + lane * 8 + 7];
-    in.oxp = src[chunk_of(1, 0, 0) * 512 + lane * 8 + 0];
+    // skip_xm / skip_xp: the -x / +x ghosts come from a chunk the caller holds
+    if (!skip_xm) in.oxm = src[chunk_of(-1, 0, 0) * 512 + lane * 8 + 7];
+    if (!skip_xp) in.oxp = src[chunk_of(1, 0, 0) * 512 + lane * 8 + 0];
     if (lane < 36) {
         int yy, zz;
         ghost_row_yz(lane, yy, zz);
@@ -872,14 +874,14 @@ __device__ inline void box_load(BoxIn<T> &in, int64_t c, const int32_t *__restri
         const T *gs = src + chunk_of(0, dy, dz) * 512 + lrow * 8;
 #pragma unroll
         for (int i = 0; i < 8; ++i) in.gr[i] = gs[i];
-        in.gxm = src[chunk_of(-1, dy, dz) * 512 + lrow * 8 + 7];
-        in.gxp = src[chunk_of(1, dy, dz) * 512 + lrow * 8 + 0];
+        if (!skip_xm) in.gxm = src[chunk_of(-1, dy, dz) * 512 + lrow * 8 + 7];
+        if (!skip_xp) in.gxp = src[chunk_of(1, dy, dz) * 512 + lrow * 8 + 0];
     }
 }
 
 // Stages chunk c's 10 x 10 x 10 tile from `in`, sums the 3 x 3 x 3 box of each of the
 // lane's 8 points and writes the chunk back through the tile.
-template <typename T>
+template <typename T, bool UNI>
 __device__ inline void box_finish(const BoxIn<T> &in, int64_t c, T *tw, T *__restrict__ dst,
                                   const int64_t *__restrict__ keys, int64_t bsz0, int64_t bsz1, T bg_dst,
                                   const StencilArgs &sa, int lane)
@@ -940,12 +942,10 @@ __device__ inline void box_finish(const BoxIn<T> &in, int64_t c, T *tw, T *__res
             // instead of once per tap
 #pragma unroll
             for (int x = 0; x < 8; ++x) isum[x] = 0;
-            bool uni = true;  // every tap has the same coefficient (uniform across the wave)
-#pragma unroll
-            for (int q = 1; q < 27; ++q) uni = uni && cf[q] == cf[0];
-            if (uni) {
-                // c (a + b + ...) = c a + c b + ... in wrapping arithmetic: the 27 values are
-                // summed (row triples first) and multiplied once, no multiply per tap
+            if constexpr (UNI) {
+                // every tap has the same coefficient: c (a + b + ...) = c a + c b + ... in
+                // wrapping arithmetic, so the 27 values are summed (row triples first) and
+                // multiplied once, no multiply per tap
 #pragma unroll
                 for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -1032,7 +1032,7 @@ __device__ inline void box_finish(const BoxIn<T> &in, int64_t c, T *tw, T *__res
 // sums its 8 points over the 3 x 3 x 3 box in stencil_generic_kernel's order (x offset
 // outermost, z innermost).  Rows whose points are all active or empty are stored whole
 // (empty points take the destination background); other rows point by point.
-template <typename T, int WAVES>
+template <typename T, int WAVES, bool UNI>
 __global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32_t *__restrict__ nn,
                                                          const uint8_t *__restrict__ mask, const T *__restrict__ src,
                                                          T *__restrict__ dst, const int64_t *__restrict__ keys,
@@ -1045,7 +1045,49 @@ __global__ __launch_bounds__(64 * WAVES) void box8_kernel(int64_t n, const int32
     if (c >= n) return;  // waves are independent
     BoxIn<T> a;
     box_load<T>(a, c, nn, mask, src, lane);
-    box_finish<T>(a, c, tile[w], dst, keys, bsz0, bsz1, bg_dst, sa, lane);
+    box_finish<T, UNI>(a, c, tile[w], dst, keys, bsz0, bsz1, bg_dst, sa, lane);
+}
+
+// Two consecutive chunks c0 = 2p, c1 = 2p + 1 per wave.  When c1 is c0's +x neighbour (the
+// table says so; consecutive chunk ids along x sort next to each other), the x ghosts
+// between them are not gathered: c0's +x ghosts of its own and ghost rows are x = 0 of
+// c1's same rows, c1's -x ghosts x = 7 of c0's, all in the lane's own registers (a ghost
+// row of c1 comes from the chunk at (1, dy, dz) from c0, whose row c0's +x ghost is).
+// Those gathers of single values from 64 or 36 rows each are most of the kernel's
+// texture-addresser work.  Otherwise each chunk gathers its own.
+template <typename T, int WAVES, bool UNI>
+__global__ __launch_bounds__(64 * WAVES) void box8_pair_kernel(int64_t n, const int32_t *__restrict__ nn,
+                                                              const uint8_t *__restrict__ mask,
+                                                              const T *__restrict__ src, T *__restrict__ dst,
+                                                              const int64_t *__restrict__ keys, int64_t bsz0,
+                                                              int64_t bsz1, T bg_dst, StencilArgs sa)
+{
+    __shared__ T tile[WAVES][100 * 10];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t c0 = ((int64_t)blockIdx.x * WAVES + w) * 2, c1 = c0 + 1;
+    if (c0 >= n) return;  // waves are independent
+    const bool two = c1 < n;
+    // full table entry (dx + 1) + 3 (dy + 1) + 9 (dz + 1) of (+1, 0, 0): 14
+    const bool adj = two && nn[c0 * 27 + 14] == (int32_t)c1;
+    BoxIn<T> a, b;
+    box_load<T>(a, c0, nn, mask, src, lane, false, adj);
+    if (two) box_load<T>(b, c1, nn, mask, src, lane, adj, false);
+    if (adj) {
+        a.oxp = b.own[0];
+        b.oxm = a.own[7];
+        if (lane < 36) {
+            a.gxp = b.gr[0];
+            b.gxm = a.gr[7];
+        }
+    }
+    box_finish<T, UNI>(a, c0, tile[w], dst, keys, bsz0, bsz1, bg_dst, sa, lane);
+    if (two) {
+        // the tile is staged again: every lane's reads of the first chunk come first
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        box_finish<T, UNI>(b, c1, tile[w], dst, keys, bsz0, bsz1, bg_dst, sa, lane);
+    }
 }
 
 // ---- 3D 8^3 star fast path ------------------------------------------------------
@@ -2101,9 +2143,22 @@ int apply_stencil_impl(sg_grid *g, const sg_stencil_desc *d)
             if constexpr (sizeof(T) >= 4) {
                 T bgd;
                 memcpy(&bgd, g->bg[d->dst[0]], sizeof(T));
-                hipLaunchKernelGGL((box8_kernel<T, 4>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, g->stream, n,
-                                   g->d_nn[type], g->d_mask, (const T *)g->d_prop[d->src[0]],
-                                   (T *)g->d_prop[d->dst[0]], g->d_keys, g->bsz[0], g->bsz[1], bgd, sa);
+                // integer boxes whose 27 coefficients are equal (as T) sum first, multiply once
+                bool uni = std::is_integral<T>::value;
+                for (int q = 1; q < 27 && uni; ++q) uni = (T)d->icoef[q] == (T)d->icoef[0];
+                auto launch = [&](auto u) {
+                    constexpr bool U = decltype(u)::value;
+                    if (g->box_pairs)
+                        hipLaunchKernelGGL((box8_pair_kernel<T, 4, U>), dim3((unsigned)((n + 7) / 8)), dim3(256), 0,
+                                           g->stream, n, g->d_nn[type], g->d_mask, (const T *)g->d_prop[d->src[0]],
+                                           (T *)g->d_prop[d->dst[0]], g->d_keys, g->bsz[0], g->bsz[1], bgd, sa);
+                    else
+                        hipLaunchKernelGGL((box8_kernel<T, 4, U>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0,
+                                           g->stream, n, g->d_nn[type], g->d_mask, (const T *)g->d_prop[d->src[0]],
+                                           (T *)g->d_prop[d->dst[0]], g->d_keys, g->bsz[0], g->bsz[1], bgd, sa);
+                };
+                if (uni) launch(std::true_type());
+                else launch(std::false_type());
                 done = true;
             }
         });

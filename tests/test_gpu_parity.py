@@ -553,11 +553,13 @@ def test_zmb_grid_parity(G, oracle, dim, edge):
     S.assert_same_state(a, b)
 
 
+@pytest.mark.parametrize("pairs", [1, 0])
 @pytest.mark.parametrize("seed", range(6))
-def test_tag_and_box_8cubed_random(G, oracle, seed):
+def test_tag_and_box_8cubed_random(G, oracle, seed, pairs):
     """The 8^3 tile kernels of tagBoundaries (bit-parallel, star and full) and of the
-    27-point box (all scalar widths >= 4 bytes, box restriction, both apply modes) on random
-    3D grids whose masks hold every bit pattern, against the oracle."""
+    27-point box (all scalar widths >= 4 bytes, box restriction, both apply modes; two
+    chunks per wave sharing x ghosts, or one) on random 3D grids whose masks hold every
+    bit pattern, against the oracle."""
     rng = np.random.default_rng(1000 + seed)
     size = [int(rng.integers(24, 60)) for _ in range(3)]
     dt = [np.float32, np.float64, np.int32, np.uint32, np.int64, np.uint64][seed]
@@ -575,6 +577,7 @@ def test_tag_and_box_8cubed_random(G, oracle, seed):
         g.tag_boundaries(nn)
         grids.append(g)
     a, b = grids
+    a.set_tuning(C.TUNE_BOX_PAIRS, pairs)
     S.assert_same_state(a, b)  # masks after tagging
     for p in range(2):
         b.set_prop(p, a.prop(p))  # PADDING-only slots have no specified value after a flush
