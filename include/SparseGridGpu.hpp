@@ -798,15 +798,21 @@ public:
         sg::check(sg_insert_chunks(h_, (int64_t)n, k.data(), masks, values, SG_MEM_HOST));
     }
 
+    // flush<v_reduce...>(ctx, opt) (SparseGridGpu.hpp:772-779).  FLUSH_ON_HOST: the
+    // reference merges on the CPU and leaves the result in host memory
+    // (map_vector_sparse.hpp:1375-1484); here the same merge runs on the device (same
+    // result), with the host copy's edits (insertFlush) sent first and the host copy
+    // refreshed afterwards, so that host reads see the merged grid either way.
     template <typename... v_reduce> void flush(gpu::ofp_context_t &, flush_type opt = FLUSH_ON_DEVICE)
     {
-        (void)opt;
         int32_t ops[nprops > 0 ? nprops : 1];
         for (unsigned int i = 0; i < nprops; ++i) ops[i] = SG_OP_NONE;
         int dummy[] = {0, (ops[v_reduce::prop] = v_reduce::op, 0)...};
         (void)dummy;
+        if (opt == FLUSH_ON_HOST && host_copy_ && host_dirty_) hostToDevice();
         sg::check(sg_flush(h_, ops));
         ins_view_ok_ = false;  // the flush took the records of the kernels' insert buffer
+        if (opt == FLUSH_ON_HOST) deviceToHost();
     }
 
     // insertFlush<p>(coord) (SparseGridGpu.hpp:2780-2845, BlockMapGpu.hpp:186-222): insert the

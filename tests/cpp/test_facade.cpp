@@ -56,6 +56,45 @@ static void testInsert()
     REQUIRE(match);
 }
 
+// flush(ctx, FLUSH_ON_HOST): the same merge as on the device, and the host copy holds it
+// (an insertFlush edit made before the flush is kept)
+static void testFlushOnHost()
+{
+    const size_t sz[3] = {32, 32, 32};
+    SparseGridGpu<3, aggregate<float, int>, 8> a(sz), b(sz);
+    gpu::ofp_context_t ctx;
+    std::vector<long> keys;
+    std::vector<float> v0;
+    std::vector<int> v1;
+    for (int i = 0; i < 500; ++i) {
+        keys.push_back((long)((i * 7919) % (32 * 32 * 32)));
+        v0.push_back(0.5f * (float)i);
+        v1.push_back(i % 13);
+    }
+    const void *v[] = {v0.data(), v1.data()};
+    a.insertPoints(keys.data(), keys.size(), v);
+    a.flush<smax_<0>, sadd_<1>>(ctx, FLUSH_ON_DEVICE);
+    b.insertPoints(keys.data(), keys.size(), v);
+    b.flush<smax_<0>, sadd_<1>>(ctx, FLUSH_ON_HOST);
+    REQUIRE(a.private_get_index_array() == b.private_get_index_array());
+    REQUIRE(a.private_get_mask_array() == b.private_get_mask_array());
+    REQUIRE(a.private_get_data_array<0>() == b.private_get_data_array<0>());
+    REQUIRE(a.private_get_data_array<1>() == b.private_get_data_array<1>());
+    // a host edit, then more inserts flushed on the host
+    b.insertFlush<1>(grid_key_dx<3>({1, 2, 3})) = 77;
+    b.insertPoints(keys.data(), 100, v);
+    b.flush<smax_<0>, sadd_<1>>(ctx, FLUSH_ON_HOST);
+    REQUIRE(b.get<1>(grid_key_dx<3>({1, 2, 3})) == 77);
+    a.insertPoints(keys.data(), 100, v);
+    a.flush<smax_<0>, sadd_<1>>(ctx, FLUSH_ON_DEVICE);
+    bool same = true;
+    for (int i = 0; i < 500; ++i) {
+        const auto c = a.getCoord(keys[i]);
+        same &= a.get<0>(c) == b.get<0>(c) && a.get<1>(c) == b.get<1>(c);
+    }
+    REQUIRE(same);
+}
+
 static void testTagBoundaries()
 {
     const size_t sz[2] = {16, 16};
@@ -652,6 +691,7 @@ static void testHostPackUnpack()
 int main()
 {
     testInsert();
+    testFlushOnHost();
     testTagBoundaries();
     testStencilHeat();
     testConv3x3x3<SparseGridGpu<3, aggregate<float, float>>>();
