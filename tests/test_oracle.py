@@ -1,4 +1,4 @@
-"""CPU tests that pin the oracle (oracle/sgcpu.cpp) before it is trusted as the checker:
+"""CPU tests that pin the oracle (oracle/sgcpu.cpp) before the GPU tests compare against it:
 the reference's own known answers, brute-force models and closed forms."""
 import numpy as np
 import pytest
