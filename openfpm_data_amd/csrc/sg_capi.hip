@@ -264,7 +264,7 @@ int sg_set_linearizer(sg_grid *g, int lin)
 {
     if (!g) return fail(SG_E_ARG, "null grid");
     if (lin != SG_LIN_SMB && lin != SG_LIN_ZMB) return fail(SG_E_ARG, "unknown linearizer");
-    if (g->nchunks != 0 || g->ins_n != 0 || g->cins_n != 0)
+    if (g->nchunks != 0 || g->ins_n != 0 || g->cins_n != 0 || g->pins_n != 0)
         return fail(SG_E_STATE, "the linearizer can only change on an empty grid");
     if (lin == SG_LIN_ZMB) {
         // Morton ids of the domain's chunks: the largest must leave room for point keys
@@ -330,7 +330,7 @@ int64_t sg_pending_inserts(sg_grid *g)
     if (!g) return 0;
     // chunk records count as the point records they stand for
     if (g->cins_n > 0 && convert_chunk_records(g) != SG_OK) return -1;
-    return g->ins_n;
+    return g->ins_n + g->pins_n;
 }
 
 int sg_flush(sg_grid *g, const int32_t *ops)
@@ -608,6 +608,13 @@ int sg_trim(sg_grid *g)
             g->d_ins_val[p] = nullptr;
         }
         g->ins_cap = 0;
+    }
+    if (g->pins_n == 0 && g->pins_cap > 0) {
+        g->pool.free(g->d_pins_ck);
+        g->pool.free(g->d_pins_rec);
+        g->d_pins_ck = nullptr;
+        g->d_pins_rec = nullptr;
+        g->pins_cap = 0;
     }
     if (g->cins_n == 0 && g->cins_cap > 0) {
         g->pool.free(g->d_cins_id);

@@ -895,6 +895,100 @@ struct FoldState {
     int voff[kMaxProps];
 };
 
+// Device-coordinate point inserts staged straight as the grouped flush's (chunk id,
+// record) pairs, in insertion order, so that the flush has no insert-buffer pass: the
+// chunk id (as stage_coords_kernel computes the key), then the record: slot offset, mask
+// (masks or EXIST) and each property's value (the source's, or the background) at the
+// layout's offsets.  A coordinate outside the grid gives chunk id 0xffffffff and sets
+// *bad.  A block takes kWG * kStageItems consecutive records, assembles them in LDS and
+// writes them out as whole 16-byte vectors.
+__global__ __launch_bounds__(kWG) void stage_coord_records_kernel(const int32_t *__restrict__ coords,
+                                                                  const uint8_t *__restrict__ masks, int64_t n,
+                                                                  int dim, int edge, int sz0, int sz1, int sz2,
+                                                                  ChunkLin L, StageVals V, RecLayout R,
+                                                                  uint32_t *__restrict__ ck,
+                                                                  unsigned char *__restrict__ rec, int *__restrict__ bad)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char s_rec[kWG * kStageItems * 32];
+    const int64_t b0 = (int64_t)blockIdx.x * (kWG * kStageItems);
+    const int sz[3] = {sz0, sz1, sz2};
+    const uint32_t e = (uint32_t)edge;
+    int c[kStageItems][3];
+#pragma unroll
+    for (int t = 0; t < kStageItems; ++t) {
+        const int64_t i = b0 + t * kWG + threadIdx.x;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c[t][d] = (d < dim && i < n) ? coords[i * dim + d] : 0;
+    }
+#pragma unroll
+    for (int t = 0; t < kStageItems; ++t) {
+        const int64_t i = b0 + t * kWG + threadIdx.x;
+        if (i >= n) break;
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ok = ok && (d >= dim || (c[t][d] >= 0 && c[t][d] < sz[d]));
+        uint32_t id = 0xffffffffu, l = 0;
+        if (ok) {
+            int64_t bc[3] = {0, 0, 0};
+#pragma unroll
+            for (int d = 2; d >= 0; --d) {
+                if (d >= dim) continue;
+                const uint32_t q = (uint32_t)c[t][d] / e;
+                bc[d] = q;
+                l = l * e + ((uint32_t)c[t][d] - q * e);
+            }
+            id = (uint32_t)chunk_id(L, bc);
+        } else {
+            *bad = 1;
+        }
+        ck[i] = id;
+        unsigned char *d = s_rec + (t * kWG + threadIdx.x) * R.bytes;
+        for (int v = 0; v < R.nv; ++v) ((uint4 *)d)[v] = make_uint4(0, 0, 0, 0);
+        *(uint16_t *)d = (uint16_t)l;
+        d[2] = masks ? masks[i] : (uint8_t)SG_EXIST;
+        for (int q = 0; q < V.np; ++q) {
+            const unsigned char *src = V.src[q];
+            unsigned char *f = d + R.off[q];
+            switch (V.ts[q]) {
+            case 8: *(uint64_t *)f = src ? ((const uint64_t *)src)[i] : V.bg[q]; break;
+            case 4: *(uint32_t *)f = src ? ((const uint32_t *)src)[i] : (uint32_t)V.bg[q]; break;
+            case 2: *(uint16_t *)f = src ? ((const uint16_t *)src)[i] : (uint16_t)V.bg[q]; break;
+            default: *f = src ? src[i] : (unsigned char)V.bg[q]; break;
+            }
+        }
+    }
+    __syncthreads();
+    const int64_t cnt = (n - b0 < kWG * kStageItems) ? (n - b0) : kWG * kStageItems;
+    const int64_t nvec = cnt * R.nv;
+    for (int64_t v = threadIdx.x; v < nvec; v += kWG) ((uint4 *)rec)[b0 * R.nv + v] = ((const uint4 *)s_rec)[v];
+}
+
+// staged (chunk id, record) pairs -> the point insert buffer (key -1 for chunk id
+// 0xffffffff, the mask, each property's value into V.dst)
+__global__ __launch_bounds__(kWG) void point_records_to_buffer_kernel(const uint32_t *__restrict__ ck,
+                                                                      const unsigned char *__restrict__ rec, int64_t n,
+                                                                      int csz, RecLayout R, StageVals V,
+                                                                      int64_t *__restrict__ keys,
+                                                                      uint8_t *__restrict__ mask)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned char *d = rec + i * R.bytes;
+    const uint32_t id = ck[i];
+    keys[i] = id == 0xffffffffu ? (int64_t)-1 : (int64_t)id * csz + *(const uint16_t *)d;
+    mask[i] = d[2];
+    for (int q = 0; q < V.np; ++q) {
+        const unsigned char *f = d + R.off[q];
+        unsigned char *o = V.dst[q];
+        switch (V.ts[q]) {
+        case 8: ((uint64_t *)o)[i] = *(const uint64_t *)f; break;
+        case 4: ((uint32_t *)o)[i] = *(const uint32_t *)f; break;
+        case 2: ((uint16_t *)o)[i] = *(const uint16_t *)f; break;
+        default: o[i] = *f; break;
+        }
+    }
+}
+
 // the records of one batch for one property, rank by rank (a rank's records have distinct
 // slots); first / ao come from the batch's mask pass.  The operator is a template
 // parameter, so the rounds run without dispatch.
@@ -1600,6 +1694,64 @@ int64_t key_limit(const sg_grid *g)
     return (chunk_id(chunk_lin(g), last) + 1) * g->csz;
 }
 
+// The grouped flush's record layout for this grid's properties: slot offset (2 bytes),
+// mask (1 byte), then each property at its natural alignment, in 16 or 32 bytes; S: the
+// fold's LDS state per wave; cbits: significant bits of a chunk id.  False when the
+// grouped flush does not take this grid.
+bool point_record_layout(const sg_grid *g, RecLayout *R, FoldState *S, int *cbits)
+{
+    const int csz = g->csz;
+    const int64_t nblocks = key_limit(g) / csz;
+    int cb = 0;
+    while (cb < 63 && (((int64_t)1 << cb) < nblocks)) ++cb;
+    *cbits = cb;
+    if (cb > 32 || csz > 65536) return false;
+    int at = 3;
+    for (int p = 0; p < g->np; ++p) {
+        at = (at + g->ts[p] - 1) / g->ts[p] * g->ts[p];
+        R->off[p] = at;
+        at += g->ts[p];
+    }
+    if (at > 32) return false;
+    R->nv = at <= 16 ? 1 : 2;
+    R->bytes = 16 * R->nv;
+    size_t b = (size_t)2 * csz;
+    for (int p = 0; p < g->np; ++p) {
+        b = (b + 15) / 16 * 16;
+        S->voff[p] = (int)b;
+        b += (size_t)csz * g->ts[p];
+    }
+    b = (b + 15) / 16 * 16;
+    S->roff = (int)b;
+    b += (size_t)64 * R->bytes;
+    if (b > 65536) return false;
+    S->bytes = (int)b;
+    return true;
+}
+
+// room for `need` staged (chunk id, record) pairs of nv 16-byte vectors, the staged ones kept
+int grow_point_records(sg_grid *g, int64_t need, int nv)
+{
+    if (need <= g->pins_cap && g->pins_nv == nv) return SG_OK;
+    const int64_t ncap = std::max<int64_t>(need, std::max<int64_t>(1024, g->pins_cap * 2));
+    uint32_t *nck = (uint32_t *)g->pool.alloc(ncap * sizeof(uint32_t));
+    void *nrec = g->pool.alloc(ncap * 16 * nv);
+    if (!nck || !nrec) return fail(SG_E_OOM, "insert buffer allocation failed");
+    if (!g->d_pins_bad && !(g->d_pins_bad = (int *)g->pool.alloc(64)))
+        return fail(SG_E_OOM, "insert buffer allocation failed");
+    if (g->pins_n) {
+        SG_HIP(hipMemcpyAsync(nck, g->d_pins_ck, g->pins_n * sizeof(uint32_t), hipMemcpyDeviceToDevice, g->stream));
+        SG_HIP(hipMemcpyAsync(nrec, g->d_pins_rec, g->pins_n * 16 * nv, hipMemcpyDeviceToDevice, g->stream));
+    }
+    g->pool.free(g->d_pins_ck);
+    g->pool.free(g->d_pins_rec);
+    g->d_pins_ck = nck;
+    g->d_pins_rec = nrec;
+    g->pins_cap = ncap;
+    g->pins_nv = nv;
+    return SG_OK;
+}
+
 // append masks and values of n records at insert position `at`
 int append_mask_values(sg_grid *g, int64_t at, int64_t n, const uint8_t *masks, const void *const *values, int mem)
 {
@@ -1628,6 +1780,34 @@ int append_mask_values(sg_grid *g, int64_t at, int64_t n, const uint8_t *masks, 
 
 }  // namespace
 
+int convert_point_records(sg_grid *g)
+{
+    if (g->pins_n == 0) return SG_OK;
+    const int64_t n = g->pins_n;
+    RecLayout R;
+    FoldState S;
+    int cbits = 0;
+    if (!point_record_layout(g, &R, &S, &cbits) || R.nv != g->pins_nv)
+        return fail(SG_E_STATE, "staged point records of another layout");
+    int rc = grow_insert_buffer(g, g->ins_n + n);
+    if (rc) return rc;
+    const int64_t at = g->ins_n;
+    StageVals V;
+    V.np = g->np;
+    for (int p = 0; p < g->np; ++p) {
+        V.ts[p] = g->ts[p];
+        V.src[p] = nullptr;
+        V.dst[p] = (unsigned char *)g->d_ins_val[p] + at * g->ts[p];
+        V.bg[p] = 0;
+    }
+    hipLaunchKernelGGL(point_records_to_buffer_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_pins_ck,
+                       (const unsigned char *)g->d_pins_rec, n, g->csz, R, V, g->d_ins_key + at, g->d_ins_mask + at);
+    SG_HIP(hipGetLastError());
+    g->ins_n += n;
+    g->pins_n = 0;
+    return SG_OK;
+}
+
 int append_points(sg_grid *g, int64_t n, const int64_t *keys, const uint8_t *masks, const void *const *values,
                   int mem)
 {
@@ -1640,6 +1820,7 @@ int append_points(sg_grid *g, int64_t n, const int64_t *keys, const uint8_t *mas
             if (keys[i] < 0 || keys[i] >= limit) return fail(SG_E_RANGE, "point key outside the grid");
     }
     int rc = convert_chunk_records(g);
+    if (!rc) rc = convert_point_records(g);
     if (rc) return rc;
     rc = grow_insert_buffer(g, g->ins_n + n);
     if (rc) return rc;
@@ -1659,6 +1840,35 @@ int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *m
     if (!coords) return fail(SG_E_ARG, "null coords");
     int rc = convert_chunk_records(g);
     if (rc) return rc;
+    if (mem == SG_MEM_DEVICE && g->ins_n == 0 && g->flush_path == 0 && g->pins_n + n < ((int64_t)1 << 31)) {
+        // nothing else pending: stage the records in the grouped flush's format
+        RecLayout R;
+        FoldState S;
+        int cbits = 0;
+        if (point_record_layout(g, &R, &S, &cbits) && cbits <= 31) {
+            if ((rc = grow_point_records(g, g->pins_n + n, R.nv))) return rc;
+            if (g->pins_n == 0) SG_HIP(hipMemsetAsync(g->d_pins_bad, 0, sizeof(int), g->stream));
+            StageVals V;
+            V.np = g->np;
+            for (int p = 0; p < g->np; ++p) {
+                V.ts[p] = g->ts[p];
+                V.src[p] = values ? (const unsigned char *)values[p] : nullptr;
+                V.dst[p] = nullptr;
+                V.bg[p] = 0;
+                memcpy(&V.bg[p], g->bg[p], g->ts[p]);
+            }
+            int szi[3];
+            for (int d = 0; d < 3; ++d) szi[d] = (int)std::min<int64_t>(g->sz[d], INT32_MAX);
+            hipLaunchKernelGGL(stage_coord_records_kernel, dim3(grid_1d(n, kWG * kStageItems)), dim3(kWG), 0,
+                               g->stream, coords, masks, n, g->dim, g->edge, szi[0], szi[1], szi[2], chunk_lin(g), V,
+                               R, g->d_pins_ck + g->pins_n,
+                               (unsigned char *)g->d_pins_rec + g->pins_n * R.bytes, g->d_pins_bad);
+            SG_HIP(hipGetLastError());
+            g->pins_n += n;
+            return SG_OK;
+        }
+    }
+    if ((rc = convert_point_records(g))) return rc;
     rc = grow_insert_buffer(g, g->ins_n + n);
     if (rc) return rc;
     const int64_t at = g->ins_n;
@@ -1829,6 +2039,7 @@ int convert_chunk_records(sg_grid *g)
 int reserve_point_records(sg_grid *g, int64_t n, int64_t *at)
 {
     int rc = convert_chunk_records(g);
+    if (!rc) rc = convert_point_records(g);
     if (rc) return rc;
     rc = grow_insert_buffer(g, g->ins_n + n);
     if (rc) return rc;
@@ -1848,6 +2059,10 @@ int append_chunks(sg_grid *g, int64_t n, const int64_t *chunk_ids, const uint8_t
         for (int64_t i = 0; i < n; ++i)
             if (chunk_ids[i] < 0 || chunk_ids[i] >= nblocks) return fail(SG_E_RANGE, "chunk id outside the grid");
     // after point records, chunk records join them as points (insertion order is kept)
+    if (g->pins_n > 0) {
+        const int rc = convert_point_records(g);
+        if (rc) return rc;
+    }
     if (g->ins_n > 0) return expand_chunks(g, n, chunk_ids, masks, values, mem);
     int rc = grow_chunk_buffer(g, g->cins_n + n);
     if (rc) return rc;
@@ -2254,43 +2469,30 @@ int flush_chunks_impl(sg_grid *g, const int32_t *ops)
 int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
 {
     *taken = false;
-    const int64_t n = g->ins_n;
     const int csz = g->csz;
     const int64_t limit = key_limit(g);
-    const int64_t nblocks = limit / csz;
-    int cbits = 0;
-    while (cbits < 63 && (((int64_t)1 << cbits) < nblocks)) ++cbits;
-    if (cbits > 32 || csz > 65536 || n >= ((int64_t)1 << 31)) return SG_OK;
     RecLayout R;
-    {
-        int at = 3;
-        for (int p = 0; p < g->np; ++p) {
-            at = (at + g->ts[p] - 1) / g->ts[p] * g->ts[p];
-            R.off[p] = at;
-            at += g->ts[p];
-        }
-        if (at > 32) return SG_OK;
-        R.nv = at <= 16 ? 1 : 2;
-        R.bytes = 16 * R.nv;
-    }
     FoldState S;
-    {
-        size_t at = (size_t)2 * csz;
-        for (int p = 0; p < g->np; ++p) {
-            at = (at + 15) / 16 * 16;
-            S.voff[p] = (int)at;
-            at += (size_t)csz * g->ts[p];
-        }
-        at = (at + 15) / 16 * 16;
-        S.roff = (int)at;
-        at += (size_t)64 * R.bytes;
-        if (at > 65536) return SG_OK;
-        S.bytes = (int)at;
-    }
+    int cbits = 0;
+    if (!point_record_layout(g, &R, &S, &cbits)) return SG_OK;
+    if (g->ins_n + g->pins_n >= ((int64_t)1 << 31)) return SG_OK;
     *taken = true;
     int rc;
     uint32_t *hw = host_words(g);
     if (!hw) return fail(SG_E_OOM, "pinned host allocation failed");
+    // records staged as (chunk id, record) pairs are sorted as they are, unless a staged
+    // coordinate was outside the grid: then they go through the insert buffer below (the
+    // flag is read before the sort, which overwrites the staged order)
+    bool staged = g->pins_n > 0;
+    if (staged) {
+        SG_HIP(hipMemcpyAsync(hw + 9, g->d_pins_bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+        SG_HIP(hipStreamSynchronize(g->stream));
+        if (hw[9] || R.nv != g->pins_nv) {
+            if ((rc = convert_point_records(g))) return rc;
+            staged = false;
+        }
+    }
+    const int64_t n = staged ? g->pins_n : g->ins_n;
     if ((rc = ensure(g->pool, g->scratch[6], 64))) return rc;
     int *bad = (int *)g->scratch[6].p;
     SG_HIP(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
@@ -2305,20 +2507,27 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
         F.bg[p] = 0;
         memcpy(&F.bg[p], g->bg[p], g->ts[p]);
     }
-    // (chunk id, record) pairs and the sort's second buffers
+    // (chunk id, record) pairs and the sort's second buffers; staged pairs are the sort's
+    // input, kept (a failure below leaves them pending), with b_ck / b_rec as its second
+    // temporaries
     DevBuf &b_ck = g->scratch[0], &b_ckt = g->scratch[2], &b_rec = g->scratch[7], &b_rect = g->scratch[13];
     if ((rc = ensure(g->pool, b_ck, (size_t)n * 4 + 64)) || (rc = ensure(g->pool, b_ckt, (size_t)n * 4 + 64)) ||
         (rc = ensure(g->pool, b_rec, (size_t)n * R.bytes + 64)) ||
         (rc = ensure(g->pool, b_rect, (size_t)n * R.bytes + 64)))
         return rc;
-    hipLaunchKernelGGL(chunk_records_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key,
-                       g->d_ins_mask, n, csz, limit, F, R, (uint32_t *)b_ck.p, (unsigned char *)b_rec.p, bad);
-    SG_HIP(hipGetLastError());
     uint32_t *ck = nullptr;
     uint4 *rec = nullptr;
-    if ((rc = radix_sort_records_u32(g, (uint32_t *)b_ck.p, (uint4 *)b_rec.p, (uint32_t *)b_ckt.p,
-                                     (uint4 *)b_rect.p, R.nv, n, cbits, &ck, &rec)))
-        return rc;
+    if (staged) {
+        rc = radix_sort_records_u32(g, g->d_pins_ck, (uint4 *)g->d_pins_rec, (uint32_t *)b_ckt.p, (uint4 *)b_rect.p,
+                                    R.nv, n, cbits, &ck, &rec, (uint32_t *)b_ck.p, (uint4 *)b_rec.p);
+    } else {
+        hipLaunchKernelGGL(chunk_records_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key,
+                           g->d_ins_mask, n, csz, limit, F, R, (uint32_t *)b_ck.p, (unsigned char *)b_rec.p, bad);
+        SG_HIP(hipGetLastError());
+        rc = radix_sort_records_u32(g, (uint32_t *)b_ck.p, (uint4 *)b_rec.p, (uint32_t *)b_ckt.p, (uint4 *)b_rect.p,
+                                    R.nv, n, cbits, &ck, &rec);
+    }
+    if (rc) return rc;
     // the range flag, read with the scan total below
     SG_HIP(hipMemcpyAsync(hw + 8, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
     // touched chunks: runs of equal sorted chunk ids, counted per tile, tile offsets
@@ -2439,6 +2648,7 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
     }
     finish_merge(g, df);
     g->ins_n = 0;
+    g->pins_n = 0;
     g->nn_valid[0] = g->nn_valid[1] = false;
     g->occ_tab_valid = false;
     g->ebits_valid = planes;
@@ -2452,17 +2662,19 @@ int flush_impl(sg_grid *g, const int32_t *ops)
 {
     // chunk records are pending only while no point record is (appending points converts them)
     if (g->cins_n > 0) return flush_chunks_impl(g, ops);
-    const int64_t n = g->ins_n;
-    if (n == 0) return SG_OK;  // nothing inserted (map_vector_sparse.hpp:1199-1213)
-    if (n >= ((int64_t)1 << 31)) return fail(SG_E_RANGE, "more than 2^31 pending inserts");
+    if (g->ins_n + g->pins_n == 0) return SG_OK;  // nothing inserted (map_vector_sparse.hpp:1199-1213)
+    if (g->ins_n + g->pins_n >= ((int64_t)1 << 31)) return fail(SG_E_RANGE, "more than 2^31 pending inserts");
     if (g->flush_path == 0) {
         bool taken = false;
         const int rc0 = flush_grouped_impl(g, ops, &taken);
         if (taken || rc0) return rc0;
     }
+    // the flush by point key reads the insert buffer
+    int rc = convert_point_records(g);
+    if (rc) return rc;
+    const int64_t n = g->ins_n;
     const int csz = g->csz;
     const int64_t limit = key_limit(g);
-    int rc;
 
     // validate device-side keys: the flag comes back with the first scan total below (the
     // grid is not touched before it is read; out-of-range keys only reorder scratch data)

@@ -110,6 +110,14 @@ struct sg_grid {
     int64_t *d_ins_key = nullptr;
     uint8_t *d_ins_mask = nullptr;
     void *d_ins_val[sg::kMaxProps] = {};
+    // point records of device-coordinate inserts staged as the grouped flush's (chunk id,
+    // record) pairs, while no other insert is pending: [n] chunk ids, [n] records of
+    // 16 * pins_nv bytes, and a flag set when a coordinate was outside the grid
+    int64_t pins_n = 0, pins_cap = 0;
+    int pins_nv = 0;
+    uint32_t *d_pins_ck = nullptr;
+    void *d_pins_rec = nullptr;
+    int *d_pins_bad = nullptr;
     // insert buffer of whole-chunk records (insertBlock), kept as chunks while no point
     // record is pending: [n] chunk ids, [n][csz] masks and values
     int64_t cins_n = 0, cins_cap = 0;
@@ -324,6 +332,9 @@ inline ChunkLin chunk_lin(const sg_grid *g);
 int flush_impl(sg_grid *g, const int32_t *ops);
 // pending whole-chunk records -> point records (before point records are appended)
 int convert_chunk_records(sg_grid *g);
+// pending staged (chunk id, record) pairs -> the point insert buffer (before anything
+// else is appended to it or reads it)
+int convert_point_records(sg_grid *g);
 // room for n more point records at *at of the insert buffer (chunk records converted first)
 int reserve_point_records(sg_grid *g, int64_t n, int64_t *at);
 int append_points(sg_grid *g, int64_t n, const int64_t *keys, const uint8_t *masks,
@@ -370,9 +381,11 @@ int radix_sort_pairs_u32(sg_grid *g, uint32_t *keys, uint32_t *vals, uint32_t *k
                          uint32_t *vals_tmp, int64_t n, int bits, uint32_t **keys_out,
                          uint32_t **vals_out);
 // the same for (key, record of nv 16-byte vectors, nv = 1 or 2): the records move with
-// their keys
+// their keys.  With keys_tmp2 / recs_tmp2 the input is only read: the passes after the
+// first alternate between the two temporaries
 int radix_sort_records_u32(sg_grid *g, uint32_t *keys, uint4 *recs, uint32_t *keys_tmp, uint4 *recs_tmp, int nv,
-                           int64_t n, int bits, uint32_t **keys_out, uint4 **recs_out);
+                           int64_t n, int bits, uint32_t **keys_out, uint4 **recs_out,
+                           uint32_t *keys_tmp2 = nullptr, uint4 *recs_tmp2 = nullptr);
 
 inline unsigned grid_1d(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 

@@ -916,6 +916,7 @@ int sg_remove_add_unpack_reset(sg_grid *g)
     g->copies.clear();
     g->ins_n = 0;
     g->cins_n = 0;
+    g->pins_n = 0;
     return SG_OK;
 }
 

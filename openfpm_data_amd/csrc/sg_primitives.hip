@@ -422,7 +422,8 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_rec_kernel(const uint32
 }  // namespace
 
 int radix_sort_records_u32(sg_grid *g, uint32_t *keys, uint4 *recs, uint32_t *keys_tmp, uint4 *recs_tmp, int nv,
-                           int64_t n, int bits, uint32_t **keys_out, uint4 **recs_out)
+                           int64_t n, int bits, uint32_t **keys_out, uint4 **recs_out, uint32_t *keys_tmp2,
+                           uint4 *recs_tmp2)
 {
     *keys_out = keys;
     *recs_out = recs;
@@ -453,6 +454,11 @@ int radix_sort_records_u32(sg_grid *g, uint32_t *keys, uint4 *recs, uint32_t *ke
         SG_HIP(hipGetLastError());
         std::swap(src_k, dst_k);
         std::swap(src_r, dst_r);
+        if (keys_tmp2 && dst_k == keys) {
+            // keep the input: the next pass writes the second temporaries
+            dst_k = keys_tmp2;
+            dst_r = recs_tmp2;
+        }
     }
     *keys_out = src_k;
     *recs_out = src_r;

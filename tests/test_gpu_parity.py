@@ -118,6 +118,110 @@ def test_flush_chunk_record_programs(G, oracle, seed, order):
         assert np.all(a.prop(p)[empty] == np.array(bgs[p], dt))
 
 
+def _device_values(vals, dtypes):
+    """Host value arrays -> device tensors (unsigned types as their signed views)."""
+    import torch
+    out = []
+    for v, dt in zip(vals, dtypes):
+        v = np.ascontiguousarray(np.asarray(v, dtype=dt).reshape(-1))
+        if v.dtype.kind == "u" and v.dtype.itemsize > 1:
+            v = v.view(np.dtype("i%d" % v.dtype.itemsize))
+        out.append(torch.from_numpy(v).cuda())
+    return out
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_staged_device_coordinate_inserts(G, oracle, seed):
+    """Device-coordinate inserts are staged straight as the grouped flush's (chunk id,
+    record) pairs while nothing else is pending.  Batches of them, mixed in any order with
+    key inserts and chunk inserts (which move the staged records to the insert buffer
+    first), the flush by point key chosen after staging, and batches holding coordinates
+    outside the grid (the flush reports and drops those, the others stay pending and merge
+    at the next flush), against the oracle fed the same records as keys."""
+    import torch
+    rng = np.random.default_rng(4000 + seed)
+    dim, edge = 3, 8
+    # whole chunks: every point key of a chunk is then a coordinate inside the grid
+    size = [8 * int(rng.integers(3, 9)) for _ in range(dim)]
+    dtypes = [[np.float32, np.int32, np.uint32], [np.float64, np.int32, np.uint8], [np.int16],
+              [np.uint8, np.float64], [np.float32], [np.int64, np.uint16, np.int8],
+              [np.float32, np.float32], [np.uint32, np.uint32, np.uint32]][seed]
+    steps = S.random_flush_program(seed, dim, edge, size, dtypes, nflush=3, nrec=3000,
+                                   with_chunks=seed % 2 == 0, order=["points_first", "chunks_first"][seed % 4 // 2])
+    a = G(dim, size, dtypes, block_edge=edge)
+    b = oracle.SparseGridCpu(dim, size, dtypes, block_edge=edge)
+    for g in (a, b):
+        for p in range(len(dtypes)):
+            g.set_background_value(p, p + 2)
+    for fi, (recs, ops) in enumerate(steps):
+        for kind, keys, m, v in recs:
+            if kind != "points":
+                a.insert_block(keys, m, v)
+                b.insert_block(keys, m, v)
+                continue
+            if rng.random() < 0.75:
+                # in two device batches, sometimes without masks (every record EXIST) or
+                # without values (the background)
+                nomask, noval = rng.random() < 0.2, rng.random() < 0.2
+                if nomask:
+                    m = np.ones_like(m)
+                if noval:
+                    v = [np.full(len(keys), p + 2, dt) for p, dt in enumerate(dtypes)]
+                b.insert(keys, v, m)
+                cut = int(rng.integers(0, len(keys) + 1))
+                for lo, hi in ((0, cut), (cut, len(keys))):
+                    if hi == lo:
+                        continue
+                    dc = torch.from_numpy(a.inv_lin_id(keys[lo:hi]).astype(np.int32)).cuda()
+                    dm = None if nomask else torch.from_numpy(np.ascontiguousarray(m[lo:hi])).cuda()
+                    dv = None if noval else _device_values([x[lo:hi] for x in v], dtypes)
+                    a.insert_coords(dc, dv, dm)
+            else:
+                b.insert(keys, v, m)
+                a.insert(keys, v, m)
+        if fi == 1:
+            # a staged batch with coordinates outside the grid: the flush drops them, the
+            # rest stays pending in order
+            n = 500
+            coords = np.stack([rng.integers(0, s, n) for s in size], axis=1).astype(np.int32)
+            bad = rng.choice(n, 7, replace=False)
+            coords[bad[:4], 0] = size[0] + rng.integers(0, 5, 4)
+            coords[bad[4:], 2] = -1 - rng.integers(0, 3, 3)
+            vals = [S._rand_vals(rng, dt, n) for dt in dtypes]
+            masks = rng.choice(np.array([1, 1, 3, 0, 2], np.uint8), n)
+            good = np.setdiff1d(np.arange(n), bad)
+            pending = a.pending_inserts()
+            a.insert_coords(torch.from_numpy(coords).cuda(), _device_values(vals, dtypes),
+                            torch.from_numpy(masks).cuda())
+            assert a.pending_inserts() == pending + n
+            with pytest.raises(C.SgError):
+                a.flush(ops)
+            assert a.pending_inserts() == pending + len(good)
+            b.insert(b.lin_id(coords[good]), [x[good] for x in vals], masks[good])
+        if fi == 2 and seed % 3 == 1:
+            a.set_tuning(C.TUNE_FLUSH_PATH, 1)  # the flush by point key takes the staged records
+        a.flush(ops)
+        b.flush(ops)
+        assert a.pending_inserts() == 0
+        # existing values equal (a PADDING slot without EXIST has no specified value), and
+        # every mask-0 slot holds the background
+        S.assert_same_state(a, b)
+        empty = a.masks() == 0
+        for p, dt in enumerate(dtypes):
+            assert np.all(a.prop(p)[empty] == np.array(p + 2, dt))
+    # staged records pending: they survive sg_trim, block a change of linearizer, and a
+    # reset drops them
+    e = G(dim, size, dtypes, block_edge=edge)
+    e.insert_coords(torch.zeros((3, dim), dtype=torch.int32).cuda(), None)
+    C.check(e._lib.sg_trim(e._h))
+    assert e.pending_inserts() == 3
+    assert e._lib.sg_set_linearizer(e._h, 1) != C.SG_OK
+    e.remove_add_unpack_reset()
+    assert e.pending_inserts() == 0
+    e.flush(["none"] * len(dtypes))
+    assert e.num_chunks() == 0
+
+
 def test_flush_empty_and_repeat(G, oracle):
     for cls in (G, oracle.SparseGridCpu):
         g = cls(3, (64, 64, 64), [np.int32], block_edge=8)
