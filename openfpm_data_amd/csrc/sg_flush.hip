@@ -900,31 +900,59 @@ struct FoldState {
 // chunk id (as stage_coords_kernel computes the key), then the record: slot offset, mask
 // (masks or EXIST) and each property's value (the source's, or the background) at the
 // layout's offsets.  A coordinate outside the grid gives chunk id 0xffffffff and sets
-// *bad.  A block takes kWG * kStageItems consecutive records, assembles them in LDS and
-// writes them out as whole 16-byte vectors.
+// *bad.  A block takes kWG * kStageItems consecutive records, a thread every kWG-th; the
+// records are assembled in registers (the layout is uniform) and stored as 16-byte
+// vectors, a wave's stores covering consecutive records.
+
+// bits `v` of `bytes` bytes at byte `off` of an NV-vector record held in registers
+template <int NV> __device__ inline void rec_put(uint4 (&r)[NV], int off, int bytes, uint64_t v)
+{
+    const int w = off >> 2, sh = (off & 3) * 8;
+    const uint32_t lo = bytes >= 4 ? (uint32_t)v : (((uint32_t)v & ((1u << (8 * bytes)) - 1u)) << sh);
+    const uint32_t keep = bytes >= 4 ? 0u : ~(((1u << (8 * bytes)) - 1u) << sh);
+    auto setq = [](uint4 &q, int k, uint32_t x, uint32_t kp) {
+        switch (k & 3) {
+        case 0: q.x = (q.x & kp) | x; break;
+        case 1: q.y = (q.y & kp) | x; break;
+        case 2: q.z = (q.z & kp) | x; break;
+        default: q.w = (q.w & kp) | x; break;
+        }
+    };
+    // the vector by a branch on the (uniform) word index, not by indexing the array
+    auto set = [&](int k, uint32_t x, uint32_t kp) {
+        if (NV == 1 || k < 4) setq(r[0], k, x, kp);
+        else setq(r[NV - 1], k, x, kp);
+    };
+    set(w, lo, keep);
+    if (bytes == 8) set(w + 1, (uint32_t)(v >> 32), 0u);
+}
+
+template <int NV>
 __global__ __launch_bounds__(kWG) void stage_coord_records_kernel(const int32_t *__restrict__ coords,
                                                                   const uint8_t *__restrict__ masks, int64_t n,
                                                                   int dim, int edge, int sz0, int sz1, int sz2,
                                                                   ChunkLin L, StageVals V, RecLayout R,
-                                                                  uint32_t *__restrict__ ck,
-                                                                  unsigned char *__restrict__ rec, int *__restrict__ bad)
+                                                                  uint32_t *__restrict__ ck, uint4 *__restrict__ rec,
+                                                                  int *__restrict__ bad)
 {
-    __shared__ __attribute__((aligned(16))) unsigned char s_rec[kWG * kStageItems * 32];
     const int64_t b0 = (int64_t)blockIdx.x * (kWG * kStageItems);
     const int sz[3] = {sz0, sz1, sz2};
     const uint32_t e = (uint32_t)edge;
+    // every load of a step is issued before its stores
     int c[kStageItems][3];
+    uint8_t mk[kStageItems];
 #pragma unroll
     for (int t = 0; t < kStageItems; ++t) {
         const int64_t i = b0 + t * kWG + threadIdx.x;
 #pragma unroll
         for (int d = 0; d < 3; ++d) c[t][d] = (d < dim && i < n) ? coords[i * dim + d] : 0;
+        mk[t] = (masks && i < n) ? masks[i] : (uint8_t)SG_EXIST;
     }
+    uint4 r[kStageItems][NV];
 #pragma unroll
     for (int t = 0; t < kStageItems; ++t) {
         const int64_t i = b0 + t * kWG + threadIdx.x;
-        if (i >= n) break;
-        bool ok = true;
+        bool ok = i < n;
 #pragma unroll
         for (int d = 0; d < 3; ++d) ok = ok && (d >= dim || (c[t][d] >= 0 && c[t][d] < sz[d]));
         uint32_t id = 0xffffffffu, l = 0;
@@ -938,29 +966,41 @@ __global__ __launch_bounds__(kWG) void stage_coord_records_kernel(const int32_t 
                 l = l * e + ((uint32_t)c[t][d] - q * e);
             }
             id = (uint32_t)chunk_id(L, bc);
-        } else {
+        } else if (i < n) {
             *bad = 1;
         }
-        ck[i] = id;
-        unsigned char *d = s_rec + (t * kWG + threadIdx.x) * R.bytes;
-        for (int v = 0; v < R.nv; ++v) ((uint4 *)d)[v] = make_uint4(0, 0, 0, 0);
-        *(uint16_t *)d = (uint16_t)l;
-        d[2] = masks ? masks[i] : (uint8_t)SG_EXIST;
-        for (int q = 0; q < V.np; ++q) {
-            const unsigned char *src = V.src[q];
-            unsigned char *f = d + R.off[q];
-            switch (V.ts[q]) {
-            case 8: *(uint64_t *)f = src ? ((const uint64_t *)src)[i] : V.bg[q]; break;
-            case 4: *(uint32_t *)f = src ? ((const uint32_t *)src)[i] : (uint32_t)V.bg[q]; break;
-            case 2: *(uint16_t *)f = src ? ((const uint16_t *)src)[i] : (uint16_t)V.bg[q]; break;
-            default: *f = src ? src[i] : (unsigned char)V.bg[q]; break;
+        if (i < n) ck[i] = id;
+        r[t][0] = make_uint4(l | ((uint32_t)mk[t] << 16), 0, 0, 0);
+        if (NV == 2) r[t][NV - 1] = make_uint4(0, 0, 0, 0);
+    }
+    for (int q = 0; q < V.np; ++q) {
+        const unsigned char *src = V.src[q];
+        const int ts = V.ts[q], off = R.off[q];
+        uint64_t v[kStageItems];
+#pragma unroll
+        for (int t = 0; t < kStageItems; ++t) {
+            const int64_t i = b0 + t * kWG + threadIdx.x;
+            v[t] = V.bg[q];
+            if (src && i < n) {
+                switch (ts) {
+                case 8: v[t] = ((const uint64_t *)src)[i]; break;
+                case 4: v[t] = ((const uint32_t *)src)[i]; break;
+                case 2: v[t] = ((const uint16_t *)src)[i]; break;
+                default: v[t] = src[i]; break;
+                }
             }
         }
+#pragma unroll
+        for (int t = 0; t < kStageItems; ++t) rec_put<NV>(r[t], off, ts, v[t]);
     }
-    __syncthreads();
-    const int64_t cnt = (n - b0 < kWG * kStageItems) ? (n - b0) : kWG * kStageItems;
-    const int64_t nvec = cnt * R.nv;
-    for (int64_t v = threadIdx.x; v < nvec; v += kWG) ((uint4 *)rec)[b0 * R.nv + v] = ((const uint4 *)s_rec)[v];
+#pragma unroll
+    for (int t = 0; t < kStageItems; ++t) {
+        const int64_t i = b0 + t * kWG + threadIdx.x;
+        if (i < n) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) rec[i * NV + k] = r[t][k];
+        }
+    }
 }
 
 // staged (chunk id, record) pairs -> the point insert buffer (key -1 for chunk id
@@ -1859,10 +1899,15 @@ int append_coords(sg_grid *g, int64_t n, const int32_t *coords, const uint8_t *m
             }
             int szi[3];
             for (int d = 0; d < 3; ++d) szi[d] = (int)std::min<int64_t>(g->sz[d], INT32_MAX);
-            hipLaunchKernelGGL(stage_coord_records_kernel, dim3(grid_1d(n, kWG * kStageItems)), dim3(kWG), 0,
-                               g->stream, coords, masks, n, g->dim, g->edge, szi[0], szi[1], szi[2], chunk_lin(g), V,
-                               R, g->d_pins_ck + g->pins_n,
-                               (unsigned char *)g->d_pins_rec + g->pins_n * R.bytes, g->d_pins_bad);
+            uint4 *rd = (uint4 *)g->d_pins_rec + g->pins_n * R.nv;
+            if (R.nv == 1)
+                hipLaunchKernelGGL(stage_coord_records_kernel<1>, dim3(grid_1d(n, kWG * kStageItems)), dim3(kWG), 0,
+                                   g->stream, coords, masks, n, g->dim, g->edge, szi[0], szi[1], szi[2], chunk_lin(g),
+                                   V, R, g->d_pins_ck + g->pins_n, rd, g->d_pins_bad);
+            else
+                hipLaunchKernelGGL(stage_coord_records_kernel<2>, dim3(grid_1d(n, kWG * kStageItems)), dim3(kWG), 0,
+                                   g->stream, coords, masks, n, g->dim, g->edge, szi[0], szi[1], szi[2], chunk_lin(g),
+                                   V, R, g->d_pins_ck + g->pins_n, rd, g->d_pins_bad);
             SG_HIP(hipGetLastError());
             g->pins_n += n;
             return SG_OK;
