@@ -849,61 +849,6 @@ __device__ inline void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// insert buffer -> (chunk id, record) pairs in insertion order; a key outside [0, limit)
-// sets *bad (the flush then drops those records before anything is merged)
-__global__ __launch_bounds__(kWG) void chunk_records_kernel(const int64_t *__restrict__ keys,
-                                                           const uint8_t *__restrict__ mask, int64_t n, int csz,
-                                                           int64_t limit, FoldProps F, RecLayout R,
-                                                           uint32_t *__restrict__ ck, unsigned char *__restrict__ rec,
-                                                           int *__restrict__ bad)
-{
-    // each thread assembles its record in LDS (small stores at the layout's offsets), then
-    // the block writes its records out as whole 16-byte vectors
-    __shared__ __attribute__((aligned(16))) unsigned char s_rec[kWG * 32];
-    const int64_t i0 = (int64_t)blockIdx.x * kWG;
-    const int64_t i = i0 + threadIdx.x;
-    if (i < n) {
-        const int64_t k = keys[i];
-        const bool ok = k >= 0 && k < limit;
-        if (!ok) *bad = 1;
-        const int64_t kk = ok ? k : 0;
-        ck[i] = (uint32_t)(kk / csz);
-        unsigned char *d = s_rec + threadIdx.x * R.bytes;
-        *(uint16_t *)d = (uint16_t)(kk % csz);
-        d[2] = mask[i];
-        for (int q = 0; q < F.np; ++q) {
-            const unsigned char *s = F.rval[q] + i * F.ts[q];
-            unsigned char *t = d + R.off[q];
-            switch (F.ts[q]) {
-            case 8: *(uint64_t *)t = *(const uint64_t *)s; break;
-            case 4: *(uint32_t *)t = *(const uint32_t *)s; break;
-            case 2: *(uint16_t *)t = *(const uint16_t *)s; break;
-            default: *t = *s; break;
-            }
-        }
-    }
-    __syncthreads();
-    const int64_t nv = ((n - i0 < kWG) ? (n - i0) : kWG) * R.nv;
-    for (int64_t v = threadIdx.x; v < nv; v += kWG) ((uint4 *)rec)[i0 * R.nv + v] = ((const uint4 *)s_rec)[v];
-}
-
-// LDS fold state of one chunk, per wave: seen[csz] (0xff where a record reached the
-// slot, else 0), am[csz] (OR of their masks), then per property csz values at voff[q]
-struct FoldState {
-    int bytes;  // per wave, a multiple of 16
-    int roff;   // staging of a batch of 64 records
-    int voff[kMaxProps];
-};
-
-// Device-coordinate point inserts staged straight as the grouped flush's (chunk id,
-// record) pairs, in insertion order, so that the flush has no insert-buffer pass: the
-// chunk id (as stage_coords_kernel computes the key), then the record: slot offset, mask
-// (masks or EXIST) and each property's value (the source's, or the background) at the
-// layout's offsets.  A coordinate outside the grid gives chunk id 0xffffffff and sets
-// *bad.  A block takes kWG * kStageItems consecutive records, a thread every kWG-th; the
-// records are assembled in registers (the layout is uniform) and stored as 16-byte
-// vectors, a wave's stores covering consecutive records.
-
 // bits `v` of `bytes` bytes at byte `off` of an NV-vector record held in registers
 template <int NV> __device__ inline void rec_put(uint4 (&r)[NV], int off, int bytes, uint64_t v)
 {
@@ -926,6 +871,60 @@ template <int NV> __device__ inline void rec_put(uint4 (&r)[NV], int off, int by
     set(w, lo, keep);
     if (bytes == 8) set(w + 1, (uint32_t)(v >> 32), 0u);
 }
+
+// insert buffer -> (chunk id, record) pairs in insertion order; a key outside [0, limit)
+// sets *bad (the flush then drops those records before anything is merged).  Each thread
+// assembles its record in registers (the layout is uniform) and stores it as 16-byte
+// vectors, a wave's stores covering consecutive records.
+template <int NV>
+__global__ __launch_bounds__(kWG) void chunk_records_kernel(const int64_t *__restrict__ keys,
+                                                           const uint8_t *__restrict__ mask, int64_t n, int csz,
+                                                           int64_t limit, FoldProps F, RecLayout R,
+                                                           uint32_t *__restrict__ ck, uint4 *__restrict__ rec,
+                                                           int *__restrict__ bad)
+{
+    const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k = keys[i];
+    const uint8_t m = mask[i];
+    const bool ok = k >= 0 && k < limit;
+    if (!ok) *bad = 1;
+    const int64_t kk = ok ? k : 0;
+    ck[i] = (uint32_t)(kk / csz);
+    uint4 r[NV];
+    r[0] = make_uint4((uint32_t)(kk % csz) | ((uint32_t)m << 16), 0, 0, 0);
+    if (NV == 2) r[NV - 1] = make_uint4(0, 0, 0, 0);
+    for (int q = 0; q < F.np; ++q) {
+        const unsigned char *s = F.rval[q];
+        uint64_t v;
+        switch (F.ts[q]) {
+        case 8: v = ((const uint64_t *)s)[i]; break;
+        case 4: v = ((const uint32_t *)s)[i]; break;
+        case 2: v = ((const uint16_t *)s)[i]; break;
+        default: v = s[i]; break;
+        }
+        rec_put<NV>(r, R.off[q], F.ts[q], v);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) rec[i * NV + v] = r[v];
+}
+
+// LDS fold state of one chunk, per wave: seen[csz] (0xff where a record reached the
+// slot, else 0), am[csz] (OR of their masks), then per property csz values at voff[q]
+struct FoldState {
+    int bytes;  // per wave, a multiple of 16
+    int roff;   // staging of a batch of 64 records
+    int voff[kMaxProps];
+};
+
+// Device-coordinate point inserts staged straight as the grouped flush's (chunk id,
+// record) pairs, in insertion order, so that the flush has no insert-buffer pass: the
+// chunk id (as stage_coords_kernel computes the key), then the record: slot offset, mask
+// (masks or EXIST) and each property's value (the source's, or the background) at the
+// layout's offsets.  A coordinate outside the grid gives chunk id 0xffffffff and sets
+// *bad.  A block takes kWG * kStageItems consecutive records, a thread every kWG-th; the
+// records are assembled in registers (the layout is uniform) and stored as 16-byte
+// vectors, a wave's stores covering consecutive records.
 
 template <int NV>
 __global__ __launch_bounds__(kWG) void stage_coord_records_kernel(const int32_t *__restrict__ coords,
@@ -2566,8 +2565,12 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
         rc = radix_sort_records_u32(g, g->d_pins_ck, (uint4 *)g->d_pins_rec, (uint32_t *)b_ckt.p, (uint4 *)b_rect.p,
                                     R.nv, n, cbits, &ck, &rec, (uint32_t *)b_ck.p, (uint4 *)b_rec.p);
     } else {
-        hipLaunchKernelGGL(chunk_records_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key,
-                           g->d_ins_mask, n, csz, limit, F, R, (uint32_t *)b_ck.p, (unsigned char *)b_rec.p, bad);
+        if (R.nv == 1)
+            hipLaunchKernelGGL(chunk_records_kernel<1>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key,
+                               g->d_ins_mask, n, csz, limit, F, R, (uint32_t *)b_ck.p, (uint4 *)b_rec.p, bad);
+        else
+            hipLaunchKernelGGL(chunk_records_kernel<2>, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_ins_key,
+                               g->d_ins_mask, n, csz, limit, F, R, (uint32_t *)b_ck.p, (uint4 *)b_rec.p, bad);
         SG_HIP(hipGetLastError());
         rc = radix_sort_records_u32(g, (uint32_t *)b_ck.p, (uint4 *)b_rec.p, (uint32_t *)b_ckt.p, (uint4 *)b_rect.p,
                                     R.nv, n, cbits, &ck, &rec);
