@@ -112,7 +112,19 @@ __device__ inline int64_t find_chunk_near(const int64_t *keys, int64_t n, int64_
     return (lo < n && keys[lo] == key) ? lo : n;
 }
 
-__global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs ga, int type, int nnn, int32_t *nn)
+// position of every chunk at its id (ids below the table size; the rest of the table -1)
+__global__ void chunk_table_kernel(const int64_t *keys, int64_t n, int32_t *tab)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tab[keys[i]] = (int32_t)i;
+}
+
+// One thread per (chunk, table entry).  DIRECT: the neighbour's position comes from the
+// chunk table (`tab`, ids [0, idmax)); otherwise from a binary search of the sorted ids
+// confined to the positions the id distance allows.
+template <bool DIRECT>
+__global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs ga, int type, int nnn,
+                                       const int32_t *__restrict__ tab, int64_t idmax, int32_t *nn)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * nnn) return;
@@ -136,6 +148,16 @@ __global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs 
         }
         return v;
     };
+    // the position of chunk id `key`, n when there is no such chunk
+    auto find = [&](int64_t key) -> int64_t {
+        if constexpr (DIRECT) {
+            if (key < 0 || key >= idmax) return n;
+            const int32_t p = tab[key];
+            return p < 0 ? n : (int64_t)p;
+        } else {
+            return find_chunk_near(keys, n, c, keys[c], key);
+        }
+    };
     int64_t pos;
     if (type == SG_NN_STAR) {
         pos = -1;
@@ -145,7 +167,7 @@ __global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs 
             bc[d] = dPos;
             int64_t bl = block_lin_id(bc);
             bl = (dPos < 0) ? -1 : bl;
-            pos = find_chunk_near(keys, n, c, keys[c], bl);
+            pos = find(bl);
         }
     } else {
         pos = n;
@@ -155,7 +177,7 @@ __global__ void find_neighbours_kernel(const int64_t *keys, int64_t n, GeomArgs 
                 bc[d] += cnt % 3 - 1;
                 cnt /= 3;
             }
-            pos = find_chunk_near(keys, n, c, keys[c], block_lin_id(bc));
+            pos = find(block_lin_id(bc));
         }
     }
     nn[t] = (int32_t)pos;
@@ -1937,9 +1959,28 @@ int find_neighbours_impl(sg_grid *g, int type)
         g->nn_cap[type] = cap;
     }
     if (n > 0) {
-        hipLaunchKernelGGL(find_neighbours_kernel, dim3(grid_1d(n * nnn, kWG)), dim3(kWG), 0, g->stream, g->d_keys, n,
-                           geom(g), type, nnn, g->d_nn[type]);
-        SG_HIP(hipGetLastError());
+        // a table of chunk positions by id when the domain's id range is not much larger
+        // than the chunk count (one read per lookup instead of a binary search)
+        int64_t last[3] = {0, 0, 0};
+        for (int d = 0; d < g->dim; ++d) last[d] = g->bsz[d] - 1;
+        const int64_t idmax = chunk_id(chunk_lin(g), last) + 1;
+        int32_t *tab = nullptr;
+        if (idmax <= ((int64_t)1 << 26) && idmax <= 64 * std::max<int64_t>(n, 65536))
+            tab = (int32_t *)g->pool.alloc((size_t)idmax * sizeof(int32_t));
+        if (tab) {
+            hipLaunchKernelGGL(fill_range_kernel<int32_t>, dim3(grid_1d(idmax, kWG)), dim3(kWG), 0, g->stream, tab,
+                               idmax, (int32_t)-1);
+            hipLaunchKernelGGL(chunk_table_kernel, dim3(grid_1d(n, kWG)), dim3(kWG), 0, g->stream, g->d_keys, n, tab);
+            hipLaunchKernelGGL(find_neighbours_kernel<true>, dim3(grid_1d(n * nnn, kWG)), dim3(kWG), 0, g->stream,
+                               g->d_keys, n, geom(g), type, nnn, tab, idmax, g->d_nn[type]);
+            SG_HIP(hipGetLastError());
+            g->pool.free(tab);  // stream order: reused only after these kernels
+        } else {
+            hipLaunchKernelGGL(find_neighbours_kernel<false>, dim3(grid_1d(n * nnn, kWG)), dim3(kWG), 0, g->stream,
+                               g->d_keys, n, geom(g), type, nnn, (const int32_t *)nullptr, (int64_t)0,
+                               g->d_nn[type]);
+            SG_HIP(hipGetLastError());
+        }
     }
     g->nn_valid[type] = true;
     return SG_OK;

@@ -252,6 +252,27 @@ def test_neighbour_tables(G, oracle, nn, dim, edge):
     np.testing.assert_array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("lin", ["smb", "zmb"])
+@pytest.mark.parametrize("nn", [C.NN_STAR, C.NN_FULL])
+def test_neighbour_tables_sparse_domain(G, oracle, nn, lin):
+    """A few clustered chunks in a domain of 2^23 chunk positions: far more ids than 64 x
+    the chunk count, where the neighbour search runs on the sorted ids instead of a table
+    of positions by id (row-major and Morton ids), against the oracle."""
+    rng = np.random.default_rng(77)
+    size = [2048, 2048, 1024]
+    centre = rng.integers(2, [254, 254, 126], size=(40, 3))
+    coords = np.concatenate([centre + rng.integers(-1, 2, size=centre.shape) for _ in range(6)])
+    coords = np.unique(coords, axis=0) * 8
+    out = []
+    for cls in (G, oracle.SparseGridCpu):
+        g = cls(3, size, [np.float32], block_edge=8, linearizer=lin)
+        g.insert_coords(coords, [np.ones(len(coords), np.float32)])
+        g.flush(["smax"])
+        out.append(g.neighbours(nn))
+    assert len(out[0]) > 0
+    np.testing.assert_array_equal(out[0], out[1])
+
+
 def _shell(cls, n, lo, hi, props, edge=8, vals=None):
     pts = W.shell_points(n, lo, hi)
     g = cls(3, (n, n, n), props, block_edge=edge)
