@@ -1910,6 +1910,22 @@ __global__ __launch_bounds__(kWG) void count_mask_kernel(const uint8_t *mask, in
     if (threadIdx.x == 0 && s) atomicAdd(out, s);
 }
 
+// points with EXIST from the 8^3 EXIST bit-planes (nw 64-bit words): 64 B per chunk
+// instead of 512
+__global__ __launch_bounds__(kWG) void count_planes_kernel(const uint64_t *eb, int64_t nw, unsigned long long *out)
+{
+    __shared__ unsigned long long s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    unsigned long long c = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += stride) c += __popcll(eb[i]);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s, c);
+    __syncthreads();
+    if (threadIdx.x == 0 && s) atomicAdd(out, s);
+}
+
 template <typename F> void dispatch_type(int dt, F &&f)
 {
     switch (dt) {
@@ -2434,7 +2450,14 @@ int count_mask_impl(sg_grid *g, int which, int64_t *out)
     unsigned long long *d = (unsigned long long *)b.p;
     SG_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), g->stream));
     const int64_t total = g->nchunks * g->csz;
-    if (total > 0) {
+    if (total > 0 && which == 0 && g->dim == 3 && g->edge == 8 && g->ebits_valid) {
+        // existing points: the EXIST bit-planes the flush keeps current
+        const int64_t nw = g->nchunks * 8;
+        unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_1d(nw, kWG), 2048));
+        hipLaunchKernelGGL(count_planes_kernel, dim3(blocks), dim3(kWG), 0, g->stream, (const uint64_t *)g->d_ebits, nw,
+                           d);
+        SG_HIP(hipGetLastError());
+    } else if (total > 0) {
         unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_1d(total / 16, kWG), 2048));
         hipLaunchKernelGGL(count_mask_kernel, dim3(blocks), dim3(kWG), 0, g->stream, g->d_mask, total, which, d);
         SG_HIP(hipGetLastError());

@@ -273,6 +273,35 @@ def test_neighbour_tables_sparse_domain(G, oracle, nn, lin):
     np.testing.assert_array_equal(out[0], out[1])
 
 
+def test_count_existing_through_planes_and_masks(G, oracle):
+    """countExistingElements on 8^3 chunks reads the EXIST bit-planes while the flush
+    keeps them current, the masks otherwise (after a removal, after the mask array was
+    handed out); countBoundaryElements reads the masks: every count equals the oracle's."""
+    rng = np.random.default_rng(91)
+    size = (72, 64, 56)
+    steps = S.random_flush_program(91, 3, 8, list(size), [np.int32], nflush=2, nrec=4000)
+    grids = [S.run_flush_program(cls, 3, 8, list(size), [np.int32], steps) for cls in (G, oracle.SparseGridCpu)]
+
+    def same():
+        for f in ("count_existing_elements", "count_boundary_elements"):
+            assert getattr(grids[0], f)() == getattr(grids[1], f)(), f
+
+    same()
+    for g in grids:
+        g.tag_boundaries(C.NN_STAR)
+    same()
+    keys = rng.choice(grids[1].chunk_ids(), 20) * 512 + rng.integers(0, 512, 20)
+    for g in grids:
+        g.remove_points(keys)
+    same()
+    grids[0].device_ptr(-1)
+    same()
+    for g in grids:
+        g.insert(keys, [np.arange(20, dtype=np.int32)])
+        g.flush(["smax"])
+    same()
+
+
 def _shell(cls, n, lo, hi, props, edge=8, vals=None):
     pts = W.shell_points(n, lo, hi)
     g = cls(3, (n, n, n), props, block_edge=edge)
