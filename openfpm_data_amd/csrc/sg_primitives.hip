@@ -299,7 +299,7 @@ constexpr int kRecTile = kSortWG * kRecIPT;
 // workgroup loaded before the first is counted)
 constexpr int kHistTiles = 4;
 __global__ __launch_bounds__(kSortWG) void radix_hist_rec_kernel(const uint32_t *keys, uint32_t *counts, int64_t n,
-                                                                 int shift, int ntiles)
+                                                                 int shift, uint32_t dmask, int ntiles)
 {
     __shared__ uint32_t s_hist[kHistTiles][256];
     const int t = threadIdx.x;
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(kSortWG) void radix_hist_rec_kernel(const uint32_t 
 #pragma unroll
         for (int j = 0; j < kRecIPT; ++j) {
             const int64_t i = base + (int64_t)j * kSortWG + t;
-            d[h][j] = (i < n) ? (keys[i] >> shift) & 255u : 256u;
+            d[h][j] = (i < n) ? (keys[i] >> shift) & dmask : 256u;
         }
     }
     __syncthreads();
@@ -336,9 +336,10 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_rec_kernel(const uint32
                                                                     uint4 *__restrict__ orecs,
                                                                     const uint32_t *__restrict__ offsets,
                                                                     const uint32_t *__restrict__ counts, int64_t n,
-                                                                    int shift, int ntiles)
+                                                                    int shift, int dbits, int ntiles)
 {
     __shared__ uint32_t s_key[kRecTile];
+    const uint32_t dmask = (1u << dbits) - 1u;
     __shared__ uint4 s_rec[kRecTile * NV];
     __shared__ uint32_t s_gbase[256];
     __shared__ uint32_t s_lstart[256];
@@ -371,10 +372,9 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_rec_kernel(const uint32
 #pragma unroll
     for (int j = 0; j < kRecIPT; ++j) {
         const bool valid = base + (int64_t)j * 64 + lane < n;
-        const uint32_t d = (kk[j] >> shift) & 255u;
+        const uint32_t d = (kk[j] >> shift) & dmask;
         uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < dbits; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_rec_kernel(const uint32
 #pragma unroll
     for (int j = 0; j < kRecIPT; ++j) {
         if (base + (int64_t)j * 64 + lane < n) {
-            const uint32_t d = (kk[j] >> shift) & 255u;
+            const uint32_t d = (kk[j] >> shift) & dmask;
             const uint32_t pos = s_lstart[d] + s_wcnt[w][d] + wr[j];
             s_key[pos] = kk[j];
 #pragma unroll
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_rec_kernel(const uint32
     const int m = (int)((n - tbase < kRecTile) ? (n - tbase) : kRecTile);
     for (int k = t; k < m; k += kSortWG) {
         const uint32_t key = s_key[k];
-        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t d = (key >> shift) & dmask;
         const uint32_t pos = s_gbase[d] + (uint32_t)k - s_lstart[d];
         okeys[pos] = key;
 #pragma unroll
@@ -438,19 +438,21 @@ int radix_sort_records_u32(sg_grid *g, uint32_t *keys, uint4 *recs, uint32_t *ke
     uint32_t *offs = counts + (size_t)256 * ntiles;
     uint32_t *src_k = keys, *dst_k = keys_tmp;
     uint4 *src_r = recs, *dst_r = recs_tmp;
-    for (int shift = 0; shift < bits; shift += 8) {
+    // as few passes as 8-bit digits need, the bits shared evenly among them (21 bits: 3
+    // passes of 7, so that a tile's run of one digit is twice as long as with 256 digits)
+    const int passes = (bits + 7) / 8, dbits = (bits + passes - 1) / passes;
+    for (int shift = 0; shift < bits; shift += dbits) {
         hipLaunchKernelGGL(radix_hist_rec_kernel, dim3((ntiles + kHistTiles - 1) / kHistTiles), dim3(kSortWG), 0,
-                           g->stream, src_k, counts, n, shift,
-                           ntiles);
+                           g->stream, src_k, counts, n, shift, (1u << dbits) - 1u, ntiles);
         SG_HIP(hipGetLastError());
         rc = exclusive_scan_u32(g, counts, offs, (int64_t)256 * ntiles, nullptr);
         if (rc) return rc;
         if (nv == 1)
             hipLaunchKernelGGL(radix_scatter_rec_kernel<1>, dim3(ntiles), dim3(kSortWG), 0, g->stream, src_k, src_r,
-                               dst_k, dst_r, offs, counts, n, shift, ntiles);
+                               dst_k, dst_r, offs, counts, n, shift, dbits, ntiles);
         else
             hipLaunchKernelGGL(radix_scatter_rec_kernel<2>, dim3(ntiles), dim3(kSortWG), 0, g->stream, src_k, src_r,
-                               dst_k, dst_r, offs, counts, n, shift, ntiles);
+                               dst_k, dst_r, offs, counts, n, shift, dbits, ntiles);
         SG_HIP(hipGetLastError());
         std::swap(src_k, dst_k);
         std::swap(src_r, dst_r);
