@@ -1918,8 +1918,15 @@ __global__ __launch_bounds__(kWG) void count_planes_kernel(const uint64_t *eb, i
     if (threadIdx.x == 0) s = 0;
     __syncthreads();
     unsigned long long c = 0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += stride) c += __popcll(eb[i]);
+    // 8 consecutive words (one chunk) per thread and step, loaded before they are counted
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * 8;
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8; i < nw; i += stride) {
+        uint64_t w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = (i + k < nw) ? eb[i + k] : 0ull;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c += __popcll(w[k]);
+    }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s, c);
     __syncthreads();
@@ -2453,12 +2460,13 @@ int count_mask_impl(sg_grid *g, int which, int64_t *out)
     if (total > 0 && which == 0 && g->dim == 3 && g->edge == 8 && g->ebits_valid) {
         // existing points: the EXIST bit-planes the flush keeps current
         const int64_t nw = g->nchunks * 8;
-        unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_1d(nw, kWG), 2048));
+        // one workgroup per CU: each adds its count to the total by one global atomic
+        unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_1d(nw, kWG * 8), 256));
         hipLaunchKernelGGL(count_planes_kernel, dim3(blocks), dim3(kWG), 0, g->stream, (const uint64_t *)g->d_ebits, nw,
                            d);
         SG_HIP(hipGetLastError());
     } else if (total > 0) {
-        unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_1d(total / 16, kWG), 2048));
+        unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_1d(total / 16, kWG), 256));
         hipLaunchKernelGGL(count_mask_kernel, dim3(blocks), dim3(kWG), 0, g->stream, g->d_mask, total, which, d);
         SG_HIP(hipGetLastError());
     }
