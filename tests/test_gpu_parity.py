@@ -140,9 +140,9 @@ def test_staged_device_coordinate_inserts(G, oracle, seed):
     at the next flush), against the oracle fed the same records as keys."""
     import torch
     rng = np.random.default_rng(4000 + seed)
-    dim, edge = 3, 8
+    dim, edge = (3, 8) if seed < 6 else (2, 16)
     # whole chunks: every point key of a chunk is then a coordinate inside the grid
-    size = [8 * int(rng.integers(3, 9)) for _ in range(dim)]
+    size = [edge * int(rng.integers(3, 9)) for _ in range(dim)]
     dtypes = [[np.float32, np.int32, np.uint32], [np.float64, np.int32, np.uint8], [np.int16],
               [np.uint8, np.float64], [np.float32], [np.int64, np.uint16, np.int8],
               [np.float32, np.float32], [np.uint32, np.uint32, np.uint32]][seed]
@@ -186,7 +186,7 @@ def test_staged_device_coordinate_inserts(G, oracle, seed):
             coords = np.stack([rng.integers(0, s, n) for s in size], axis=1).astype(np.int32)
             bad = rng.choice(n, 7, replace=False)
             coords[bad[:4], 0] = size[0] + rng.integers(0, 5, 4)
-            coords[bad[4:], 2] = -1 - rng.integers(0, 3, 3)
+            coords[bad[4:], dim - 1] = -1 - rng.integers(0, 3, 3)
             vals = [S._rand_vals(rng, dt, n) for dt in dtypes]
             masks = rng.choice(np.array([1, 1, 3, 0, 2], np.uint8), n)
             good = np.setdiff1d(np.arange(n), bad)
