@@ -178,7 +178,9 @@ int sg_insert_points(sg_grid *g, int64_t n, const int64_t *keys,
 /* same, with integer coordinates [n][dim] instead of linear keys; host coordinates
  * outside the grid fail here (SG_E_RANGE), device ones fail the next sg_flush (which then
  * drops those records only: the other records stay pending, unmerged, for the next
- * sg_flush), so the call does not wait for the device */
+ * sg_flush), so the call does not wait for the device.  Device coordinates inserted
+ * while nothing else is pending are staged in the flush's own (chunk id, record) form;
+ * any other insert moves them to the insert buffer first (same order, same result). */
 int sg_insert_coords(sg_grid *g, int64_t n, const int32_t *coords,
                      const uint8_t *masks, const void *const *values, int mem);
 /* Chunk insertion: insertBlock(blockId) per chunk slot
