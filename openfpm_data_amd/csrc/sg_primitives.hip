@@ -325,7 +325,7 @@ __global__ __launch_bounds__(kSortWG) void radix_hist_rec_kernel(const uint32_t 
 #pragma unroll
     for (int h = 0; h < kHistTiles; ++h) {
         const int64_t tile = (int64_t)blockIdx.x * kHistTiles + h;
-        if (tile < ntiles) counts[(int64_t)t * ntiles + tile] = s_hist[h][t];
+        if (tile < ntiles && (uint32_t)t <= dmask) counts[(int64_t)t * ntiles + tile] = s_hist[h][t];
     }
 }
 
@@ -345,9 +345,11 @@ __global__ __launch_bounds__(kSortWG) void radix_scatter_rec_kernel(const uint32
     __shared__ uint32_t s_lstart[256];
     __shared__ uint32_t s_wcnt[kSortWG / 64][256];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    s_gbase[t] = offsets[(int64_t)t * ntiles + blockIdx.x];
+    // the count table holds the 2^dbits digits of this pass
+    const bool dig = (uint32_t)t <= dmask;
+    s_gbase[t] = dig ? offsets[(int64_t)t * ntiles + blockIdx.x] : 0u;
     uint32_t tot;
-    s_lstart[t] = block_exclusive_scan(counts[(int64_t)t * ntiles + blockIdx.x], &tot);
+    s_lstart[t] = block_exclusive_scan(dig ? counts[(int64_t)t * ntiles + blockIdx.x] : 0u, &tot);
 #pragma unroll
     for (int q = 0; q < kSortWG / 64; ++q) s_wcnt[q][t] = 0;
     // Wave w owns the tile's items [w * 64 * IPT, (w + 1) * 64 * IPT), item j of a lane at
@@ -445,7 +447,7 @@ int radix_sort_records_u32(sg_grid *g, uint32_t *keys, uint4 *recs, uint32_t *ke
         hipLaunchKernelGGL(radix_hist_rec_kernel, dim3((ntiles + kHistTiles - 1) / kHistTiles), dim3(kSortWG), 0,
                            g->stream, src_k, counts, n, shift, (1u << dbits) - 1u, ntiles);
         SG_HIP(hipGetLastError());
-        rc = exclusive_scan_u32(g, counts, offs, (int64_t)256 * ntiles, nullptr);
+        rc = exclusive_scan_u32(g, counts, offs, ((int64_t)1 << dbits) * ntiles, nullptr);
         if (rc) return rc;
         if (nv == 1)
             hipLaunchKernelGGL(radix_scatter_rec_kernel<1>, dim3(ntiles), dim3(kSortWG), 0, g->stream, src_k, src_r,
