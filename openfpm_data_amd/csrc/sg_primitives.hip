@@ -102,14 +102,21 @@ __global__ __launch_bounds__(kScanWG) void scan_reduce_kernel(const uint32_t *in
     if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
-// exclusive scan of the tile (in place allowed), offset by block_offsets[tile]
+// exclusive scan of the tile (in place allowed), offset by block_offsets[tile], or with
+// block_sums (the tiles' totals, unscanned) by the sum of the earlier tiles' totals
 template <bool VEC>
 __global__ __launch_bounds__(kScanWG) void scan_down_kernel(const uint32_t *in, uint32_t *out,
                                                             const uint32_t *block_offsets, int64_t n,
-                                                            uint32_t *total_out)
+                                                            uint32_t *total_out, const uint32_t *block_sums)
 {
     __shared__ uint32_t s_wave[kScanPass][kScanWG / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t prior = 0;
+    if (block_sums) {
+        uint32_t p = 0;
+        for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += kScanWG) p += block_sums[i];
+        block_exclusive_scan(p, &prior);
+    }
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanVec;
     uint32_t v[kScanPass][kScanVec], inc[kScanPass];
 #pragma unroll
@@ -119,7 +126,7 @@ __global__ __launch_bounds__(kScanWG) void scan_down_kernel(const uint32_t *in, 
         if (lane == 63) s_wave[j][w] = inc[j];
     }
     __syncthreads();
-    uint32_t off = block_offsets ? block_offsets[blockIdx.x] : 0u;
+    uint32_t off = block_offsets ? block_offsets[blockIdx.x] : prior;
 #pragma unroll
     for (int j = 0; j < kScanPass; ++j) {
         uint32_t pre = 0, tot = 0;
@@ -520,21 +527,29 @@ int exclusive_scan_u32_ws(hipStream_t stream, const uint32_t *in, uint32_t *out,
         SG_HIP(hipGetLastError());
         lin = sums[l + 1];
     }
-    auto down = [&](const uint32_t *src, uint32_t *dst, const uint32_t *bo, int64_t len, uint32_t *tot) {
+    auto down = [&](const uint32_t *src, uint32_t *dst, const uint32_t *bo, int64_t len, uint32_t *tot,
+                    const uint32_t *bs) {
         const dim3 grid((unsigned)((len + kScanTile - 1) / kScanTile));
         if (aligned(src) && aligned(dst))
-            hipLaunchKernelGGL(scan_down_kernel<true>, grid, dim3(kScanWG), 0, stream, src, dst, bo, len, tot);
+            hipLaunchKernelGGL(scan_down_kernel<true>, grid, dim3(kScanWG), 0, stream, src, dst, bo, len, tot, bs);
         else
-            hipLaunchKernelGGL(scan_down_kernel<false>, grid, dim3(kScanWG), 0, stream, src, dst, bo, len, tot);
+            hipLaunchKernelGGL(scan_down_kernel<false>, grid, dim3(kScanWG), 0, stream, src, dst, bo, len, tot, bs);
     };
+    if (nl == 2) {
+        // one level of tiles (up to kScanTile of them): each tile adds up the earlier
+        // tiles' totals itself, no scan of the totals between the two launches
+        down(in, out, nullptr, n, d_total, sums[1]);
+        SG_HIP(hipGetLastError());
+        return SG_OK;
+    }
     for (int l = nl - 1; l >= 1; --l) {
         // scan level l (its single top tile or using offsets from level l+1)
         const uint32_t *bo = (l + 1 < nl) ? offs[l + 1] : nullptr;
-        down(sums[l], offs[l], bo, sizes[l], (uint32_t *)nullptr);
+        down(sums[l], offs[l], bo, sizes[l], (uint32_t *)nullptr, (const uint32_t *)nullptr);
         SG_HIP(hipGetLastError());
     }
     // the inclusive total is written by the thread owning item n-1 (works in place)
-    down(in, out, nl > 1 ? offs[1] : nullptr, n, d_total);
+    down(in, out, nl > 1 ? offs[1] : nullptr, n, d_total, (const uint32_t *)nullptr);
     SG_HIP(hipGetLastError());
     return SG_OK;
 }
