@@ -2524,17 +2524,13 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
     int rc;
     uint32_t *hw = host_words(g);
     if (!hw) return fail(SG_E_OOM, "pinned host allocation failed");
-    // records staged as (chunk id, record) pairs are sorted as they are, unless a staged
-    // coordinate was outside the grid: then they go through the insert buffer below (the
-    // flag is read before the sort, which overwrites the staged order)
+    // records staged as (chunk id, record) pairs are sorted as they are (the sort only
+    // reads them); when a staged coordinate was outside the grid (the flag comes back with
+    // the first scan total below) they go through the insert buffer instead
     bool staged = g->pins_n > 0;
-    if (staged) {
-        SG_HIP(hipMemcpyAsync(hw + 9, g->d_pins_bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
-        SG_HIP(hipStreamSynchronize(g->stream));
-        if (hw[9] || R.nv != g->pins_nv) {
-            if ((rc = convert_point_records(g))) return rc;
-            staged = false;
-        }
+    if (staged && R.nv != g->pins_nv) {
+        if ((rc = convert_point_records(g))) return rc;
+        staged = false;
     }
     const int64_t n = staged ? g->pins_n : g->ins_n;
     if ((rc = ensure(g->pool, g->scratch[6], 64))) return rc;
@@ -2576,8 +2572,9 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
                                     R.nv, n, cbits, &ck, &rec);
     }
     if (rc) return rc;
-    // the range flag, read with the scan total below
+    // the range flags, read with the scan total below
     SG_HIP(hipMemcpyAsync(hw + 8, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+    if (staged) SG_HIP(hipMemcpyAsync(hw + 9, g->d_pins_bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
     // touched chunks: runs of equal sorted chunk ids, counted per tile, tile offsets
     // scanned, then each run's id and first record written at its offset
     const int64_t rtiles = (n + kRunTile - 1) / kRunTile;
@@ -2589,6 +2586,12 @@ int flush_grouped_impl(sg_grid *g, const int32_t *ops, bool *taken)
     SG_HIP(hipGetLastError());
     uint32_t nn32 = 0;
     if ((rc = exclusive_scan_u32(g, rcnt, roff, rtiles, &nn32))) return rc;
+    if (staged && hw[9]) {
+        // the staged pairs, intact, through the insert buffer: the flush by chunk id over it
+        // drops the records outside the grid and reports them
+        if ((rc = convert_point_records(g))) return rc;
+        return flush_grouped_impl(g, ops, taken);
+    }
     if (hw[8]) {
         // the records outside the grid are dropped, the others stay pending (nothing is merged)
         int64_t kept = 0;
